@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark: assembly + GMRES time per Newton step to relative residual
+1e-8, FP64 vs mixed precision, on one MI355X.
+
+Workload (default): incompressible Navier-Stokes VMS (FLUID3D_M/C), tet4,
+straight cylinder r = 1, L = 10, ~200k tets from a seeded Delaunay
+tetrahedralisation of jittered lattice points; rho = 1.06, mu = 0.04,
+parabolic inlet peak 30, zero-traction outlet with backflow stabilisation,
+no-slip wall, generalized-alpha rho_inf = 0.2, dt = 1e-3, random initial
+velocity U(-1, 1); one Newton step = zero + element assembly + face terms +
+FSILS_SOLVE with GMRES(50) and diagonal scaling to 1e-8 in the reference's
+scaled norm.  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  `value` is the mixed-precision path's time per
+Newton step (ms, lower is better); the FP64 path, the SpMV roofline and the
+CPU baseline (the FP64 C oracle on this host, one core) are reported beside
+it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(nnzb: int, nNo: int, dof: int, s: int) -> int:
+    """Algorithmic bytes of one block SpMV: values, column indices, row
+    pointers, one read of U and one write of KU."""
+    return nnzb * dof * dof * s + 4 * nnzb + 4 * (nNo + 1) + 2 * nNo * dof * s
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--h", type=float, default=0.11, help="cylinder lattice spacing (0.11 -> ~200k tets)")
+    ap.add_argument("--sD", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the multi-GPU partitioning is a design note, not an implemented path", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from svfsi_amd import problems
+    from svfsi_amd._lib import lib
+    from svfsi_amd.step import NewtonStep
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+
+    t0 = time.time()
+    prob = problems.fluid_cylinder(h=args.h, sD=args.sD)
+    st = NewtonStep(prob)
+    t_setup = time.time() - t0
+    m = prob.mesh
+    nnzb = st.nnz
+    dof = prob.dof
+    print(f"# setup {t_setup:.1f}s nNo={m.nNo} nEl={m.nEl} nnzb={nnzb} colors={st.ncolors}", file=sys.stderr,
+          flush=True)
+
+    def run(mixed: bool):
+        ls = st.step(mixed=mixed)
+        return ls
+
+    # warmup both paths
+    for _ in range(args.warmup):
+        run(False)
+        run(True)
+    torch.cuda.synchronize()
+
+    def timed(mixed: bool, k: int):
+        L = lib()
+        L.svfsi_prof_reset()
+        L.svfsi_prof_enable(1)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        lss = []
+        for _ in range(k):
+            lss.append(run(mixed))
+        ev1.record()
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        L.svfsi_prof_enable(0)
+        import ctypes as C
+
+        cnt = [C.c_longlong(), C.c_longlong()]
+        tot = [C.c_double(), C.c_double()]
+        for kk in (0, 1):
+            L.svfsi_prof_get(kk, C.byref(cnt[kk]), C.byref(tot[kk]))
+        return (b - a) * 1e3 / k, lss, [(cnt[q].value, tot[q].value) for q in (0, 1)]
+
+    # phase split (outside the timed loops): assembly alone
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(max(1, args.steps)):
+        st.assemble()
+    torch.cuda.synchronize()
+    asm_ms = (time.perf_counter() - a) * 1e3 / max(1, args.steps)
+
+    fp64_ms, ls64, prof64 = timed(False, args.steps)
+    mix_ms, lsmx, profmx = timed(True, args.steps)
+
+    # SpMV roofline: FP32 SpMV of the mixed path (the inner solver's kernel)
+    n32, t32 = profmx[1]
+    n64, t64 = prof64[0]
+    b32 = spmv_bytes(nnzb, m.nNo, dof, 4)
+    b64 = spmv_bytes(nnzb, m.nNo, dof, 8)
+    gbs32 = b32 / (t32 / n32 * 1e-3) / 1e9 if n32 else None
+    gbs64 = b64 / (t64 / n64 * 1e-3) / 1e9 if n64 else None
+
+    # accuracy of this step against the CPU oracle (outside the timed region)
+    cpu = None
+    err_fp64 = err_mixed = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        st.assemble()
+        torch.cuda.synchronize()
+        V0 = st.Val.cpu().numpy().copy()
+        R0 = st.R.cpu().numpy().copy()
+        ls = st.solve(mixed=False)
+        x64 = st.R.cpu().numpy().copy()
+        st.assemble()
+        st.solve(mixed=True)
+        xmx = st.R.cpu().numpy().copy()
+        # CPU baseline: the FP64 oracle Newton step (assembly + FSILS_SOLVE) on one core
+        c0 = time.perf_counter()
+        Vo, Ro = orc.assemble(prob, st.rowPtr - 1, st.colPtr - 1)
+        c1 = time.perf_counter()
+        xo, _, so = orc.fsils_solve(st.rowPtr - 1, st.colPtr - 1, dof, prob.dir_mask(), prob.ls, Ro, Vo)
+        c2 = time.perf_counter()
+        nrm = np.abs(xo).max()
+        err_fp64 = float(np.abs(x64 - xo).max() / nrm)
+        err_mixed = float(np.abs(xmx - xo).max() / nrm)
+        asm_err = float(np.abs(V0 - Vo).max() / np.abs(Vo).max())
+        cpu = {
+            "value": round((c2 - c0) * 1e3, 1), "unit": "ms/step", "cores": 1, "kind": "port",
+            "sample": f"one full FP64 Newton step of the same workload on the C oracle "
+                      f"(assembly {round((c1 - c0) * 1e3)} ms + GMRES {round((c2 - c1) * 1e3)} ms, "
+                      f"{so.itr} iterations)",
+            "assembly_rel_err_vs_gpu_fp64": asm_err,
+        }
+
+    speed = fp64_ms / mix_ms if mix_ms > 0 else None
+    roof = None
+    if gbs32 is not None:
+        roof = {"kernel": "k_bspmv<4,float,float> (FP32 block SpMV, inner GMRES)",
+                "bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": b32, "avg_launch_us": round(t32 / n32 * 1e3, 2), "launches": n32}
+    out = {
+        "metric": "Assembly+GMRES time per Newton step at 1e-8 residual; FP64 vs mixed; SpMV GB/s",
+        "value": round(mix_ms, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mix_ms, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64+f32 (mixed: FP32 inner solve, FP64 residual/refinement)",
+        "data": "synthetic",
+        "config": {"workload": "FLUID3D VMS tet4 cylinder r=1 L=10, one Newton step, GMRES(%d) to 1e-8" % args.sD,
+                   "nNo": m.nNo, "nEl": m.nEl, "nnzb": nnzb, "dof": dof, "h": args.h, "colors": st.ncolors},
+        "fp64_ms_per_step": round(fp64_ms, 3),
+        "assembly_ms": round(asm_ms, 3),
+        "speedup_mixed_vs_fp64": None if speed is None else round(speed, 3),
+        "fp64_gmres_itr": ls64[-1].RI.itr,
+        "mixed_outer_itr": lsmx[-1].RI.itr,
+        "mixed_inner_spmv": lsmx[-1].RI.cM,
+        "suc": [int(ls64[-1].RI.suc), int(lsmx[-1].RI.suc)],
+        "spmv_fp64_gbs": None if gbs64 is None else round(gbs64, 1),
+        "spmv_fp32_gbs": None if gbs32 is None else round(gbs32, 1),
+        "rel_err_vs_cpu_fp64": err_fp64,
+        "rel_err_vs_cpu_mixed": err_mixed,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu is not None:
+        out["speedup_vs_cpu_mixed"] = round(cpu["value"] / mix_ms, 1)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

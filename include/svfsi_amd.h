@@ -1,0 +1,275 @@
+/*
+ * svfsi_amd.h -- C ABI of libsvfsi_amd.so, the MI355X (gfx950) implementation
+ * of svFSI's per-Newton-step linear-system path: element tangent/residual
+ * construction + DOASSEM scatter into svFSILS block-CSR, then the svFSILS
+ * Krylov solve with diagonal (Jacobi) scaling.
+ *
+ * Reference interfaces replaced (paths under Code/Source/ of SimVascular/svFSI):
+ *   svFSILS/FSILS_api_c.h:52-64      C prototypes fsils_*_ (gfortran mangling)
+ *   svFSILS/FSILS_API.h:51-135       Fortran explicit interfaces FSILS_*
+ *   svFSILS/FSILS_struct_c.h:115-140 C layout of FSILS_lsType / subLsType
+ *   svFSI/EQASSEM.f:38-87            GLOBALEQASSEM element loop (no C ABI in
+ *                                    the reference; precedent: the Trilinos
+ *                                    sink trilinos_doassem_,
+ *                                    svFSI/trilinos_linear_solver.h:201-208)
+ *
+ * Conventions
+ *   - Plain C types only; no torch or HIP types in signatures.  A stream is
+ *     passed as void* (a hipStream_t, NULL = the default stream).
+ *   - Functions with a _dev suffix take DEVICE pointers (memory resident in
+ *     HBM, e.g. from hipMalloc or a torch tensor); the others take HOST
+ *     pointers and copy.
+ *   - Indices that cross the ABI follow the reference: 1-based rowPtr /
+ *     colPtr / gNodes / IEN, Fortran column-major arrays: Ri(dof,nNo),
+ *     Val(dof*dof,nnz) with each dof x dof block row-major
+ *     (Val((i-1)*dof+j,k) = dR_i/dU_j), x(3,nNo), IEN(eNoN,nEl).
+ *   - Every entry point returns 0 on success or a negative SVFSI_ERR_* code;
+ *     svfsi_last_error() describes the failure.  The reference STOPs on
+ *     misuse (e.g. svFSILS/LS.f:60-63, SOLVE.f:78-81); this library returns
+ *     the error instead.  Non-convergence is not an error: as in the
+ *     reference it is reported through ls->RI.suc = 0.
+ *   - There is no CPU fallback: every compute entry point runs HIP kernels
+ *     and fails with SVFSI_ERR_NODEVICE when no GPU is present.
+ */
+#ifndef SVFSI_AMD_H
+#define SVFSI_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Codes (svFSILS/FSILS_STRUCT.h:53-62)                                    */
+/* ---------------------------------------------------------------------- */
+#define LS_TYPE_CG     798
+#define LS_TYPE_GMRES  797
+#define LS_TYPE_NS     796
+#define LS_TYPE_BICGS  795
+#define PRECOND_FSILS  701
+#define PRECOND_RCS    709
+#define BC_TYPE_Dir    0
+#define BC_TYPE_Neu    1
+
+#define SVFSI_OK            0
+#define SVFSI_ERR_ARG      -1  /* bad argument (reference: FATAL ERROR STOP) */
+#define SVFSI_ERR_HIP      -2  /* HIP runtime error */
+#define SVFSI_ERR_STATE    -3  /* object not created / already created */
+#define SVFSI_ERR_NOTSUP   -4  /* combination the reference supports but this build does not */
+#define SVFSI_ERR_NODEVICE -5  /* no HIP device */
+
+/* FSILS_subLsType, same layout as svFSILS/FSILS_struct_c.h:115-130 */
+typedef struct {
+    int suc;       /* successful solve (OUT) */
+    int mItr;      /* max iterations / restarts (IN) */
+    int sD;        /* Krylov space dimension (IN) */
+    int itr;       /* iterations performed (OUT) */
+    int cM, cN, cD;/* counters (OUT) */
+    int reserve;
+    double absTol; /* absolute tolerance (IN) */
+    double relTol; /* relative tolerance (IN) */
+    double iNorm;  /* initial residual norm (OUT) */
+    double fNorm;  /* final residual norm (OUT) */
+    double dB;     /* residual reduction (OUT) */
+    double callD;  /* wall time of the call, s (OUT) */
+} FSILS_subLsType;
+
+/* FSILS_lsType, same layout as svFSILS/FSILS_struct_c.h:132-140 */
+typedef struct {
+    int foC;
+    int LS_type;
+    int Resm, Resc;
+    FSILS_subLsType GM;
+    FSILS_subLsType CG;
+    FSILS_subLsType RI;
+} FSILS_lsType;
+
+const char* svfsi_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int svfsi_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* svFSILS drop-in layer                                                   */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces fsils_ls_create_ (FSILS_api_c.h:58; FSILS_LS_CREATE, svFSILS/LS.f:50-119).
+ * Optional arguments are passed as NULL.  Fills defaults exactly as the
+ * reference: GMRES relTol 0.1, mItr 4, sD 250; CG relTol 1e-2, mItr 1000;
+ * absTol 1e-10. */
+int fsils_gpu_ls_create(FSILS_lsType* ls, int LS_type, const double* relTol,
+                        const double* absTol, const int* maxItr,
+                        const int* dimKry, const double* relTolIn,
+                        const double* absTolIn, const int* maxItrIn);
+
+/* Replaces fsils_lhs_create_c_ (FSILS_api_c.h:55; FSILS_LHS_CREATE,
+ * svFSILS/LHS.f:51-111 one-rank branch).  rowPtr(nNo+1), colPtr(nnz) are
+ * 1-based host arrays; the pattern is copied to the device, diagPtr is
+ * built, and a transpose-position map is built for the adjoint.  gNodes is
+ * accepted for signature parity (one rank: identity map, LHS.f:106). */
+int fsils_gpu_lhs_create(void** lhs, int gnNo, int nNo, int nnz,
+                         const int* gNodes, const int* rowPtr,
+                         const int* colPtr, int nFaces);
+/* Replaces fsils_lhs_free_ (FSILS_api_c.h:56; LHS.f). */
+int fsils_gpu_lhs_free(void* lhs);
+
+/* Replaces fsils_bc_create_ (FSILS_api_c.h:61; FSILS_BC_CREATE,
+ * svFSILS/BC.f:50-121).  faIn is 1-based; gNodes 1-based; Val(dof,nNo)
+ * may be NULL (then 0, a homogeneous Dirichlet mask).  BC_TYPE_Neu faces
+ * are stored but coupled Neumann (res != 0) solves are not supported. */
+int fsils_gpu_bc_create(void* lhs, int faIn, int nNo, int dof, int BC_type,
+                        const int* gNodes, const double* Val);
+/* Replaces fsils_bc_free_ (FSILS_api_c.h:62). */
+int fsils_gpu_bc_free(void* lhs, int faIn);
+
+/* Replaces fsils_solve_ (FSILS_api_c.h:64; FSILS_SOLVE, svFSILS/SOLVE.f:51-143)
+ * with HOST arrays: Ri(dof,nNo) is the RHS on entry and the solution on
+ * exit; Val(dof*dof,nnz) is left scaled in place (W K W), as the reference
+ * does (SOLVE.f:57, PRECOND.f:124-130).  prec must be PRECOND_FSILS;
+ * LS_type GMRES or CG (dof >= 2).  incL(nFaces) may be NULL (all faces
+ * included); res(nFaces) may be NULL. */
+int fsils_gpu_solve(void* lhs, FSILS_lsType* ls, int dof, double* Ri,
+                    double* Val, int prec, const int* incL, const double* res);
+
+/* Same as fsils_gpu_solve with DEVICE arrays dRi, dVal (no host copies). */
+int fsils_gpu_solve_dev(void* lhs, FSILS_lsType* ls, int dof, double* dRi,
+                        double* dVal, int prec, const int* incL,
+                        const double* res, void* stream);
+
+/* Mixed-precision form of fsils_gpu_solve_dev (no reference counterpart;
+ * the reference is FP64-only, svFSILS/FSILS_TYPEDEF.h:55-57).  Same
+ * arguments, convergence test and outputs.  The scaled matrix is copied to
+ * FP32 and used for the inner Krylov iterations (FP32 SpMV, FP32 basis
+ * storage, FP64 dot accumulation, FP64 Hessenberg/Givens); an FP64 outer
+ * loop (flexible GMRES / iterative refinement) recomputes the residual
+ * W(R - K x) with the FP64 matrix, so the reported residual is the FP64
+ * one in the reference's scaled norm.  ls->RI.itr counts FP64 outer
+ * residual evaluations; ls->RI.cM counts inner FP32 SpMVs. */
+int fsils_gpu_solve_mixed_dev(void* lhs, FSILS_lsType* ls, int dof,
+                              double* dRi, double* dVal, int prec,
+                              const int* incL, const double* res, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Kernel-level entry points of the solve (device arrays)                  */
+/* ---------------------------------------------------------------------- */
+
+/* FSILS_SPARMULVV (svFSILS/SPARMUL.f:51-133): KU = K U, one rank. */
+int svfsi_spmv_dev(void* lhs, int dof, const double* dK, const double* dU,
+                   double* dKU, void* stream);
+/* Same product with FP32 values and vectors (mixed path). */
+int svfsi_spmv_f32_dev(void* lhs, int dof, const float* dK, const float* dU,
+                       float* dKU, void* stream);
+/* Transpose product KU = K^T U (adjoint solve; no reference counterpart). */
+int svfsi_spmv_t_dev(void* lhs, int dof, const double* dK, const double* dU,
+                     double* dKU, void* stream);
+/* FSILS_DOTV / FSILS_NORMV one rank (svFSILS/DOT.f:56-113, NORM.f:55-113):
+ * deterministic two-stage reduction; the result is written to *out (host). */
+int svfsi_dot_dev(int n, const double* dU, const double* dV, double* out,
+                  void* stream);
+/* PRECONDDIAG (svFSILS/PRECOND.f:50-145): scales dVal and dR in place and
+ * writes W(dof,nNo) to dW, honouring the Dirichlet faces of lhs. */
+int svfsi_precond_diag_dev(void* lhs, int dof, double* dVal, double* dR,
+                           double* dW, const int* incL, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Sparsity pattern                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* LHSA (svFSI/LHSA.f:38-262): node-adjacency block pattern of a mesh.
+ * IEN(eNoN,nEl) 1-based.  Output 1-based rowPtr(nNo+1) and colPtr(nnz),
+ * columns sorted ascending and unique per row, diagonal included.  Call
+ * with colPtr = NULL to obtain *nnz and rowPtr, then again with colPtr of
+ * length *nnz.  Host setup routine (runs once per mesh; needs no GPU). */
+int svfsi_lhsa(int nNo, int nEl, int eNoN, const int* IEN, int* nnz,
+               int* rowPtr, int* colPtr);
+
+/* ---------------------------------------------------------------------- */
+/* Assembly (replaces the element loop of GLOBALEQASSEM, EQASSEM.f:45-84)  */
+/* ---------------------------------------------------------------------- */
+
+/* Equation parameters (eq(cEq) / dmn props of COMMOD, svFSI/MOD.f). */
+typedef struct {
+    double am, af, beta, gam, dt; /* generalized-alpha, INITIALIZE.f:76-139 */
+    double rho;                   /* solid or fluid density */
+    double E, nu;                 /* LELAS (LELAS.f:146-155) */
+    double fb[3];                 /* body force f_x,f_y,f_z */
+    double mu;                    /* FLUID constant viscosity */
+    double bstab;                 /* FLUID backflow stabilization (default 0.2) */
+    double C10, Kpen;             /* STRUCT nHK (MATMODELS.f:136-150) */
+    int volType;                  /* 1 Quad, 2 ST91 (default), 3 M94 */
+    int pad;
+} svfsi_eq_params;
+
+#define SVFSI_PHYS_FLUID   1  /* CONSTRUCT_FLUID, FLUID.f:40-190 (tet4 VMS) */
+#define SVFSI_PHYS_LELAS   2  /* CONSTRUCT_LELAS, LELAS.f:38-128 (tet4) */
+#define SVFSI_PHYS_STRUCT  3  /* CONSTRUCT_dSOLID, STRUCT.f:39-162 (nHK) */
+
+/* Device-resident mesh bound to an lhs pattern: coordinates, connectivity,
+ * a deterministic element colouring and the per-element DOASSEM scatter map
+ * (the column search of LHSA.f:282-292, done once).  IEN(eNoN,nEl) 1-based
+ * host array; x(3,nNo) host array.  eNoN 4 (tet4) or 10 (tet10). */
+int svfsi_mesh_create(void** mesh, void* lhs, int nNo, int nEl, int eNoN,
+                      const int* IEN, const double* x);
+int svfsi_mesh_free(void* mesh);
+/* Number of element colours (for reporting). */
+int svfsi_mesh_ncolors(void* mesh);
+
+/* Boundary faces of the mesh (tri3): fIEN(3,nFe) 1-based node ids, fEl(nFe)
+ * 1-based adjacent volume element.  Returns a face id >= 0 through *faceId. */
+int svfsi_mesh_add_face(void* mesh, int nFe, const int* fIEN, const int* fEl,
+                        int* faceId);
+
+/* Zero dVal and dR (LSALLOC, svFSI/LS.f:338-366). */
+int svfsi_zero_dev(void* mesh, int dof, double* dVal, double* dR, void* stream);
+
+/* Element loop + DOASSEM into device dVal(dof*dof,nnz), dR(dof,nNo)
+ * (accumulates; call svfsi_zero_dev first).  State arrays Ag,Yg,Dg are
+ * device (tDof,nNo).  prop_el: optional per-element property (LELAS E_e,
+ * FLUID mu_e, NULL = uniform).  quad: 0 = the reference rule (tet4 4-pt,
+ * tet10 15-pt), 1 = tet10 4-point rule. */
+int svfsi_assemble_dev(void* mesh, int phys, const svfsi_eq_params* p,
+                       int tDof, const double* dAg, const double* dYg,
+                       const double* dDg, const double* dprop_el, int quad,
+                       double* dVal, double* dR, void* stream);
+
+/* Face terms (device arrays):
+ *   traction: SETBCTRACL (svFSI/SETBC.f:319-416), uniform steady h(3).
+ *   backflow: BASSEMNEUBC+BFLUID with h = 0 (EQASSEM.f:90-192,
+ *             FLUID.f:1280-1336), outlet backflow stabilization. */
+int svfsi_assemble_traction_dev(void* mesh, int faceId, int dof,
+                                const double* h, double* dR, void* stream);
+int svfsi_assemble_backflow_dev(void* mesh, int faceId,
+                                const svfsi_eq_params* p, int tDof,
+                                const double* dYg, double* dVal, double* dR,
+                                void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Adjoint (no reference counterpart)                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Element sensitivity of the LELAS tangent to the per-element modulus:
+ * dg[e] = -lam_e^T (dK_e/dE_e) u_e, with dK_e/dE_e the stiffness part at
+ * unit modulus (the mass part does not depend on E).  Recomputes element
+ * matrices; dlam, du are device (3,nNo), dg device (nEl). */
+int svfsi_lelas_dkde_dot_dev(void* mesh, const svfsi_eq_params* p,
+                             const double* dlam, const double* du,
+                             double* dg, void* stream);
+
+/* Transposed solve K^T lam = rhs with the same scaling and Dirichlet mask as
+ * fsils_gpu_solve_dev (W K^T W, GMRES or CG by ls->LS_type).  dVal is the
+ * UNSCALED assembled matrix and is not modified; drhs(dof,nNo) in,
+ * dlam(dof,nNo) out. */
+int fsils_gpu_solve_t_dev(void* lhs, FSILS_lsType* ls, int dof,
+                          const double* dVal, const double* drhs,
+                          double* dlam, const int* incL, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Profiling hook (HIP events around the SpMV launches of the solvers)     */
+/* ---------------------------------------------------------------------- */
+int svfsi_prof_enable(int on);
+/* kernel: 0 = FP64 SpMV, 1 = FP32 SpMV.  Returns launches and total ms. */
+int svfsi_prof_get(int kernel, long long* count, double* total_ms);
+int svfsi_prof_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,205 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY -- the checker for the HIP product path.  The
+restated algorithms are in svfsi_oracle.c, each function citing the
+reference routine it follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+SO = HERE / "liboracle.so"
+SRC = HERE / "svfsi_oracle.c"
+
+
+def build() -> Path:
+    """Compile the oracle with gcc (no external libraries)."""
+    tmp = SO.with_suffix(".so.tmp")
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-Wextra", "-o", str(tmp), str(SRC), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stderr}")
+    tmp.replace(SO)
+    return SO
+
+
+class Eq(C.Structure):
+    _fields_ = [
+        ("am", C.c_double), ("af", C.c_double), ("beta", C.c_double), ("gam", C.c_double),
+        ("dt", C.c_double), ("rho", C.c_double), ("E", C.c_double), ("nu", C.c_double),
+        ("fb", C.c_double * 3), ("mu", C.c_double), ("bstab", C.c_double),
+        ("C10", C.c_double), ("Kpen", C.c_double), ("volType", C.c_int), ("pad", C.c_int),
+    ]
+
+
+class SubLs(C.Structure):
+    _fields_ = [
+        ("suc", C.c_int), ("mItr", C.c_int), ("sD", C.c_int), ("itr", C.c_int),
+        ("cM", C.c_int), ("cN", C.c_int), ("cD", C.c_int), ("reserve", C.c_int),
+        ("absTol", C.c_double), ("relTol", C.c_double), ("iNorm", C.c_double),
+        ("fNorm", C.c_double), ("dB", C.c_double), ("callD", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not SO.exists() or SO.stat().st_mtime < SRC.stat().st_mtime:
+            build()
+        _lib = C.CDLL(str(SO))
+        d = C.c_double
+        _lib.orc_dot.restype = d
+        _lib.orc_norm.restype = d
+        _lib.orc_iszero.restype = C.c_int
+        _lib.orc_lhsa_count.restype = C.c_int
+        _lib.orc_fsils_solve.restype = C.c_int
+        _lib.orc_diagptr.restype = C.c_int
+    return _lib
+
+
+def eq_struct(d: dict) -> Eq:
+    e = Eq()
+    for k in ("am", "af", "beta", "gam", "dt", "rho", "E", "nu", "mu", "bstab", "C10", "Kpen"):
+        setattr(e, k, float(d.get(k, 0.0)))
+    fb = d.get("fb", (0.0, 0.0, 0.0))
+    for i in range(3):
+        e.fb[i] = float(fb[i])
+    e.volType = int(d.get("volType", 2))
+    return e
+
+
+def P(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- pattern
+def lhsa(nNo: int, ien: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """LHSA restatement: 0-based rowPtr, colPtr."""
+    ien = i32(ien)
+    nEl, eNoN = ien.shape
+    rowPtr = np.zeros(nNo + 1, np.int32)
+    nnz = lib().orc_lhsa_count(nNo, nEl, eNoN, P(ien), P(rowPtr))
+    if nnz < 0:
+        raise ValueError(f"node {-nnz} is isolated")
+    colPtr = np.zeros(nnz, np.int32)
+    lib().orc_lhsa_fill(nNo, nEl, eNoN, P(ien), P(rowPtr), P(colPtr))
+    return rowPtr, colPtr
+
+
+# ---------------------------------------------------------------- assembly
+def assemble(prob, rowPtr: np.ndarray, colPtr: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Element loop + DOASSEM + face terms of a Problem (0-based pattern)."""
+    m = prob.mesh
+    dof = prob.dof
+    Val = np.zeros((len(colPtr), dof * dof))
+    R = np.zeros((m.nNo, dof))
+    eq = eq_struct(prob.eq)
+    x, ien = f64(m.x), i32(m.ien)
+    Ag, Yg, Dg = f64(prob.Ag), f64(prob.Yg), f64(prob.Dg)
+    prop = None if prob.prop_el is None else P(f64(prob.prop_el))
+    L = lib()
+    if prob.phys == 2:
+        L.orc_construct_lelas(m.nNo, m.nEl, P(ien), P(x), P(Ag), P(Dg), Ag.shape[1], C.byref(eq), prop,
+                              P(rowPtr), P(colPtr), P(Val), P(R))
+    elif prob.phys == 1:
+        L.orc_construct_fluid(m.nNo, m.nEl, P(ien), P(x), P(Ag), P(Yg), Ag.shape[1], C.byref(eq), prop,
+                              P(rowPtr), P(colPtr), P(Val), P(R))
+    elif prob.phys == 3:
+        L.orc_construct_struct(m.nNo, m.nEl, m.eNoN, P(ien), P(x), P(Ag), P(Yg), P(Dg), Ag.shape[1],
+                               C.byref(eq), prob.quad, P(rowPtr), P(colPtr), P(Val), P(R))
+    for fien, fel, h in prob.trac_faces:
+        fi, fe, hh = i32(fien), i32(fel), f64(h)
+        L.orc_traction_tri3(len(fi), P(fi), P(fe), P(ien), m.eNoN, P(x), P(hh), dof, P(rowPtr), P(colPtr),
+                            P(Val), P(R))
+    for fien, fel in prob.bflow_faces:
+        fi, fe = i32(fien), i32(fel)
+        L.orc_backflow_tri3(len(fi), P(fi), P(fe), P(ien), m.eNoN, P(x), P(Yg), Yg.shape[1], C.byref(eq),
+                            P(rowPtr), P(colPtr), P(Val), P(R))
+    return Val, R
+
+
+# ---------------------------------------------------------------- solver
+def spmv(rowPtr, colPtr, dof, K, U) -> np.ndarray:
+    nNo = len(rowPtr) - 1
+    out = np.zeros((nNo, dof))
+    lib().orc_spmv(nNo, dof, P(i32(rowPtr)), P(i32(colPtr)), P(f64(K)), P(f64(U)), P(out))
+    return out
+
+
+def spmv_t(rowPtr, colPtr, dof, K, U) -> np.ndarray:
+    nNo = len(rowPtr) - 1
+    out = np.zeros((nNo, dof))
+    lib().orc_spmv_t(nNo, dof, P(i32(rowPtr)), P(i32(colPtr)), P(f64(K)), P(f64(U)), P(out))
+    return out
+
+
+def precond_diag(rowPtr, colPtr, dof, Val, R, mask):
+    """Returns (scaled Val, scaled R, W)."""
+    nNo = len(rowPtr) - 1
+    rp, cp = i32(rowPtr), i32(colPtr)
+    dg = np.zeros(nNo, np.int32)
+    assert lib().orc_diagptr(nNo, P(rp), P(cp), P(dg)) == 0
+    V, Rr = f64(Val).copy(), f64(R).copy()
+    W = np.zeros((nNo, dof))
+    lib().orc_precond_diag(nNo, dof, P(rp), P(cp), P(dg), None if mask is None else P(f64(mask)), P(V), P(Rr), P(W))
+    return V, Rr, W
+
+
+def _ls(settings: dict) -> SubLs:
+    s = SubLs()
+    s.relTol = settings["relTol"]
+    s.absTol = settings["absTol"]
+    s.mItr = settings["mItr"]
+    s.sD = settings.get("sD", 0) or 0
+    return s
+
+
+def cgradv(rowPtr, colPtr, dof, K, R, settings: dict):
+    x = f64(R).copy()
+    s = _ls(settings)
+    lib().orc_cgradv(len(rowPtr) - 1, dof, P(i32(rowPtr)), P(i32(colPtr)), P(f64(K)), P(x), C.byref(s))
+    return x, s
+
+
+def gmresv(rowPtr, colPtr, dof, K, R, settings: dict):
+    x = f64(R).copy()
+    s = _ls(settings)
+    lib().orc_gmresv(len(rowPtr) - 1, dof, P(i32(rowPtr)), P(i32(colPtr)), P(f64(K)), P(x), C.byref(s))
+    return x, s
+
+
+def fsils_solve(rowPtr, colPtr, dof, mask, settings: dict, R, Val):
+    """FSILS_SOLVE restatement.  Returns (x, scaled Val, SubLs)."""
+    x = f64(R).copy()
+    V = f64(Val).copy()
+    s = _ls(settings)
+    rc = lib().orc_fsils_solve(len(rowPtr) - 1, dof, P(i32(rowPtr)), P(i32(colPtr)),
+                               None if mask is None else P(f64(mask)), settings["LS_type"], C.byref(s),
+                               P(x), P(V))
+    if rc != 0:
+        raise RuntimeError(f"orc_fsils_solve rc={rc}")
+    return x, V, s
+
+
+def lelas_dkde_dot(prob, lam, u) -> np.ndarray:
+    m = prob.mesh
+    g = np.zeros(m.nEl)
+    eq = eq_struct(prob.eq)
+    lib().orc_lelas_dkde_dot(m.nEl, P(i32(m.ien)), P(f64(m.x)), C.byref(eq), None, P(f64(lam)), P(f64(u)), P(g))
+    return g

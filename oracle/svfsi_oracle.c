@@ -1,0 +1,1134 @@
+/*
+ * svfsi_oracle.c -- CPU restatement of the svFSI per-Newton-step
+ * linear-system path (element kernels, DOASSEM, svFSILS Krylov solve).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product library.
+ * Loaded only by tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg.  See svfsi_oracle.h for layouts.  Citations are
+ * file:line under Code/Source/ of SimVascular/svFSI.
+ */
+#include "svfsi_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include <time.h>
+
+#define EPS DBL_EPSILON
+
+/* ===================================================================== */
+/* NN.f                                                                   */
+/* ===================================================================== */
+
+/* ISZERO (svFSI/MATFUN.f:763-779, UTIL.f:879-903 with ib absent):
+ * true iff |a| / max(|a|, eps) < 10 eps. */
+int orc_iszero(double a)
+{
+    double aa = fabs(a);
+    double nrm = aa > EPS ? aa : EPS;
+    return (aa / nrm) < 10.0 * EPS;
+}
+
+/* GETGIP, TET4 (svFSI/NN.f:266-275). */
+void orc_rule_tet4(double w[4], double xi[4][3])
+{
+    double s = (5.0 + 3.0 * sqrt(5.0)) / 20.0;
+    double t = (5.0 - sqrt(5.0)) / 20.0;
+    for (int g = 0; g < 4; ++g) {
+        w[g] = 1.0 / 24.0;
+        for (int i = 0; i < 3; ++i) xi[g][i] = t;
+        if (g < 3) xi[g][g] = s;
+    }
+}
+
+/* Four-point rule for TET10 (the configuration's reduced rule; the
+ * reference always uses 15 points for TET10, svFSI/NN.f:71-76).  Same
+ * points and weights as the TET4 rule; exact for quadratic integrands. */
+void orc_rule_tet10_4pt(double w[4], double xi[4][3]) { orc_rule_tet4(w, xi); }
+
+/* GETGIP, TET10 15-point rule (svFSI/NN.f:277-307). */
+void orc_rule_tet10(double w[15], double xi[15][3])
+{
+    w[0] = 0.030283678097089;
+    for (int g = 1; g < 5; ++g) w[g] = 0.006026785714286;
+    for (int g = 5; g < 9; ++g) w[g] = 0.011645249086029;
+    for (int g = 9; g < 15; ++g) w[g] = 0.010949141561386;
+    double s = 0.25;
+    xi[0][0] = s; xi[0][1] = s; xi[0][2] = s;
+    s = 0.333333333333333; double t = 0.0;
+    double p1[4][3] = {{t, s, s}, {s, t, s}, {s, s, t}, {s, s, s}};
+    for (int g = 0; g < 4; ++g) for (int i = 0; i < 3; ++i) xi[1 + g][i] = p1[g][i];
+    s = 0.090909090909091; t = 0.727272727272727;
+    double p2[4][3] = {{t, s, s}, {s, t, s}, {s, s, t}, {s, s, s}};
+    for (int g = 0; g < 4; ++g) for (int i = 0; i < 3; ++i) xi[5 + g][i] = p2[g][i];
+    s = 0.066550153573664; t = 0.433449846426336;
+    double p3[6][3] = {{s, s, t}, {s, t, s}, {s, t, t}, {t, t, s}, {t, s, t}, {t, s, s}};
+    for (int g = 0; g < 6; ++g) for (int i = 0; i < 3; ++i) xi[9 + g][i] = p3[g][i];
+}
+
+/* GETGIP, TRI3 (svFSI/NN.f:416-422). */
+void orc_rule_tri3(double w[3], double xi[3][2])
+{
+    double s = 2.0 / 3.0, t = 1.0 / 6.0;
+    w[0] = w[1] = w[2] = 1.0 / 6.0;
+    xi[0][0] = t; xi[0][1] = t;
+    xi[1][0] = s; xi[1][1] = t;
+    xi[2][0] = t; xi[2][1] = s;
+}
+
+/* GETGNN, TET4 (svFSI/NN.f:655-671). */
+void orc_shape_tet4(const double xi[3], double N[4], double Nxi[4][3])
+{
+    N[0] = xi[0]; N[1] = xi[1]; N[2] = xi[2];
+    N[3] = 1.0 - xi[0] - xi[1] - xi[2];
+    memset(Nxi, 0, sizeof(double) * 12);
+    Nxi[0][0] = 1.0; Nxi[1][1] = 1.0; Nxi[2][2] = 1.0;
+    Nxi[3][0] = Nxi[3][1] = Nxi[3][2] = -1.0;
+}
+
+/* GETGNN, TET10 (svFSI/NN.f:674-715).  Mid-edge order:
+ * 5:(1,2) 6:(2,3) 7:(1,3) 8:(1,4) 9:(2,4) 10:(3,4). */
+void orc_shape_tet10(const double xi[3], double N[10], double Nxi[10][3])
+{
+    double x = xi[0], y = xi[1], z = xi[2];
+    double s = 1.0 - x - y - z;
+    N[0] = x * (2.0 * x - 1.0);
+    N[1] = y * (2.0 * y - 1.0);
+    N[2] = z * (2.0 * z - 1.0);
+    N[3] = s * (2.0 * s - 1.0);
+    N[4] = 4.0 * x * y;
+    N[5] = 4.0 * y * z;
+    N[6] = 4.0 * x * z;
+    N[7] = 4.0 * x * s;
+    N[8] = 4.0 * y * s;
+    N[9] = 4.0 * z * s;
+    double d[10][3] = {
+        {4.0 * x - 1.0, 0.0, 0.0},
+        {0.0, 4.0 * y - 1.0, 0.0},
+        {0.0, 0.0, 4.0 * z - 1.0},
+        {1.0 - 4.0 * s, 1.0 - 4.0 * s, 1.0 - 4.0 * s},
+        {4.0 * y, 4.0 * x, 0.0},
+        {0.0, 4.0 * z, 4.0 * y},
+        {4.0 * z, 0.0, 4.0 * x},
+        {4.0 * (s - x), -4.0 * x, -4.0 * x},
+        {-4.0 * y, 4.0 * (s - y), -4.0 * y},
+        {-4.0 * z, -4.0 * z, 4.0 * (s - z)}};
+    memcpy(Nxi, d, sizeof(d));
+}
+
+/* GETGNN, TRI3 (svFSI/NN.f:1100-1110). */
+void orc_shape_tri3(const double xi[2], double N[3], double Nxi[3][2])
+{
+    N[0] = xi[0]; N[1] = xi[1]; N[2] = 1.0 - xi[0] - xi[1];
+    Nxi[0][0] = 1.0; Nxi[0][1] = 0.0;
+    Nxi[1][0] = 0.0; Nxi[1][1] = 1.0;
+    Nxi[2][0] = -1.0; Nxi[2][1] = -1.0;
+}
+
+/* GNN, insd = 3 (svFSI/NN.f:1469-1565): isoparametric Jacobian, its
+ * cofactor inverse, metric ks = xiX^T xiX and physical gradients. */
+void orc_gnn(int eNoN, const double (*Nxi)[3], const double (*xl)[3],
+             double (*Nx)[3], double* Jac, double ks[3][3])
+{
+    double A[3][3] = {{0}}; /* A[i][j] = xXi(i,j) = sum_a x_i(a) dN_a/dxi_j */
+    for (int a = 0; a < eNoN; ++a)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) A[i][j] += xl[a][i] * Nxi[a][j];
+    double J = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0]
+             + A[0][2] * A[1][0] * A[2][1] - A[0][0] * A[1][2] * A[2][1]
+             - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
+    double B[3][3]; /* B[i][j] = xiX(i,j) = d xi_i / d x_j */
+    B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / J;
+    B[0][1] = (A[2][1] * A[0][2] - A[2][2] * A[0][1]) / J;
+    B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / J;
+    B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / J;
+    B[1][1] = (A[2][2] * A[0][0] - A[2][0] * A[0][2]) / J;
+    B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / J;
+    B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / J;
+    B[2][1] = (A[2][0] * A[0][1] - A[2][1] * A[0][0]) / J;
+    B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / J;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            ks[i][j] = B[0][i] * B[0][j] + B[1][i] * B[1][j] + B[2][i] * B[2][j];
+    for (int a = 0; a < eNoN; ++a)
+        for (int j = 0; j < 3; ++j)
+            Nx[a][j] = Nxi[a][0] * B[0][j] + Nxi[a][1] * B[1][j] + Nxi[a][2] * B[2][j];
+    *Jac = J;
+}
+
+/* ===================================================================== */
+/* LHSA.f                                                                 */
+/* ===================================================================== */
+
+/* LHSA / ADDCOL (svFSI/LHSA.f:38-262): for every element, every (a,b)
+ * node pair inserts column IEN(b) in row IEN(a), kept sorted and unique by
+ * insertion sort.  Pass 1 counts per-row lengths into rowPtr and returns
+ * nnz; pass 2 (orc_lhsa_fill) writes the columns. */
+static int* g_uind = NULL;   /* [nNo][cap] insertion lists */
+static int* g_ulen = NULL;
+static int  g_cap = 0;
+
+static void addcol(int row, int col)
+{
+    int* u = g_uind + (size_t)row * g_cap;
+    int n = g_ulen[row];
+    int i = 0;
+    while (i < n && u[i] < col) ++i;
+    if (i < n && u[i] == col) return;
+    if (n == g_cap) abort(); /* capacity chosen large enough below */
+    for (int j = n; j > i; --j) u[j] = u[j - 1];
+    u[i] = col;
+    g_ulen[row] = n + 1;
+}
+
+static void lhsa_build(int nNo, int nEl, int eNoN, const int* IEN)
+{
+    /* node-to-element degree bounds the row length: <= deg*eNoN */
+    int* deg = calloc((size_t)nNo, sizeof(int));
+    for (size_t k = 0; k < (size_t)nEl * eNoN; ++k) deg[IEN[k]]++;
+    int mx = 0;
+    for (int a = 0; a < nNo; ++a) if (deg[a] > mx) mx = deg[a];
+    free(deg);
+    g_cap = mx * eNoN + 1;
+    free(g_uind); free(g_ulen);
+    g_uind = malloc(sizeof(int) * (size_t)nNo * g_cap);
+    g_ulen = calloc((size_t)nNo, sizeof(int));
+    for (int e = 0; e < nEl; ++e)
+        for (int a = 0; a < eNoN; ++a)
+            for (int b = 0; b < eNoN; ++b)
+                addcol(IEN[(size_t)e * eNoN + a], IEN[(size_t)e * eNoN + b]);
+}
+
+int orc_lhsa_count(int nNo, int nEl, int eNoN, const int* IEN, int* rowPtr)
+{
+    lhsa_build(nNo, nEl, eNoN, IEN);
+    rowPtr[0] = 0;
+    for (int r = 0; r < nNo; ++r) {
+        if (g_ulen[r] == 0) return -1 - r; /* "Node is isolated" LHSA.f:177 */
+        rowPtr[r + 1] = rowPtr[r] + g_ulen[r];
+    }
+    return rowPtr[nNo];
+}
+
+void orc_lhsa_fill(int nNo, int nEl, int eNoN, const int* IEN,
+                   const int* rowPtr, int* colPtr)
+{
+    (void)nEl; (void)eNoN; (void)IEN;
+    for (int r = 0; r < nNo; ++r)
+        memcpy(colPtr + rowPtr[r], g_uind + (size_t)r * g_cap, sizeof(int) * g_ulen[r]);
+    free(g_uind); free(g_ulen);
+    g_uind = NULL; g_ulen = NULL;
+}
+
+/* DOASSEM (svFSI/LHSA.f:266-298): R(:,row) += lR(:,a); per (a,b) a
+ * bisection for the column inside the row, then Val(:,ptr) += lK(:,a,b). */
+void orc_doassem(int d, const int* eqN, int dof, const double* lK,
+                 const double* lR, const int* rowPtr, const int* colPtr,
+                 double* Val, double* R)
+{
+    int dd = dof * dof;
+    for (int a = 0; a < d; ++a) {
+        int rowN = eqN[a];
+        for (int i = 0; i < dof; ++i) R[(size_t)rowN * dof + i] += lR[a * dof + i];
+        for (int b = 0; b < d; ++b) {
+            int colN = eqN[b];
+            int left = rowPtr[rowN], right = rowPtr[rowN + 1];
+            int ptr = (left + right) / 2;
+            while (colN != colPtr[ptr]) {
+                if (colN > colPtr[ptr]) left = ptr; else right = ptr;
+                ptr = (left + right) / 2;
+            }
+            const double* k = lK + ((size_t)b * d + a) * dd;
+            double* v = Val + (size_t)ptr * dd;
+            for (int q = 0; q < dd; ++q) v[q] += k[q];
+        }
+    }
+}
+
+/* ===================================================================== */
+/* LELAS.f                                                                */
+/* ===================================================================== */
+
+/* LELAS3D (svFSI/LELAS.f:130-242), no prestress, body-force field Bf = 0,
+ * elasticity modulus passed per element. */
+void orc_lelas3d(int eNoN, double w, const double* N, const double (*Nx)[3],
+                 const double* al, const double* dl, int tDof,
+                 const orc_eq* eq, double E, double* lR, double* lK)
+{
+    const int dof = 3, dd = 9;
+    double rho = eq->rho, nu = eq->nu;
+    double lambda = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu);
+    double mu = E * 0.5 / (1.0 + nu);
+    double lDm = lambda / mu;
+    double T1 = eq->af * eq->beta * eq->dt * eq->dt;
+    double amd = eq->am / T1 * rho;
+    double wl = w * T1 * mu;
+    double ud[3] = {-eq->fb[0], -eq->fb[1], -eq->fb[2]};
+    double ed[6] = {0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < eNoN; ++a) {
+        const double* A = al + a * tDof;
+        const double* D = dl + a * tDof;
+        for (int i = 0; i < 3; ++i) ud[i] += N[a] * A[i];
+        ed[0] += Nx[a][0] * D[0];
+        ed[1] += Nx[a][1] * D[1];
+        ed[2] += Nx[a][2] * D[2];
+        ed[3] += Nx[a][1] * D[0] + Nx[a][0] * D[1];
+        ed[4] += Nx[a][2] * D[1] + Nx[a][1] * D[2];
+        ed[5] += Nx[a][0] * D[2] + Nx[a][2] * D[0];
+    }
+    double divD = lambda * (ed[0] + ed[1] + ed[2]);
+    double S[6];
+    S[0] = divD + 2.0 * mu * ed[0];
+    S[1] = divD + 2.0 * mu * ed[1];
+    S[2] = divD + 2.0 * mu * ed[2];
+    S[3] = mu * ed[3];
+    S[4] = mu * ed[4];
+    S[5] = mu * ed[5];
+    for (int a = 0; a < eNoN; ++a) {
+        lR[a * dof + 0] += w * (rho * N[a] * ud[0] + Nx[a][0] * S[0] + Nx[a][1] * S[3] + Nx[a][2] * S[5]);
+        lR[a * dof + 1] += w * (rho * N[a] * ud[1] + Nx[a][0] * S[3] + Nx[a][1] * S[1] + Nx[a][2] * S[4]);
+        lR[a * dof + 2] += w * (rho * N[a] * ud[2] + Nx[a][0] * S[5] + Nx[a][1] * S[4] + Nx[a][2] * S[2]);
+        for (int b = 0; b < eNoN; ++b) {
+            double* k = lK + ((size_t)b * eNoN + a) * dd;
+            double NxdNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
+            double T = amd * N[a] * N[b] / mu + NxdNx;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double v = lDm * Nx[a][i] * Nx[b][j] + Nx[a][j] * Nx[b][i];
+                    if (i == j) v = T + (1.0 + lDm) * Nx[a][i] * Nx[b][i];
+                    k[i * dof + j] += wl * v;
+                }
+        }
+    }
+}
+
+/* CONSTRUCT_LELAS (svFSI/LELAS.f:38-128): tet4 only, GNN once (lShpF). */
+void orc_construct_lelas(int nNo, int nEl, const int* IEN, const double* x,
+                         const double* Ag, const double* Dg, int tDof,
+                         const orc_eq* eq, const double* E_el,
+                         const int* rowPtr, const int* colPtr,
+                         double* Val, double* R)
+{
+    (void)nNo;
+    const int eNoN = 4, dof = 3;
+    double wg[4], xi[4][3], Ng[4][4], Nxi[4][3];
+    orc_rule_tet4(wg, xi);
+    for (int g = 0; g < 4; ++g) orc_shape_tet4(xi[g], Ng[g], Nxi);
+    double xl[4][3], al[4 * 3], dl[4 * 3], Nx[4][3], ks[3][3], Jac = 0.0;
+    double lR[4 * 3], lK[16 * 9];
+    int ptr[4];
+    for (int e = 0; e < nEl; ++e) {
+        for (int a = 0; a < eNoN; ++a) {
+            int Ac = IEN[(size_t)e * eNoN + a];
+            ptr[a] = Ac;
+            for (int i = 0; i < 3; ++i) {
+                xl[a][i] = x[(size_t)Ac * 3 + i];
+                al[a * 3 + i] = Ag[(size_t)Ac * tDof + i];
+                dl[a * 3 + i] = Dg[(size_t)Ac * tDof + i];
+            }
+        }
+        memset(lR, 0, sizeof(lR));
+        memset(lK, 0, sizeof(lK));
+        double E = E_el ? E_el[e] : eq->E;
+        for (int g = 0; g < 4; ++g) {
+            if (g == 0) orc_gnn(eNoN, (const double (*)[3])Nxi, (const double (*)[3])xl, Nx, &Jac, ks);
+            double w = wg[g] * Jac;
+            orc_lelas3d(eNoN, w, Ng[g], (const double (*)[3])Nx, al, dl, 3, eq, E, lR, lK);
+        }
+        orc_doassem(eNoN, ptr, dof, lK, lR, rowPtr, colPtr, Val, R);
+    }
+}
+
+/* ===================================================================== */
+/* FLUID.f                                                                */
+/* ===================================================================== */
+
+/* Kinematics shared by FLUID3D_M and FLUID3D_C (FLUID.f:228-412 and
+ * 847-1000), Newtonian viscosity (GETVISCOSITY viscType_Const,
+ * FLUID.f:1700-1703: mu_x = mu_g = 0). */
+typedef struct {
+    double ud[3], u[3], ux[3][3], d2u2[3], divU, p, px[3];
+    double es[3][3], tauM, up[3];
+} fl_kin;
+
+static void fluid_kin(int eNoN, const double ks[3][3], const double* N,
+                      const double (*Nx)[3], const double (*Nxx)[6],
+                      const double* al, const double* yl, int tDof,
+                      const orc_eq* eq, double mu, fl_kin* k)
+{
+    const double ctM = 1.0, ctC = 36.0, rho = eq->rho;
+    memset(k, 0, sizeof(*k));
+    for (int i = 0; i < 3; ++i) k->ud[i] = -eq->fb[i];
+    double uxx[3][3][3] = {{{0}}}; /* uxx[i][c][j] = d2 u_c / dx_i dx_j */
+    for (int a = 0; a < eNoN; ++a) {
+        const double* A = al + a * tDof;
+        const double* Y = yl + a * tDof;
+        for (int c = 0; c < 3; ++c) {
+            k->ud[c] += N[a] * A[c];
+            k->u[c] += N[a] * Y[c];
+            for (int i = 0; i < 3; ++i) k->ux[i][c] += Nx[a][i] * Y[c];
+            uxx[0][c][0] += Nxx[a][0] * Y[c];
+            uxx[1][c][1] += Nxx[a][1] * Y[c];
+            uxx[2][c][2] += Nxx[a][2] * Y[c];
+            uxx[1][c][0] += Nxx[a][3] * Y[c];
+            uxx[2][c][1] += Nxx[a][4] * Y[c];
+            uxx[0][c][2] += Nxx[a][5] * Y[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        uxx[0][c][1] = uxx[1][c][0];
+        uxx[1][c][2] = uxx[2][c][1];
+        uxx[2][c][0] = uxx[0][c][2];
+        k->d2u2[c] = uxx[0][c][0] + uxx[1][c][1] + uxx[2][c][2];
+    }
+    k->divU = k->ux[0][0] + k->ux[1][1] + k->ux[2][2];
+    for (int a = 0; a < eNoN; ++a) {
+        const double* Y = yl + a * tDof;
+        k->p += N[a] * Y[3];
+        for (int i = 0; i < 3; ++i) k->px[i] += Nx[a][i] * Y[3];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) k->es[i][j] = k->ux[i][j] + k->ux[j][i];
+
+    double kT = 4.0 * (ctM / eq->dt) * (ctM / eq->dt);
+    double kU = 0.0, kS = 0.0;
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            kU += k->u[i] * k->u[j] * ks[i][j];
+            kS += ks[i][j] * ks[i][j];
+        }
+    kS = ctC * kS * (mu / rho) * (mu / rho);
+    k->tauM = 1.0 / (rho * sqrt(kT + kU + kS));
+    for (int c = 0; c < 3; ++c) {
+        double rV = k->ud[c] + k->u[0] * k->ux[0][c] + k->u[1] * k->ux[1][c] + k->u[2] * k->ux[2][c];
+        double rS = mu * k->d2u2[c];
+        k->up[c] = -k->tauM * (rho * rV + k->px[c] - rS);
+    }
+}
+
+/* FLUID3D_M (svFSI/FLUID.f:192-560), vmsFlag = .TRUE. (nFs = 1). */
+void orc_fluid3d_m(int eNoN, double w, const double ks[3][3], const double* N,
+                   const double (*Nx)[3], const double (*Nxx)[6],
+                   const double* al, const double* yl, int tDof,
+                   const orc_eq* eq, double mu, double* lR, double* lK)
+{
+    const int dof = 4, dd = 16;
+    const double rho = eq->rho;
+    double T1c = eq->af * eq->gam * eq->dt;
+    double amd = eq->am / T1c, wl = w * T1c, wr = w * rho;
+    fl_kin k;
+    fluid_kin(eNoN, ks, N, Nx, Nxx, al, yl, tDof, eq, mu, &k);
+    double tauM = k.tauM;
+    double tauC = 1.0 / (tauM * (ks[0][0] + ks[1][1] + ks[2][2]));
+    double tauB = 0.0;
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) tauB += k.up[i] * k.up[j] * ks[i][j];
+    if (orc_iszero(tauB)) tauB = EPS;
+    tauB = rho / sqrt(tauB);
+    double ua[3], pa = k.p - tauC * k.divU;
+    for (int c = 0; c < 3; ++c) ua[c] = k.u[c] + k.up[c];
+    double rV[3], rM[3][3];
+    for (int c = 0; c < 3; ++c)
+        rV[c] = tauB * (k.up[0] * k.ux[0][c] + k.up[1] * k.ux[1][c] + k.up[2] * k.ux[2][c]);
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < 3; ++i)
+            rM[i][c] = mu * k.es[i][c] - rho * k.up[c] * ua[i] + rV[c] * k.up[i] - (i == c ? pa : 0.0);
+    for (int c = 0; c < 3; ++c)
+        rV[c] = k.ud[c] + ua[0] * k.ux[0][c] + ua[1] * k.ux[1][c] + ua[2] * k.ux[2][c];
+
+    double uNx[32], upNx[32], uaNx[32], T1a[32];
+    for (int a = 0; a < eNoN; ++a) {
+        for (int c = 0; c < 3; ++c)
+            lR[a * dof + c] += wr * N[a] * rV[c]
+                + w * (Nx[a][0] * rM[0][c] + Nx[a][1] * rM[1][c] + Nx[a][2] * rM[2][c]);
+        uNx[a] = k.u[0] * Nx[a][0] + k.u[1] * Nx[a][1] + k.u[2] * Nx[a][2];
+        upNx[a] = k.up[0] * Nx[a][0] + k.up[1] * Nx[a][1] + k.up[2] * Nx[a][2];
+        uaNx[a] = uNx[a] + upNx[a];
+        /* updu(i,c,a) = delta_ic * T1a (mu_x = mu_g = 0) */
+        T1a[a] = -rho * uNx[a] + mu * (Nxx[a][0] + Nxx[a][1] + Nxx[a][2]);
+    }
+    for (int b = 0; b < eNoN; ++b)
+        for (int a = 0; a < eNoN; ++a) {
+            double* kk = lK + ((size_t)b * eNoN + a) * dd;
+            double NxNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
+            double T1 = mu * NxNx + rho * amd * N[b] * (N[a] + rho * tauM * uaNx[a])
+                      + rho * N[a] * (uNx[b] + upNx[b]) + tauB * upNx[a] * upNx[b];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    double updu = (r == c) ? T1a[b] : 0.0;
+                    double T2 = mu * Nx[a][c] * Nx[b][r] + tauC * Nx[a][r] * Nx[b][c]
+                              - rho * tauM * uaNx[a] * updu;
+                    kk[r * dof + c] += wl * (r == c ? T2 + T1 : T2);
+                }
+        }
+    for (int b = 0; b < eNoN; ++b)
+        for (int a = 0; a < eNoN; ++a) {
+            double* kk = lK + ((size_t)b * eNoN + a) * dd;
+            double T1 = rho * tauM * uaNx[a];
+            for (int r = 0; r < 3; ++r)
+                kk[r * dof + 3] -= wl * (Nx[a][r] * N[b] - Nx[b][r] * T1);
+        }
+}
+
+/* FLUID3D_C (svFSI/FLUID.f:813-1084), vmsFlag = .TRUE. */
+void orc_fluid3d_c(int eNoN, double w, const double ks[3][3], const double* N,
+                   const double (*Nx)[3], const double (*Nxx)[6],
+                   const double* al, const double* yl, int tDof,
+                   const orc_eq* eq, double mu, double* lR, double* lK)
+{
+    const int dof = 4, dd = 16;
+    const double rho = eq->rho;
+    double T1c = eq->af * eq->gam * eq->dt;
+    double amd = eq->am / T1c, wl = w * T1c;
+    fl_kin k;
+    fluid_kin(eNoN, ks, N, Nx, Nxx, al, yl, tDof, eq, mu, &k);
+    double tauM = k.tauM;
+    double T1a[32];
+    for (int a = 0; a < eNoN; ++a) {
+        double uNx = k.u[0] * Nx[a][0] + k.u[1] * Nx[a][1] + k.u[2] * Nx[a][2];
+        T1a[a] = -rho * uNx + mu * (Nxx[a][0] + Nxx[a][1] + Nxx[a][2]);
+    }
+    for (int a = 0; a < eNoN; ++a) {
+        double upNx = k.up[0] * Nx[a][0] + k.up[1] * Nx[a][1] + k.up[2] * Nx[a][2];
+        lR[a * dof + 3] += w * (N[a] * k.divU - upNx);
+    }
+    for (int b = 0; b < eNoN; ++b) {
+        double T1 = rho * amd * N[b];
+        for (int a = 0; a < eNoN; ++a) {
+            double* kk = lK + ((size_t)b * eNoN + a) * dd;
+            for (int c = 0; c < 3; ++c) {
+                /* T2 = sum_i Nqx(i,a) (updu(c,i,b) - delta_ic T1) */
+                double T2 = Nx[a][c] * (T1a[b] - T1);
+                kk[12 + c] += wl * (N[a] * Nx[b][c] - tauM * T2);
+            }
+        }
+    }
+    for (int b = 0; b < eNoN; ++b)
+        for (int a = 0; a < eNoN; ++a) {
+            double NxNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
+            lK[((size_t)b * eNoN + a) * dd + 15] += wl * tauM * NxNx;
+        }
+}
+
+/* CONSTRUCT_FLUID (svFSI/FLUID.f:40-190), tet4 VMS (nFs = 1): both
+ * function spaces are the mesh tet4 basis (GETTHOODFS, FS.f:257-286);
+ * GNN at g = 1 only (lShpF); GNNxx of an affine tet is zero. */
+void orc_construct_fluid(int nNo, int nEl, const int* IEN, const double* x,
+                         const double* Ag, const double* Yg, int tDof,
+                         const orc_eq* eq, const double* mu_el,
+                         const int* rowPtr, const int* colPtr,
+                         double* Val, double* R)
+{
+    (void)nNo;
+    const int eNoN = 4, dof = 4;
+    double wg[4], xi[4][3], Ng[4][4], Nxi[4][3];
+    orc_rule_tet4(wg, xi);
+    for (int g = 0; g < 4; ++g) orc_shape_tet4(xi[g], Ng[g], Nxi);
+    double Nxx[4][6];
+    memset(Nxx, 0, sizeof(Nxx));
+    double xl[4][3], al[16], yl[16], Nx[4][3], ks[3][3], Jac = 0.0;
+    double lR[16], lK[16 * 16];
+    int ptr[4];
+    for (int e = 0; e < nEl; ++e) {
+        for (int a = 0; a < eNoN; ++a) {
+            int Ac = IEN[(size_t)e * eNoN + a];
+            ptr[a] = Ac;
+            for (int i = 0; i < 3; ++i) xl[a][i] = x[(size_t)Ac * 3 + i];
+            for (int i = 0; i < 4; ++i) {
+                al[a * 4 + i] = Ag[(size_t)Ac * tDof + i];
+                yl[a * 4 + i] = Yg[(size_t)Ac * tDof + i];
+            }
+        }
+        memset(lR, 0, sizeof(lR));
+        memset(lK, 0, sizeof(lK));
+        double mu = mu_el ? mu_el[e] : eq->mu;
+        orc_gnn(eNoN, (const double (*)[3])Nxi, (const double (*)[3])xl, Nx, &Jac, ks);
+        for (int g = 0; g < 4; ++g)
+            orc_fluid3d_m(eNoN, wg[g] * Jac, (const double (*)[3])ks, Ng[g],
+                          (const double (*)[3])Nx, (const double (*)[6])Nxx,
+                          al, yl, 4, eq, mu, lR, lK);
+        for (int g = 0; g < 4; ++g)
+            orc_fluid3d_c(eNoN, wg[g] * Jac, (const double (*)[3])ks, Ng[g],
+                          (const double (*)[3])Nx, (const double (*)[6])Nxx,
+                          al, yl, 4, eq, mu, lR, lK);
+        orc_doassem(eNoN, ptr, dof, lK, lR, rowPtr, colPtr, Val, R);
+    }
+}
+
+/* ===================================================================== */
+/* Face terms: GNNB + SETBCTRACL / BFLUID                                 */
+/* ===================================================================== */
+
+/* GNNB (svFSI/NN.f:1856-1996), tri3 face of a volume element: n is the
+ * cross product of the two face tangents, flipped to point away from the
+ * first element node that is not on the face.  Returns |n| via nV scale. */
+static void gnnb_tri3(const int* fn, int Ec, const int* IEN, int eNoN,
+                      const double* x, double n[3])
+{
+    const double* x1 = x + (size_t)fn[0] * 3;
+    const double* x2 = x + (size_t)fn[1] * 3;
+    const double* x3 = x + (size_t)fn[2] * 3;
+    double t1[3], t2[3];
+    for (int i = 0; i < 3; ++i) { t1[i] = x1[i] - x3[i]; t2[i] = x2[i] - x3[i]; }
+    n[0] = t1[1] * t2[2] - t1[2] * t2[1];
+    n[1] = t1[2] * t2[0] - t1[0] * t2[2];
+    n[2] = t1[0] * t2[1] - t1[1] * t2[0];
+    int inner = -1;
+    for (int b = 0; b < eNoN && inner < 0; ++b) {
+        int Bc = IEN[(size_t)Ec * eNoN + b];
+        if (Bc != fn[0] && Bc != fn[1] && Bc != fn[2]) inner = Bc;
+    }
+    const double* xb = x + (size_t)inner * 3;
+    double v0 = x1[0] - xb[0], v1 = x1[1] - xb[1], v2 = x1[2] - xb[2];
+    if (n[0] * v0 + n[1] * v1 + n[2] * v2 < 0.0) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+}
+
+/* SETBCTRACL (svFSI/SETBC.f:319-416), steady uniform traction h. */
+void orc_traction_tri3(int nFe, const int* fIEN, const int* fEl,
+                       const int* IEN, int eNoN, const double* x,
+                       const double h[3], int dof, const int* rowPtr,
+                       const int* colPtr, double* Val, double* R)
+{
+    double wg[3], xi[3][2], N[3], Nxi[3][2];
+    orc_rule_tri3(wg, xi);
+    double lR[3 * 4], lK[9 * 16];
+    for (int e = 0; e < nFe; ++e) {
+        memset(lR, 0, sizeof(lR));
+        memset(lK, 0, sizeof(lK));
+        for (int g = 0; g < 3; ++g) {
+            double n[3];
+            gnnb_tri3(fIEN + 3 * e, fEl[e], IEN, eNoN, x, n);
+            double Jac = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            double w = wg[g] * Jac;
+            orc_shape_tri3(xi[g], N, Nxi);
+            for (int a = 0; a < 3; ++a)
+                for (int i = 0; i < 3; ++i) lR[a * dof + i] -= w * N[a] * h[i];
+        }
+        orc_doassem(3, fIEN + 3 * e, dof, lK, lR, rowPtr, colPtr, Val, R);
+    }
+}
+
+/* BASSEMNEUBC (svFSI/EQASSEM.f:90-192) with h = 0 calling BFLUID
+ * (svFSI/FLUID.f:1280-1336): outlet backflow stabilization. */
+void orc_backflow_tri3(int nFe, const int* fIEN, const int* fEl,
+                       const int* IEN, int eNoN, const double* x,
+                       const double* Yg, int tDof, const orc_eq* eq,
+                       const int* rowPtr, const int* colPtr,
+                       double* Val, double* R)
+{
+    const int dof = 4, dd = 16;
+    double wg[3], xi[3][2], N[3], Nxi[3][2];
+    orc_rule_tri3(wg, xi);
+    double lR[12], lK[9 * 16];
+    for (int e = 0; e < nFe; ++e) {
+        memset(lR, 0, sizeof(lR));
+        memset(lK, 0, sizeof(lK));
+        const int* fn = fIEN + 3 * e;
+        for (int g = 0; g < 3; ++g) {
+            double n[3];
+            gnnb_tri3(fn, fEl[e], IEN, eNoN, x, n);
+            double Jac = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            double nV[3] = {n[0] / Jac, n[1] / Jac, n[2] / Jac};
+            double w = wg[g] * Jac;
+            orc_shape_tri3(xi[g], N, Nxi);
+            double y[3] = {0, 0, 0};
+            for (int a = 0; a < 3; ++a)
+                for (int i = 0; i < 3; ++i) y[i] += N[a] * Yg[(size_t)fn[a] * tDof + i];
+            double wl = w * eq->af * eq->gam * eq->dt;
+            double udn = y[0] * nV[0] + y[1] * nV[1] + y[2] * nV[2];
+            udn = 0.5 * eq->bstab * eq->rho * (udn - fabs(udn));
+            double hc[3] = {udn * y[0], udn * y[1], udn * y[2]};
+            for (int a = 0; a < 3; ++a) {
+                for (int i = 0; i < 3; ++i) lR[a * dof + i] -= w * N[a] * hc[i];
+                for (int b = 0; b < 3; ++b) {
+                    double T1 = wl * N[a] * N[b] * udn;
+                    double* kk = lK + ((size_t)b * 3 + a) * dd;
+                    kk[0] -= T1; kk[5] -= T1; kk[10] -= T1;
+                }
+            }
+        }
+        orc_doassem(3, fn, dof, lK, lR, rowPtr, colPtr, Val, R);
+    }
+}
+
+/* ===================================================================== */
+/* STRUCT.f + MATMODELS.f (neo-Hookean)                                   */
+/* ===================================================================== */
+
+static double det3(const double A[3][3])
+{
+    /* MAT_DET (svFSI/MATFUN.f:177-205): cofactor expansion on row 1 */
+    return A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1])
+         - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
+         + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+}
+
+static void inv3(const double A[3][3], double Ai[3][3])
+{
+    /* MAT_INV nd = 3 (svFSI/MATFUN.f:257-271) */
+    double d = det3(A);
+    Ai[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / d;
+    Ai[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / d;
+    Ai[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / d;
+    Ai[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / d;
+    Ai[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / d;
+    Ai[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / d;
+    Ai[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / d;
+    Ai[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / d;
+    Ai[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / d;
+}
+
+/* GETPK2CC, stIso_nHook branch without fibers or active stress
+ * (svFSI/MATMODELS.f:41-150, 520-537), GETSVOLP (539-564), CCTOVOIGT
+ * (1588-1647).  CC is formed as the reference does from TEN_DYADPROD and
+ * TEN_SYMMPROD (svFSI/MATFUN.f:438-478). */
+void orc_getpk2cc_nhk(const orc_eq* eq, const double F[3][3],
+                      double S[3][3], double Dm[6][6])
+{
+    const double nd = 3.0;
+    double Kp = eq->Kpen;
+    double J = det3(F);
+    double J2d = pow(J, -2.0 / nd);
+    double C[3][3], Ci[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+    inv3(C, Ci);
+    double p = 0.0, pl = 0.0;
+    if (!orc_iszero(Kp)) {
+        switch (eq->volType) {
+        case 1: p = Kp * (J - 1.0); pl = Kp * (2.0 * J - 1.0); break;
+        case 3: p = Kp * (1.0 - 1.0 / J); pl = Kp; break;
+        default: p = 0.5 * Kp * (J - 1.0 / J); pl = Kp * J; break;
+        }
+    }
+    double g1 = 2.0 * eq->C10;
+    double Sb[3][3] = {{g1, 0, 0}, {0, g1, 0}, {0, 0, g1}};
+    double CSb = 0.0;
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) CSb += C[i][j] * Sb[i][j];
+    double r1 = J2d * CSb / nd;
+    double Sd[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Sd[i][j] = J2d * Sb[i][j] - r1 * Ci[i][j];
+    static double CC[3][3][3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l) {
+                    double dy1 = Ci[i][j] * Sd[k][l] + Sd[i][j] * Ci[k][l];
+                    double sym = 0.5 * (Ci[i][k] * Ci[j][l] + Ci[i][l] * Ci[j][k]);
+                    double dyc = Ci[i][j] * Ci[k][l];
+                    CC[i][j][k][l] = (-2.0 / nd) * dy1 + 2.0 * (r1 - p * J) * sym
+                                   + (pl * J - 2.0 * r1 / nd) * dyc;
+                }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) S[i][j] = Sd[i][j] + p * J * Ci[i][j];
+    /* Voigt order 11,22,33,12,23,31 */
+    static const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {2, 0}};
+    for (int a = 0; a < 6; ++a)
+        for (int b = a; b < 6; ++b) {
+            Dm[a][b] = CC[vi[a][0]][vi[a][1]][vi[b][0]][vi[b][1]];
+            Dm[b][a] = Dm[a][b];
+        }
+}
+
+/* STRUCT3D (svFSI/STRUCT.f:164-496) for solid viscosity = 0, damping = 0,
+ * no prestress and no fibers (the viscous Tv terms vanish). */
+void orc_struct3d(int eNoN, double w, const double* N, const double (*Nx)[3],
+                  const double* al, const double* yl, const double* dl,
+                  int tDof, const orc_eq* eq, double* lR, double* lK)
+{
+    (void)yl;
+    const int dof = 3, dd = 9;
+    double rho = eq->rho;
+    double afu = eq->af * eq->beta * eq->dt * eq->dt;
+    double amd = eq->am * rho;
+    double ud[3], F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int i = 0; i < 3; ++i) ud[i] = -rho * eq->fb[i];
+    for (int a = 0; a < eNoN; ++a) {
+        const double* A = al + a * tDof;
+        const double* D = dl + a * tDof;
+        for (int i = 0; i < 3; ++i) {
+            ud[i] += N[a] * (rho * A[i]);
+            for (int j = 0; j < 3; ++j) F[i][j] += Nx[a][j] * D[i];
+        }
+    }
+    double S[3][3], Dm[6][6], P[3][3];
+    orc_getpk2cc_nhk(eq, (const double (*)[3])F, S, Dm);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            P[i][j] = F[i][0] * S[0][j] + F[i][1] * S[1][j] + F[i][2] * S[2][j];
+    for (int a = 0; a < eNoN; ++a)
+        for (int i = 0; i < 3; ++i)
+            lR[a * dof + i] += w * (N[a] * ud[i] + Nx[a][0] * P[i][0] + Nx[a][1] * P[i][1] + Nx[a][2] * P[i][2]);
+    /* Bm(6,3,a) (STRUCT.f:295-319) */
+    double Bm[64][6][3];
+    for (int a = 0; a < eNoN; ++a)
+        for (int c = 0; c < 3; ++c) {
+            Bm[a][0][c] = Nx[a][0] * F[c][0];
+            Bm[a][1][c] = Nx[a][1] * F[c][1];
+            Bm[a][2][c] = Nx[a][2] * F[c][2];
+            Bm[a][3][c] = Nx[a][0] * F[c][1] + F[c][0] * Nx[a][1];
+            Bm[a][4][c] = Nx[a][1] * F[c][2] + F[c][1] * Nx[a][2];
+            Bm[a][5][c] = Nx[a][2] * F[c][0] + F[c][2] * Nx[a][0];
+        }
+    for (int b = 0; b < eNoN; ++b)
+        for (int a = 0; a < eNoN; ++a) {
+            double NxSNx = 0.0;
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) NxSNx += Nx[a][i] * S[i][j] * Nx[b][j];
+            double T1 = amd * N[a] * N[b] + afu * NxSNx;
+            double DBm[6][3];
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    double s = 0.0;
+                    for (int q = 0; q < 6; ++q) s += Dm[r][q] * Bm[b][q][c];
+                    DBm[r][c] = s;
+                }
+            double* kk = lK + ((size_t)b * eNoN + a) * dd;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    double BmDBm = 0.0;
+                    for (int q = 0; q < 6; ++q) BmDBm += Bm[a][q][r] * DBm[q][c];
+                    kk[r * dof + c] += w * ((r == c ? T1 : 0.0) + afu * BmDBm);
+                }
+        }
+}
+
+/* CONSTRUCT_dSOLID (svFSI/STRUCT.f:39-162): GNN at every Gauss point. */
+void orc_construct_struct(int nNo, int nEl, int eNoN, const int* IEN,
+                          const double* x, const double* Ag, const double* Yg,
+                          const double* Dg, int tDof, const orc_eq* eq,
+                          int rule, const int* rowPtr, const int* colPtr,
+                          double* Val, double* R)
+{
+    (void)nNo;
+    const int dof = 3;
+    int nG;
+    double wg[15], xi[15][3];
+    if (eNoN == 4) { orc_rule_tet4(wg, xi); nG = 4; }
+    else if (rule == 1) { orc_rule_tet10_4pt(wg, xi); nG = 4; }
+    else { orc_rule_tet10(wg, xi); nG = 15; }
+    double Ng[15][10], Nxig[15][10][3];
+    for (int g = 0; g < nG; ++g) {
+        if (eNoN == 4) orc_shape_tet4(xi[g], Ng[g], Nxig[g]);
+        else orc_shape_tet10(xi[g], Ng[g], Nxig[g]);
+    }
+    double xl[10][3], al[30], yl[30], dl[30], Nx[10][3], ks[3][3], Jac;
+    double* lR = malloc(sizeof(double) * 30);
+    double* lK = malloc(sizeof(double) * 900);
+    int ptr[10];
+    for (int e = 0; e < nEl; ++e) {
+        for (int a = 0; a < eNoN; ++a) {
+            int Ac = IEN[(size_t)e * eNoN + a];
+            ptr[a] = Ac;
+            for (int i = 0; i < 3; ++i) {
+                xl[a][i] = x[(size_t)Ac * 3 + i];
+                al[a * 3 + i] = Ag[(size_t)Ac * tDof + i];
+                yl[a * 3 + i] = Yg[(size_t)Ac * tDof + i];
+                dl[a * 3 + i] = Dg[(size_t)Ac * tDof + i];
+            }
+        }
+        memset(lR, 0, sizeof(double) * eNoN * dof);
+        memset(lK, 0, sizeof(double) * eNoN * eNoN * 9);
+        for (int g = 0; g < nG; ++g) {
+            orc_gnn(eNoN, (const double (*)[3])Nxig[g], (const double (*)[3])xl, Nx, &Jac, ks);
+            double w = wg[g] * Jac;
+            orc_struct3d(eNoN, w, Ng[g], (const double (*)[3])Nx, al, yl, dl, 3, eq, lR, lK);
+        }
+        orc_doassem(eNoN, ptr, dof, lK, lR, rowPtr, colPtr, Val, R);
+    }
+    free(lR); free(lK);
+}
+
+/* ===================================================================== */
+/* svFSILS                                                                */
+/* ===================================================================== */
+
+/* FSILS_SPARMULVV (svFSILS/SPARMUL.f:51-133), one rank. */
+void orc_spmv(int nNo, int dof, const int* rowPtr, const int* colPtr,
+              const double* K, const double* U, double* KU)
+{
+    int dd = dof * dof;
+    for (int i = 0; i < nNo; ++i) {
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
+            const double* k = K + (size_t)j * dd;
+            const double* u = U + (size_t)colPtr[j] * dof;
+            for (int l = 0; l < dof; ++l) {
+                double s = 0.0;
+                for (int m = 0; m < dof; ++m) s += k[l * dof + m] * u[m];
+                acc[l] += s;
+            }
+        }
+        for (int l = 0; l < dof; ++l) KU[(size_t)i * dof + l] = acc[l];
+    }
+}
+
+/* Transpose product K^T U of the block-CSR matrix (no reference
+ * counterpart: the adjoint solve).  Block (i,j) contributes
+ * K_ij^T u_i to row j. */
+void orc_spmv_t(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                const double* K, const double* U, double* KtU)
+{
+    int dd = dof * dof;
+    memset(KtU, 0, sizeof(double) * (size_t)nNo * dof);
+    for (int i = 0; i < nNo; ++i)
+        for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
+            const double* k = K + (size_t)j * dd;
+            const double* u = U + (size_t)i * dof;
+            double* y = KtU + (size_t)colPtr[j] * dof;
+            for (int m = 0; m < dof; ++m) {
+                double s = 0.0;
+                for (int l = 0; l < dof; ++l) s += k[l * dof + m] * u[l];
+                y[m] += s;
+            }
+        }
+}
+
+/* FSILS_NCDOTV / FSILS_DOTV one rank (svFSILS/DOT.f:56-191). */
+double orc_dot(int n, const double* U, const double* V)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += U[i] * V[i];
+    return s;
+}
+
+/* FSILS_NORMV one rank (svFSILS/NORM.f:55-113). */
+double orc_norm(int n, const double* U) { return sqrt(orc_dot(n, U, U)); }
+
+/* diagPtr of FSILS_LHS_CREATE (svFSILS/LHS.f:92-104). */
+int orc_diagptr(int nNo, const int* rowPtr, const int* colPtr, int* diagPtr)
+{
+    for (int i = 0; i < nNo; ++i) {
+        diagPtr[i] = -1;
+        for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j)
+            if (colPtr[j] == i) { diagPtr[i] = j; break; }
+        if (diagPtr[i] < 0) return -1 - i;
+    }
+    return 0;
+}
+
+/* PRECONDDIAG + PREMUL/POSMUL (svFSILS/PRECOND.f:50-145, 372-489). */
+void orc_precond_diag(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                      const int* diagPtr, const double* dirMask,
+                      double* Val, double* R, double* W)
+{
+    int dd = dof * dof;
+    for (int a = 0; a < nNo; ++a)
+        for (int i = 0; i < dof; ++i) {
+            double d = Val[(size_t)diagPtr[a] * dd + i * dof + i];
+            if (d == 0.0) d = 1.0;
+            W[(size_t)a * dof + i] = 1.0 / sqrt(fabs(d));
+        }
+    if (dirMask)
+        for (size_t q = 0; q < (size_t)nNo * dof; ++q) W[q] *= dirMask[q];
+    for (int a = 0; a < nNo; ++a)
+        for (int j = rowPtr[a]; j < rowPtr[a + 1]; ++j)
+            for (int i = 0; i < dof; ++i)
+                for (int m = 0; m < dof; ++m)
+                    Val[(size_t)j * dd + i * dof + m] *= W[(size_t)a * dof + i];
+    for (size_t q = 0; q < (size_t)nNo * dof; ++q) R[q] *= W[q];
+    for (int a = 0; a < nNo; ++a)
+        for (int j = rowPtr[a]; j < rowPtr[a + 1]; ++j) {
+            int c = colPtr[j];
+            for (int i = 0; i < dof; ++i)
+                for (int m = 0; m < dof; ++m)
+                    Val[(size_t)j * dd + i * dof + m] *= W[(size_t)c * dof + m];
+        }
+}
+
+static double wall_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* CGRADV (svFSILS/CGRAD.f:184-242). R in: RHS, out: solution. */
+void orc_cgradv(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                const double* K, double* R, orc_subls* ls)
+{
+    size_t n = (size_t)nNo * dof;
+    double* P = malloc(sizeof(double) * n);
+    double* KP = malloc(sizeof(double) * n);
+    double* X = calloc(n, sizeof(double));
+    ls->callD = wall_s();
+    ls->suc = 0;
+    ls->iNorm = orc_norm((int)n, R);
+    double eps = fmax(ls->absTol, ls->relTol * ls->iNorm);
+    eps = eps * eps;
+    double errO = ls->iNorm * ls->iNorm, err = errO;
+    memcpy(P, R, sizeof(double) * n);
+    int i;
+    for (i = 1; i <= ls->mItr; ++i) {
+        if (err < eps) { ls->suc = 1; break; }
+        errO = err;
+        orc_spmv(nNo, dof, rowPtr, colPtr, K, P, KP);
+        double alpha = errO / orc_dot((int)n, P, KP);
+        for (size_t q = 0; q < n; ++q) X[q] += alpha * P[q];
+        for (size_t q = 0; q < n; ++q) R[q] += -alpha * KP[q];
+        err = orc_norm((int)n, R);
+        err = err * err;
+        double r = errO / err;
+        for (size_t q = 0; q < n; ++q) P[q] += r * R[q];
+        r = err / errO;
+        for (size_t q = 0; q < n; ++q) P[q] *= r;
+    }
+    memcpy(R, X, sizeof(double) * n);
+    ls->itr = i - 1;
+    ls->fNorm = sqrt(err);
+    ls->callD = wall_s() - ls->callD;
+    ls->dB = (errO < EPS) ? 0.0 : 5.0 * log(err / errO);
+    free(P); free(KP); free(X);
+}
+
+/* GMRESV (svFSILS/GMRES.f:273-431) without coupled faces: restarted
+ * GMRES(sD), classical Gram-Schmidt with all dots first, Pythagorean
+ * sub-diagonal, Givens rotations, back substitution after each cycle. */
+void orc_gmresv(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                const double* K, double* R, orc_subls* ls)
+{
+    size_t n = (size_t)nNo * dof;
+    int sD = ls->sD;
+    double* u = malloc(sizeof(double) * n * (sD + 1));
+    double* X = calloc(n, sizeof(double));
+    double* h = calloc((size_t)(sD + 1) * sD, sizeof(double)); /* h[(i)*(sD+1)+j] = h(j,i) */
+    double* y = malloc(sizeof(double) * sD);
+    double* c = malloc(sizeof(double) * sD);
+    double* s = malloc(sizeof(double) * sD);
+    double* err = malloc(sizeof(double) * (sD + 1));
+#define H(j, i) h[(size_t)(i) * (sD + 1) + (j)]
+    ls->callD = wall_s();
+    ls->suc = 0;
+    double eps = orc_norm((int)n, R);
+    ls->iNorm = eps;
+    ls->fNorm = eps;
+    eps = fmax(ls->absTol, ls->relTol * eps);
+    ls->itr = 0;
+    if (ls->iNorm <= ls->absTol) {
+        /* GMRES.f:307-311 returns before R = X: R keeps the scaled RHS */
+        ls->callD = EPS;
+        ls->dB = 0.0;
+        goto done;
+    }
+    for (int l = 1; l <= ls->mItr; ++l) {
+        ls->dB = ls->fNorm;
+        ls->itr++;
+        orc_spmv(nNo, dof, rowPtr, colPtr, K, X, u);
+        for (size_t q = 0; q < n; ++q) u[q] = R[q] - u[q];
+        err[0] = orc_norm((int)n, u);
+        for (size_t q = 0; q < n; ++q) u[q] = u[q] / err[0];
+        int i;
+        for (i = 1; i <= sD; ++i) {
+            ls->itr++;
+            double* ui = u + n * (i - 1);
+            double* un = u + n * i;
+            orc_spmv(nNo, dof, rowPtr, colPtr, K, ui, un);
+            for (int j = 1; j <= i + 1; ++j) H(j - 1, i - 1) = orc_dot((int)n, u + n * (j - 1), un);
+            for (int j = 1; j <= i; ++j) {
+                double hj = H(j - 1, i - 1);
+                const double* uj = u + n * (j - 1);
+                for (size_t q = 0; q < n; ++q) un[q] += -hj * uj[q];
+                H(i, i - 1) = H(i, i - 1) - hj * hj;
+            }
+            H(i, i - 1) = sqrt(fabs(H(i, i - 1)));
+            double rr = 1.0 / H(i, i - 1);
+            for (size_t q = 0; q < n; ++q) un[q] *= rr;
+            for (int j = 1; j <= i - 1; ++j) {
+                double tmp = c[j - 1] * H(j - 1, i - 1) + s[j - 1] * H(j, i - 1);
+                H(j, i - 1) = -s[j - 1] * H(j - 1, i - 1) + c[j - 1] * H(j, i - 1);
+                H(j - 1, i - 1) = tmp;
+            }
+            double tmp = sqrt(H(i - 1, i - 1) * H(i - 1, i - 1) + H(i, i - 1) * H(i, i - 1));
+            c[i - 1] = H(i - 1, i - 1) / tmp;
+            s[i - 1] = H(i, i - 1) / tmp;
+            H(i - 1, i - 1) = tmp;
+            H(i, i - 1) = 0.0;
+            err[i] = -s[i - 1] * err[i - 1];
+            err[i - 1] = c[i - 1] * err[i - 1];
+            if (fabs(err[i]) < eps) { ls->suc = 1; break; }
+        }
+        if (i > sD) i = sD;
+        for (int j = 0; j < i; ++j) y[j] = err[j];
+        for (int j = i; j >= 1; --j) {
+            for (int k = j + 1; k <= i; ++k) y[j - 1] = y[j - 1] - H(j - 1, k - 1) * y[k - 1];
+            y[j - 1] = y[j - 1] / H(j - 1, j - 1);
+        }
+        for (int j = 1; j <= i; ++j) {
+            const double* uj = u + n * (j - 1);
+            for (size_t q = 0; q < n; ++q) X[q] += y[j - 1] * uj[q];
+        }
+        ls->fNorm = fabs(err[i]);
+        if (ls->suc) break;
+    }
+    memcpy(R, X, sizeof(double) * n);
+    ls->callD = wall_s() - ls->callD;
+    ls->dB = 10.0 * log(ls->fNorm / ls->dB);
+done:
+#undef H
+    free(u); free(X); free(h); free(y); free(c); free(s); free(err);
+}
+
+/* FSILS_SOLVE (svFSILS/SOLVE.f:51-143), one rank, PRECOND_FSILS, no
+ * coupled Neumann faces: scaling, CG or GMRES, un-scaling x = W y. */
+int orc_fsils_solve(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                    const double* dirMask, int LS_type, orc_subls* ls,
+                    double* Ri, double* Val)
+{
+    int* diagPtr = malloc(sizeof(int) * nNo);
+    if (orc_diagptr(nNo, rowPtr, colPtr, diagPtr) != 0) { free(diagPtr); return -1; }
+    size_t n = (size_t)nNo * dof;
+    double* W = malloc(sizeof(double) * n);
+    orc_precond_diag(nNo, dof, rowPtr, colPtr, diagPtr, dirMask, Val, Ri, W);
+    if (LS_type == 797) orc_gmresv(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
+    else if (LS_type == 798) orc_cgradv(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
+    else { free(W); free(diagPtr); return -2; }
+    for (size_t q = 0; q < n; ++q) Ri[q] = W[q] * Ri[q];
+    free(W); free(diagPtr);
+    return 0;
+}
+
+/* ===================================================================== */
+/* Adjoint helper                                                         */
+/* ===================================================================== */
+
+/* g_e = -lam_e^T (dK_e/dE_e) u_e for the LELAS tet4 tangent.  The
+ * stiffness part of LELAS3D is linear in E (lambda, mu proportional to E)
+ * and the mass part amd*N_a*N_b/mu*wl = w*am*rho*N_a*N_b does not depend
+ * on E, so dK_e/dE_e = (K_e(E_e) - M_e)/E_e = K_e(1) - M_e/... computed
+ * here as the stiffness tangent at unit modulus. */
+void orc_lelas_dkde_dot(int nEl, const int* IEN, const double* x,
+                        const orc_eq* eq, const double* E_el,
+                        const double* lam, const double* u, double* g)
+{
+    (void)E_el;
+    const int eNoN = 4;
+    double wg[4], xi[4][3], Ng[4][4], Nxi[4][3];
+    orc_rule_tet4(wg, xi);
+    for (int gq = 0; gq < 4; ++gq) orc_shape_tet4(xi[gq], Ng[gq], Nxi);
+    orc_eq eq0 = *eq;
+    eq0.rho = 0.0; /* drop the mass part: dK/dE is the unit-modulus stiffness */
+    double xl[4][3], Nx[4][3], ks[3][3], Jac = 0.0, lR[12], lK[144], zero[12] = {0};
+    for (int e = 0; e < nEl; ++e) {
+        for (int a = 0; a < eNoN; ++a)
+            for (int i = 0; i < 3; ++i) xl[a][i] = x[(size_t)IEN[e * 4 + a] * 3 + i];
+        memset(lK, 0, sizeof(lK));
+        memset(lR, 0, sizeof(lR));
+        orc_gnn(eNoN, (const double (*)[3])Nxi, (const double (*)[3])xl, Nx, &Jac, ks);
+        for (int gq = 0; gq < 4; ++gq)
+            orc_lelas3d(eNoN, wg[gq] * Jac, Ng[gq], (const double (*)[3])Nx, zero, zero, 3, &eq0, 1.0, lR, lK);
+        double s = 0.0;
+        for (int b = 0; b < 4; ++b)
+            for (int a = 0; a < 4; ++a) {
+                const double* k = lK + (b * 4 + a) * 9;
+                const double* la = lam + (size_t)IEN[e * 4 + a] * 3;
+                const double* ub = u + (size_t)IEN[e * 4 + b] * 3;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) s += la[i] * k[i * 3 + j] * ub[j];
+            }
+        g[e] = -s;
+    }
+}

@@ -1,0 +1,138 @@
+"""ctypes binding of libsvfsi_amd.so (declarations in include/svfsi_amd.h).
+
+The library is loaded from the package directory.  There is no fallback: if
+the shared object is missing the import of the binding raises, so nothing
+can silently run on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libsvfsi_amd.so"
+
+LS_TYPE_CG = 798
+LS_TYPE_GMRES = 797
+LS_TYPE_NS = 796
+LS_TYPE_BICGS = 795
+PRECOND_FSILS = 701
+PRECOND_RCS = 709
+BC_TYPE_Dir = 0
+BC_TYPE_Neu = 1
+
+SVFSI_PHYS_FLUID = 1
+SVFSI_PHYS_LELAS = 2
+SVFSI_PHYS_STRUCT = 3
+
+SVFSI_OK = 0
+SVFSI_ERR_ARG = -1
+SVFSI_ERR_HIP = -2
+SVFSI_ERR_STATE = -3
+SVFSI_ERR_NOTSUP = -4
+SVFSI_ERR_NODEVICE = -5
+
+
+class FSILS_subLsType(C.Structure):
+    """FSILS_subLsType (svFSILS/FSILS_struct_c.h:115-130)."""
+
+    _fields_ = [
+        ("suc", C.c_int), ("mItr", C.c_int), ("sD", C.c_int), ("itr", C.c_int),
+        ("cM", C.c_int), ("cN", C.c_int), ("cD", C.c_int), ("reserve", C.c_int),
+        ("absTol", C.c_double), ("relTol", C.c_double), ("iNorm", C.c_double),
+        ("fNorm", C.c_double), ("dB", C.c_double), ("callD", C.c_double),
+    ]
+
+
+class FSILS_lsType(C.Structure):
+    """FSILS_lsType (svFSILS/FSILS_struct_c.h:132-140)."""
+
+    _fields_ = [
+        ("foC", C.c_int), ("LS_type", C.c_int), ("Resm", C.c_int), ("Resc", C.c_int),
+        ("GM", FSILS_subLsType), ("CG", FSILS_subLsType), ("RI", FSILS_subLsType),
+    ]
+
+
+class EqParams(C.Structure):
+    """svfsi_eq_params: eq(cEq) and domain properties of COMMOD."""
+
+    _fields_ = [
+        ("am", C.c_double), ("af", C.c_double), ("beta", C.c_double), ("gam", C.c_double),
+        ("dt", C.c_double), ("rho", C.c_double), ("E", C.c_double), ("nu", C.c_double),
+        ("fb", C.c_double * 3), ("mu", C.c_double), ("bstab", C.c_double),
+        ("C10", C.c_double), ("Kpen", C.c_double), ("volType", C.c_int), ("pad", C.c_int),
+    ]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+# name -> argtypes ; every entry point returns int (0 = success)
+SIGNATURES = {
+    "svfsi_device_count": [],
+    "fsils_gpu_ls_create": [C.POINTER(FSILS_lsType), _I, _DP, _DP, _IP, _IP, _DP, _DP, _IP],
+    "fsils_gpu_lhs_create": [C.POINTER(_P), _I, _I, _I, _IP, _IP, _IP, _I],
+    "fsils_gpu_lhs_free": [_P],
+    "fsils_gpu_bc_create": [_P, _I, _I, _I, _I, _IP, _DP],
+    "fsils_gpu_bc_free": [_P, _I],
+    "fsils_gpu_solve": [_P, C.POINTER(FSILS_lsType), _I, _DP, _DP, _I, _IP, _DP],
+    "fsils_gpu_solve_dev": [_P, C.POINTER(FSILS_lsType), _I, _P, _P, _I, _IP, _DP, _P],
+    "fsils_gpu_solve_mixed_dev": [_P, C.POINTER(FSILS_lsType), _I, _P, _P, _I, _IP, _DP, _P],
+    "fsils_gpu_solve_t_dev": [_P, C.POINTER(FSILS_lsType), _I, _P, _P, _P, _IP, _P],
+    "svfsi_spmv_dev": [_P, _I, _P, _P, _P, _P],
+    "svfsi_spmv_f32_dev": [_P, _I, _P, _P, _P, _P],
+    "svfsi_spmv_t_dev": [_P, _I, _P, _P, _P, _P],
+    "svfsi_dot_dev": [_I, _P, _P, _DP, _P],
+    "svfsi_precond_diag_dev": [_P, _I, _P, _P, _P, _IP, _P],
+    "svfsi_lhsa": [_I, _I, _I, _IP, _IP, _IP, _IP],
+    "svfsi_mesh_create": [C.POINTER(_P), _P, _I, _I, _I, _IP, _DP],
+    "svfsi_mesh_free": [_P],
+    "svfsi_mesh_ncolors": [_P],
+    "svfsi_mesh_add_face": [_P, _I, _IP, _IP, _IP],
+    "svfsi_zero_dev": [_P, _I, _P, _P, _P],
+    "svfsi_assemble_dev": [_P, _I, C.POINTER(EqParams), _I, _P, _P, _P, _P, _I, _P, _P, _P],
+    "svfsi_assemble_traction_dev": [_P, _I, _I, _DP, _P, _P],
+    "svfsi_assemble_backflow_dev": [_P, _I, C.POINTER(EqParams), _I, _P, _P, _P, _P],
+    "svfsi_lelas_dkde_dot_dev": [_P, C.POINTER(EqParams), _P, _P, _P, _P],
+    "svfsi_prof_enable": [_I],
+    "svfsi_prof_get": [_I, C.POINTER(C.c_longlong), _DP],
+    "svfsi_prof_reset": [],
+}
+
+
+class SvfsiError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libsvfsi_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SvfsiError(
+                f"{LIB_PATH} is missing: build it with `python -m svfsi_amd.build` "
+                "(the HIP library is required; there is no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.svfsi_last_error.argtypes = []
+        L.svfsi_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != SVFSI_OK:
+        msg = lib().svfsi_last_error().decode(errors="replace")
+        raise SvfsiError(f"{what} failed (rc={rc}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)

@@ -1,0 +1,571 @@
+// assemble.hip -- element tangent/residual construction and DOASSEM scatter
+// for gfx950.
+//
+// Elements are processed colour by colour (no two elements of a colour share
+// a node), so every block of Val and every row of R receives at most one
+// contribution per launch and the summation order is fixed: the assembly is
+// deterministic bit for bit.  Within a colour, one thread owns one (a,b)
+// node pair of one element: it recomputes the element geometry and the
+// Gauss-point kinematics (cheap, register resident) and accumulates its
+// dof x dof block over the Gauss points exactly as the reference routine
+// does for that block, then adds it to Val through the precomputed scatter
+// map (the bisection of DOASSEM, svFSI/LHSA.f:282-292, done once at mesh
+// creation).  The thread with b = 0 also builds the residual rows of node a.
+#include "internal.hpp"
+#include "devutil.hpp"
+#include "assemble.hpp"
+
+#include <cmath>
+
+namespace svfsi {
+
+// GNN for an affine tet4 (svFSI/NN.f:1469-1565): Nxi of TET4 are constant
+// (NN.f:663-671): dN1 = e1, dN2 = e2, dN3 = e3, dN4 = -(1,1,1).
+__device__ __forceinline__ void gnn_tet4(const double xl[4][3], double Nx[4][3], double& Jac,
+                                         double ks[3][3])
+{
+    double A[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        A[i][0] = xl[0][i] - xl[3][i];
+        A[i][1] = xl[1][i] - xl[3][i];
+        A[i][2] = xl[2][i] - xl[3][i];
+    }
+    // the reference sums x_a * Nxi_a over a; for tet4 that is x_j - x_4
+    const double J = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0]
+                   + A[0][2] * A[1][0] * A[2][1] - A[0][0] * A[1][2] * A[2][1]
+                   - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
+    double B[3][3];
+    B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / J;
+    B[0][1] = (A[2][1] * A[0][2] - A[2][2] * A[0][1]) / J;
+    B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / J;
+    B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / J;
+    B[1][1] = (A[2][2] * A[0][0] - A[2][0] * A[0][2]) / J;
+    B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / J;
+    B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / J;
+    B[2][1] = (A[2][0] * A[0][1] - A[2][1] * A[0][0]) / J;
+    B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / J;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ks[i][j] = B[0][i] * B[0][j] + B[1][i] * B[1][j] + B[2][i] * B[2][j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        Nx[0][j] = B[0][j];
+        Nx[1][j] = B[1][j];
+        Nx[2][j] = B[2][j];
+        Nx[3][j] = -B[0][j] - B[1][j] - B[2][j];
+    }
+    Jac = J;
+}
+
+// ---------------------------------------------------------------------------
+// LELAS3D (svFSI/LELAS.f:130-242) in CONSTRUCT_LELAS (LELAS.f:38-128), tet4.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_asm_lelas(int begin, int count, const int* __restrict__ ien,
+                                                   const int* __restrict__ elemId,
+                                                   const int* __restrict__ emap,
+                                                   const double* __restrict__ x, int tDof,
+                                                   const double* __restrict__ Ag,
+                                                   const double* __restrict__ Dg,
+                                                   const double* __restrict__ Eel, AsmParams P,
+                                                   Tet4Rule rule, double* __restrict__ val,
+                                                   double* __restrict__ R)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * 16) return;
+    const int p = begin + (t >> 4);
+    const int ab = t & 15;
+    const int a = ab >> 2, b = ab & 3;
+    int nd[4];
+    double xl[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        nd[q] = ien[(size_t)p * 4 + q];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) xl[q][i] = x[(size_t)nd[q] * 3 + i];
+    }
+    double Nx[4][3], ks[3][3], Jac;
+    gnn_tet4(xl, Nx, Jac, ks);
+    const double E = Eel ? Eel[elemId[p]] : P.E;
+    const double nu = P.nu, rho = P.rho;
+    const double lambda = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu);
+    const double mu = E * 0.5 / (1.0 + nu);
+    const double lDm = lambda / mu;
+    const double T1 = P.af * P.beta * P.dt * P.dt;
+    const double amd = P.am / T1 * rho;
+    double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double NxdNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const double w = rule.w[g] * Jac;
+        const double wl = w * T1 * mu;
+        const double Tm = amd * rule.N[g][a] * rule.N[g][b] / mu + NxdNx;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double v = (i == j) ? Tm + (1.0 + lDm) * Nx[a][i] * Nx[b][i]
+                                          : lDm * Nx[a][i] * Nx[b][j] + Nx[a][j] * Nx[b][i];
+                k[i * 3 + j] += wl * v;
+            }
+    }
+    double* vb = val + (size_t)emap[(size_t)p * 16 + ab] * 9;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) vb[q] += k[q];
+
+    if (b != 0) return;
+    // residual row a (LELAS.f:161-205 and 208-215)
+    double ed[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double* D = Dg + (size_t)nd[q] * tDof;
+        ed[0] += Nx[q][0] * D[0];
+        ed[1] += Nx[q][1] * D[1];
+        ed[2] += Nx[q][2] * D[2];
+        ed[3] += Nx[q][1] * D[0] + Nx[q][0] * D[1];
+        ed[4] += Nx[q][2] * D[1] + Nx[q][1] * D[2];
+        ed[5] += Nx[q][0] * D[2] + Nx[q][2] * D[0];
+    }
+    const double divD = lambda * (ed[0] + ed[1] + ed[2]);
+    double S[6];
+    S[0] = divD + 2.0 * mu * ed[0];
+    S[1] = divD + 2.0 * mu * ed[1];
+    S[2] = divD + 2.0 * mu * ed[2];
+    S[3] = mu * ed[3];
+    S[4] = mu * ed[4];
+    S[5] = mu * ed[5];
+    double r[3] = {0, 0, 0};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const double w = rule.w[g] * Jac;
+        double ud[3] = {-P.fb[0], -P.fb[1], -P.fb[2]};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double* A = Ag + (size_t)nd[q] * tDof;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) ud[i] += rule.N[g][q] * A[i];
+        }
+        const double Na = rule.N[g][a];
+        r[0] += w * (rho * Na * ud[0] + Nx[a][0] * S[0] + Nx[a][1] * S[3] + Nx[a][2] * S[5]);
+        r[1] += w * (rho * Na * ud[1] + Nx[a][0] * S[3] + Nx[a][1] * S[1] + Nx[a][2] * S[4]);
+        r[2] += w * (rho * Na * ud[2] + Nx[a][0] * S[5] + Nx[a][1] * S[4] + Nx[a][2] * S[2]);
+    }
+    double* Rr = R + (size_t)nd[a] * 3;
+    Rr[0] += r[0];
+    Rr[1] += r[1];
+    Rr[2] += r[2];
+}
+
+// ---------------------------------------------------------------------------
+// FLUID3D_M + FLUID3D_C (svFSI/FLUID.f:192-560, 813-1084) in CONSTRUCT_FLUID
+// (FLUID.f:40-190), tet4 VMS (nFs = 1), Newtonian viscosity: GNNxx of an
+// affine tet is zero, so uxx = d2u2 = 0, and mu_x = mu_g = 0 (GETVISCOSITY
+// viscType_Const, FLUID.f:1700-1703).  The momentum blocks (1..12) and the
+// continuity blocks (13..16) are disjoint, so one Gauss loop builds both in
+// the same per-entry order as the reference's two loops.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool iszero_d(double a)
+{
+    const double eps = 2.220446049250313e-16;
+    const double aa = fabs(a);
+    const double nrm = aa > eps ? aa : eps;
+    return (aa / nrm) < 10.0 * eps;
+}
+
+__global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const int* __restrict__ ien,
+                                                   const int* __restrict__ elemId,
+                                                   const int* __restrict__ emap,
+                                                   const double* __restrict__ x, int tDof,
+                                                   const double* __restrict__ Ag,
+                                                   const double* __restrict__ Yg,
+                                                   const double* __restrict__ muEl, AsmParams P,
+                                                   Tet4Rule rule, double* __restrict__ val,
+                                                   double* __restrict__ R)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * 16) return;
+    const int p = begin + (t >> 4);
+    const int ab = t & 15;
+    const int a = ab >> 2, b = ab & 3;
+    int nd[4];
+    double xl[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        nd[q] = ien[(size_t)p * 4 + q];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) xl[q][i] = x[(size_t)nd[q] * 3 + i];
+    }
+    double Nx[4][3], ks[3][3], Jac;
+    gnn_tet4(xl, Nx, Jac, ks);
+    const double mu = muEl ? muEl[elemId[p]] : P.mu;
+    const double rho = P.rho;
+    const double ctM = 1.0, ctC = 36.0;
+    const double T1c = P.af * P.gam * P.dt;
+    const double amd = P.am / T1c;
+
+    // element-constant kinematics: ux(i,c) = sum Nx(i,q) y(c,q), px, divU, es
+    double yl[4][4], al[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) yl[q][c] = Yg[(size_t)nd[q] * tDof + c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) al[q][c] = Ag[(size_t)nd[q] * tDof + c];
+    }
+    double ux[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double px[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ux[i][c] += Nx[q][i] * yl[q][c];
+        }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) px[i] += Nx[q][i] * yl[q][3];
+    const double divU = ux[0][0] + ux[1][1] + ux[2][2];
+    double kS = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) kS += ks[i][j] * ks[i][j];
+    kS = ctC * kS * (mu / rho) * (mu / rho);
+    const double kT = 4.0 * (ctM / P.dt) * (ctM / P.dt);
+    const double trK = ks[0][0] + ks[1][1] + ks[2][2];
+    const double NxNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
+
+    double k[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) k[q] = 0.0;
+    double r[4] = {0, 0, 0, 0};
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+        const double w = rule.w[g] * Jac;
+        const double wl = w * T1c;
+        const double wr = w * rho;
+        double ud[3] = {-P.fb[0], -P.fb[1], -P.fb[2]};
+        double u[3] = {0, 0, 0};
+        double pp = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double Nq = rule.N[g][q];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ud[c] += Nq * al[q][c];
+                u[c] += Nq * yl[q][c];
+            }
+            pp += Nq * yl[q][3];
+        }
+        double kU = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) kU += u[i] * u[j] * ks[i][j];
+        const double tauM = 1.0 / (rho * sqrt(kT + kU + kS));
+        double up[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double rV = ud[c] + u[0] * ux[0][c] + u[1] * ux[1][c] + u[2] * ux[2][c];
+            up[c] = -tauM * (rho * rV + px[c] - mu * 0.0);
+        }
+        const double tauC = 1.0 / (tauM * trK);
+        double tauB = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tauB += up[i] * up[j] * ks[i][j];
+        if (iszero_d(tauB)) tauB = 2.220446049250313e-16;
+        tauB = rho / sqrt(tauB);
+        const double ua[3] = {u[0] + up[0], u[1] + up[1], u[2] + up[2]};
+
+        const double Na = rule.N[g][a], Nb = rule.N[g][b];
+        const double uNxa = u[0] * Nx[a][0] + u[1] * Nx[a][1] + u[2] * Nx[a][2];
+        const double uNxb = u[0] * Nx[b][0] + u[1] * Nx[b][1] + u[2] * Nx[b][2];
+        const double upNxa = up[0] * Nx[a][0] + up[1] * Nx[a][1] + up[2] * Nx[a][2];
+        const double upNxb = up[0] * Nx[b][0] + up[1] * Nx[b][1] + up[2] * Nx[b][2];
+        const double uaNxa = uNxa + upNxa;
+        const double T1b = -rho * uNxb;   // updu(i,c,b) = delta_ic T1b
+
+        // momentum u-u blocks (FLUID.f:494-545)
+        const double T1 = mu * NxNx + rho * amd * Nb * (Na + rho * tauM * uaNxa)
+                        + rho * Na * (uNxb + upNxb) + tauB * upNxa * upNxb;
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double updu = (rr == c) ? T1b : 0.0;
+                const double T2 = mu * Nx[a][c] * Nx[b][rr] + tauC * Nx[a][rr] * Nx[b][c]
+                                - rho * tauM * uaNxa * updu;
+                k[rr * 4 + c] += wl * ((rr == c) ? T2 + T1 : T2);
+            }
+        // momentum u-p blocks (FLUID.f:548-557)
+        {
+            const double T1p = rho * tauM * uaNxa;
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) k[rr * 4 + 3] -= wl * (Nx[a][rr] * Nb - Nx[b][rr] * T1p);
+        }
+        // continuity p-u and p-p blocks (FLUID.f:1052-1081)
+        {
+            const double T1q = rho * amd * Nb;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double T2 = Nx[a][c] * (T1b - T1q);
+                k[12 + c] += wl * (Na * Nx[b][c] - tauM * T2);
+            }
+            k[15] += wl * tauM * NxNx;
+        }
+        if (b == 0) {
+            // momentum residual (FLUID.f:429-465) and continuity (1046-1049)
+            const double pa = pp - tauC * divU;
+            double rVb[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                rVb[c] = tauB * (up[0] * ux[0][c] + up[1] * ux[1][c] + up[2] * ux[2][c]);
+            double rM[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const double es = ux[i][c] + ux[c][i];
+                    rM[i][c] = mu * es - rho * up[c] * ua[i] + rVb[c] * up[i] - (i == c ? pa : 0.0);
+                }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double rV = ud[c] + ua[0] * ux[0][c] + ua[1] * ux[1][c] + ua[2] * ux[2][c];
+                r[c] += wr * Na * rV + w * (Nx[a][0] * rM[0][c] + Nx[a][1] * rM[1][c] + Nx[a][2] * rM[2][c]);
+            }
+            r[3] += w * (Na * divU - upNxa);
+        }
+    }
+    double* vb = val + (size_t)emap[(size_t)p * 16 + ab] * 16;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) vb[q] += k[q];
+    if (b == 0) {
+        double* Rr = R + (size_t)nd[a] * 4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) Rr[c] += r[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Face terms on tri3 faces (GNNB, svFSI/NN.f:1856-1996).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gnnb_tri3(const double* x, int n0, int n1, int n2, int inner,
+                                          double n[3])
+{
+    double t1[3], t2[3], v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        t1[i] = x[(size_t)n0 * 3 + i] - x[(size_t)n2 * 3 + i];
+        t2[i] = x[(size_t)n1 * 3 + i] - x[(size_t)n2 * 3 + i];
+        v[i] = x[(size_t)n0 * 3 + i] - x[(size_t)inner * 3 + i];
+    }
+    n[0] = t1[1] * t2[2] - t1[2] * t2[1];
+    n[1] = t1[2] * t2[0] - t1[0] * t2[2];
+    n[2] = t1[0] * t2[1] - t1[1] * t2[0];
+    if (n[0] * v[0] + n[1] * v[1] + n[2] * v[2] < 0.0) {
+        n[0] = -n[0];
+        n[1] = -n[1];
+        n[2] = -n[2];
+    }
+}
+
+// SETBCTRACL (svFSI/SETBC.f:319-416): lR(:,a) -= w N_a h ; one thread per
+// (face, a).
+__global__ void k_face_traction(int begin, int count, const int* __restrict__ order,
+                                const int* __restrict__ fien, const int* __restrict__ inner,
+                                const double* __restrict__ x, int dof, double h0, double h1,
+                                double h2, Tri3Rule rule, double* __restrict__ R)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * 3) return;
+    const int f = order[begin + t / 3];
+    const int a = t % 3;
+    const int n0 = fien[f * 3], n1 = fien[f * 3 + 1], n2 = fien[f * 3 + 2];
+    double n[3];
+    gnnb_tri3(x, n0, n1, n2, inner[f], n);
+    const double Jac = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double h[3] = {h0, h1, h2};
+    double r[3] = {0, 0, 0};
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        const double w = rule.w[g] * Jac;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) r[i] -= w * rule.N[g][a] * h[i];
+    }
+    const int node = fien[f * 3 + a];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) R[(size_t)node * dof + i] += r[i];
+}
+
+// BFLUID with h = 0 (svFSI/FLUID.f:1280-1336); one thread per (face, a, b).
+__global__ void k_face_backflow(int begin, int count, const int* __restrict__ order,
+                                const int* __restrict__ fien, const int* __restrict__ inner,
+                                const int* __restrict__ fmap, const double* __restrict__ x,
+                                int tDof, const double* __restrict__ Yg, AsmParams P,
+                                Tri3Rule rule, double* __restrict__ val, double* __restrict__ R)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * 9) return;
+    const int f = order[begin + t / 9];
+    const int ab = t % 9;
+    const int a = ab / 3, b = ab % 3;
+    const int nd[3] = {fien[f * 3], fien[f * 3 + 1], fien[f * 3 + 2]};
+    double n[3];
+    gnnb_tri3(x, nd[0], nd[1], nd[2], inner[f], n);
+    const double Jac = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double nV[3] = {n[0] / Jac, n[1] / Jac, n[2] / Jac};
+    double kd = 0.0;
+    double r[3] = {0, 0, 0};
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        const double w = rule.w[g] * Jac;
+        double y[3] = {0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) y[i] += rule.N[g][q] * Yg[(size_t)nd[q] * tDof + i];
+        const double wl = w * P.af * P.gam * P.dt;
+        double udn = y[0] * nV[0] + y[1] * nV[1] + y[2] * nV[2];
+        udn = 0.5 * P.bstab * P.rho * (udn - fabs(udn));
+        kd -= wl * rule.N[g][a] * rule.N[g][b] * udn;
+        if (b == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) r[i] -= w * rule.N[g][a] * (udn * y[i]);
+        }
+    }
+    double* vb = val + (size_t)fmap[f * 9 + ab] * 16;
+    vb[0] += kd;
+    vb[5] += kd;
+    vb[10] += kd;
+    if (b == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) R[(size_t)nd[a] * 4 + i] += r[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint: g_e = -lam_e^T (dK_e/dE_e) u_e for the LELAS tet4 tangent.  The
+// stiffness part of LELAS3D is proportional to E (lambda, mu ~ E) and the
+// mass part does not depend on E, so dK_e/dE_e is the stiffness tangent at
+// unit modulus.  One thread per element; element matrices are recomputed,
+// never stored.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lelas_dkde(int nEl, const int* __restrict__ ien,
+                                                    const int* __restrict__ elemId,
+                                                    const double* __restrict__ x, AsmParams P,
+                                                    Tet4Rule rule, const double* __restrict__ lam,
+                                                    const double* __restrict__ u,
+                                                    double* __restrict__ g)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nEl) return;
+    int nd[4];
+    double xl[4][3], la[4][3], ub[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        nd[q] = ien[(size_t)p * 4 + q];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            xl[q][i] = x[(size_t)nd[q] * 3 + i];
+            la[q][i] = lam[(size_t)nd[q] * 3 + i];
+            ub[q][i] = u[(size_t)nd[q] * 3 + i];
+        }
+    }
+    double Nx[4][3], ks[3][3], Jac;
+    gnn_tet4(xl, Nx, Jac, ks);
+    const double nu = P.nu;
+    const double lambda = nu / (1.0 + nu) / (1.0 - 2.0 * nu);
+    const double mu = 0.5 / (1.0 + nu);
+    const double lDm = lambda / mu;
+    const double T1 = P.af * P.beta * P.dt * P.dt;
+    double wsum = 0.0;
+#pragma unroll
+    for (int gq = 0; gq < 4; ++gq) wsum += rule.w[gq] * Jac * T1 * mu;
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const double NxdNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double v = (i == j) ? NxdNx + (1.0 + lDm) * Nx[a][i] * Nx[b][i]
+                                              : lDm * Nx[a][i] * Nx[b][j] + Nx[a][j] * Nx[b][i];
+                    s += la[a][i] * (wsum * v) * ub[b][j];
+                }
+        }
+    g[elemId[p]] = -s;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int launch_asm_tet4(const Mesh& M, int phys, const AsmParams& P, const Tet4Rule& rule, int tDof,
+                    const double* Ag, const double* Yg, const double* Dg, const double* prop,
+                    double* val, double* R, hipStream_t s)
+{
+    const int nC = (int)M.colorOff.size() - 1;
+    for (int c = 0; c < nC; ++c) {
+        const int begin = M.colorOff[c], count = M.colorOff[c + 1] - begin;
+        if (count == 0) continue;
+        const unsigned grid = (unsigned)((count * 16 + 255) / 256);
+        if (phys == SVFSI_PHYS_LELAS)
+            hipLaunchKernelGGL(k_asm_lelas, dim3(grid), dim3(256), 0, s, begin, count, M.ien.as<int>(),
+                               M.elemId.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, prop,
+                               P, rule, val, R);
+        else
+            hipLaunchKernelGGL(k_asm_fluid, dim3(grid), dim3(256), 0, s, begin, count, M.ien.as<int>(),
+                               M.elemId.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Yg, prop,
+                               P, rule, val, R);
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_face_traction(const Mesh& M, const Mesh::FaceSet& F, int dof, const double* h,
+                         const Tri3Rule& rule, double* R, hipStream_t s)
+{
+    const int nC = (int)F.colorOff.size() - 1;
+    for (int c = 0; c < nC; ++c) {
+        const int begin = F.colorOff[c], count = F.colorOff[c + 1] - begin;
+        if (count == 0) continue;
+        hipLaunchKernelGGL(k_face_traction, dim3((count * 3 + 255) / 256), dim3(256), 0, s, begin, count,
+                           F.order.as<int>(), F.fien.as<int>(), F.inner.as<int>(), M.x.as<double>(), dof,
+                           h[0], h[1], h[2], rule, R);
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_face_backflow(const Mesh& M, const Mesh::FaceSet& F, const AsmParams& P, int tDof,
+                         const double* Yg, const Tri3Rule& rule, double* val, double* R, hipStream_t s)
+{
+    const int nC = (int)F.colorOff.size() - 1;
+    for (int c = 0; c < nC; ++c) {
+        const int begin = F.colorOff[c], count = F.colorOff[c + 1] - begin;
+        if (count == 0) continue;
+        hipLaunchKernelGGL(k_face_backflow, dim3((count * 9 + 255) / 256), dim3(256), 0, s, begin, count,
+                           F.order.as<int>(), F.fien.as<int>(), F.inner.as<int>(), F.fmap.as<int>(),
+                           M.x.as<double>(), tDof, Yg, P, rule, val, R);
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_lelas_dkde(const Mesh& M, const AsmParams& P, const Tet4Rule& rule, const double* lam,
+                      const double* u, double* g, hipStream_t s)
+{
+    if (M.nEl == 0) return SVFSI_OK;
+    hipLaunchKernelGGL(k_lelas_dkde, dim3((M.nEl + 255) / 256), dim3(256), 0, s, M.nEl, M.ien.as<int>(),
+                       M.elemId.as<int>(), M.x.as<double>(), P, rule, lam, u, g);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+} // namespace svfsi

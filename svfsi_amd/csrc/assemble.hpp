@@ -1,0 +1,47 @@
+// assemble.hpp -- kernel-side parameter blocks of the assembly kernels.
+#pragma once
+
+#include "internal.hpp"
+
+namespace svfsi {
+
+// Equation parameters passed by value to the assembly kernels.
+struct AsmParams {
+    double am, af, beta, gam, dt;
+    double rho, E, nu;
+    double fb[3];
+    double mu, bstab;
+    double C10, Kpen;
+    int volType;
+    int pad;
+};
+
+// TET4 4-point rule and shape values (GETGIP/GETGNN, svFSI/NN.f:266-275,
+// 655-671), computed on the host with the reference's formulas.
+struct Tet4Rule {
+    double w[4];
+    double N[4][4]; // N[g][a]
+};
+
+// TRI3 3-point face rule (svFSI/NN.f:416-422, 1100-1110).
+struct Tri3Rule {
+    double w[3];
+    double N[3][3];
+};
+
+Tet4Rule make_tet4_rule();
+Tri3Rule make_tri3_rule();
+AsmParams to_asm(const svfsi_eq_params& p);
+
+int launch_asm_tet4(const Mesh& M, int phys, const AsmParams& P, const Tet4Rule& rule, int tDof,
+                    const double* Ag, const double* Yg, const double* Dg, const double* prop,
+                    double* val, double* R, hipStream_t s);
+int launch_face_traction(const Mesh& M, const Mesh::FaceSet& F, int dof, const double* h,
+                         const Tri3Rule& rule, double* R, hipStream_t s);
+int launch_face_backflow(const Mesh& M, const Mesh::FaceSet& F, const AsmParams& P, int tDof,
+                         const double* Yg, const Tri3Rule& rule, double* val, double* R,
+                         hipStream_t s);
+int launch_lelas_dkde(const Mesh& M, const AsmParams& P, const Tet4Rule& rule, const double* lam,
+                      const double* u, double* g, hipStream_t s);
+
+} // namespace svfsi

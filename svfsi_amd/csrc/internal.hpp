@@ -1,0 +1,153 @@
+// internal.hpp -- shared definitions of libsvfsi_amd (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/svfsi_amd.h"
+
+namespace svfsi {
+
+void set_error(const char* fmt, ...);
+
+#define SV_HIP(x)                                                              \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) {                                                \
+            ::svfsi::set_error("%s:%d %s: %s", __FILE__, __LINE__, #x,         \
+                               hipGetErrorString(e_));                         \
+            return SVFSI_ERR_HIP;                                              \
+        }                                                                      \
+    } while (0)
+
+#define SV_TRY(x)                                                              \
+    do {                                                                       \
+        int r_ = (x);                                                          \
+        if (r_ != SVFSI_OK) return r_;                                         \
+    } while (0)
+
+#define SV_ARG(cond, ...)                                                      \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            ::svfsi::set_error(__VA_ARGS__);                                   \
+            return SVFSI_ERR_ARG;                                              \
+        }                                                                      \
+    } while (0)
+
+int require_device();
+
+// Device buffer owned by the library (hipMalloc'd, freed in destructor).
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t nbytes) {
+        if (nbytes <= bytes && p) return SVFSI_OK;
+        release();
+        if (nbytes == 0) nbytes = 8;
+        SV_HIP(hipMalloc(&p, nbytes));
+        bytes = nbytes;
+        return SVFSI_OK;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Face {
+    bool foC = false;
+    int nNo = 0, dof = 0, bGrp = BC_TYPE_Dir;
+    std::vector<int> glob;   // 0-based nodes
+    std::vector<double> val; // (dof, nNo)
+};
+
+// Krylov workspace, grown on demand and reused across solves.
+struct Work {
+    DBuf W, X, R0, tmp, basis, basis32, Z32, val32, x32, y32;
+    DBuf partial, state, hess;
+    DBuf mask;
+    int maskDof = -1;
+    std::vector<int> maskInc;
+    int maskVersion = -1;
+};
+
+struct Lhs {
+    int gnNo = 0, nNo = 0, nnz = 0, nFaces = 0;
+    DBuf rowPtr, colPtr, diagPtr, blockRow, trans;
+    std::vector<int> hRowPtr, hColPtr; // 0-based host copies
+    std::vector<Face> faces;
+    int facesVersion = 0;
+    bool symmetric = false; // every block (i,j) has a (j,i) partner: trans[] valid
+    Work ws;
+};
+
+struct Mesh {
+    Lhs* lhs = nullptr;
+    int nNo = 0, nEl = 0, eNoN = 0;
+    DBuf x;          // (3, nNo)
+    DBuf ien;        // (eNoN, nEl) colour-sorted, 0-based
+    DBuf elemId;     // original element id of each colour-sorted slot
+    DBuf emap;       // (eNoN*eNoN, nEl) colour-sorted nnz index of block (a,b)
+    std::vector<int> colorOff; // nColors+1 offsets into the colour-sorted list
+    struct FaceSet {
+        int nFe = 0;
+        DBuf fien;   // (3, nFe) 0-based
+        DBuf fel;    // (nFe) 0-based volume element
+        DBuf inner;  // (nFe) first element node not on the face
+        std::vector<int> colorOff;
+        DBuf fmap;   // (9, nFe) nnz index of face block (a,b)
+        DBuf order;  // colour-sorted face ids
+    };
+    std::vector<FaceSet*> faceSets;
+    ~Mesh() {
+        for (auto* f : faceSets) delete f;
+    }
+};
+
+// Profiling of SpMV launches (HIP events).
+struct ProfPending {
+    hipEvent_t a, b;
+    int k;
+};
+struct Prof {
+    bool on = false;
+    long long count[2] = {0, 0};
+    double ms[2] = {0.0, 0.0};
+    std::vector<ProfPending> pending;
+};
+Prof& prof();
+void prof_resolve();
+
+// ---- launchers (device pointers; stream may be null) ----
+int launch_spmv(const Lhs& L, int dof, const double* K, const double* U,
+                double* KU, hipStream_t s, const int* stop = nullptr);
+int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
+                    float* KU, hipStream_t s, const int* stop = nullptr);
+int launch_spmv_t(const Lhs& L, int dof, const double* K, const double* U,
+                  double* KU, hipStream_t s, const int* stop = nullptr);
+int launch_dot(int n, const double* U, const double* V, double* partial,
+               double* result, hipStream_t s);
+int build_mask(Lhs& L, int dof, const int* incL, hipStream_t s);
+int precond_diag(Lhs& L, int dof, double* Val, double* R, double* W,
+                 float* Val32, const int* incL, hipStream_t s);
+int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s);
+
+int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
+             double* R, hipStream_t s);
+int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
+                double* R, hipStream_t s, bool transpose);
+int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
+                const float* K32, double* R, hipStream_t s);
+
+double wall_seconds();
+
+} // namespace svfsi

@@ -1,0 +1,741 @@
+// krylov.hip -- device-resident CGRADV / GMRESV for gfx950.
+//
+// The iteration scalars (alpha, err, Hessenberg column, Givens rotations,
+// convergence flag) live in device memory and are produced by the last
+// workgroup of each reduction kernel, so an iteration issues no host
+// round trip: CG is three kernels per iteration, GMRES three kernels per
+// Arnoldi step (SpMV; multi-dot + Pythagorean norm + Givens; update).  The
+// host enqueues a chunk of iterations and polls a pinned copy of the state
+// of the previous chunk; kernels after convergence exit at their first
+// instruction.  All reductions are fixed-order, so results are bitwise
+// reproducible run to run.
+#include "internal.hpp"
+#include "devutil.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace svfsi {
+
+double wall_seconds()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ===========================================================================
+// CGRADV (svFSILS/CGRAD.f:184-242)
+// ===========================================================================
+struct CgState {
+    double err, errO, alpha, eps;
+    int done, suc, itr, mItr;
+    unsigned int counter;
+    int pad;
+};
+
+// K P for the rows of this block, <P, K P>; last block: alpha = errO / <P,KP>.
+template <int DOF>
+__global__ __launch_bounds__(256) void k_cg_spmv_dot(int nNo, const int* __restrict__ rowPtr,
+                                                     const int* __restrict__ colPtr,
+                                                     const double* __restrict__ val,
+                                                     const double* __restrict__ p,
+                                                     double* __restrict__ kp,
+                                                     double* __restrict__ partial,
+                                                     CgState* __restrict__ st)
+{
+    if (st->done) return;
+    constexpr int DD = DOF * DOF;
+    constexpr int G = kWave / DD;
+    __shared__ double lds[4];
+    __shared__ int flag;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int g = lane / DD;
+    const int r = lane - g * DD;
+    const int i = r / DOF;
+    const int j = r - i * DOF;
+    const int row = (blockIdx.x * 4 + wave) * G + g;
+    const bool active = (g < G) && (row < nNo);
+    double acc = 0.0;
+    if (active) {
+        int k = rowPtr[row];
+        const int e = rowPtr[row + 1];
+        double a0 = 0.0, a1 = 0.0;
+        for (; k + 1 < e; k += 2) {
+            const int c0 = colPtr[k], c1 = colPtr[k + 1];
+            a0 += __builtin_nontemporal_load(val + (size_t)k * DD + r) * p[(size_t)c0 * DOF + j];
+            a1 += __builtin_nontemporal_load(val + (size_t)(k + 1) * DD + r) * p[(size_t)c1 * DOF + j];
+        }
+        if (k < e) a0 += __builtin_nontemporal_load(val + (size_t)k * DD + r) * p[(size_t)colPtr[k] * DOF + j];
+        acc = a0 + a1;
+    }
+    if constexpr (DOF == 4) {
+        acc += __shfl_xor(acc, 1, kWave);
+        acc += __shfl_xor(acc, 2, kWave);
+    } else if constexpr (DOF == 2) {
+        acc += __shfl_xor(acc, 1, kWave);
+    } else {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 1; q < DOF; ++q) t += __shfl_down(acc, q, kWave);
+        acc += t;
+    }
+    double contrib = 0.0;
+    if (active && j == 0) {
+        kp[(size_t)row * DOF + i] = acc;
+        contrib = p[(size_t)row * DOF + i] * acc;
+    }
+    double s = block_sum<256>(contrib, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (last_block_done(&st->counter, &flag)) {
+        double t = 0.0;
+        for (int q = threadIdx.x; q < (int)gridDim.x; q += 256) t += load_agent(partial + q);
+        t = block_sum<256>(t, lds);
+        if (threadIdx.x == 0) st->alpha = st->err / t;
+    }
+}
+
+// X += alpha P ; R -= alpha KP ; err = |R|^2 ; convergence test of the next
+// loop top (CGRAD.f:205-215).
+__global__ __launch_bounds__(256) void k_cg_update(size_t n, double* __restrict__ x,
+                                                   double* __restrict__ rr,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ kp,
+                                                   double* __restrict__ partial,
+                                                   CgState* __restrict__ st)
+{
+    if (st->done) return;
+    __shared__ double lds[4];
+    __shared__ int flag;
+    const double alpha = st->alpha;
+    double s = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        x[q] = x[q] + alpha * p[q];
+        const double rv = rr[q] + (-alpha) * kp[q];
+        rr[q] = rv;
+        s += rv * rv;
+    }
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (last_block_done(&st->counter, &flag)) {
+        double t = 0.0;
+        for (int q = threadIdx.x; q < (int)gridDim.x; q += 256) t += load_agent(partial + q);
+        t = block_sum<256>(t, lds);
+        if (threadIdx.x == 0) {
+            // FSILS_NORMV returns sqrt(sum); CGRADV squares it again.
+            const double nrm = sqrt(t);
+            const double err = nrm * nrm;
+            st->errO = st->err;
+            st->err = err;
+            st->itr += 1;
+            if (st->itr >= st->mItr) {
+                st->done = 1;             // loop exhausted: suc stays false
+            } else if (err < st->eps) {
+                st->done = 1;
+                st->suc = 1;
+            }
+        }
+    }
+}
+
+// P = (P + (errO/err) R) * (err/errO)  (OMPSUMV then OMPMULV, CGRAD.f:227-229)
+__global__ __launch_bounds__(256) void k_cg_pupdate(size_t n, double* __restrict__ p,
+                                                    const double* __restrict__ rr,
+                                                    const CgState* __restrict__ st)
+{
+    if (st->done) return;
+    const double r1 = st->errO / st->err;
+    const double r2 = st->err / st->errO;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        p[q] = (p[q] + r1 * rr[q]) * r2;
+}
+
+static unsigned vec_grid(size_t n)
+{
+    size_t g = (n + 255) / 256;
+    if (g > 2048) g = 2048;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+// Pinned host mirror of a small device state, double-buffered.
+struct Poll {
+    void* h[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    int init(size_t b) {
+        bytes = b;
+        for (int q = 0; q < 2; ++q) {
+            SV_HIP(hipHostMalloc(&h[q], b, hipHostMallocDefault));
+            SV_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
+        }
+        return SVFSI_OK;
+    }
+    ~Poll() {
+        for (int q = 0; q < 2; ++q) {
+            if (h[q]) (void)hipHostFree(h[q]);
+            if (ev[q]) (void)hipEventDestroy(ev[q]);
+        }
+    }
+    int post(int q, const void* dsrc, hipStream_t s) {
+        SV_HIP(hipMemcpyAsync(h[q], dsrc, bytes, hipMemcpyDeviceToHost, s));
+        SV_HIP(hipEventRecord(ev[q], s));
+        return SVFSI_OK;
+    }
+    int wait(int q) {
+        SV_HIP(hipEventSynchronize(ev[q]));
+        return SVFSI_OK;
+    }
+};
+
+int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s)
+{
+    const size_t n = (size_t)L.nNo * dof;
+    Work& w = L.ws;
+    const int gs = (L.nNo + 4 * (kWave / (dof * dof)) - 1) / (4 * (kWave / (dof * dof)));
+    const unsigned gv = vec_grid(n);
+    SV_TRY(w.X.ensure(n * sizeof(double)));
+    SV_TRY(w.tmp.ensure(2 * n * sizeof(double)));
+    // partials of the widest reduction (SpMV grid, vector grid, or the
+    // launch_dot grid of 512) followed by the scalar result slot
+    size_t nPart = (size_t)(gs > (int)gv ? gs : gv);
+    if (nPart < 512) nPart = 512;
+    SV_TRY(w.partial.ensure((nPart + 16) * sizeof(double)));
+    SV_TRY(w.state.ensure(sizeof(CgState)));
+    double* X = w.X.as<double>();
+    double* P = w.tmp.as<double>();
+    double* KP = P + n;
+    double* part = w.partial.as<double>();
+    double* scal = part + nPart + 8;
+    CgState* st = w.state.as<CgState>();
+
+    ls.callD = wall_seconds();
+    ls.suc = 0;
+    SV_TRY(launch_dot((int)n, R, R, part, scal, s));
+    double rr = 0.0;
+    SV_HIP(hipMemcpyAsync(&rr, scal, sizeof(double), hipMemcpyDeviceToHost, s));
+    SV_HIP(hipStreamSynchronize(s));
+    ls.iNorm = std::sqrt(rr);
+    double eps = std::fmax(ls.absTol, ls.relTol * ls.iNorm);
+    eps = eps * eps;
+    CgState h0{};
+    h0.errO = ls.iNorm * ls.iNorm;
+    h0.err = h0.errO;
+    h0.eps = eps;
+    h0.mItr = ls.mItr;
+    h0.itr = 0;
+    h0.done = 0;
+    h0.suc = 0;
+    if (h0.err < eps) { h0.done = 1; h0.suc = 1; }
+    if (ls.mItr <= 0) h0.done = 1;
+    SV_HIP(hipMemcpyAsync(st, &h0, sizeof(CgState), hipMemcpyHostToDevice, s));
+    SV_HIP(hipMemsetAsync(X, 0, n * sizeof(double), s));
+    SV_HIP(hipMemcpyAsync(P, R, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+
+    Poll poll;
+    SV_TRY(poll.init(sizeof(CgState)));
+    const int chunk = 16;
+    int posted = 0;
+    CgState hs = h0;
+    bool finished = h0.done != 0;
+    int launched = 0;
+    while (!finished) {
+        const int q = posted & 1;
+        for (int it = 0; it < chunk; ++it) {
+            switch (dof) {
+            case 2: hipLaunchKernelGGL(k_cg_spmv_dot<2>, dim3(gs), dim3(256), 0, s, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, P, KP, part, st); break;
+            case 3: hipLaunchKernelGGL(k_cg_spmv_dot<3>, dim3(gs), dim3(256), 0, s, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, P, KP, part, st); break;
+            case 4: hipLaunchKernelGGL(k_cg_spmv_dot<4>, dim3(gs), dim3(256), 0, s, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, P, KP, part, st); break;
+            default: set_error("CG: dof %d not supported", dof); return SVFSI_ERR_NOTSUP;
+            }
+            hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, s, n, X, R, P, KP, part, st);
+            hipLaunchKernelGGL(k_cg_pupdate, dim3(gv), dim3(256), 0, s, n, P, R, (const CgState*)st);
+        }
+        launched += chunk;
+        SV_HIP(hipGetLastError());
+        SV_TRY(poll.post(q, st, s));
+        ++posted;
+        // check the previous chunk while this one runs
+        if (posted >= 2) {
+            SV_TRY(poll.wait(q ^ 1));
+            std::memcpy(&hs, poll.h[q ^ 1], sizeof(CgState));
+            if (hs.done) finished = true;
+        }
+        if (launched > ls.mItr + 2 * chunk) break; // safety: flag must be set by now
+    }
+    SV_HIP(hipStreamSynchronize(s));
+    SV_HIP(hipMemcpy(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost));
+    SV_HIP(hipMemcpyAsync(R, X, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SV_HIP(hipStreamSynchronize(s));
+    ls.suc = hs.suc;
+    ls.itr = hs.itr;
+    ls.cM = hs.itr;
+    ls.fNorm = std::sqrt(hs.err);
+    ls.callD = wall_seconds() - ls.callD;
+    ls.dB = (hs.errO < 2.220446049250313e-16) ? 0.0 : 5.0 * std::log(hs.err / hs.errO);
+    return SVFSI_OK;
+}
+
+// ===========================================================================
+// GMRESV (svFSILS/GMRES.f:273-431), templated on the basis storage type BT
+// (double for the FP64 path; float for the inner solver of the mixed path,
+// whose dot products and Hessenberg arithmetic stay FP64).
+// ===========================================================================
+struct GmState {
+    double eps;
+    double fNorm;
+    int stop, suc, iconv, pad;
+    unsigned int counter;
+    int pad2;
+};
+
+struct GmArrays {
+    double* H;     // (sD+1, sD) column-major: H(j,i) at (i-1)*(sD+1)+(j-1)
+    double* c;     // sD
+    double* sn;    // sD
+    double* err;   // sD+1
+    double* y;     // sD
+    double* hraw;  // sD+2: CGS coefficients of the current step, then 1/h(i+1,i)
+};
+
+static size_t gm_arrays_doubles(int sD) { return (size_t)(sD + 1) * sD + 3 * sD + (sD + 1) + (sD + 3); }
+
+static GmArrays gm_carve(double* base, int sD)
+{
+    GmArrays a;
+    a.H = base;
+    a.c = a.H + (size_t)(sD + 1) * sD;
+    a.sn = a.c + sD;
+    a.err = a.sn + sD;
+    a.y = a.err + (sD + 1);
+    a.hraw = a.y + sD;
+    return a;
+}
+
+// u0 = R - u0 and |u0|; last block: err(1) = |u0|, cycle state reset.
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_resid(size_t n, const BT* __restrict__ R,
+                                                  BT* __restrict__ u0,
+                                                  double* __restrict__ partial,
+                                                  GmState* __restrict__ st, GmArrays A)
+{
+    __shared__ double lds[4];
+    __shared__ int flag;
+    double s = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const BT v = R[q] - u0[q];
+        u0[q] = v;
+        s += (double)v * (double)v;
+    }
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (last_block_done(&st->counter, &flag)) {
+        double t = 0.0;
+        for (int q = threadIdx.x; q < (int)gridDim.x; q += 256) t += load_agent(partial + q);
+        t = block_sum<256>(t, lds);
+        if (threadIdx.x == 0) {
+            A.err[0] = sqrt(t);
+            st->stop = 0;
+            st->iconv = 0;
+        }
+    }
+}
+
+// u0 = u0 / err(1)   (GMRES.f:336: a division, not a reciprocal product)
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_div(size_t n, BT* __restrict__ u0, GmArrays A)
+{
+    const double e0 = A.err[0];
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        u0[q] = (BT)((double)u0[q] / e0);
+}
+
+constexpr int kMdotPerThread = 8;
+constexpr int kMaxSD = 1000;
+
+// h(j,i) = <u_j, u_{i+1}> for j = 1..i+1 (GMRES.f:337-340), then in the last
+// block: Pythagorean h(i+1,i) (342-347), previous rotations, new Givens
+// rotation, residual estimate and convergence test (351-366).
+// i is the 1-based Arnoldi step; u_j is basis + (j-1)*n.
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_mdot(size_t n, int i, int sD, const BT* __restrict__ basis,
+                                                 double* __restrict__ partial,
+                                                 GmState* __restrict__ st, GmArrays A)
+{
+    if (st->stop) return;
+    __shared__ double lds[4];
+    __shared__ int flag;
+    const int nb = gridDim.x;
+    const size_t chunk = (n + nb - 1) / nb;
+    const size_t q0 = (size_t)blockIdx.x * chunk;
+    const size_t q1 = (q0 + chunk < n) ? q0 + chunk : n;
+    const BT* w = basis + (size_t)i * n;
+    double wv[kMdotPerThread];
+#pragma unroll
+    for (int m = 0; m < kMdotPerThread; ++m) {
+        const size_t q = q0 + threadIdx.x + (size_t)m * 256;
+        wv[m] = (q < q1) ? (double)w[q] : 0.0;
+    }
+    for (int jj = 0; jj <= i; ++jj) {
+        const BT* u = basis + (size_t)jj * n;
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < kMdotPerThread; ++m) {
+            const size_t q = q0 + threadIdx.x + (size_t)m * 256;
+            if (q < q1) s += (double)u[q] * wv[m];
+        }
+        s = block_sum<256>(s, lds);
+        if (threadIdx.x == 0) partial[(size_t)jj * nb + blockIdx.x] = s;
+    }
+    if (!last_block_done(&st->counter, &flag)) return;
+    // reduce partials: 4 lanes per coefficient, fixed order
+    __shared__ double hsh[kMaxSD + 2];
+    for (int jj = threadIdx.x >> 2; jj <= i; jj += 64) {
+        double t = 0.0;
+        for (int b = threadIdx.x & 3; b < nb; b += 4) t += load_agent(partial + (size_t)jj * nb + b);
+        t += __shfl_xor(t, 1, kWave);
+        t += __shfl_xor(t, 2, kWave);
+        if ((threadIdx.x & 3) == 0) hsh[jj] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double* Hc = A.H + (size_t)(i - 1) * (sD + 1); // column i: Hc[j-1] = h(j,i)
+    for (int jj = 0; jj <= i; ++jj) {
+        Hc[jj] = hsh[jj];
+        A.hraw[jj] = hsh[jj];
+    }
+    double hii = Hc[i];
+    for (int jj = 0; jj < i; ++jj) hii = hii - Hc[jj] * Hc[jj];
+    hii = sqrt(fabs(hii));
+    Hc[i] = hii;
+    A.hraw[sD + 2] = 1.0 / hii;  // OMPMULV(1/h(i+1,i)) factor for k_gm_update
+    for (int jj = 1; jj <= i - 1; ++jj) {
+        const double tmp = A.c[jj - 1] * Hc[jj - 1] + A.sn[jj - 1] * Hc[jj];
+        Hc[jj] = -A.sn[jj - 1] * Hc[jj - 1] + A.c[jj - 1] * Hc[jj];
+        Hc[jj - 1] = tmp;
+    }
+    const double tmp = sqrt(Hc[i - 1] * Hc[i - 1] + Hc[i] * Hc[i]);
+    A.c[i - 1] = Hc[i - 1] / tmp;
+    A.sn[i - 1] = Hc[i] / tmp;
+    Hc[i - 1] = tmp;
+    Hc[i] = 0.0;
+    A.err[i] = -A.sn[i - 1] * A.err[i - 1];
+    A.err[i - 1] = A.c[i - 1] * A.err[i - 1];
+    if (fabs(A.err[i]) < st->eps) {
+        st->suc = 1;
+        st->stop = 1;
+        st->iconv = i;
+    } else if (i >= sD) {
+        st->stop = 1;
+        st->iconv = sD;
+    }
+}
+
+// u_{i+1} = (u_{i+1} - sum_j h(j,i) u_j) * (1/h(i+1,i))   (GMRES.f:342-349)
+// The pre-rotation coefficients were saved in hraw by k_gm_mdot.
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_update(size_t n, int i, int sD, BT* __restrict__ basis,
+                                                   const GmState* __restrict__ st, GmArrays A)
+{
+    if (st->stop) return;
+    BT* w = basis + (size_t)i * n;
+    const double inv = A.hraw[sD + 2];
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        double v = (double)w[q];
+        for (int jj = 0; jj < i; ++jj) v = v + (-A.hraw[jj]) * (double)basis[(size_t)jj * n + q];
+        w[q] = (BT)(v * inv);
+    }
+}
+
+// Back substitution of the cycle (GMRES.f:369-376), one thread.
+__global__ void k_gm_backsolve(int sD, GmState* __restrict__ st, GmArrays A)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int i = st->iconv;
+    if (i > sD) i = sD;
+    for (int jj = 0; jj < i; ++jj) A.y[jj] = A.err[jj];
+    for (int jj = i; jj >= 1; --jj) {
+        double yj = A.y[jj - 1];
+        for (int k = jj + 1; k <= i; ++k) yj = yj - A.H[(size_t)(k - 1) * (sD + 1) + (jj - 1)] * A.y[k - 1];
+        A.y[jj - 1] = yj / A.H[(size_t)(jj - 1) * (sD + 1) + (jj - 1)];
+    }
+    st->fNorm = fabs(A.err[i]);
+}
+
+// X = X + y_j u_j, j = 1..i (GMRES.f:378-381)
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_xupdate(size_t n, BT* __restrict__ X, const BT* __restrict__ basis,
+                                                    const GmState* __restrict__ st, GmArrays A)
+{
+    const int i = st->iconv;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        double v = (double)X[q];
+        for (int jj = 0; jj < i; ++jj) v = v + A.y[jj] * (double)basis[(size_t)jj * n + q];
+        X[q] = (BT)v;
+    }
+}
+
+template <typename BT>
+__global__ void k_set_zero(size_t n, BT* __restrict__ x)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) x[q] = (BT)0;
+}
+
+template <typename BT>
+__global__ __launch_bounds__(256) void k_normsq_partial(size_t n, const BT* __restrict__ u,
+                                                        double* __restrict__ partial)
+{
+    __shared__ double lds[4];
+    double s = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        s += (double)u[q] * (double)u[q];
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void k_sum_to(int m, const double* __restrict__ partial, double* __restrict__ out)
+{
+    __shared__ double lds[4];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < m; q += 256) s += partial[q];
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// Generic operator: y = A x on the device (FP64 or FP32 matrix, plain or
+// transposed), optionally gated by the device stop flag.
+template <typename BT> struct GmOp;
+
+template <> struct GmOp<double> {
+    const Lhs* L; int dof; const double* K; bool transpose;
+    int apply(const double* x, double* y, hipStream_t s, const int* stop) const {
+        return transpose ? launch_spmv_t(*L, dof, K, x, y, s, stop) : launch_spmv(*L, dof, K, x, y, s, stop);
+    }
+};
+template <> struct GmOp<float> {
+    const Lhs* L; int dof; const float* K;
+    int apply(const float* x, float* y, hipStream_t s, const int* stop) const {
+        return launch_spmv_f32(*L, dof, K, x, y, s, stop);
+    }
+};
+
+// Restarted GMRES(sD) on n = nNo*dof unknowns.  R: RHS in, solution out
+// (GMRESV overwrites R with X, GMRES.f:383).  Fills suc/itr/iNorm/fNorm/dB.
+template <typename BT>
+static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf, DBuf& xBuf,
+                      FSILS_subLsType& ls, hipStream_t s)
+{
+    const size_t n = (size_t)L.nNo * dof;
+    const int sD = ls.sD > 0 ? ls.sD : 1;
+    if (sD > kMaxSD) {
+        set_error("GMRES: Krylov dimension %d exceeds %d", sD, kMaxSD);
+        return SVFSI_ERR_NOTSUP;
+    }
+    Work& w = L.ws;
+    size_t nb = (n + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    const size_t nbMin = (n + 256 * kMdotPerThread - 1) / (256 * kMdotPerThread);
+    if (nb < nbMin) nb = nbMin;
+    if (nb < 1) nb = 1;
+    const unsigned gv = vec_grid(n);
+    SV_TRY(basisBuf.ensure((size_t)(sD + 1) * n * sizeof(BT)));
+    SV_TRY(xBuf.ensure(n * sizeof(BT)));
+    const size_t partN = ((size_t)(sD + 2) * nb > gv ? (size_t)(sD + 2) * nb : gv) + 16;
+    SV_TRY(w.partial.ensure(partN * sizeof(double)));
+    SV_TRY(w.state.ensure(256));
+    SV_TRY(w.hess.ensure(gm_arrays_doubles(sD) * sizeof(double)));
+    BT* basis = basisBuf.as<BT>();
+    BT* X = xBuf.as<BT>();
+    double* part = w.partial.as<double>();
+    double* scal = part + partN - 8;
+    GmState* st = w.state.as<GmState>();
+    GmArrays A = gm_carve(w.hess.as<double>(), sD);
+    const int* stopFlag = &st->stop;
+
+    ls.callD = wall_seconds();
+    ls.suc = 0;
+    hipLaunchKernelGGL(k_normsq_partial<BT>, dim3(gv), dim3(256), 0, s, n, (const BT*)R, part);
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, (int)gv, part, scal);
+    double rr = 0.0;
+    SV_HIP(hipMemcpyAsync(&rr, scal, sizeof(double), hipMemcpyDeviceToHost, s));
+    SV_HIP(hipStreamSynchronize(s));
+    double eps = std::sqrt(rr);
+    ls.iNorm = eps;
+    ls.fNorm = eps;
+    eps = std::fmax(ls.absTol, ls.relTol * eps);
+    ls.itr = 0;
+    ls.cM = 0;
+    hipLaunchKernelGGL(k_set_zero<BT>, dim3(gv), dim3(256), 0, s, n, X);
+    if (ls.iNorm <= ls.absTol) {
+        // GMRES.f:307-311: return before R = X (R keeps the scaled RHS)
+        ls.callD = 2.220446049250313e-16;
+        ls.dB = 0.0;
+        return SVFSI_OK;
+    }
+    GmState h0{};
+    h0.eps = eps;
+    SV_HIP(hipMemcpyAsync(st, &h0, sizeof(GmState), hipMemcpyHostToDevice, s));
+
+    Poll poll;
+    SV_TRY(poll.init(sizeof(GmState)));
+    GmState hs{};
+    const int chunk = 8;
+    for (int l = 1; l <= ls.mItr; ++l) {
+        ls.dB = ls.fNorm;
+        ls.itr += 1;
+        // u1 = R - K X ; err(1) = |u1| ; u1 = u1/err(1)
+        SV_TRY(op.apply(X, basis, s, nullptr));
+        hipLaunchKernelGGL(k_gm_resid<BT>, dim3(gv), dim3(256), 0, s, n, (const BT*)R, basis, part, st, A);
+        hipLaunchKernelGGL(k_gm_div<BT>, dim3(gv), dim3(256), 0, s, n, basis, A);
+        ls.cM += 1;
+        int posted = 0;
+        bool stopped = false;
+        for (int i0 = 1; i0 <= sD && !stopped; i0 += chunk) {
+            const int i1 = (i0 + chunk - 1 < sD) ? i0 + chunk - 1 : sD;
+            for (int i = i0; i <= i1; ++i) {
+                SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, s, stopFlag));
+                hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nb), dim3(256), 0, s, n, i, sD, (const BT*)basis, part, st, A);
+                hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, s, n, i, sD, basis, (const GmState*)st, A);
+            }
+            SV_HIP(hipGetLastError());
+            const int q = posted & 1;
+            SV_TRY(poll.post(q, st, s));
+            ++posted;
+            if (posted >= 2) {
+                SV_TRY(poll.wait(q ^ 1));
+                std::memcpy(&hs, poll.h[q ^ 1], sizeof(GmState));
+                if (hs.stop) stopped = true;
+            }
+        }
+        hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, sD, st, A);
+        hipLaunchKernelGGL(k_gm_xupdate<BT>, dim3(gv), dim3(256), 0, s, n, X, (const BT*)basis, (const GmState*)st, A);
+        SV_HIP(hipGetLastError());
+        SV_HIP(hipMemcpyAsync(&hs, st, sizeof(GmState), hipMemcpyDeviceToHost, s));
+        SV_HIP(hipStreamSynchronize(s));
+        ls.itr += hs.iconv;
+        ls.cM += hs.iconv;
+        ls.fNorm = hs.fNorm;
+        if (hs.suc) {
+            ls.suc = 1;
+            break;
+        }
+    }
+    SV_HIP(hipMemcpyAsync(R, X, n * sizeof(BT), hipMemcpyDeviceToDevice, s));
+    SV_HIP(hipStreamSynchronize(s));
+    ls.callD = wall_seconds() - ls.callD;
+    ls.dB = 10.0 * std::log(ls.fNorm / ls.dB);
+    return SVFSI_OK;
+}
+
+int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s,
+                bool transpose)
+{
+    GmOp<double> op{&L, dof, K, transpose};
+    return gmres_core<double>(L, op, dof, R, L.ws.basis, L.ws.X, ls, s);
+}
+
+// ===========================================================================
+// Mixed precision: FP64 iterative refinement around an inner GMRES on the
+// FP32 copy of the scaled matrix (FP32 SpMV and basis storage, FP64 dot
+// accumulation and Hessenberg least squares).  The outer residual
+// r = b - A x uses the FP64 scaled matrix; convergence is the FP64 test
+// |r| < max(absTol, relTol |b|) of GMRESV.
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_resid64(size_t n, const double* __restrict__ b,
+                                                 const double* __restrict__ ax,
+                                                 double* __restrict__ r, float* __restrict__ r32,
+                                                 double* __restrict__ partial)
+{
+    __shared__ double lds[4];
+    double s = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const double v = b[q] - ax[q];
+        r[q] = v;
+        r32[q] = (float)v;
+        s += v * v;
+    }
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void k_add_f32(size_t n, double* __restrict__ x, const float* __restrict__ d)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        x[q] += (double)d[q];
+}
+
+int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const float* K32, double* R,
+                hipStream_t s)
+{
+    const size_t n = (size_t)L.nNo * dof;
+    Work& w = L.ws;
+    const unsigned gv = vec_grid(n);
+    SV_TRY(w.X.ensure(n * sizeof(double)));
+    SV_TRY(w.R0.ensure(n * sizeof(double)));
+    SV_TRY(w.tmp.ensure(n * sizeof(double)));
+    SV_TRY(w.y32.ensure(n * sizeof(float)));
+    double* x = w.X.as<double>();
+    double* r = w.R0.as<double>();
+    double* ax = w.tmp.as<double>();
+    float* r32 = w.y32.as<float>();
+    // outer reductions use their own buffer: gmres_core owns w.partial
+    SV_TRY(w.Z32.ensure((size_t)(gv + 16) * sizeof(double)));
+    double* part = w.Z32.as<double>();
+    double* scal = part + gv + 8;
+
+    ls.callD = wall_seconds();
+    ls.suc = 0;
+    hipLaunchKernelGGL(k_normsq_partial<double>, dim3(gv), dim3(256), 0, s, n, (const double*)R, part);
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, (int)gv, part, scal);
+    double rr = 0.0;
+    SV_HIP(hipMemcpyAsync(&rr, scal, sizeof(double), hipMemcpyDeviceToHost, s));
+    SV_HIP(hipStreamSynchronize(s));
+    ls.iNorm = std::sqrt(rr);
+    ls.fNorm = ls.iNorm;
+    const double eps = std::fmax(ls.absTol, ls.relTol * ls.iNorm);
+    ls.itr = 0;
+    ls.cM = 0;
+    SV_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
+    if (ls.iNorm <= ls.absTol) {
+        ls.callD = 2.220446049250313e-16;
+        ls.dB = 0.0;
+        return SVFSI_OK;
+    }
+    GmOp<float> op{&L, dof, K32};
+    double rnorm = ls.iNorm;
+    // r = b (x = 0)
+    SV_HIP(hipMemsetAsync(ax, 0, n * sizeof(double), s));
+    const int maxOuter = ls.mItr > 0 ? ls.mItr : 1;
+    double prev = rnorm;
+    for (int o = 0; o < maxOuter; ++o) {
+        hipLaunchKernelGGL(k_resid64, dim3(gv), dim3(256), 0, s, n, (const double*)R, (const double*)ax, r, r32, part);
+        hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, (int)gv, part, scal);
+        SV_HIP(hipMemcpyAsync(&rr, scal, sizeof(double), hipMemcpyDeviceToHost, s));
+        SV_HIP(hipStreamSynchronize(s));
+        prev = rnorm;
+        rnorm = std::sqrt(rr);
+        ls.itr += 1;
+        ls.fNorm = rnorm;
+        if (rnorm < eps) { ls.suc = 1; break; }
+        // inner: A32 d = r32 to a relative reduction that reaches eps with margin
+        FSILS_subLsType in{};
+        in.sD = ls.sD > 0 ? ls.sD : 50;
+        in.mItr = 50;
+        in.absTol = 0.0;
+        double target = 0.5 * eps / rnorm;
+        in.relTol = target > 1e-4 ? target : 1e-4;
+        SV_TRY(gmres_core<float>(L, op, dof, r32, w.basis32, w.x32, in, s));
+        ls.cM += in.cM;
+        hipLaunchKernelGGL(k_add_f32, dim3(gv), dim3(256), 0, s, n, x, (const float*)r32);
+        SV_TRY(launch_spmv(L, dof, K, x, ax, s));
+    }
+    (void)prev;
+    SV_HIP(hipMemcpyAsync(R, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SV_HIP(hipStreamSynchronize(s));
+    ls.callD = wall_seconds() - ls.callD;
+    ls.dB = 10.0 * std::log(ls.fNorm / ls.iNorm);
+    return SVFSI_OK;
+}
+
+} // namespace svfsi

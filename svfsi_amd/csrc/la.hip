@@ -1,0 +1,328 @@
+// la.hip -- block-CSR SpMV, reductions and diagonal scaling for gfx950.
+//
+// Block SpMV (FSILS_SPARMULVV, svFSILS/SPARMUL.f:51-133).  Mapping: a group
+// of DOF*DOF lanes owns one block row; lane r = i*DOF + j always reads entry
+// (i,j) of each block, so a group streams the row's contiguous blocks with
+// one coalesced DOF*DOF-element load per block (72 B FP64 dof 3, 128 B FP64
+// dof 4) and 64/(DOF*DOF) rows share a wavefront (7 rows for dof 3, 4 rows
+// for dof 4).  The per-lane partial sums over j are combined with lane
+// shuffles and the j == 0 lane stores KU(i,row).  Val is read with
+// non-temporal loads (streamed once per product) so that the gathered
+// vector U keeps the L2.
+#include "internal.hpp"
+#include "devutil.hpp"
+
+#include <cmath>
+
+namespace svfsi {
+
+template <int DOF, typename VT, typename XT>
+__global__ __launch_bounds__(256) void k_bspmv(int nNo, const int* __restrict__ rowPtr,
+                                               const int* __restrict__ colPtr,
+                                               const VT* __restrict__ val,
+                                               const XT* __restrict__ x,
+                                               XT* __restrict__ y,
+                                               const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    constexpr int DD = DOF * DOF;
+    constexpr int G = kWave / DD;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int g = lane / DD;
+    const int r = lane - g * DD;
+    const int i = r / DOF;
+    const int j = r - i * DOF;
+    const int row = (blockIdx.x * (blockDim.x >> 6) + wave) * G + g;
+    const bool active = (g < G) && (row < nNo);
+    double acc = 0.0;
+    if (active) {
+        int k = rowPtr[row];
+        const int e = rowPtr[row + 1];
+        double a0 = 0.0, a1 = 0.0;
+        for (; k + 1 < e; k += 2) {
+            const int c0 = colPtr[k], c1 = colPtr[k + 1];
+            const double v0 = (double)__builtin_nontemporal_load(val + (size_t)k * DD + r);
+            const double v1 = (double)__builtin_nontemporal_load(val + (size_t)(k + 1) * DD + r);
+            a0 += v0 * (double)x[(size_t)c0 * DOF + j];
+            a1 += v1 * (double)x[(size_t)c1 * DOF + j];
+        }
+        if (k < e) {
+            const int c0 = colPtr[k];
+            a0 += (double)__builtin_nontemporal_load(val + (size_t)k * DD + r) * (double)x[(size_t)c0 * DOF + j];
+        }
+        acc = a0 + a1;
+    }
+    if constexpr (DOF == 4) {
+        acc += __shfl_xor(acc, 1, kWave);
+        acc += __shfl_xor(acc, 2, kWave);
+    } else if constexpr (DOF == 2) {
+        acc += __shfl_xor(acc, 1, kWave);
+    } else {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 1; q < DOF; ++q) t += __shfl_down(acc, q, kWave);
+        acc += t;
+    }
+    if (active && j == 0) y[(size_t)row * DOF + i] = (XT)acc;
+}
+
+// Transpose product: row i of K^T gathers block (col,i) through the
+// transpose-position map trans[k] (pattern is structurally symmetric, as the
+// node-adjacency pattern of LHSA.f always is) and reads it transposed.
+template <int DOF>
+__global__ __launch_bounds__(256) void k_bspmv_t(int nNo, const int* __restrict__ rowPtr,
+                                                 const int* __restrict__ colPtr,
+                                                 const int* __restrict__ trans,
+                                                 const double* __restrict__ val,
+                                                 const double* __restrict__ x,
+                                                 double* __restrict__ y,
+                                                 const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    constexpr int DD = DOF * DOF;
+    constexpr int G = kWave / DD;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int g = lane / DD;
+    const int r = lane - g * DD;
+    const int i = r / DOF;
+    const int j = r - i * DOF;
+    const int row = (blockIdx.x * (blockDim.x >> 6) + wave) * G + g;
+    const bool active = (g < G) && (row < nNo);
+    double acc = 0.0;
+    if (active) {
+        for (int k = rowPtr[row]; k < rowPtr[row + 1]; ++k) {
+            const int c = colPtr[k];
+            acc += val[(size_t)trans[k] * DD + j * DOF + i] * x[(size_t)c * DOF + j];
+        }
+    }
+    if constexpr (DOF == 4) {
+        acc += __shfl_xor(acc, 1, kWave);
+        acc += __shfl_xor(acc, 2, kWave);
+    } else if constexpr (DOF == 2) {
+        acc += __shfl_xor(acc, 1, kWave);
+    } else {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 1; q < DOF; ++q) t += __shfl_down(acc, q, kWave);
+        acc += t;
+    }
+    if (active && j == 0) y[(size_t)row * DOF + i] = acc;
+}
+
+static inline int spmv_grid(int nNo, int dof)
+{
+    const int rowsPerBlock = 4 * (kWave / (dof * dof));
+    return (nNo + rowsPerBlock - 1) / rowsPerBlock;
+}
+
+// Event pair recorded around a profiled launch; resolved lazily by
+// prof_resolve() so that profiling adds no host synchronisation.
+struct ProfScope {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    int k;
+    ProfScope(int kernel, hipStream_t st) : s(st), k(kernel) {
+        if (prof().on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, s);
+        }
+    }
+    ~ProfScope() {
+        if (a) {
+            (void)hipEventRecord(b, s);
+            prof().pending.push_back({a, b, k});
+        }
+    }
+};
+
+template <typename VT, typename XT>
+static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU,
+                         hipStream_t s, int profKernel, const int* stop)
+{
+    if (L.nNo == 0) return SVFSI_OK;
+    ProfScope ps(profKernel, s);
+    const int grid = spmv_grid(L.nNo, dof);
+    const int* rp = L.rowPtr.as<int>();
+    const int* cp = L.colPtr.as<int>();
+    switch (dof) {
+    case 2: hipLaunchKernelGGL((k_bspmv<2, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
+    case 3: hipLaunchKernelGGL((k_bspmv<3, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
+    case 4: hipLaunchKernelGGL((k_bspmv<4, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
+    default: set_error("spmv: dof %d not supported (2..4)", dof); return SVFSI_ERR_NOTSUP;
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_spmv(const Lhs& L, int dof, const double* K, const double* U, double* KU,
+                hipStream_t s, const int* stop)
+{
+    return spmv_dispatch<double, double>(L, dof, K, U, KU, s, 0, stop);
+}
+
+int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U, float* KU,
+                    hipStream_t s, const int* stop)
+{
+    return spmv_dispatch<float, float>(L, dof, K, U, KU, s, 1, stop);
+}
+
+int launch_spmv_t(const Lhs& L, int dof, const double* K, const double* U, double* KU,
+                  hipStream_t s, const int* stop)
+{
+    if (L.nNo == 0) return SVFSI_OK;
+    if (!L.symmetric) {
+        set_error("transpose product needs a structurally symmetric pattern");
+        return SVFSI_ERR_ARG;
+    }
+    const int grid = spmv_grid(L.nNo, dof);
+    const int* rp = L.rowPtr.as<int>();
+    const int* cp = L.colPtr.as<int>();
+    const int* tr = L.trans.as<int>();
+    switch (dof) {
+    case 2: hipLaunchKernelGGL(k_bspmv_t<2>, dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, tr, K, U, KU, stop); break;
+    case 3: hipLaunchKernelGGL(k_bspmv_t<3>, dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, tr, K, U, KU, stop); break;
+    case 4: hipLaunchKernelGGL(k_bspmv_t<4>, dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, tr, K, U, KU, stop); break;
+    default: set_error("spmv_t: dof %d not supported (2..4)", dof); return SVFSI_ERR_NOTSUP;
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic dot product: fixed grid, per-block partials, fixed-order sum.
+// ---------------------------------------------------------------------------
+constexpr int kDotBlocks = 512;
+
+__global__ __launch_bounds__(256) void k_dot_partial(size_t n, const double* __restrict__ u,
+                                                     const double* __restrict__ v,
+                                                     double* __restrict__ partial)
+{
+    __shared__ double lds[4];
+    double s = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        s += u[q] * v[q];
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(int m, const double* __restrict__ partial,
+                                                      double* __restrict__ out)
+{
+    __shared__ double lds[4];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < m; q += 256) s += partial[q];
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) *out = s;
+}
+
+int launch_dot(int n, const double* U, const double* V, double* partial, double* result, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(256), 0, s, (size_t)n, U, V, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, kDotBlocks, partial, result);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PRECONDDIAG (svFSILS/PRECOND.f:50-145) and PREMUL/POSMUL (372-489).
+// ---------------------------------------------------------------------------
+__global__ void k_diag_w(int nNo, int dof, const int* __restrict__ diagPtr,
+                         const double* __restrict__ val, const double* __restrict__ mask,
+                         double* __restrict__ W)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nNo * dof) return;
+    const int a = q / dof, i = q - a * dof;
+    double d = val[(size_t)diagPtr[a] * dof * dof + i * dof + i];
+    if (d == 0.0) d = 1.0;                 // PRECOND.f:104-109
+    double w = 1.0 / sqrt(fabs(d));        // PRECOND.f:111
+    W[q] = w * mask[q];                    // Dirichlet faces, PRECOND.f:112-121
+}
+
+// Val(:,k) <- (W_row * Val) * W_col, the PREMUL then POSMUL rounding order;
+// optionally also writes an FP32 copy of the scaled matrix.
+template <int DOF>
+__global__ __launch_bounds__(256) void k_scale_val(size_t nEnt, const int* __restrict__ blockRow,
+                                                   const int* __restrict__ colPtr,
+                                                   const double* __restrict__ W,
+                                                   double* __restrict__ val,
+                                                   float* __restrict__ val32)
+{
+    constexpr int DD = DOF * DOF;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nEnt) return;
+    const size_t k = t / DD;
+    const int r = (int)(t - k * DD);
+    const int i = r / DOF, j = r - (r / DOF) * DOF;
+    const double wr = W[(size_t)blockRow[k] * DOF + i];
+    const double wc = W[(size_t)colPtr[k] * DOF + j];
+    const double v = (val[t] * wr) * wc;
+    val[t] = v;
+    if (val32) val32[t] = (float)v;
+}
+
+__global__ void k_scale_vec(size_t n, const double* __restrict__ W, double* __restrict__ R)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) R[q] = W[q] * R[q];
+}
+
+int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s)
+{
+    if (n == 0) return SVFSI_OK;
+    hipLaunchKernelGGL(k_scale_vec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, W, R);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+// Dense Dirichlet multiplier mask(dof,nNo): product of face%val over the
+// included Dirichlet faces (PRECOND.f:112-121, i = MIN(face%dof, dof)).
+int build_mask(Lhs& L, int dof, const int* incL, hipStream_t s)
+{
+    std::vector<int> inc(L.faces.size(), 1);
+    if (incL)
+        for (size_t f = 0; f < L.faces.size(); ++f) inc[f] = incL[f] != 0;
+    Work& w = L.ws;
+    if (w.maskDof == dof && w.maskVersion == L.facesVersion && w.maskInc == inc) return SVFSI_OK;
+    std::vector<double> m((size_t)L.nNo * dof, 1.0);
+    for (size_t f = 0; f < L.faces.size(); ++f) {
+        const Face& F = L.faces[f];
+        if (!F.foC || !inc[f] || F.bGrp != BC_TYPE_Dir) continue;
+        const int nd = F.dof < dof ? F.dof : dof;
+        for (int a = 0; a < F.nNo; ++a)
+            for (int i = 0; i < nd; ++i) m[(size_t)F.glob[a] * dof + i] *= F.val[(size_t)a * F.dof + i];
+    }
+    SV_TRY(w.mask.ensure(m.size() * sizeof(double)));
+    SV_HIP(hipMemcpyAsync(w.mask.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    SV_HIP(hipStreamSynchronize(s));
+    w.maskDof = dof;
+    w.maskVersion = L.facesVersion;
+    w.maskInc = inc;
+    return SVFSI_OK;
+}
+
+int precond_diag(Lhs& L, int dof, double* Val, double* R, double* W, float* Val32,
+                 const int* incL, hipStream_t s)
+{
+    SV_TRY(build_mask(L, dof, incL, s));
+    const int n = L.nNo * dof;
+    if (n == 0) return SVFSI_OK;
+    hipLaunchKernelGGL(k_diag_w, dim3((n + 255) / 256), dim3(256), 0, s, L.nNo, dof,
+                       L.diagPtr.as<int>(), Val, L.ws.mask.as<double>(), W);
+    const size_t nEnt = (size_t)L.nnz * dof * dof;
+    const unsigned g = (unsigned)((nEnt + 255) / 256);
+    switch (dof) {
+    case 2: hipLaunchKernelGGL(k_scale_val<2>, dim3(g), dim3(256), 0, s, nEnt, L.blockRow.as<int>(), L.colPtr.as<int>(), W, Val, Val32); break;
+    case 3: hipLaunchKernelGGL(k_scale_val<3>, dim3(g), dim3(256), 0, s, nEnt, L.blockRow.as<int>(), L.colPtr.as<int>(), W, Val, Val32); break;
+    case 4: hipLaunchKernelGGL(k_scale_val<4>, dim3(g), dim3(256), 0, s, nEnt, L.blockRow.as<int>(), L.colPtr.as<int>(), W, Val, Val32); break;
+    default: set_error("precond: dof %d not supported (2..4)", dof); return SVFSI_ERR_NOTSUP;
+    }
+    SV_TRY(launch_scale_vec((size_t)n, W, R, s));
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+} // namespace svfsi

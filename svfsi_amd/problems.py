@@ -1,0 +1,125 @@
+"""Inputs of the benchmark configurations: mesh, equation parameters, state,
+boundary conditions and solver settings for one Newton step.
+
+Each problem is plain data (numpy arrays, 0-based connectivity); the HIP
+path (svfsi_amd.assembly / svfsi_amd.fsils) and the oracle consume the same
+arrays.  Generalized-alpha constants follow svFSI/INITIALIZE.f:76-139.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import mesh as meshmod
+from ._lib import LS_TYPE_CG, LS_TYPE_GMRES, SVFSI_PHYS_FLUID, SVFSI_PHYS_LELAS, SVFSI_PHYS_STRUCT
+
+
+def gen_alpha(rho_inf: float, second_order: bool) -> dict:
+    """am, af, beta, gam of INITIALIZE.f:76-139 (first-order default am,
+    second-order am for lElas/struct)."""
+    if second_order:
+        am = (2.0 - rho_inf) / (1.0 + rho_inf)
+    else:
+        am = 0.5 * (3.0 - rho_inf) / (1.0 + rho_inf)
+    af = 1.0 / (1.0 + rho_inf)
+    beta = 0.25 * (1.0 + am - af) ** 2
+    gam = 0.5 + am - af
+    return dict(am=am, af=af, beta=beta, gam=gam)
+
+
+@dataclass
+class Problem:
+    name: str
+    phys: int
+    dof: int
+    mesh: meshmod.TetMesh
+    eq: dict
+    Ag: np.ndarray
+    Yg: np.ndarray
+    Dg: np.ndarray
+    dir_faces: list            # [(nodes 0-based, face dof)] homogeneous Dirichlet masks
+    trac_faces: list = field(default_factory=list)   # [(fIEN, fEl, h(3))]
+    bflow_faces: list = field(default_factory=list)  # [(fIEN, fEl)]
+    ls: dict = field(default_factory=dict)
+    prop_el: np.ndarray | None = None
+    quad: int = 0
+
+    @property
+    def nNo(self) -> int:
+        return self.mesh.nNo
+
+    def dir_mask(self) -> np.ndarray:
+        """Dense product of the Dirichlet face values (PRECOND.f:112-121)."""
+        m = np.ones((self.nNo, self.dof))
+        for nodes, fdof in self.dir_faces:
+            m[nodes, : min(fdof, self.dof)] = 0.0
+        return m
+
+
+def _faces_where(m: meshmod.TetMesh, pred_nodes: np.ndarray):
+    f, e = meshmod.boundary_faces(m.ien)
+    sel = pred_nodes[f].all(axis=1)
+    return f[sel], e[sel]
+
+
+def lelas_cube(n: int = 20, E: float = 1e6, nu: float = 0.3, rho: float = 1.0, dt: float = 1.0,
+               traction: float = 1e3, rho_inf: float = 0.2) -> Problem:
+    """Linear elasticity on the Kuhn cube: x = 0 clamped (all dofs), uniform
+    traction (traction, 0, 0) on x = 1, zero state (the RHS is the traction
+    load alone), CG with diagonal scaling to relative residual 1e-8."""
+    m = meshmod.cube_tet4(n)
+    eq = dict(gen_alpha(rho_inf, True), dt=dt, rho=rho, E=E, nu=nu)
+    z = np.zeros((m.nNo, 3))
+    x0 = np.where(m.x[:, 0] == 0.0)[0].astype(np.int32)
+    ff, fe = _faces_where(m, m.x[:, 0] == 1.0)
+    return Problem(
+        name=f"lelas_cube{n}", phys=SVFSI_PHYS_LELAS, dof=3, mesh=m, eq=eq, Ag=z, Yg=z.copy(), Dg=z.copy(),
+        dir_faces=[(x0, 3)], trac_faces=[(ff, fe, np.array([traction, 0.0, 0.0]))],
+        ls=dict(LS_type=LS_TYPE_CG, relTol=1e-8, absTol=1e-30, mItr=20000, sD=0))
+
+
+def fluid_cylinder(h: float = 0.11, seed: int = 0, rho: float = 1.06, mu: float = 0.04,
+                   u_peak: float = 30.0, dt: float = 1e-3, rho_inf: float = 0.2, sD: int = 50,
+                   bstab: float = 0.2) -> Problem:
+    """VMS Navier-Stokes (FLUID3D_M/C) in a straight cylinder r = 1, L = 10:
+    parabolic inlet u_z = u_peak (1 - r^2) at z = 0, no-slip wall, zero
+    traction outlet with backflow stabilisation.  State: Ag = 0, Yg velocity
+    drawn from U(-1, 1) (seeded) with the Dirichlet values imposed, p = 0.
+    GMRES(sD) with diagonal scaling to relative residual 1e-8."""
+    m = meshmod.cylinder_tet4(h, seed=seed)
+    eq = dict(gen_alpha(rho_inf, False), dt=dt, rho=rho, mu=mu, bstab=bstab)
+    rng = np.random.default_rng(seed + 1)
+    Y = np.zeros((m.nNo, 4))
+    Y[:, :3] = rng.uniform(-1.0, 1.0, (m.nNo, 3))
+    inlet, wall, outlet = m.node_tags["inlet"], m.node_tags["wall"], m.node_tags["outlet"]
+    r2 = m.x[:, 0] ** 2 + m.x[:, 1] ** 2
+    Y[inlet, :3] = 0.0
+    Y[inlet, 2] = u_peak * (1.0 - r2[inlet])
+    Y[wall, :3] = 0.0
+    A = np.zeros((m.nNo, 4))
+    ff, fe = _faces_where(m, outlet)
+    return Problem(
+        name=f"fluid_cyl_h{h}", phys=SVFSI_PHYS_FLUID, dof=4, mesh=m, eq=eq, Ag=A, Yg=Y, Dg=np.zeros_like(Y),
+        dir_faces=[(np.where(inlet)[0].astype(np.int32), 3), (np.where(wall)[0].astype(np.int32), 3)],
+        bflow_faces=[(ff, fe)],
+        ls=dict(LS_type=LS_TYPE_GMRES, relTol=1e-8, absTol=1e-30, mItr=200, sD=sD))
+
+
+def struct_cube(n: int = 4, C10: float = 1e5, Kpen: float = 1e7, rho: float = 1.0, dt: float = 1e-3,
+                amp: float = 1e-3, seed: int = 0, rho_inf: float = 0.2, quad: int = 1,
+                volType: int = 2, sD: int = 100) -> Problem:
+    """Neo-Hookean hyperelasticity (STRUCT3D + GETPK2CC nHK) on the tet10
+    Kuhn cube: x = 0 clamped, random nodal displacement U(-amp, amp)
+    (seeded) as Dg, Ag = Yg = 0, GMRES to relative residual 1e-8."""
+    m = meshmod.cube_tet10(n)
+    eq = dict(gen_alpha(rho_inf, True), dt=dt, rho=rho, C10=C10, Kpen=Kpen, volType=volType)
+    rng = np.random.default_rng(seed)
+    D = rng.uniform(-amp, amp, (m.nNo, 3))
+    x0 = np.where(m.x[:, 0] == 0.0)[0].astype(np.int32)
+    D[x0] = 0.0
+    z = np.zeros((m.nNo, 3))
+    return Problem(
+        name=f"struct_cube{n}", phys=SVFSI_PHYS_STRUCT, dof=3, mesh=m, eq=eq, Ag=z, Yg=z.copy(), Dg=D,
+        dir_faces=[(x0, 3)], quad=quad,
+        ls=dict(LS_type=LS_TYPE_GMRES, relTol=1e-8, absTol=1e-30, mItr=100, sD=sD))

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Integer/index work is compared exactly, FP64 kernels to
+1e-12 relative (global, max-norm), FP32 kernels to 1e-5 relative."""
+import sys
+
+import numpy as np
+import pytest
+
+from svfsi_amd import fsils, mesh, problems
+from svfsi_amd.assembly import DeviceMesh
+from svfsi_amd.step import NewtonStep
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def T(a, gpu, dtype=None):
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(gpu)
+
+
+def relerr(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def cube(orc):
+    m = mesh.cube_tet4(5)
+    rp, cp = fsils.lhsa(m.nNo, m.ien)
+    return m, rp, cp
+
+
+@pytest.mark.parametrize("dof", [3, 4])
+def test_spmv_fp64_fp32_transpose(gpu, orc, cube, dof):
+    import torch
+
+    m, rp, cp = cube
+    say("spmv", dof, m.nNo, len(cp))
+    lhs = fsils.fsils_lhs_create(m.nNo, len(cp), rp, cp)
+    rng = np.random.default_rng(dof)
+    Val = rng.normal(size=(len(cp), dof * dof))
+    U = rng.normal(size=(m.nNo, dof))
+    ref = orc.spmv(rp - 1, cp - 1, dof, Val, U)
+    out = torch.zeros((m.nNo, dof), dtype=torch.float64, device=gpu)
+    fsils.call("svfsi_spmv_dev", lhs.handle, dof, fsils.dev_ptr(T(Val, gpu)), fsils.dev_ptr(T(U, gpu)),
+               fsils.dev_ptr(out), fsils.cur_stream())
+    torch.cuda.synchronize()
+    assert relerr(out.cpu().numpy(), ref) < 1e-13
+    # transpose product
+    reft = orc.spmv_t(rp - 1, cp - 1, dof, Val, U)
+    fsils.call("svfsi_spmv_t_dev", lhs.handle, dof, fsils.dev_ptr(T(Val, gpu)), fsils.dev_ptr(T(U, gpu)),
+               fsils.dev_ptr(out), fsils.cur_stream())
+    torch.cuda.synchronize()
+    assert relerr(out.cpu().numpy(), reft) < 1e-13
+    # FP32 values/vectors: compare with the FP64 product of the rounded inputs
+    V32 = Val.astype(np.float32)
+    U32 = U.astype(np.float32)
+    ref32 = orc.spmv(rp - 1, cp - 1, dof, V32.astype(float), U32.astype(float))
+    o32 = torch.zeros((m.nNo, dof), dtype=torch.float32, device=gpu)
+    fsils.call("svfsi_spmv_f32_dev", lhs.handle, dof, fsils.dev_ptr(T(V32, gpu)), fsils.dev_ptr(T(U32, gpu)),
+               fsils.dev_ptr(o32), fsils.cur_stream())
+    torch.cuda.synchronize()
+    assert relerr(o32.cpu().numpy(), ref32) < 1e-5
+    lhs.free()
+
+
+def test_dot(gpu):
+    import torch
+
+    from svfsi_amd._lib import lib
+    import ctypes as C
+
+    rng = np.random.default_rng(0)
+    for n in (1, 255, 1000, 123457):
+        a, b = rng.normal(size=n), rng.normal(size=n)
+        out = C.c_double()
+        fsils.call("svfsi_dot_dev", n, fsils.dev_ptr(T(a, gpu)), fsils.dev_ptr(T(b, gpu)), C.byref(out),
+                   fsils.cur_stream())
+        assert abs(out.value - a @ b) < 1e-12 * np.abs(a * b).sum()
+
+
+def test_precond(gpu, orc, cube):
+    import torch
+
+    m, rp, cp = cube
+    dof = 3
+    lhs = fsils.fsils_lhs_create(m.nNo, len(cp), rp, cp, nFaces=1)
+    fixed = np.where(m.x[:, 0] == 0.0)[0]
+    fsils.fsils_bc_create(lhs, 1, len(fixed), 3, fsils.BC_TYPE_Dir, fixed + 1)
+    rng = np.random.default_rng(1)
+    Val = rng.normal(size=(len(cp), 9))
+    R = rng.normal(size=(m.nNo, 3))
+    mask = np.ones((m.nNo, 3))
+    mask[fixed] = 0
+    V2, R2, W2 = orc.precond_diag(rp - 1, cp - 1, dof, Val, R, mask)
+    dV, dR = T(Val, gpu), T(R, gpu)
+    dW = torch.zeros((m.nNo, 3), dtype=torch.float64, device=gpu)
+    fsils.call("svfsi_precond_diag_dev", lhs.handle, dof, fsils.dev_ptr(dV), fsils.dev_ptr(dR), fsils.dev_ptr(dW),
+               None, fsils.cur_stream())
+    assert np.array_equal(dW.cpu().numpy(), W2)
+    assert np.array_equal(dR.cpu().numpy(), R2)
+    assert relerr(dV.cpu().numpy(), V2) < 1e-15
+    lhs.free()
+
+
+def _assembled(prob, orc):
+    st = NewtonStep(prob)
+    st.assemble()
+    import torch
+
+    torch.cuda.synchronize()
+    Vg, Rg = st.Val.cpu().numpy(), st.R.cpu().numpy()
+    Vo, Ro = orc.assemble(prob, st.rowPtr - 1, st.colPtr - 1)
+    return st, Vg, Rg, Vo, Ro
+
+
+def test_lelas_assembly_parity_and_determinism(gpu, orc):
+    prob = problems.lelas_cube(6)
+    rng = np.random.default_rng(3)
+    prob.Dg = rng.normal(size=prob.Dg.shape) * 1e-3
+    prob.Ag = rng.normal(size=prob.Ag.shape)
+    say("lelas assembly", prob.nNo)
+    st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
+    assert relerr(Vg, Vo) < 1e-12
+    assert relerr(Rg, Ro) < 1e-12
+    st.assemble()
+    import torch
+
+    torch.cuda.synchronize()
+    assert np.array_equal(st.Val.cpu().numpy(), Vg)  # bitwise reproducible
+    assert np.array_equal(st.R.cpu().numpy(), Rg)
+
+
+def test_lelas_per_element_modulus(gpu, orc):
+    prob = problems.lelas_cube(4)
+    prob.prop_el = 1e6 * np.exp(np.random.default_rng(2).normal(size=prob.mesh.nEl) * 0.3)
+    st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
+    assert relerr(Vg, Vo) < 1e-12
+
+
+def test_fluid_assembly_parity(gpu, orc):
+    prob = problems.fluid_cylinder(h=0.4)
+    rng = np.random.default_rng(4)
+    prob.Ag[:, :3] = rng.normal(size=(prob.nNo, 3))
+    prob.Yg[:, 3] = rng.normal(size=prob.nNo)
+    say("fluid assembly", prob.nNo, prob.mesh.nEl)
+    st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
+    assert relerr(Vg, Vo) < 1e-12
+    assert relerr(Rg, Ro) < 1e-12
+
+
+def _solve_pair(prob, orc, mixed=False):
+    st = NewtonStep(prob)
+    st.assemble()
+    import torch
+
+    torch.cuda.synchronize()
+    V0, R0 = st.Val.cpu().numpy().copy(), st.R.cpu().numpy().copy()
+    ls = st.solve(mixed=mixed)
+    torch.cuda.synchronize()
+    xg = st.R.cpu().numpy()
+    xo, Vs, so = orc.fsils_solve(st.rowPtr - 1, st.colPtr - 1, prob.dof, prob.dir_mask(), prob.ls, R0, V0)
+    return st, ls, xg, xo, so, V0, R0, Vs
+
+
+def test_cg_solve_parity(gpu, orc):
+    prob = problems.lelas_cube(6)
+    say("cg solve", prob.nNo)
+    st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc)
+    assert ls.RI.suc == 1 and so.suc == 1
+    assert abs(ls.RI.itr - so.itr) <= 2, (ls.RI.itr, so.itr)
+    assert relerr(xg, xo) < 1e-7
+    assert relerr(st.Val.cpu().numpy(), Vs) < 1e-13  # Val scaled in place like the reference
+    assert abs(ls.RI.iNorm - so.iNorm) <= 1e-12 * so.iNorm
+
+
+def test_gmres_solve_parity(gpu, orc):
+    prob = problems.fluid_cylinder(h=0.4)
+    prob.ls["sD"] = 20
+    say("gmres solve", prob.nNo)
+    st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc)
+    assert ls.RI.suc == 1 and so.suc == 1
+    assert abs(ls.RI.itr - so.itr) <= 2, (ls.RI.itr, so.itr)
+    assert relerr(xg, xo) < 1e-6
+    assert abs(ls.RI.iNorm - so.iNorm) <= 1e-12 * so.iNorm
+
+
+def test_host_entry_point(gpu, orc):
+    prob = problems.lelas_cube(4)
+    Vo, Ro = orc.assemble(prob, *[a for a in orc.lhsa(prob.nNo, prob.mesh.ien)])
+    rp, cp = fsils.lhsa(prob.nNo, prob.mesh.ien)
+    lhs = fsils.fsils_lhs_create(prob.nNo, len(cp), rp, cp, nFaces=1)
+    nodes, fdof = prob.dir_faces[0]
+    fsils.fsils_bc_create(lhs, 1, len(nodes), fdof, fsils.BC_TYPE_Dir, nodes + 1)
+    ls = fsils.fsils_ls_create(fsils.LS_TYPE_CG, relTol=1e-8, absTol=1e-30, maxItr=5000)
+    Ri, Val = Ro.copy(), Vo.copy()
+    fsils.fsils_solve(lhs, ls, 3, Ri, Val)
+    xo, Vs, so = orc.fsils_solve(rp - 1, cp - 1, 3, prob.dir_mask(), dict(LS_type=798, relTol=1e-8, absTol=1e-30, mItr=5000), Ro, Vo)
+    assert ls.RI.suc == 1
+    assert relerr(Ri, xo) < 1e-7
+    assert relerr(Val, Vs) < 1e-13
+
+
+def test_mixed_reaches_fp64_tolerance(gpu, orc):
+    prob = problems.fluid_cylinder(h=0.4)
+    say("mixed solve", prob.nNo)
+    st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc, mixed=True)
+    assert ls.RI.suc == 1
+    # FP64 residual of the scaled system, recomputed on the CPU
+    mask = prob.dir_mask()
+    _, Rs, W = orc.precond_diag(st.rowPtr - 1, st.colPtr - 1, prob.dof, V0, R0, mask)
+    y = np.where(W != 0, xg / np.where(W != 0, W, 1), 0.0)
+    r = Rs - orc.spmv(st.rowPtr - 1, st.colPtr - 1, prob.dof, Vs, y)
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(Rs) * 1.0001
+    assert relerr(xg, xo) < 1e-6
+
+
+def test_lelas_dkde(gpu, orc):
+    import torch
+
+    prob = problems.lelas_cube(4)
+    st = NewtonStep(prob)
+    rng = np.random.default_rng(9)
+    lam = rng.normal(size=(prob.nNo, 3))
+    u = rng.normal(size=(prob.nNo, 3))
+    g = torch.zeros(prob.mesh.nEl, dtype=torch.float64, device=gpu)
+    st.dmesh.lelas_dkde_dot(prob.eq, T(lam, gpu), T(u, gpu), g)
+    torch.cuda.synchronize()
+    go = orc.lelas_dkde_dot(prob, lam, u)
+    assert relerr(g.cpu().numpy(), go) < 1e-12
