@@ -117,6 +117,12 @@ def lib() -> C.CDLL:
             raise SvfsiError(
                 f"{LIB_PATH} is missing: build it with `python -m svfsi_amd.build` "
                 "(the HIP library is required; there is no CPU fallback)")
+        # torch ships its own libamdhip64.so.7.  Loading it first makes the
+        # library's NEEDED libamdhip64.so.7 resolve to the same runtime, so
+        # torch tensors, streams and this library share one HIP runtime
+        # (two runtimes in one process cannot share device pointers).
+        import torch  # noqa: F401
+
         L = C.CDLL(str(LIB_PATH))
         for name, args in SIGNATURES.items():
             f = getattr(L, name)
