@@ -136,13 +136,14 @@ int fsils_gpu_solve_dev(void* lhs, FSILS_lsType* ls, int dof, double* dRi,
 
 /* Mixed-precision form of fsils_gpu_solve_dev (no reference counterpart;
  * the reference is FP64-only, svFSILS/FSILS_TYPEDEF.h:55-57).  Same
- * arguments, convergence test and outputs.  The scaled matrix is copied to
- * FP32 and used for the inner Krylov iterations (FP32 SpMV, FP32 basis
- * storage, FP64 dot accumulation, FP64 Hessenberg/Givens); an FP64 outer
- * loop (flexible GMRES / iterative refinement) recomputes the residual
- * W(R - K x) with the FP64 matrix, so the reported residual is the FP64
- * one in the reference's scaled norm.  ls->RI.itr counts FP64 outer
- * residual evaluations; ls->RI.cM counts inner FP32 SpMVs. */
+ * arguments, scaling, convergence test and outputs.  Flexible GMRES in
+ * FP64 (outer residual, Arnoldi, Hessenberg/Givens, solution) whose right
+ * preconditioner is an inner restarted GMRES(ls->RI.sD) on an FP32 copy of
+ * the scaled matrix (FP32 SpMV, basis and directions; FP64 dot
+ * accumulation and least squares), stopped at a loose relative tolerance.
+ * The reported residual is the FP64 one in the reference's scaled norm.
+ * ls->RI.itr counts outer (FP64) Arnoldi steps and restarts;
+ * ls->RI.cM counts all SpMVs (inner FP32 plus outer FP64). */
 int fsils_gpu_solve_mixed_dev(void* lhs, FSILS_lsType* ls, int dof,
                               double* dRi, double* dVal, int prec,
                               const int* incL, const double* res, void* stream);

@@ -74,6 +74,7 @@ struct Face {
 struct Work {
     DBuf W, X, R0, tmp, basis, basis32, Z32, val32, x32, y32;
     DBuf partial, state, hess;
+    DBuf zdir, opartial, ostate, ohess; // flexible GMRES: FP32 directions, outer state
     DBuf mask;
     int maskDof = -1;
     std::vector<int> maskInc;

@@ -14,6 +14,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace svfsi {
@@ -351,78 +352,155 @@ __global__ __launch_bounds__(256) void k_gm_div(size_t n, BT* __restrict__ u0, G
         u0[q] = (BT)((double)u0[q] / e0);
 }
 
-constexpr int kMdotPerThread = 8;
-constexpr int kMaxSD = 1000;
+constexpr int kMaxSD = 256;
 
-// h(j,i) = <u_j, u_{i+1}> for j = 1..i+1 (GMRES.f:337-340), then in the last
-// block: Pythagorean h(i+1,i) (342-347), previous rotations, new Givens
-// rotation, residual estimate and convergence test (351-366).
-// i is the 1-based Arnoldi step; u_j is basis + (j-1)*n.
+// Vector-load helpers: E consecutive elements as one 16-byte load.
+template <typename BT> struct VecE;
+template <> struct VecE<double> {
+    static constexpr int E = 2;
+    typedef double v __attribute__((ext_vector_type(2)));
+};
+template <> struct VecE<float> {
+    static constexpr int E = 4;
+    typedef float v __attribute__((ext_vector_type(4)));
+};
+
 template <typename BT>
-__global__ __launch_bounds__(256) void k_gm_mdot(size_t n, int i, int sD, const BT* __restrict__ basis,
+__device__ __forceinline__ void loadE(const BT* __restrict__ p, size_t q, size_t q1, double* o)
+{
+    constexpr int E = VecE<BT>::E;
+    if (q + E <= q1 && ((q & (E - 1)) == 0)) {
+        const typename VecE<BT>::v t = *reinterpret_cast<const typename VecE<BT>::v*>(p + q);
+#pragma unroll
+        for (int e = 0; e < E; ++e) o[e] = (double)t[e];
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) o[e] = (q + e < q1) ? (double)p[q + e] : 0.0;
+    }
+}
+
+// h(j,i) = <u_j, u_{i+1}> for j = 1..i+1 (GMRES.f:337-340), per-block
+// partial sums.  i is the 1-based Arnoldi step; u_j is basis + (j-1)*n.
+// Grid: (row chunks) x (groups of 8 coefficients).  A block keeps its chunk
+// of the new vector in registers (E elements per lane, 16-byte loads) and
+// forms eight dot products at once, so their loads and lane-shuffle
+// reductions interleave; the four wavefront sums meet in LDS.  There is no
+// intra-kernel cross-block synchronisation (on a multi-XCD part an
+// agent-scope fence writes back the XCD's L2): the partials are summed by
+// k_gm_givens.
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_mdot(size_t n, int i, const BT* __restrict__ basis,
                                                  double* __restrict__ partial,
-                                                 GmState* __restrict__ st, GmArrays A)
+                                                 const GmState* __restrict__ st)
 {
     if (st->stop) return;
-    __shared__ double lds[4];
-    __shared__ int flag;
-    const int nb = gridDim.x;
-    const size_t chunk = (n + nb - 1) / nb;
+    constexpr int E = VecE<BT>::E;
+    __shared__ double red[4][8];
+    const int nbr = gridDim.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t chunk = (size_t)256 * E;
     const size_t q0 = (size_t)blockIdx.x * chunk;
     const size_t q1 = (q0 + chunk < n) ? q0 + chunk : n;
-    const BT* w = basis + (size_t)i * n;
-    double wv[kMdotPerThread];
+    const size_t q = q0 + (size_t)threadIdx.x * E;
+    const int j0 = blockIdx.y * 8;
+    double wv[E];
+    loadE(basis + (size_t)i * n, q, q1, wv);
+    double s[8];
 #pragma unroll
-    for (int m = 0; m < kMdotPerThread; ++m) {
-        const size_t q = q0 + threadIdx.x + (size_t)m * 256;
-        wv[m] = (q < q1) ? (double)w[q] : 0.0;
-    }
-    for (int jj = 0; jj <= i; ++jj) {
-        const BT* u = basis + (size_t)jj * n;
-        double s = 0.0;
+    for (int b = 0; b < 8; ++b) {
+        s[b] = 0.0;
+        const int jj = j0 + b;
+        if (jj <= i) {
+            double uv[E];
+            loadE(basis + (size_t)jj * n, q, q1, uv);
 #pragma unroll
-        for (int m = 0; m < kMdotPerThread; ++m) {
-            const size_t q = q0 + threadIdx.x + (size_t)m * 256;
-            if (q < q1) s += (double)u[q] * wv[m];
+            for (int e = 0; e < E; ++e) s[b] += uv[e] * wv[e];
         }
-        s = block_sum<256>(s, lds);
-        if (threadIdx.x == 0) partial[(size_t)jj * nb + blockIdx.x] = s;
     }
-    if (!last_block_done(&st->counter, &flag)) return;
-    // reduce partials: 4 lanes per coefficient, fixed order
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) s[b] += __shfl_xor(s[b], o, kWave);
+    if (lane == 0)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) red[wave][b] = s[b];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int jj = j0 + threadIdx.x;
+        if (jj <= i)
+            partial[(size_t)jj * nbr + blockIdx.x] =
+                ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+    }
+}
+
+// One block: sums the per-block partials in block order (deterministic),
+// then the Pythagorean h(i+1,i) (GMRES.f:342-347), the previous rotations,
+// the new Givens rotation, the residual estimate and the convergence test
+// (351-366), all in LDS.
+__global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, const double* __restrict__ partial,
+                                                   GmState* __restrict__ st, GmArrays A)
+{
+    if (st->stop) return;
     __shared__ double hsh[kMaxSD + 2];
-    for (int jj = threadIdx.x >> 2; jj <= i; jj += 64) {
-        double t = 0.0;
-        for (int b = threadIdx.x & 3; b < nb; b += 4) t += load_agent(partial + (size_t)jj * nb + b);
-        t += __shfl_xor(t, 1, kWave);
-        t += __shfl_xor(t, 2, kWave);
-        if ((threadIdx.x & 3) == 0) hsh[jj] = t;
+    __shared__ double cs[2][kMaxSD];
+    // T lanes per coefficient (power of two <= 64), fixed-order sums
+    int npow = 1;
+    while (npow < i + 1) npow <<= 1;
+    int T = 256 / npow;
+    if (T > 64) T = 64;
+    if (T < 1) T = 1;
+    const int per = 256 / T;
+    for (int jj0 = 0; jj0 <= i; jj0 += per) {
+        const int jj = jj0 + threadIdx.x / T;
+        const int b0 = threadIdx.x % T;
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        if (jj <= i) {
+            const double* pp = partial + (size_t)jj * nb;
+            int b = b0;
+            for (; b + 3 * T < nb; b += 4 * T) {
+                t0 += pp[b];
+                t1 += pp[b + T];
+                t2 += pp[b + 2 * T];
+                t3 += pp[b + 3 * T];
+            }
+            for (; b < nb; b += T) t0 += pp[b];
+        }
+        double t = (t0 + t1) + (t2 + t3);
+        for (int o = T >> 1; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+        if (jj <= i && b0 == 0) hsh[jj] = t;
+    }
+    for (int jj = threadIdx.x; jj < i - 1; jj += 256) {
+        cs[0][jj] = A.c[jj];
+        cs[1][jj] = A.sn[jj];
     }
     __syncthreads();
+    // CGS coefficients for k_gm_update (pre-rotation values)
+    for (int jj = threadIdx.x; jj <= i; jj += 256) A.hraw[jj] = hsh[jj];
     if (threadIdx.x != 0) return;
-    double* Hc = A.H + (size_t)(i - 1) * (sD + 1); // column i: Hc[j-1] = h(j,i)
-    for (int jj = 0; jj <= i; ++jj) {
-        Hc[jj] = hsh[jj];
-        A.hraw[jj] = hsh[jj];
-    }
-    double hii = Hc[i];
-    for (int jj = 0; jj < i; ++jj) hii = hii - Hc[jj] * Hc[jj];
+    double hii = hsh[i];
+    for (int jj = 0; jj < i; ++jj) hii = hii - hsh[jj] * hsh[jj];
     hii = sqrt(fabs(hii));
-    Hc[i] = hii;
+    hsh[i] = hii;
     A.hraw[sD + 2] = 1.0 / hii;  // OMPMULV(1/h(i+1,i)) factor for k_gm_update
     for (int jj = 1; jj <= i - 1; ++jj) {
-        const double tmp = A.c[jj - 1] * Hc[jj - 1] + A.sn[jj - 1] * Hc[jj];
-        Hc[jj] = -A.sn[jj - 1] * Hc[jj - 1] + A.c[jj - 1] * Hc[jj];
-        Hc[jj - 1] = tmp;
+        const double tmp = cs[0][jj - 1] * hsh[jj - 1] + cs[1][jj - 1] * hsh[jj];
+        hsh[jj] = -cs[1][jj - 1] * hsh[jj - 1] + cs[0][jj - 1] * hsh[jj];
+        hsh[jj - 1] = tmp;
     }
-    const double tmp = sqrt(Hc[i - 1] * Hc[i - 1] + Hc[i] * Hc[i]);
-    A.c[i - 1] = Hc[i - 1] / tmp;
-    A.sn[i - 1] = Hc[i] / tmp;
-    Hc[i - 1] = tmp;
-    Hc[i] = 0.0;
-    A.err[i] = -A.sn[i - 1] * A.err[i - 1];
-    A.err[i - 1] = A.c[i - 1] * A.err[i - 1];
-    if (fabs(A.err[i]) < st->eps) {
+    const double tmp = sqrt(hsh[i - 1] * hsh[i - 1] + hsh[i] * hsh[i]);
+    const double ci = hsh[i - 1] / tmp;
+    const double si = hsh[i] / tmp;
+    hsh[i - 1] = tmp;
+    hsh[i] = 0.0;
+    A.c[i - 1] = ci;
+    A.sn[i - 1] = si;
+    double* Hc = A.H + (size_t)(i - 1) * (sD + 1); // column i: Hc[j-1] = h(j,i)
+    for (int jj = 0; jj <= i; ++jj) Hc[jj] = hsh[jj];
+    const double ei = A.err[i - 1];
+    const double eNew = -si * ei;
+    A.err[i] = eNew;
+    A.err[i - 1] = ci * ei;
+    if (fabs(eNew) < st->eps) {
         st->suc = 1;
         st->stop = 1;
         st->iconv = i;
@@ -433,34 +511,105 @@ __global__ __launch_bounds__(256) void k_gm_mdot(size_t n, int i, int sD, const 
 }
 
 // u_{i+1} = (u_{i+1} - sum_j h(j,i) u_j) * (1/h(i+1,i))   (GMRES.f:342-349)
-// The pre-rotation coefficients were saved in hraw by k_gm_mdot.
+// The pre-rotation coefficients were saved in hraw by k_gm_givens.  Each
+// lane updates E consecutive elements with 16-byte loads; the sweep over j
+// keeps the reference's order per element.
 template <typename BT>
 __global__ __launch_bounds__(256) void k_gm_update(size_t n, int i, int sD, BT* __restrict__ basis,
                                                    const GmState* __restrict__ st, GmArrays A)
 {
     if (st->stop) return;
-    BT* w = basis + (size_t)i * n;
+    constexpr int E = VecE<BT>::E;
+    __shared__ double hc[kMaxSD + 1];
+    for (int jj = threadIdx.x; jj < i; jj += 256) hc[jj] = -A.hraw[jj];
+    __syncthreads();
     const double inv = A.hraw[sD + 2];
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
-        double v = (double)w[q];
-        for (int jj = 0; jj < i; ++jj) v = v + (-A.hraw[jj]) * (double)basis[(size_t)jj * n + q];
-        w[q] = (BT)(v * inv);
+    BT* w = basis + (size_t)i * n;
+    for (size_t q = ((size_t)blockIdx.x * 256 + threadIdx.x) * E; q < n; q += (size_t)gridDim.x * 256 * E) {
+        double v[E];
+        loadE(w, q, n, v);
+        int jj = 0;
+        for (; jj + 4 <= i; jj += 4) {   // four loads in flight, same order
+            double u0[E], u1[E], u2[E], u3[E];
+            loadE(basis + (size_t)jj * n, q, n, u0);
+            loadE(basis + (size_t)(jj + 1) * n, q, n, u1);
+            loadE(basis + (size_t)(jj + 2) * n, q, n, u2);
+            loadE(basis + (size_t)(jj + 3) * n, q, n, u3);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                v[e] = v[e] + hc[jj] * u0[e];
+                v[e] = v[e] + hc[jj + 1] * u1[e];
+                v[e] = v[e] + hc[jj + 2] * u2[e];
+                v[e] = v[e] + hc[jj + 3] * u3[e];
+            }
+        }
+        for (; jj < i; ++jj) {
+            double u[E];
+            loadE(basis + (size_t)jj * n, q, n, u);
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = v[e] + hc[jj] * u[e];
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (q + e < n) w[q + e] = (BT)(v[e] * inv);
     }
 }
 
-// Back substitution of the cycle (GMRES.f:369-376), one thread.
-__global__ void k_gm_backsolve(int sD, GmState* __restrict__ st, GmArrays A)
+// Back substitution of the cycle (GMRES.f:369-376).  The block stages the
+// upper triangle of H in LDS (packed by columns) when it fits, then one
+// thread runs the reference's row-oriented substitution.
+constexpr int kBackLds = 6144; // doubles of LDS for the packed triangle (48 KiB, sD <= 110)
+
+__global__ __launch_bounds__(64) void k_gm_backsolve(int sD, GmState* __restrict__ st, GmArrays A)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // One wavefront.  Lane l owns rows l, l+64, ...; columns are eliminated
+    // from the last one: y_j = acc_j / h(j,j), then every owner of a row
+    // r < j subtracts h(r,j) y_j (the column-oriented form of GMRES.f:370-376;
+    // each row accumulates the same products, in descending column order).
+    extern __shared__ double Hs[];
     int i = st->iconv;
     if (i > sD) i = sD;
-    for (int jj = 0; jj < i; ++jj) A.y[jj] = A.err[jj];
-    for (int jj = i; jj >= 1; --jj) {
-        double yj = A.y[jj - 1];
-        for (int k = jj + 1; k <= i; ++k) yj = yj - A.H[(size_t)(k - 1) * (sD + 1) + (jj - 1)] * A.y[k - 1];
-        A.y[jj - 1] = yj / A.H[(size_t)(jj - 1) * (sD + 1) + (jj - 1)];
+    const bool lds = (size_t)sD * (sD + 1) / 2 <= (size_t)kBackLds;
+    const int lane = threadIdx.x;
+    if (lds) {
+        for (int k = 0; k < i; ++k)
+            for (int r = lane; r <= k; r += 64) Hs[k * (k + 1) / 2 + r] = A.H[(size_t)k * (sD + 1) + r];
     }
-    st->fNorm = fabs(A.err[i]);
+    __syncthreads();
+    auto H = [&](int r, int k) -> double {  // 0-based h(r+1,k+1), r <= k
+        return lds ? Hs[k * (k + 1) / 2 + r] : A.H[(size_t)k * (sD + 1) + r];
+    };
+    constexpr int kRows = (kMaxSD + 63) / 64;
+    double acc[kRows];
+#pragma unroll
+    for (int m = 0; m < kRows; ++m) {
+        const int r = lane + 64 * m;
+        acc[m] = (r < i) ? A.err[r] : 0.0;
+    }
+    for (int j = i - 1; j >= 0; --j) {
+        const int owner = j & 63, mj = j >> 6;
+        double yj = 0.0;
+#pragma unroll
+        for (int m = 0; m < kRows; ++m)
+            if (m == mj) yj = acc[m] / H(j, j);
+        yj = __shfl(yj, owner, kWave);
+        if (lane == owner) {
+#pragma unroll
+            for (int m = 0; m < kRows; ++m)
+                if (m == mj) acc[m] = yj;
+        }
+#pragma unroll
+        for (int m = 0; m < kRows; ++m) {
+            const int r = lane + 64 * m;
+            if (r < j) acc[m] = acc[m] - H(r, j) * yj;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kRows; ++m) {
+        const int r = lane + 64 * m;
+        if (r < i) A.y[r] = acc[m];
+    }
+    if (lane == 0) st->fNorm = fabs(A.err[i]);
 }
 
 // X = X + y_j u_j, j = 1..i (GMRES.f:378-381)
@@ -533,15 +682,12 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
         return SVFSI_ERR_NOTSUP;
     }
     Work& w = L.ws;
-    size_t nb = (n + 255) / 256;
-    if (nb > 1024) nb = 1024;
-    const size_t nbMin = (n + 256 * kMdotPerThread - 1) / (256 * kMdotPerThread);
-    if (nb < nbMin) nb = nbMin;
-    if (nb < 1) nb = 1;
+    // row chunks of the multi-dot: 256 lanes x E elements each
+    const size_t nbr = (n + (size_t)256 * VecE<BT>::E - 1) / ((size_t)256 * VecE<BT>::E);
     const unsigned gv = vec_grid(n);
     SV_TRY(basisBuf.ensure((size_t)(sD + 1) * n * sizeof(BT)));
     SV_TRY(xBuf.ensure(n * sizeof(BT)));
-    const size_t partN = ((size_t)(sD + 2) * nb > gv ? (size_t)(sD + 2) * nb : gv) + 16;
+    const size_t partN = ((size_t)(sD + 2) * nbr > gv ? (size_t)(sD + 2) * nbr : gv) + 16;
     SV_TRY(w.partial.ensure(partN * sizeof(double)));
     SV_TRY(w.state.ensure(256));
     SV_TRY(w.hess.ensure(gm_arrays_doubles(sD) * sizeof(double)));
@@ -595,7 +741,9 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
             const int i1 = (i0 + chunk - 1 < sD) ? i0 + chunk - 1 : sD;
             for (int i = i0; i <= i1; ++i) {
                 SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, s, stopFlag));
-                hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nb), dim3(256), 0, s, n, i, sD, (const BT*)basis, part, st, A);
+                hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
+                                   (const BT*)basis, part, (const GmState*)st);
+                hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, sD, (int)nbr, (const double*)part, st, A);
                 hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, s, n, i, sD, basis, (const GmState*)st, A);
             }
             SV_HIP(hipGetLastError());
@@ -608,7 +756,11 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
                 if (hs.stop) stopped = true;
             }
         }
-        hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, sD, st, A);
+        {
+            const size_t tri = (size_t)sD * (sD + 1) / 2;
+            const size_t ldsB = (tri <= (size_t)kBackLds ? tri : 0) * sizeof(double);
+            hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), ldsB, s, sD, st, A);
+        }
         hipLaunchKernelGGL(k_gm_xupdate<BT>, dim3(gv), dim3(256), 0, s, n, X, (const BT*)basis, (const GmState*)st, A);
         SV_HIP(hipGetLastError());
         SV_HIP(hipMemcpyAsync(&hs, st, sizeof(GmState), hipMemcpyDeviceToHost, s));
@@ -635,54 +787,82 @@ int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R
     return gmres_core<double>(L, op, dof, R, L.ws.basis, L.ws.X, ls, s);
 }
 
-// ===========================================================================
-// Mixed precision: FP64 iterative refinement around an inner GMRES on the
-// FP32 copy of the scaled matrix (FP32 SpMV and basis storage, FP64 dot
-// accumulation and Hessenberg least squares).  The outer residual
-// r = b - A x uses the FP64 scaled matrix; convergence is the FP64 test
-// |r| < max(absTol, relTol |b|) of GMRESV.
-// ===========================================================================
-__global__ __launch_bounds__(256) void k_resid64(size_t n, const double* __restrict__ b,
-                                                 const double* __restrict__ ax,
-                                                 double* __restrict__ r, float* __restrict__ r32,
-                                                 double* __restrict__ partial)
+__global__ void k_cvt_f64_f32(size_t n, const double* __restrict__ a, float* __restrict__ b)
 {
-    __shared__ double lds[4];
-    double s = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) b[q] = (float)a[q];
+}
+
+__global__ void k_cvt_f32_f64(size_t n, const float* __restrict__ a, double* __restrict__ b)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) b[q] = (double)a[q];
+}
+
+// x += sum_j y_j z_j with FP32 directions z_j (flexible GMRES update)
+__global__ __launch_bounds__(256) void k_fg_xupdate(size_t n, double* __restrict__ X, const float* __restrict__ Z,
+                                                    const GmState* __restrict__ st, GmArrays A)
+{
+    const int i = st->iconv;
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
-        const double v = b[q] - ax[q];
-        r[q] = v;
-        r32[q] = (float)v;
-        s += v * v;
+        double v = X[q];
+        for (int jj = 0; jj < i; ++jj) v = v + A.y[jj] * (double)Z[(size_t)jj * n + q];
+        X[q] = v;
     }
-    s = block_sum<256>(s, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ void k_add_f32(size_t n, double* __restrict__ x, const float* __restrict__ d)
+static int env_int(const char* k, int d)
 {
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
-        x[q] += (double)d[q];
+    const char* e = getenv(k);
+    return e ? atoi(e) : d;
 }
 
+static double env_dbl(const char* k, double d)
+{
+    const char* e = getenv(k);
+    return e ? atof(e) : d;
+}
+
+// Mixed precision: flexible GMRES (FGMRES(m)) in FP64 whose right
+// preconditioner is an inner GMRES on the FP32 copy of the scaled matrix.
+// FP64: the outer residual b - A x, the outer Arnoldi (classical
+// Gram-Schmidt with the Pythagorean norm, as GMRESV), the Hessenberg /
+// Givens least squares and the solution x.  FP32: the inner matrix, SpMV,
+// basis storage and the flexible directions z_j (the inner solver's dot
+// products and Hessenberg arithmetic are FP64).  The inner solve stops at
+// a loose relative tolerance; the outer Krylov space keeps what restarted
+// inner cycles would lose.  Convergence is GMRESV's FP64 test
+// |W(R - K x)| < max(absTol, relTol |W R|) on the outer estimate.
 int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const float* K32, double* R,
                 hipStream_t s)
 {
     const size_t n = (size_t)L.nNo * dof;
     Work& w = L.ws;
+    const int mo = env_int("SVFSI_FGMRES_M", 30);           // outer Krylov dimension
+    const double tin = env_dbl("SVFSI_FGMRES_INNER_TOL", 0.1);
+    const int capCycles = env_int("SVFSI_FGMRES_INNER_CYCLES", 1); // one inner cycle, no restarts
+    const int mi = ls.sD > 0 ? ls.sD : 50;                  // inner restart length
+    if (mo > kMaxSD || mi > kMaxSD) {
+        set_error("FGMRES: Krylov dimension exceeds %d", kMaxSD);
+        return SVFSI_ERR_NOTSUP;
+    }
     const unsigned gv = vec_grid(n);
+    const size_t nbr = (n + (size_t)256 * VecE<double>::E - 1) / ((size_t)256 * VecE<double>::E);
     SV_TRY(w.X.ensure(n * sizeof(double)));
-    SV_TRY(w.R0.ensure(n * sizeof(double)));
     SV_TRY(w.tmp.ensure(n * sizeof(double)));
-    SV_TRY(w.y32.ensure(n * sizeof(float)));
+    SV_TRY(w.basis.ensure((size_t)(mo + 1) * n * sizeof(double)));
+    SV_TRY(w.zdir.ensure((size_t)mo * n * sizeof(float)));
+    const size_t partN = ((size_t)(mo + 2) * nbr > gv ? (size_t)(mo + 2) * nbr : gv) + 16;
+    SV_TRY(w.opartial.ensure(partN * sizeof(double)));
+    SV_TRY(w.ostate.ensure(256));
+    SV_TRY(w.ohess.ensure(gm_arrays_doubles(mo) * sizeof(double)));
     double* x = w.X.as<double>();
-    double* r = w.R0.as<double>();
-    double* ax = w.tmp.as<double>();
-    float* r32 = w.y32.as<float>();
-    // outer reductions use their own buffer: gmres_core owns w.partial
-    SV_TRY(w.Z32.ensure((size_t)(gv + 16) * sizeof(double)));
-    double* part = w.Z32.as<double>();
-    double* scal = part + gv + 8;
+    double* t64 = w.tmp.as<double>();
+    double* Q = w.basis.as<double>();
+    float* Z = w.zdir.as<float>();
+    double* part = w.opartial.as<double>();
+    double* scal = part + partN - 8;
+    GmState* st = w.ostate.as<GmState>();
+    GmArrays A = gm_carve(w.ohess.as<double>(), mo);
+    GmOp<float> op32{&L, dof, K32};
 
     ls.callD = wall_seconds();
     ls.suc = 0;
@@ -702,39 +882,59 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
         ls.dB = 0.0;
         return SVFSI_OK;
     }
-    GmOp<float> op{&L, dof, K32};
-    double rnorm = ls.iNorm;
-    // r = b (x = 0)
-    SV_HIP(hipMemsetAsync(ax, 0, n * sizeof(double), s));
-    const int maxOuter = ls.mItr > 0 ? ls.mItr : 1;
-    double prev = rnorm;
-    for (int o = 0; o < maxOuter; ++o) {
-        hipLaunchKernelGGL(k_resid64, dim3(gv), dim3(256), 0, s, n, (const double*)R, (const double*)ax, r, r32, part);
-        hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, (int)gv, part, scal);
-        SV_HIP(hipMemcpyAsync(&rr, scal, sizeof(double), hipMemcpyDeviceToHost, s));
-        SV_HIP(hipStreamSynchronize(s));
-        prev = rnorm;
-        rnorm = std::sqrt(rr);
+    GmState h0{};
+    h0.eps = eps;
+    SV_HIP(hipMemcpyAsync(st, &h0, sizeof(GmState), hipMemcpyHostToDevice, s));
+    GmState hs{};
+    for (int l = 1; l <= ls.mItr; ++l) {
+        ls.dB = ls.fNorm;
         ls.itr += 1;
-        ls.fNorm = rnorm;
-        if (rnorm < eps) { ls.suc = 1; break; }
-        // inner: A32 d = r32 to a relative reduction that reaches eps with margin
-        FSILS_subLsType in{};
-        in.sD = ls.sD > 0 ? ls.sD : 50;
-        in.mItr = 50;
-        in.absTol = 0.0;
-        double target = 0.5 * eps / rnorm;
-        in.relTol = target > 1e-4 ? target : 1e-4;
-        SV_TRY(gmres_core<float>(L, op, dof, r32, w.basis32, w.x32, in, s));
-        ls.cM += in.cM;
-        hipLaunchKernelGGL(k_add_f32, dim3(gv), dim3(256), 0, s, n, x, (const float*)r32);
-        SV_TRY(launch_spmv(L, dof, K, x, ax, s));
+        SV_TRY(launch_spmv(L, dof, K, x, Q, s));
+        hipLaunchKernelGGL(k_gm_resid<double>, dim3(gv), dim3(256), 0, s, n, (const double*)R, Q, part, st, A);
+        hipLaunchKernelGGL(k_gm_div<double>, dim3(gv), dim3(256), 0, s, n, Q, A);
+        ls.cM += 1;
+        for (int i = 1; i <= mo; ++i) {
+            ls.itr += 1;
+            float* zi = Z + (size_t)(i - 1) * n;
+            hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, s, n, (const double*)(Q + (size_t)(i - 1) * n), zi);
+            FSILS_subLsType in{};
+            in.sD = mi;
+            in.mItr = capCycles;
+            in.absTol = 0.0;
+            in.relTol = tin;
+            SV_TRY(gmres_core<float>(L, op32, dof, zi, w.basis32, w.x32, in, s));
+            ls.cM += in.cM;
+            hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, s, n, (const float*)zi, t64);
+            SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, s));
+            ls.cM += 1;
+            hipLaunchKernelGGL(k_gm_mdot<double>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
+                               (const double*)Q, part, (const GmState*)st);
+            hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, mo, (int)nbr, (const double*)part, st, A);
+            hipLaunchKernelGGL(k_gm_update<double>, dim3(gv), dim3(256), 0, s, n, i, mo, Q, (const GmState*)st, A);
+            SV_HIP(hipGetLastError());
+            SV_HIP(hipMemcpyAsync(&hs, st, sizeof(GmState), hipMemcpyDeviceToHost, s));
+            SV_HIP(hipStreamSynchronize(s));
+            if (hs.stop) break;
+        }
+        {
+            const size_t tri = (size_t)mo * (mo + 1) / 2;
+            const size_t ldsB = (tri <= (size_t)kBackLds ? tri : 0) * sizeof(double);
+            hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), ldsB, s, mo, st, A);
+        }
+        hipLaunchKernelGGL(k_fg_xupdate, dim3(gv), dim3(256), 0, s, n, x, (const float*)Z, (const GmState*)st, A);
+        SV_HIP(hipGetLastError());
+        SV_HIP(hipMemcpyAsync(&hs, st, sizeof(GmState), hipMemcpyDeviceToHost, s));
+        SV_HIP(hipStreamSynchronize(s));
+        ls.fNorm = hs.fNorm;
+        if (hs.suc) {
+            ls.suc = 1;
+            break;
+        }
     }
-    (void)prev;
     SV_HIP(hipMemcpyAsync(R, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     SV_HIP(hipStreamSynchronize(s));
     ls.callD = wall_seconds() - ls.callD;
-    ls.dB = 10.0 * std::log(ls.fNorm / ls.iNorm);
+    ls.dB = 10.0 * std::log(ls.fNorm / ls.dB);
     return SVFSI_OK;
 }
 

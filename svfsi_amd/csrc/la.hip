@@ -13,6 +13,8 @@
 #include "devutil.hpp"
 
 #include <cmath>
+#include <cstdlib>
+#include <type_traits>
 
 namespace svfsi {
 
@@ -65,6 +67,62 @@ __global__ __launch_bounds__(256) void k_bspmv(int nNo, const int* __restrict__ 
         acc += t;
     }
     if (active && j == 0) y[(size_t)row * DOF + i] = (XT)acc;
+}
+
+// dof = 4 specialisation: 4 lanes per block row, lane i owns output
+// component i and streams row i of every block of the row (4 contiguous
+// values: 32 B FP64 / 16 B FP32) plus the 4-vector U(:,col) with vector
+// loads; no cross-lane reduction.  16 block rows per wavefront, two blocks
+// in flight per lane.
+__device__ __forceinline__ double dot4_nt(const double* __restrict__ v, const double* __restrict__ x)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 a = __builtin_nontemporal_load(reinterpret_cast<const d2*>(v));
+    const d2 b = __builtin_nontemporal_load(reinterpret_cast<const d2*>(v) + 1);
+    const d2 xa = *reinterpret_cast<const d2*>(x);
+    const d2 xb = *(reinterpret_cast<const d2*>(x) + 1);
+    return a.x * xa.x + a.y * xa.y + b.x * xb.x + b.y * xb.y;
+}
+
+__device__ __forceinline__ double dot4_nt(const float* __restrict__ v, const float* __restrict__ x)
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 a = __builtin_nontemporal_load(reinterpret_cast<const f4*>(v));
+    const f4 xa = *reinterpret_cast<const f4*>(x);
+    return (double)a.x * (double)xa.x + (double)a.y * (double)xa.y + (double)a.z * (double)xa.z
+         + (double)a.w * (double)xa.w;
+}
+
+template <typename VT, int H>
+__global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__ rowPtr,
+                                                const int* __restrict__ colPtr,
+                                                const VT* __restrict__ val,
+                                                const VT* __restrict__ x,
+                                                VT* __restrict__ y,
+                                                const int* __restrict__ stop)
+{
+    // 4*H lanes per block row: lane (h, i) takes blocks k = s + h, s + h + H,
+    // ... and output component i; the H partial sums meet by lane shuffles.
+    if (stop && *stop) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / (4 * H);
+    const int l = t - row * 4 * H;
+    const int h = l >> 2, i = l & 3;
+    double a0 = 0.0, a1 = 0.0;
+    if (row < nNo) {
+        const int e = rowPtr[row + 1];
+        int k = rowPtr[row] + h;
+        for (; k + H < e; k += 2 * H) {
+            const int c0 = colPtr[k], c1 = colPtr[k + H];
+            a0 += dot4_nt(val + (size_t)k * 16 + i * 4, x + (size_t)c0 * 4);
+            a1 += dot4_nt(val + (size_t)(k + H) * 16 + i * 4, x + (size_t)c1 * 4);
+        }
+        if (k < e) a0 += dot4_nt(val + (size_t)k * 16 + i * 4, x + (size_t)colPtr[k] * 4);
+    }
+    double acc = a0 + a1;
+#pragma unroll
+    for (int o = 4; o < 4 * H; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+    if (row < nNo && h == 0) y[(size_t)row * 4 + i] = (VT)acc;
 }
 
 // Transpose product: row i of K^T gathers block (col,i) through the
@@ -144,9 +202,29 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
 {
     if (L.nNo == 0) return SVFSI_OK;
     ProfScope ps(profKernel, s);
-    const int grid = spmv_grid(L.nNo, dof);
     const int* rp = L.rowPtr.as<int>();
     const int* cp = L.colPtr.as<int>();
+    if constexpr (std::is_same<VT, XT>::value) {
+        if (dof == 4) {
+            // lanes per output component (tuning knob SVFSI_SPMV_H = 1, 2 or 4;
+            // 4 measured fastest on the 200k-tet cylinder)
+            static const int H = [] {
+                const char* e = getenv("SVFSI_SPMV_H");
+                const int v = e ? atoi(e) : 4;
+                return (v == 1 || v == 2) ? v : 4;
+            }();
+            const int g4 = (int)(((size_t)L.nNo * 4 * H + 255) / 256);
+            if (H == 1)
+                hipLaunchKernelGGL((k_bspmv4<VT, 1>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+            else if (H == 2)
+                hipLaunchKernelGGL((k_bspmv4<VT, 2>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+            else
+                hipLaunchKernelGGL((k_bspmv4<VT, 4>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+            SV_HIP(hipGetLastError());
+            return SVFSI_OK;
+        }
+    }
+    const int grid = spmv_grid(L.nNo, dof);
     switch (dof) {
     case 2: hipLaunchKernelGGL((k_bspmv<2, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
     case 3: hipLaunchKernelGGL((k_bspmv<3, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;

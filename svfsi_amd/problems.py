@@ -103,7 +103,7 @@ def fluid_cylinder(h: float = 0.11, seed: int = 0, rho: float = 1.06, mu: float 
         name=f"fluid_cyl_h{h}", phys=SVFSI_PHYS_FLUID, dof=4, mesh=m, eq=eq, Ag=A, Yg=Y, Dg=np.zeros_like(Y),
         dir_faces=[(np.where(inlet)[0].astype(np.int32), 3), (np.where(wall)[0].astype(np.int32), 3)],
         bflow_faces=[(ff, fe)],
-        ls=dict(LS_type=LS_TYPE_GMRES, relTol=1e-8, absTol=1e-30, mItr=200, sD=sD))
+        ls=dict(LS_type=LS_TYPE_GMRES, relTol=1e-8, absTol=1e-30, mItr=1000, sD=sD))
 
 
 def struct_cube(n: int = 4, C10: float = 1e5, Kpen: float = 1e7, rho: float = 1.0, dt: float = 1e-3,
