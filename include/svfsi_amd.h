@@ -136,12 +136,13 @@ int fsils_gpu_solve_dev(void* lhs, FSILS_lsType* ls, int dof, double* dRi,
 
 /* Mixed-precision form of fsils_gpu_solve_dev (no reference counterpart;
  * the reference is FP64-only, svFSILS/FSILS_TYPEDEF.h:55-57).  Same
- * arguments, scaling, convergence test and outputs.  Flexible GMRES in
- * FP64 (outer residual, Arnoldi, Hessenberg/Givens, solution) whose right
- * preconditioner is an inner restarted GMRES(ls->RI.sD) on an FP32 copy of
- * the scaled matrix (FP32 SpMV, basis and directions; FP64 dot
- * accumulation and least squares), stopped at a loose relative tolerance.
- * The reported residual is the FP64 one in the reference's scaled norm.
+ * arguments, scaling, convergence test and outputs.  Flexible GMRES(30) in
+ * FP64 (outer residual, two-pass classical Gram-Schmidt Arnoldi,
+ * Hessenberg/Givens, solution) whose right preconditioner is one cycle of
+ * an inner GMRES(30) on an FP32 copy of the scaled matrix (FP32 SpMV, basis
+ * and directions; FP64 dot accumulation and least squares), stopped at a
+ * relative reduction of 0.1.  The reported residual is the FP64 one in the
+ * reference's scaled norm; ls->RI.mItr bounds the outer restarts.
  * ls->RI.itr counts outer (FP64) Arnoldi steps and restarts;
  * ls->RI.cM counts all SpMVs (inner FP32 plus outer FP64). */
 int fsils_gpu_solve_mixed_dev(void* lhs, FSILS_lsType* ls, int dof,

@@ -70,8 +70,25 @@ struct Face {
     std::vector<double> val; // (dof, nNo)
 };
 
+// Pinned, double-buffered host mirror of a small device state: the host
+// polls the state of the previous chunk of iterations while the next one
+// runs.  Allocated once per workspace.
+struct PinnedPoll {
+    void* h[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    PinnedPoll() = default;
+    PinnedPoll(const PinnedPoll&) = delete;
+    PinnedPoll& operator=(const PinnedPoll&) = delete;
+    ~PinnedPoll();
+    int ensure(size_t b);
+    int post(int q, const void* dsrc, size_t n, hipStream_t s);
+    int wait(int q);
+};
+
 // Krylov workspace, grown on demand and reused across solves.
 struct Work {
+    PinnedPoll poll;
     DBuf W, X, R0, tmp, basis, basis32, Z32, val32, x32, y32;
     DBuf partial, state, hess;
     DBuf zdir, opartial, ostate, ohess; // flexible GMRES: FP32 directions, outer state

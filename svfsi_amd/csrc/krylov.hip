@@ -160,35 +160,39 @@ static unsigned vec_grid(size_t n)
     return (unsigned)g;
 }
 
-// Pinned host mirror of a small device state, double-buffered.
-struct Poll {
-    void* h[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    size_t bytes = 0;
-    int init(size_t b) {
-        bytes = b;
-        for (int q = 0; q < 2; ++q) {
-            SV_HIP(hipHostMalloc(&h[q], b, hipHostMallocDefault));
-            SV_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
-        }
-        return SVFSI_OK;
+PinnedPoll::~PinnedPoll()
+{
+    for (int q = 0; q < 2; ++q) {
+        if (h[q]) (void)hipHostFree(h[q]);
+        if (ev[q]) (void)hipEventDestroy(ev[q]);
     }
-    ~Poll() {
-        for (int q = 0; q < 2; ++q) {
-            if (h[q]) (void)hipHostFree(h[q]);
-            if (ev[q]) (void)hipEventDestroy(ev[q]);
-        }
+}
+
+int PinnedPoll::ensure(size_t b)
+{
+    if (h[0] && b <= bytes) return SVFSI_OK;
+    for (int q = 0; q < 2; ++q) {
+        if (h[q]) (void)hipHostFree(h[q]);
+        h[q] = nullptr;
+        SV_HIP(hipHostMalloc(&h[q], b, hipHostMallocDefault));
+        if (!ev[q]) SV_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
     }
-    int post(int q, const void* dsrc, hipStream_t s) {
-        SV_HIP(hipMemcpyAsync(h[q], dsrc, bytes, hipMemcpyDeviceToHost, s));
-        SV_HIP(hipEventRecord(ev[q], s));
-        return SVFSI_OK;
-    }
-    int wait(int q) {
-        SV_HIP(hipEventSynchronize(ev[q]));
-        return SVFSI_OK;
-    }
-};
+    bytes = b;
+    return SVFSI_OK;
+}
+
+int PinnedPoll::post(int q, const void* dsrc, size_t n, hipStream_t s)
+{
+    SV_HIP(hipMemcpyAsync(h[q], dsrc, n, hipMemcpyDeviceToHost, s));
+    SV_HIP(hipEventRecord(ev[q], s));
+    return SVFSI_OK;
+}
+
+int PinnedPoll::wait(int q)
+{
+    SV_HIP(hipEventSynchronize(ev[q]));
+    return SVFSI_OK;
+}
 
 int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s)
 {
@@ -234,8 +238,8 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
     SV_HIP(hipMemsetAsync(X, 0, n * sizeof(double), s));
     SV_HIP(hipMemcpyAsync(P, R, n * sizeof(double), hipMemcpyDeviceToDevice, s));
 
-    Poll poll;
-    SV_TRY(poll.init(sizeof(CgState)));
+    PinnedPoll& poll = w.poll;
+    SV_TRY(poll.ensure(256));
     const int chunk = 16;
     int posted = 0;
     CgState hs = h0;
@@ -255,7 +259,7 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
         }
         launched += chunk;
         SV_HIP(hipGetLastError());
-        SV_TRY(poll.post(q, st, s));
+        SV_TRY(poll.post(q, st, sizeof(CgState), s));
         ++posted;
         // check the previous chunk while this one runs
         if (posted >= 2) {
@@ -286,7 +290,7 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
 struct GmState {
     double eps;
     double fNorm;
-    int stop, suc, iconv, pad;
+    int stop, suc, iconv, iters;  // iters: Arnoldi steps run (accumulates)
     unsigned int counter;
     int pad2;
 };
@@ -312,6 +316,19 @@ static GmArrays gm_carve(double* base, int sD)
     a.y = a.err + (sD + 1);
     a.hraw = a.y + sD;
     return a;
+}
+
+// Initial state of a GMRES solve.
+__global__ void k_gm_init(GmState* __restrict__ st, double eps)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    st->eps = eps;
+    st->fNorm = 0.0;
+    st->stop = 0;
+    st->suc = 0;
+    st->iconv = 0;
+    st->iters = 0;
+    st->counter = 0u;
 }
 
 // u0 = R - u0 and |u0|; last block: err(1) = |u0|, cycle state reset.
@@ -437,8 +454,13 @@ __global__ __launch_bounds__(256) void k_gm_mdot(size_t n, int i, const BT* __re
 // then the Pythagorean h(i+1,i) (GMRES.f:342-347), the previous rotations,
 // the new Givens rotation, the residual estimate and the convergence test
 // (351-366), all in LDS.
+// pass 0: single classical Gram-Schmidt pass with the Pythagorean norm
+// (GMRESV).  pass 1 / 2: the two passes of CGS2 (used by the flexible outer
+// solver): pass 1 only stores the first coefficients (in y, free during a
+// cycle) for k_gm_update; pass 2 takes h(i+1,i) from the twice-orthogonalised
+// vector and adds the two coefficient sets before the Givens rotations.
 __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, const double* __restrict__ partial,
-                                                   GmState* __restrict__ st, GmArrays A)
+                                                   GmState* __restrict__ st, GmArrays A, int pass)
 {
     if (st->stop) return;
     __shared__ double hsh[kMaxSD + 2];
@@ -476,10 +498,18 @@ __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, const 
     __syncthreads();
     // CGS coefficients for k_gm_update (pre-rotation values)
     for (int jj = threadIdx.x; jj <= i; jj += 256) A.hraw[jj] = hsh[jj];
+    if (pass == 1) {
+        for (int jj = threadIdx.x; jj < i; jj += 256) A.y[jj] = hsh[jj];
+        if (threadIdx.x == 0) A.hraw[sD + 2] = 1.0;
+        return;
+    }
     if (threadIdx.x != 0) return;
+    st->iters += 1;
     double hii = hsh[i];
     for (int jj = 0; jj < i; ++jj) hii = hii - hsh[jj] * hsh[jj];
     hii = sqrt(fabs(hii));
+    if (pass == 2)
+        for (int jj = 0; jj < i; ++jj) hsh[jj] = A.y[jj] + hsh[jj];
     hsh[i] = hii;
     A.hraw[sD + 2] = 1.0 / hii;  // OMPMULV(1/h(i+1,i)) factor for k_gm_update
     for (int jj = 1; jj <= i - 1; ++jj) {
@@ -719,12 +749,10 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
         ls.dB = 0.0;
         return SVFSI_OK;
     }
-    GmState h0{};
-    h0.eps = eps;
-    SV_HIP(hipMemcpyAsync(st, &h0, sizeof(GmState), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, st, eps);
 
-    Poll poll;
-    SV_TRY(poll.init(sizeof(GmState)));
+    PinnedPoll& poll = w.poll;
+    SV_TRY(poll.ensure(256));
     GmState hs{};
     const int chunk = 8;
     for (int l = 1; l <= ls.mItr; ++l) {
@@ -743,12 +771,12 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
                 SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, s, stopFlag));
                 hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
                                    (const BT*)basis, part, (const GmState*)st);
-                hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, sD, (int)nbr, (const double*)part, st, A);
+                hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, sD, (int)nbr, (const double*)part, st, A, 0);
                 hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, s, n, i, sD, basis, (const GmState*)st, A);
             }
             SV_HIP(hipGetLastError());
             const int q = posted & 1;
-            SV_TRY(poll.post(q, st, s));
+            SV_TRY(poll.post(q, st, sizeof(GmState), s));
             ++posted;
             if (posted >= 2) {
                 SV_TRY(poll.wait(q ^ 1));
@@ -785,6 +813,91 @@ int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R
 {
     GmOp<double> op{&L, dof, K, transpose};
     return gmres_core<double>(L, op, dof, R, L.ws.basis, L.ws.X, ls, s);
+}
+
+// ---------------------------------------------------------------------------
+// Device-driven GMRES cycle with zero initial guess (the inner solver of the
+// flexible GMRES): no host round trip.  The relative tolerance is applied on
+// the device to |R|; `gate` (an outer stop flag) cancels the whole cycle.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_gm_start(int m, const double* __restrict__ partial,
+                                                  GmState* __restrict__ st, GmArrays A, double relTol,
+                                                  const int* __restrict__ gate)
+{
+    __shared__ double lds[4];
+    double t = 0.0;
+    for (int q = threadIdx.x; q < m; q += 256) t += partial[q];
+    t = block_sum<256>(t, lds);
+    if (threadIdx.x == 0) {
+        const double e0 = sqrt(t);
+        A.err[0] = e0;
+        st->eps = relTol * e0;
+        st->iconv = 0;
+        st->suc = 0;
+        st->stop = ((gate && *gate) || e0 == 0.0) ? 1 : 0;
+    }
+}
+
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_scale_copy(size_t n, const BT* __restrict__ R, BT* __restrict__ u0,
+                                                       const GmState* __restrict__ st, GmArrays A)
+{
+    if (st->stop) return;
+    const double e0 = A.err[0];
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        u0[q] = (BT)((double)R[q] / e0);
+}
+
+// R = sum_{j < iconv} y_j u_j (zero initial guess)
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_xsolve(size_t n, BT* __restrict__ R, const BT* __restrict__ basis,
+                                                   const GmState* __restrict__ st, GmArrays A)
+{
+    const int i = st->iconv;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        double v = 0.0;
+        for (int jj = 0; jj < i; ++jj) v = v + A.y[jj] * (double)basis[(size_t)jj * n + q];
+        R[q] = (BT)v;
+    }
+}
+
+// One cycle of GMRES(sD) on R (RHS in, approximate solution out), enqueued
+// without any host synchronisation.  Uses the workspace's inner state.
+template <typename BT>
+static int gmres_inner_dev(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf, int sD, double relTol,
+                           const int* gate, hipStream_t s)
+{
+    const size_t n = (size_t)L.nNo * dof;
+    Work& w = L.ws;
+    const size_t nbr = (n + (size_t)256 * VecE<BT>::E - 1) / ((size_t)256 * VecE<BT>::E);
+    const unsigned gv = vec_grid(n);
+    SV_TRY(basisBuf.ensure((size_t)(sD + 1) * n * sizeof(BT)));
+    const size_t partN = ((size_t)(sD + 2) * nbr > gv ? (size_t)(sD + 2) * nbr : gv) + 16;
+    SV_TRY(w.partial.ensure(partN * sizeof(double)));
+    SV_TRY(w.hess.ensure(gm_arrays_doubles(sD) * sizeof(double)));
+    BT* basis = basisBuf.as<BT>();
+    double* part = w.partial.as<double>();
+    GmState* st = w.state.as<GmState>();
+    GmArrays A = gm_carve(w.hess.as<double>(), sD);
+    hipLaunchKernelGGL(k_normsq_partial<BT>, dim3(gv), dim3(256), 0, s, n, (const BT*)R, part);
+    hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(256), 0, s, (int)gv, (const double*)part, st, A, relTol, gate);
+    hipLaunchKernelGGL(k_gm_scale_copy<BT>, dim3(gv), dim3(256), 0, s, n, (const BT*)R, basis, (const GmState*)st, A);
+    for (int i = 1; i <= sD; ++i) {
+        SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, s, &st->stop));
+        hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
+                           (const BT*)basis, part, (const GmState*)st);
+        hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, sD, (int)nbr, (const double*)part, st, A, 0);
+        hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, s, n, i, sD, basis, (const GmState*)st, A);
+    }
+    {
+        const size_t tri = (size_t)sD * (sD + 1) / 2;
+        const size_t ldsB = (tri <= (size_t)kBackLds ? tri : 0) * sizeof(double);
+        hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), ldsB, s, sD, st, A);
+    }
+    hipLaunchKernelGGL(k_gm_xsolve<BT>, dim3(gv), dim3(256), 0, s, n, R, (const BT*)basis, (const GmState*)st, A);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
 }
 
 __global__ void k_cvt_f64_f32(size_t n, const double* __restrict__ a, float* __restrict__ b)
@@ -838,8 +951,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     Work& w = L.ws;
     const int mo = env_int("SVFSI_FGMRES_M", 30);           // outer Krylov dimension
     const double tin = env_dbl("SVFSI_FGMRES_INNER_TOL", 0.1);
-    const int capCycles = env_int("SVFSI_FGMRES_INNER_CYCLES", 1); // one inner cycle, no restarts
-    const int mi = ls.sD > 0 ? ls.sD : 50;                  // inner restart length
+    const int mi = env_int("SVFSI_FGMRES_INNER_M", 30);     // inner Krylov dimension
     if (mo > kMaxSD || mi > kMaxSD) {
         set_error("FGMRES: Krylov dimension exceeds %d", kMaxSD);
         return SVFSI_ERR_NOTSUP;
@@ -882,39 +994,46 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
         ls.dB = 0.0;
         return SVFSI_OK;
     }
-    GmState h0{};
-    h0.eps = eps;
-    SV_HIP(hipMemcpyAsync(st, &h0, sizeof(GmState), hipMemcpyHostToDevice, s));
+    SV_TRY(w.state.ensure(256));
+    hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, st, eps);
+    hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, w.state.as<GmState>(), 0.0);
+    PinnedPoll& poll = w.poll;
+    SV_TRY(poll.ensure(256));
     GmState hs{};
+    int outerMv = 0;
     for (int l = 1; l <= ls.mItr; ++l) {
         ls.dB = ls.fNorm;
         ls.itr += 1;
         SV_TRY(launch_spmv(L, dof, K, x, Q, s));
         hipLaunchKernelGGL(k_gm_resid<double>, dim3(gv), dim3(256), 0, s, n, (const double*)R, Q, part, st, A);
         hipLaunchKernelGGL(k_gm_div<double>, dim3(gv), dim3(256), 0, s, n, Q, A);
-        ls.cM += 1;
+        outerMv += 1;
+        int posted = 0;
         for (int i = 1; i <= mo; ++i) {
-            ls.itr += 1;
             float* zi = Z + (size_t)(i - 1) * n;
             hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, s, n, (const double*)(Q + (size_t)(i - 1) * n), zi);
-            FSILS_subLsType in{};
-            in.sD = mi;
-            in.mItr = capCycles;
-            in.absTol = 0.0;
-            in.relTol = tin;
-            SV_TRY(gmres_core<float>(L, op32, dof, zi, w.basis32, w.x32, in, s));
-            ls.cM += in.cM;
+            SV_TRY(gmres_inner_dev<float>(L, op32, dof, zi, w.basis32, mi, tin, &st->stop, s));
             hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, s, n, (const float*)zi, t64);
-            SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, s));
-            ls.cM += 1;
-            hipLaunchKernelGGL(k_gm_mdot<double>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
-                               (const double*)Q, part, (const GmState*)st);
-            hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, mo, (int)nbr, (const double*)part, st, A);
-            hipLaunchKernelGGL(k_gm_update<double>, dim3(gv), dim3(256), 0, s, n, i, mo, Q, (const GmState*)st, A);
+            SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, s, &st->stop));
+            outerMv += 1;
+            // outer Arnoldi: CGS2 (two classical passes)
+            for (int pass = 1; pass <= 2; ++pass) {
+                hipLaunchKernelGGL(k_gm_mdot<double>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n,
+                                   i, (const double*)Q, part, (const GmState*)st);
+                hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, mo, (int)nbr, (const double*)part, st, A,
+                                   pass);
+                hipLaunchKernelGGL(k_gm_update<double>, dim3(gv), dim3(256), 0, s, n, i, mo, Q, (const GmState*)st, A);
+            }
             SV_HIP(hipGetLastError());
-            SV_HIP(hipMemcpyAsync(&hs, st, sizeof(GmState), hipMemcpyDeviceToHost, s));
-            SV_HIP(hipStreamSynchronize(s));
-            if (hs.stop) break;
+            // lagged poll: the state of step i-1 is read while step i runs
+            const int q = posted & 1;
+            SV_TRY(poll.post(q, st, sizeof(GmState), s));
+            ++posted;
+            if (posted >= 2) {
+                SV_TRY(poll.wait(q ^ 1));
+                std::memcpy(&hs, poll.h[q ^ 1], sizeof(GmState));
+                if (hs.stop) break;
+            }
         }
         {
             const size_t tri = (size_t)mo * (mo + 1) / 2;
@@ -925,11 +1044,18 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
         SV_HIP(hipGetLastError());
         SV_HIP(hipMemcpyAsync(&hs, st, sizeof(GmState), hipMemcpyDeviceToHost, s));
         SV_HIP(hipStreamSynchronize(s));
+        ls.itr += hs.iconv;
         ls.fNorm = hs.fNorm;
         if (hs.suc) {
             ls.suc = 1;
             break;
         }
+    }
+    {
+        GmState in{};
+        SV_HIP(hipMemcpyAsync(&in, w.state.p, sizeof(GmState), hipMemcpyDeviceToHost, s));
+        SV_HIP(hipStreamSynchronize(s));
+        ls.cM = in.iters + outerMv;
     }
     SV_HIP(hipMemcpyAsync(R, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     SV_HIP(hipStreamSynchronize(s));
