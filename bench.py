@@ -38,6 +38,22 @@ def spmv_bytes(nnzb: int, nNo: int, dof: int, s: int) -> int:
     return nnzb * dof * dof * s + 4 * nnzb + 4 * (nNo + 1) + 2 * nNo * dof * s
 
 
+def measured_traffic(nnzb: int, kernel: str):
+    """HBM bytes per launch of `kernel` from the PMC summary in profiles/
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy kernel of known
+    size), when it was measured on a matrix of the same size; else None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rec.get("kernels", []):
+        if r.get("kernel") == kernel and r.get("nnzb") == nnzb:
+            return r.get("traffic_bytes")
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +62,10 @@ def main() -> int:
     ap.add_argument("--h", type=float, default=0.11, help="cylinder lattice spacing (0.11 -> ~200k tets)")
     ap.add_argument("--sD", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (calibrates PMC byte counters)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="skip the FP64 leg (for counter-collection runs, which serialise every dispatch)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -63,6 +83,17 @@ def main() -> int:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
 
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "reps": 5}))
+        return 0
+
     t0 = time.time()
     prob = problems.fluid_cylinder(h=args.h, sD=args.sD)
     st = NewtonStep(prob)
@@ -79,28 +110,27 @@ def main() -> int:
 
     # warmup both paths
     for _ in range(args.warmup):
-        run(False)
+        if not args.mixed_only:
+            run(False)
         run(True)
     torch.cuda.synchronize()
 
-    def timed(mixed: bool, k: int):
+    def timed(mixed: bool, k: int, profile: bool):
+        """Wall time per step over k steps, bracketed by device
+        synchronisation.  With profile=True the SpMV launches carry
+        dispatch-attached events (kernel durations); such runs are kept out
+        of the timed figures."""
+        import ctypes as C
+
         L = lib()
         L.svfsi_prof_reset()
-        L.svfsi_prof_enable(1)
+        L.svfsi_prof_enable(1 if profile else 0)
         torch.cuda.synchronize()
         a = time.perf_counter()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev0.record()
-        lss = []
-        for _ in range(k):
-            lss.append(run(mixed))
-        ev1.record()
+        lss = [run(mixed) for _ in range(k)]
         torch.cuda.synchronize()
         b = time.perf_counter()
         L.svfsi_prof_enable(0)
-        import ctypes as C
-
         cnt = [C.c_longlong(), C.c_longlong()]
         tot = [C.c_double(), C.c_double()]
         for kk in (0, 1):
@@ -115,8 +145,13 @@ def main() -> int:
     torch.cuda.synchronize()
     asm_ms = (time.perf_counter() - a) * 1e3 / max(1, args.steps)
 
-    fp64_ms, ls64, prof64 = timed(False, args.steps)
-    mix_ms, lsmx, profmx = timed(True, args.steps)
+    if args.mixed_only:
+        fp64_ms, ls64, prof64 = None, None, [(0, 0.0), (0, 0.0)]
+    else:
+        fp64_ms, ls64, _ = timed(False, args.steps, False)
+        _, _, prof64 = timed(False, 1, True)
+    mix_ms, lsmx, _ = timed(True, args.steps, False)
+    _, _, profmx = timed(True, 1, True)
 
     # SpMV roofline: FP32 SpMV of the mixed path (the inner solver's kernel)
     n32, t32 = profmx[1]
@@ -129,7 +164,7 @@ def main() -> int:
     # accuracy and CPU baseline (outside the timed regions)
     cpu = None
     acc = {}
-    if not args.no_cpu:
+    if not args.no_cpu and not args.mixed_only:
         from oracle import oracle as orc
 
         st.assemble()
@@ -172,12 +207,12 @@ def main() -> int:
                        f"{itr64} iterations of the FP64 step"),
         }
 
-    speed = fp64_ms / mix_ms if mix_ms > 0 else None
+    speed = fp64_ms / mix_ms if (fp64_ms and mix_ms > 0) else None
     roof = None
     if gbs32 is not None:
-        roof = {"kernel": "k_bspmv<4,float,float> (FP32 block SpMV, inner GMRES)",
+        roof = {"kernel": "k_bspmv4<float,4> (FP32 block SpMV, inner GMRES)",
                 "bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(nnzb, "k_bspmv4<float, 4>"),
                 "bytes_per_launch": b32, "avg_launch_us": round(t32 / n32 * 1e3, 2), "launches": n32}
     out = {
         "metric": "Assembly+GMRES time per Newton step at 1e-8 residual; FP64 vs mixed; SpMV GB/s",
@@ -194,13 +229,13 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "FLUID3D VMS tet4 cylinder r=1 L=10, one Newton step, GMRES(%d) to 1e-8" % args.sD,
                    "nNo": m.nNo, "nEl": m.nEl, "nnzb": nnzb, "dof": dof, "h": args.h, "colors": st.ncolors},
-        "fp64_ms_per_step": round(fp64_ms, 3),
+        "fp64_ms_per_step": None if fp64_ms is None else round(fp64_ms, 3),
         "assembly_ms": round(asm_ms, 3),
         "speedup_mixed_vs_fp64": None if speed is None else round(speed, 3),
-        "fp64_gmres_itr": ls64[-1].RI.itr,
+        "fp64_gmres_itr": None if ls64 is None else ls64[-1].RI.itr,
         "mixed_outer_itr": lsmx[-1].RI.itr,
         "mixed_inner_spmv": lsmx[-1].RI.cM,
-        "suc": [int(ls64[-1].RI.suc), int(lsmx[-1].RI.suc)],
+        "suc": [None if ls64 is None else int(ls64[-1].RI.suc), int(lsmx[-1].RI.suc)],
         "spmv_fp64_gbs": None if gbs64 is None else round(gbs64, 1),
         "spmv_fp32_gbs": None if gbs32 is None else round(gbs32, 1),
         "accuracy": acc,

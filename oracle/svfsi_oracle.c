@@ -7,6 +7,7 @@
  * baseline leg.  See svfsi_oracle.h for layouts.  Citations are
  * file:line under Code/Source/ of SimVascular/svFSI.
  */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime under -std=c11 */
 #include "svfsi_oracle.h"
 
 #include <math.h>

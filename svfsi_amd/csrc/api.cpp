@@ -74,6 +74,71 @@ Tet4Rule make_tet4_rule()
     return r;
 }
 
+// GETGIP + GETGNN for TET10 (svFSI/NN.f:277-307, 674-715): the 15-point
+// rule (quad 0) or the 4-point TET4 rule (quad 1), tabulated as
+// w, N(10), Nxi(10,3) per Gauss point.
+int make_tet10_rule(int quad, std::vector<double>& tab)
+{
+    double w[15], xi[15][3];
+    int nG;
+    if (quad == 1) {
+        const double s = (5.0 + 3.0 * std::sqrt(5.0)) / 20.0;
+        const double t = (5.0 - std::sqrt(5.0)) / 20.0;
+        nG = 4;
+        for (int g = 0; g < 4; ++g) {
+            w[g] = 1.0 / 24.0;
+            xi[g][0] = xi[g][1] = xi[g][2] = t;
+            if (g < 3) xi[g][g] = s;
+        }
+    } else {
+        nG = 15;
+        w[0] = 0.030283678097089;
+        for (int g = 1; g < 5; ++g) w[g] = 0.006026785714286;
+        for (int g = 5; g < 9; ++g) w[g] = 0.011645249086029;
+        for (int g = 9; g < 15; ++g) w[g] = 0.010949141561386;
+        xi[0][0] = xi[0][1] = xi[0][2] = 0.25;
+        // corner-type points: one coordinate t, the others s (last: all s)
+        const double cs[2][2] = {{0.333333333333333, 0.0}, {0.090909090909091, 0.727272727272727}};
+        for (int q = 0; q < 2; ++q)
+            for (int g = 0; g < 4; ++g)
+                for (int i = 0; i < 3; ++i) xi[1 + 4 * q + g][i] = (i == g) ? cs[q][1] : cs[q][0];
+        const double s = 0.066550153573664, t = 0.433449846426336;
+        const double e6[6][3] = {{s, s, t}, {s, t, s}, {s, t, t}, {t, t, s}, {t, s, t}, {t, s, s}};
+        for (int g = 0; g < 6; ++g)
+            for (int i = 0; i < 3; ++i) xi[9 + g][i] = e6[g][i];
+    }
+    tab.assign((size_t)nG * kTet10Rec, 0.0);
+    for (int g = 0; g < nG; ++g) {
+        double* r = tab.data() + (size_t)g * kTet10Rec;
+        const double x = xi[g][0], y = xi[g][1], z = xi[g][2], s = 1.0 - x - y - z;
+        r[0] = w[g];
+        double* N = r + 1;
+        N[0] = x * (2.0 * x - 1.0);
+        N[1] = y * (2.0 * y - 1.0);
+        N[2] = z * (2.0 * z - 1.0);
+        N[3] = s * (2.0 * s - 1.0);
+        N[4] = 4.0 * x * y;
+        N[5] = 4.0 * y * z;
+        N[6] = 4.0 * x * z;
+        N[7] = 4.0 * x * s;
+        N[8] = 4.0 * y * s;
+        N[9] = 4.0 * z * s;
+        const double d[10][3] = {{4.0 * x - 1.0, 0.0, 0.0},
+                                 {0.0, 4.0 * y - 1.0, 0.0},
+                                 {0.0, 0.0, 4.0 * z - 1.0},
+                                 {1.0 - 4.0 * s, 1.0 - 4.0 * s, 1.0 - 4.0 * s},
+                                 {4.0 * y, 4.0 * x, 0.0},
+                                 {0.0, 4.0 * z, 4.0 * y},
+                                 {4.0 * z, 0.0, 4.0 * x},
+                                 {4.0 * (s - x), -4.0 * x, -4.0 * x},
+                                 {-4.0 * y, 4.0 * (s - y), -4.0 * y},
+                                 {-4.0 * z, -4.0 * z, 4.0 * (s - z)}};
+        for (int a = 0; a < 10; ++a)
+            for (int j = 0; j < 3; ++j) r[11 + a * 3 + j] = d[a][j];
+    }
+    return nG;
+}
+
 Tri3Rule make_tri3_rule()
 {
     Tri3Rule r;
@@ -613,13 +678,24 @@ int svfsi_assemble_dev(void* mesh, int phys, const svfsi_eq_params* p, int tDof,
     SV_TRY(require_device());
     Mesh& M = *static_cast<Mesh*>(mesh);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    (void)quad;
     if (phys == SVFSI_PHYS_LELAS) {
         SV_ARG(M.eNoN == 4, "LELAS assembly supports tet4 only");
         SV_ARG(tDof >= 3 && dAg && dDg, "LELAS needs Ag, Dg with tDof >= 3");
     } else if (phys == SVFSI_PHYS_FLUID) {
         SV_ARG(M.eNoN == 4, "FLUID assembly supports tet4 only");
         SV_ARG(tDof >= 4 && dAg && dYg, "FLUID needs Ag, Yg with tDof >= 4");
+    } else if (phys == SVFSI_PHYS_STRUCT) {
+        SV_ARG(M.eNoN == 10, "STRUCT assembly supports tet10 only");
+        SV_ARG(tDof >= 3 && dAg && dDg, "STRUCT needs Ag, Dg with tDof >= 3");
+        SV_ARG(quad == 0 || quad == 1, "quad must be 0 (15-point) or 1 (4-point)");
+        if (!M.rule10nG[quad]) {
+            std::vector<double> tab;
+            const int nG = make_tet10_rule(quad, tab);
+            SV_TRY(upload(M.rule10[quad], tab));
+            M.rule10nG[quad] = nG;
+        }
+        return launch_asm_struct(M, to_asm(*p), M.rule10nG[quad], M.rule10[quad].as<double>(), tDof, dAg, dDg,
+                                 dVal, dR, s);
     } else {
         set_error("phys %d not supported", phys);
         return SVFSI_ERR_NOTSUP;

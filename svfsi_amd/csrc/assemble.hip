@@ -504,6 +504,246 @@ __global__ __launch_bounds__(256) void k_lelas_dkde(int nEl, const int* __restri
 }
 
 // ---------------------------------------------------------------------------
+// STRUCT3D + GETPK2CC nHK (svFSI/STRUCT.f:164-496, MATMODELS.f:41-150,
+// 539-564, 1588-1647) in CONSTRUCT_dSOLID (STRUCT.f:39-162), TET10, solid
+// viscosity 0, no damping, prestress or fibres.  One wavefront handles EPW
+// elements of one colour.  Phase 1: a lane per (element, Gauss point)
+// evaluates GNN, F, the neo-Hookean S and Voigt Dm, P = F S and the inertia
+// term into an LDS record.  Phase 2: lanes stride over the (element, a, b)
+// node pairs, accumulate the 3x3 block over the Gauss points as the
+// reference does for that block, and add it to Val through the scatter map;
+// the b = 0 lane also builds the residual row of node a.
+// ---------------------------------------------------------------------------
+// record layout (doubles)
+constexpr int SR_W = 0, SR_N = 1, SR_NX = 11, SR_S = 41, SR_DM = 50, SR_P = 86, SR_UD = 95, SR_F = 98, SR_SZ = 107;
+
+__device__ __forceinline__ double det3d(const double A[3][3])
+{
+    return A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1])
+         - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
+         + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+}
+
+template <int NG, int EPW>
+__global__ __launch_bounds__(64) void k_asm_struct(int begin, int count, const int* __restrict__ ien,
+                                                   const int* __restrict__ emap, const double* __restrict__ x,
+                                                   int tDof, const double* __restrict__ Ag,
+                                                   const double* __restrict__ Dg, AsmParams P,
+                                                   const double* __restrict__ rule, double* __restrict__ val,
+                                                   double* __restrict__ R)
+{
+    __shared__ double rec[EPW * NG][SR_SZ];
+    const int lane = threadIdx.x;
+    const int e0 = begin + blockIdx.x * EPW;
+    const int eEnd = begin + count;
+    if (lane < EPW * NG) {
+        const int k = lane / NG, g = lane - (lane / NG) * NG;
+        const int e = e0 + k;
+        double* rc = rec[lane];
+        if (e < eEnd) {
+            const double* rg = rule + g * kTet10Rec;
+            double xl[10][3];
+            int nd[10];
+#pragma unroll
+            for (int a = 0; a < 10; ++a) {
+                nd[a] = ien[(size_t)e * 10 + a];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) xl[a][i] = x[(size_t)nd[a] * 3 + i];
+            }
+            // GNN (NN.f:1469-1565)
+            double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int a = 0; a < 10; ++a)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) A[i][j] += xl[a][i] * rg[11 + a * 3 + j];
+            const double J = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0]
+                           + A[0][2] * A[1][0] * A[2][1] - A[0][0] * A[1][2] * A[2][1]
+                           - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
+            double B[3][3];
+            B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / J;
+            B[0][1] = (A[2][1] * A[0][2] - A[2][2] * A[0][1]) / J;
+            B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / J;
+            B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / J;
+            B[1][1] = (A[2][2] * A[0][0] - A[2][0] * A[0][2]) / J;
+            B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / J;
+            B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / J;
+            B[2][1] = (A[2][0] * A[0][1] - A[2][1] * A[0][0]) / J;
+            B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / J;
+            double Nx[10][3];
+#pragma unroll
+            for (int a = 0; a < 10; ++a)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    Nx[a][j] = rg[11 + a * 3] * B[0][j] + rg[11 + a * 3 + 1] * B[1][j] + rg[11 + a * 3 + 2] * B[2][j];
+            // F, inertia (STRUCT.f:195-235)
+            double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+            double ud[3] = {-P.rho * P.fb[0], -P.rho * P.fb[1], -P.rho * P.fb[2]};
+#pragma unroll
+            for (int a = 0; a < 10; ++a) {
+                const double Na = rg[1 + a];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const double Dai = Dg[(size_t)nd[a] * tDof + i];
+                    ud[i] += Na * (P.rho * Ag[(size_t)nd[a] * tDof + i]);
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) F[i][j] += Nx[a][j] * Dai;
+                }
+            }
+            // GETPK2CC, neo-Hookean + GETSVOLP
+            const double nd3 = 3.0;
+            const double Jf = det3d(F);
+            const double J2d = pow(Jf, -2.0 / nd3);
+            double C[3][3], Ci[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+            {
+                const double d = det3d(C);
+                Ci[0][0] = (C[1][1] * C[2][2] - C[1][2] * C[2][1]) / d;
+                Ci[0][1] = (C[0][2] * C[2][1] - C[0][1] * C[2][2]) / d;
+                Ci[0][2] = (C[0][1] * C[1][2] - C[0][2] * C[1][1]) / d;
+                Ci[1][0] = (C[1][2] * C[2][0] - C[1][0] * C[2][2]) / d;
+                Ci[1][1] = (C[0][0] * C[2][2] - C[0][2] * C[2][0]) / d;
+                Ci[1][2] = (C[0][2] * C[1][0] - C[0][0] * C[1][2]) / d;
+                Ci[2][0] = (C[1][0] * C[2][1] - C[1][1] * C[2][0]) / d;
+                Ci[2][1] = (C[0][1] * C[2][0] - C[0][0] * C[2][1]) / d;
+                Ci[2][2] = (C[0][0] * C[1][1] - C[0][1] * C[1][0]) / d;
+            }
+            double p = 0.0, pl = 0.0;
+            const double Kp = P.Kpen;
+            if (!iszero_d(Kp)) {
+                if (P.volType == 1) { p = Kp * (Jf - 1.0); pl = Kp * (2.0 * Jf - 1.0); }
+                else if (P.volType == 3) { p = Kp * (1.0 - 1.0 / Jf); pl = Kp; }
+                else { p = 0.5 * Kp * (Jf - 1.0 / Jf); pl = Kp * Jf; }
+            }
+            const double g1 = 2.0 * P.C10;
+            double CSb = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) CSb += C[i][j] * (i == j ? g1 : 0.0);
+            const double r1 = J2d * CSb / nd3;
+            double Sd[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) Sd[i][j] = J2d * (i == j ? g1 : 0.0) - r1 * Ci[i][j];
+            const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {2, 0}};
+#pragma unroll
+            for (int aa = 0; aa < 6; ++aa)
+#pragma unroll
+                for (int bb = 0; bb < 6; ++bb) {
+                    const int i = vi[aa][0], j = vi[aa][1], kk = vi[bb][0], l = vi[bb][1];
+                    const double dy1 = Ci[i][j] * Sd[kk][l] + Sd[i][j] * Ci[kk][l];
+                    const double sym = 0.5 * (Ci[i][kk] * Ci[j][l] + Ci[i][l] * Ci[j][kk]);
+                    const double dyc = Ci[i][j] * Ci[kk][l];
+                    rc[SR_DM + aa * 6 + bb] = (-2.0 / nd3) * dy1 + 2.0 * (r1 - p * Jf) * sym + (pl * Jf - 2.0 * r1 / nd3) * dyc;
+                }
+            double S[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    S[i][j] = Sd[i][j] + p * Jf * Ci[i][j];
+                    rc[SR_S + i * 3 + j] = S[i][j];
+                    rc[SR_F + i * 3 + j] = F[i][j];
+                }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) rc[SR_P + i * 3 + j] = F[i][0] * S[0][j] + F[i][1] * S[1][j] + F[i][2] * S[2][j];
+            rc[SR_W] = rg[0] * J;
+#pragma unroll
+            for (int a = 0; a < 10; ++a) {
+                rc[SR_N + a] = rg[1 + a];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) rc[SR_NX + a * 3 + j] = Nx[a][j];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) rc[SR_UD + i] = ud[i];
+        }
+    }
+    __syncthreads();
+    const double afu = P.af * P.beta * P.dt * P.dt;
+    const double amd = P.am * P.rho;
+    for (int pidx = lane; pidx < EPW * 100; pidx += 64) {
+        const int k = pidx / 100, ab = pidx - (pidx / 100) * 100;
+        const int a = ab / 10, b = ab - (ab / 10) * 10;
+        const int e = e0 + k;
+        if (e >= eEnd) continue;
+        double kk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double r[3] = {0, 0, 0};
+        for (int g = 0; g < NG; ++g) {
+            const double* rc = rec[k * NG + g];
+            const double w = rc[SR_W];
+            const double* Nxa = rc + SR_NX + a * 3;
+            const double* Nxb = rc + SR_NX + b * 3;
+            const double* Sg = rc + SR_S;
+            const double* Fg = rc + SR_F;
+            double NxSNx = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) NxSNx += Nxa[i] * Sg[i * 3 + j] * Nxb[j];
+            const double T1 = amd * rc[SR_N + a] * rc[SR_N + b] + afu * NxSNx;
+            double Ba[6][3], Bb[6][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Ba[0][c] = Nxa[0] * Fg[c * 3 + 0];
+                Ba[1][c] = Nxa[1] * Fg[c * 3 + 1];
+                Ba[2][c] = Nxa[2] * Fg[c * 3 + 2];
+                Ba[3][c] = Nxa[0] * Fg[c * 3 + 1] + Fg[c * 3 + 0] * Nxa[1];
+                Ba[4][c] = Nxa[1] * Fg[c * 3 + 2] + Fg[c * 3 + 1] * Nxa[2];
+                Ba[5][c] = Nxa[2] * Fg[c * 3 + 0] + Fg[c * 3 + 2] * Nxa[0];
+                Bb[0][c] = Nxb[0] * Fg[c * 3 + 0];
+                Bb[1][c] = Nxb[1] * Fg[c * 3 + 1];
+                Bb[2][c] = Nxb[2] * Fg[c * 3 + 2];
+                Bb[3][c] = Nxb[0] * Fg[c * 3 + 1] + Fg[c * 3 + 0] * Nxb[1];
+                Bb[4][c] = Nxb[1] * Fg[c * 3 + 2] + Fg[c * 3 + 1] * Nxb[2];
+                Bb[5][c] = Nxb[2] * Fg[c * 3 + 0] + Fg[c * 3 + 2] * Nxb[0];
+            }
+            double DBm[6][3];
+#pragma unroll
+            for (int rr = 0; rr < 6; ++rr)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double sacc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) sacc += rc[SR_DM + rr * 6 + q] * Bb[q][c];
+                    DBm[rr][c] = sacc;
+                }
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double BmDBm = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) BmDBm += Ba[q][rr] * DBm[q][c];
+                    kk[rr * 3 + c] += w * ((rr == c ? T1 : 0.0) + afu * BmDBm);
+                }
+            if (b == 0) {
+                const double Na = rc[SR_N + a];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    r[i] += w * (Na * rc[SR_UD + i] + Nxa[0] * rc[SR_P + i * 3] + Nxa[1] * rc[SR_P + i * 3 + 1]
+                                 + Nxa[2] * rc[SR_P + i * 3 + 2]);
+            }
+        }
+        double* vb = val + (size_t)emap[(size_t)e * 100 + ab] * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) vb[q] += kk[q];
+        if (b == 0) {
+            double* Rr = R + (size_t)ien[(size_t)e * 10 + a] * 3;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Rr[i] += r[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 int launch_asm_tet4(const Mesh& M, int phys, const AsmParams& P, const Tet4Rule& rule, int tDof,
@@ -564,6 +804,30 @@ int launch_lelas_dkde(const Mesh& M, const AsmParams& P, const Tet4Rule& rule, c
     if (M.nEl == 0) return SVFSI_OK;
     hipLaunchKernelGGL(k_lelas_dkde, dim3((M.nEl + 255) / 256), dim3(256), 0, s, M.nEl, M.ien.as<int>(),
                        M.elemId.as<int>(), M.x.as<double>(), P, rule, lam, u, g);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_asm_struct(const Mesh& M, const AsmParams& P, int nG, const double* rule, int tDof,
+                      const double* Ag, const double* Dg, double* val, double* R, hipStream_t s)
+{
+    const int nC = (int)M.colorOff.size() - 1;
+    for (int c = 0; c < nC; ++c) {
+        const int begin = M.colorOff[c], count = M.colorOff[c + 1] - begin;
+        if (count == 0) continue;
+        if (nG == 4) {
+            constexpr int EPW = 8;
+            hipLaunchKernelGGL((k_asm_struct<4, EPW>), dim3((count + EPW - 1) / EPW), dim3(64), 0, s, begin, count,
+                               M.ien.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, P, rule, val, R);
+        } else if (nG == 15) {
+            constexpr int EPW = 2;
+            hipLaunchKernelGGL((k_asm_struct<15, EPW>), dim3((count + EPW - 1) / EPW), dim3(64), 0, s, begin, count,
+                               M.ien.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, P, rule, val, R);
+        } else {
+            set_error("STRUCT: nG %d not supported", nG);
+            return SVFSI_ERR_NOTSUP;
+        }
+    }
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }

@@ -45,3 +45,14 @@ int launch_lelas_dkde(const Mesh& M, const AsmParams& P, const Tet4Rule& rule, c
                       const double* u, double* g, hipStream_t s);
 
 } // namespace svfsi
+
+namespace svfsi {
+
+// Quadrature table of a TET10 rule (GETGIP/GETGNN, svFSI/NN.f:277-307,
+// 674-715): per Gauss point w, N(10), Nxi(10,3) -> 41 doubles.
+constexpr int kTet10Rec = 41;
+int make_tet10_rule(int quad, std::vector<double>& tab); // returns nG
+int launch_asm_struct(const Mesh& M, const AsmParams& P, int nG, const double* rule, int tDof,
+                      const double* Ag, const double* Dg, double* val, double* R, hipStream_t s);
+
+} // namespace svfsi

@@ -126,6 +126,8 @@ struct Mesh {
         DBuf order;  // colour-sorted face ids
     };
     std::vector<FaceSet*> faceSets;
+    DBuf rule10[2];  // TET10 quadrature tables (15-point, 4-point), built on first use
+    int rule10nG[2] = {0, 0};
     ~Mesh() {
         for (auto* f : faceSets) delete f;
     }

@@ -12,6 +12,8 @@
 #include "internal.hpp"
 #include "devutil.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
@@ -175,33 +177,38 @@ static inline int spmv_grid(int nNo, int dof)
     return (nNo + rowsPerBlock - 1) / rowsPerBlock;
 }
 
-// Event pair recorded around a profiled launch; resolved lazily by
+// Kernel-accurate timing of a profiled launch: the events are attached to
+// the dispatch itself (hipExtLaunchKernelGGL), so they bracket the kernel's
+// execution and not the launch gaps around it.  Resolved lazily by
 // prof_resolve() so that profiling adds no host synchronisation.
 struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
-    hipStream_t s;
     int k;
-    ProfScope(int kernel, hipStream_t st) : s(st), k(kernel) {
+    explicit ProfScope(int kernel) : k(kernel) {
         if (prof().on) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
-            (void)hipEventRecord(a, s);
         }
     }
     ~ProfScope() {
-        if (a) {
-            (void)hipEventRecord(b, s);
-            prof().pending.push_back({a, b, k});
-        }
+        if (a) prof().pending.push_back({a, b, k});
     }
 };
+
+#define SV_LAUNCH(ps, kern, grid, block, shm, st, ...)                                           \
+    do {                                                                                          \
+        if ((ps).a)                                                                               \
+            hipExtLaunchKernelGGL(kern, grid, block, shm, st, (ps).a, (ps).b, 0, __VA_ARGS__);    \
+        else                                                                                      \
+            hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);                          \
+    } while (0)
 
 template <typename VT, typename XT>
 static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU,
                          hipStream_t s, int profKernel, const int* stop)
 {
     if (L.nNo == 0) return SVFSI_OK;
-    ProfScope ps(profKernel, s);
+    ProfScope ps(profKernel);
     const int* rp = L.rowPtr.as<int>();
     const int* cp = L.colPtr.as<int>();
     if constexpr (std::is_same<VT, XT>::value) {
@@ -215,20 +222,20 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
             }();
             const int g4 = (int)(((size_t)L.nNo * 4 * H + 255) / 256);
             if (H == 1)
-                hipLaunchKernelGGL((k_bspmv4<VT, 1>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 1>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
             else if (H == 2)
-                hipLaunchKernelGGL((k_bspmv4<VT, 2>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 2>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
             else
-                hipLaunchKernelGGL((k_bspmv4<VT, 4>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 4>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
             SV_HIP(hipGetLastError());
             return SVFSI_OK;
         }
     }
     const int grid = spmv_grid(L.nNo, dof);
     switch (dof) {
-    case 2: hipLaunchKernelGGL((k_bspmv<2, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
-    case 3: hipLaunchKernelGGL((k_bspmv<3, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
-    case 4: hipLaunchKernelGGL((k_bspmv<4, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
+    case 2: SV_LAUNCH(ps, (k_bspmv<2, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
+    case 3: SV_LAUNCH(ps, (k_bspmv<3, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
+    case 4: SV_LAUNCH(ps, (k_bspmv<4, VT, XT>), dim3(grid), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop); break;
     default: set_error("spmv: dof %d not supported (2..4)", dof); return SVFSI_ERR_NOTSUP;
     }
     SV_HIP(hipGetLastError());

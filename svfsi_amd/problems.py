@@ -123,3 +123,29 @@ def struct_cube(n: int = 4, C10: float = 1e5, Kpen: float = 1e7, rho: float = 1.
         name=f"struct_cube{n}", phys=SVFSI_PHYS_STRUCT, dof=3, mesh=m, eq=eq, Ag=z, Yg=z.copy(), Dg=D,
         dir_faces=[(x0, 3)], quad=quad,
         ls=dict(LS_type=LS_TYPE_GMRES, relTol=1e-8, absTol=1e-30, mItr=100, sD=sD))
+
+
+def random_field(points: np.ndarray, seed: int, sigma: float, ell: float, modes: int = 512) -> np.ndarray:
+    """Seeded Gaussian random field with squared-exponential covariance
+    sigma^2 exp(-|d|^2 / (2 ell^2)), sampled at `points` by random Fourier
+    features: z(x) = sigma sqrt(2/M) sum_m cos(w_m . x + phi_m), w_m ~ N(0, I/ell^2)."""
+    rng = np.random.default_rng(seed)
+    w = rng.normal(size=(modes, points.shape[1])) / ell
+    phi = rng.uniform(0.0, 2.0 * np.pi, modes)
+    z = np.empty(points.shape[0])
+    for i in range(0, points.shape[0], 32768):
+        z[i:i + 32768] = np.cos(points[i:i + 32768] @ w.T + phi).sum(axis=1)
+    return sigma * np.sqrt(2.0 / modes) * z
+
+
+def lelas_adjoint_cube(n: int = 55, seed: int = 1, sigma: float = 0.3, ell: float = 0.1, E0: float = 1e6,
+                       relTol: float = 1e-8) -> Problem:
+    """Linear elasticity on the Kuhn cube with a per-element modulus
+    E_e = E0 exp(z_e), z a seeded Gaussian random field over the element
+    centroids.  Same load and supports as lelas_cube."""
+    p = lelas_cube(n, E=E0)
+    z = random_field(p.mesh.x[p.mesh.ien].mean(axis=1), seed, sigma, ell)
+    p.prop_el = E0 * np.exp(z)
+    p.name = f"lelas_adjoint_cube{n}"
+    p.ls = dict(p.ls, relTol=relTol)
+    return p

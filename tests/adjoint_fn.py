@@ -1,0 +1,25 @@
+"""Test-only torch.autograd.Function over the C ABI: z -> u = K(E0 exp(z))^-1 f.
+
+Forward: device assembly (svfsi_assemble_dev with per-element modulus) and
+the FP64 device solve.  Backward: transposed device solve
+(fsils_gpu_solve_t_dev) and the element dK/dE . u kernel
+(svfsi_lelas_dkde_dot_dev), chained with dE/dz = E."""
+import torch
+
+from svfsi_amd.adjoint import LelasAdjoint
+
+
+class LelasSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, z, adj: LelasAdjoint, E0: float):
+        E = E0 * torch.exp(z)
+        u, _ = adj.forward(E.contiguous())
+        ctx.save_for_backward(E, u)
+        ctx.adj = adj
+        return u
+
+    @staticmethod
+    def backward(ctx, gu):
+        E, u = ctx.saved_tensors
+        gE, _, _ = ctx.adj.grad_E(u, gu)
+        return gE * E, None, None

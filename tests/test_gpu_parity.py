@@ -162,6 +162,23 @@ def test_fluid_assembly_parity(gpu, orc):
     assert relerr(Rg, Ro) < 1e-12
 
 
+@pytest.mark.parametrize("quad", [0, 1])
+def test_struct_assembly_parity(gpu, orc, quad):
+    import torch
+
+    prob = problems.struct_cube(3, quad=quad)
+    prob.Ag = np.random.default_rng(5).normal(size=prob.Ag.shape)
+    say("struct assembly", prob.nNo, prob.mesh.nEl, "quad", quad)
+    st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
+    say("struct relerr", relerr(Vg, Vo), relerr(Rg, Ro))
+    assert relerr(Vg, Vo) < 1e-12
+    assert relerr(Rg, Ro) < 1e-12
+    st.assemble()
+    torch.cuda.synchronize()
+    assert np.array_equal(st.Val.cpu().numpy(), Vg)
+    assert np.array_equal(st.R.cpu().numpy(), Rg)
+
+
 def _solve_pair(prob, orc, mixed=False):
     st = NewtonStep(prob)
     st.assemble()
@@ -204,6 +221,24 @@ def test_gmres_solve_parity(gpu, orc):
     assert np.linalg.norm(r) <= 2e-8 * np.linalg.norm(Rs)
     assert relerr(xg, xo) < 1e-4
     assert abs(ls.RI.iNorm - so.iNorm) <= 1e-12 * so.iNorm
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_struct_solve(gpu, orc, mixed):
+    prob = problems.struct_cube(4, quad=1)
+    say("struct solve", prob.nNo, "mixed", mixed)
+    st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc, mixed=mixed)
+    say("itr gpu", ls.RI.itr, "cpu", so.itr, "x relerr", relerr(xg, xo))
+    assert ls.RI.suc == 1 and so.suc == 1
+    if not mixed:
+        assert abs(ls.RI.itr - so.itr) <= max(3, so.itr // 50), (ls.RI.itr, so.itr)
+        assert abs(ls.RI.iNorm - so.iNorm) <= 1e-12 * so.iNorm
+    mask = prob.dir_mask()
+    _, Rs, W = orc.precond_diag(st.rowPtr - 1, st.colPtr - 1, prob.dof, V0, R0, mask)
+    y = np.where(W != 0, xg / np.where(W != 0, W, 1), 0.0)
+    r = Rs - orc.spmv(st.rowPtr - 1, st.colPtr - 1, prob.dof, Vs, y)
+    assert np.linalg.norm(r) <= 2e-8 * np.linalg.norm(Rs)
+    assert relerr(xg, xo) < 1e-4
 
 
 def test_host_entry_point(gpu, orc):
