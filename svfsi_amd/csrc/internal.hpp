@@ -86,8 +86,29 @@ struct PinnedPoll {
     int wait(int q);
 };
 
+// Instantiated HIP graphs of fixed kernel sequences (one per key), captured
+// on a private stream and replayed on the caller's stream.  `sig` fingerprints
+// every pointer and size baked into the captured launches; a different
+// signature drops all entries.
+struct GraphCache {
+    struct Entry {
+        int key;
+        hipGraphExec_t exec;
+    };
+    std::vector<Entry> e;
+    std::vector<uintptr_t> sig;
+    hipStream_t cap = nullptr;
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache();
+    void clear();
+};
+
 // Krylov workspace, grown on demand and reused across solves.
 struct Work {
+    GraphCache graphs;   // mixed solve: one graph per outer step
+    GraphCache gmGraphs; // FP64 GMRES: one graph per chunk of Arnoldi steps
     PinnedPoll poll;
     DBuf W, X, R0, tmp, basis, basis32, Z32, val32, x32, y32;
     DBuf partial, state, hess;

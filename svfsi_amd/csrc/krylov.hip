@@ -585,6 +585,123 @@ __global__ __launch_bounds__(256) void k_gm_update(size_t n, int i, int sD, BT* 
     }
 }
 
+// Step end of the device-driven inner cycle, fused: every block sums the
+// multi-dot partials (the fixed order of k_gm_givens), forms h(i+1,i) by the
+// Pythagorean rule and updates its rows of u_{i+1}
+// (u_{i+1} - sum_j h(j,i) u_j) / h(i+1,i); block 0 then applies the Givens
+// rotations and the convergence test.  Once block 0 has set the stop flag a
+// block that starts later skips its rows: u_{i+1} is not used after a stop.
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, int nb,
+                                                      const double* __restrict__ partial, BT* __restrict__ basis,
+                                                      GmState* __restrict__ st, GmArrays A)
+{
+    // block 0 may set the stop flag while other blocks start: one read per
+    // block keeps the exit uniform across the block's barriers
+    __shared__ int stop_s;
+    if (threadIdx.x == 0) stop_s = *(volatile int*)&st->stop;
+    __syncthreads();
+    if (stop_s) return;
+    constexpr int E = VecE<BT>::E;
+    __shared__ double hsh[kMaxSD + 2];
+    __shared__ double hc[kMaxSD + 1];
+    __shared__ double inv_s;
+    int npow = 1;
+    while (npow < i + 1) npow <<= 1;
+    int T = 256 / npow;
+    if (T > 64) T = 64;
+    if (T < 1) T = 1;
+    const int per = 256 / T;
+    for (int jj0 = 0; jj0 <= i; jj0 += per) {
+        const int jj = jj0 + threadIdx.x / T;
+        const int b0 = threadIdx.x % T;
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        if (jj <= i) {
+            const double* pp = partial + (size_t)jj * nb;
+            int b = b0;
+            for (; b + 3 * T < nb; b += 4 * T) {
+                t0 += pp[b];
+                t1 += pp[b + T];
+                t2 += pp[b + 2 * T];
+                t3 += pp[b + 3 * T];
+            }
+            for (; b < nb; b += T) t0 += pp[b];
+        }
+        double t = (t0 + t1) + (t2 + t3);
+        for (int o = T >> 1; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+        if (jj <= i && b0 == 0) hsh[jj] = t;
+    }
+    __syncthreads();
+    for (int jj = threadIdx.x; jj < i; jj += 256) hc[jj] = -hsh[jj];
+    if (threadIdx.x == 0) {
+        double hii = hsh[i];
+        for (int jj = 0; jj < i; ++jj) hii = hii - hsh[jj] * hsh[jj];
+        hii = sqrt(fabs(hii));
+        hsh[i] = hii;
+        inv_s = 1.0 / hii;
+    }
+    __syncthreads();
+    const double inv = inv_s;
+    BT* w = basis + (size_t)i * n;
+    for (size_t q = ((size_t)blockIdx.x * 256 + threadIdx.x) * E; q < n; q += (size_t)gridDim.x * 256 * E) {
+        double v[E];
+        loadE(w, q, n, v);
+        int jj = 0;
+        for (; jj + 4 <= i; jj += 4) {
+            double u0[E], u1[E], u2[E], u3[E];
+            loadE(basis + (size_t)jj * n, q, n, u0);
+            loadE(basis + (size_t)(jj + 1) * n, q, n, u1);
+            loadE(basis + (size_t)(jj + 2) * n, q, n, u2);
+            loadE(basis + (size_t)(jj + 3) * n, q, n, u3);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                v[e] = v[e] + hc[jj] * u0[e];
+                v[e] = v[e] + hc[jj + 1] * u1[e];
+                v[e] = v[e] + hc[jj + 2] * u2[e];
+                v[e] = v[e] + hc[jj + 3] * u3[e];
+            }
+        }
+        for (; jj < i; ++jj) {
+            double u[E];
+            loadE(basis + (size_t)jj * n, q, n, u);
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = v[e] + hc[jj] * u[e];
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (q + e < n) w[q + e] = (BT)(v[e] * inv);
+    }
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    st->iters += 1;
+    for (int jj = 1; jj <= i - 1; ++jj) {
+        const double c = A.c[jj - 1], sn = A.sn[jj - 1];
+        const double tmp = c * hsh[jj - 1] + sn * hsh[jj];
+        hsh[jj] = -sn * hsh[jj - 1] + c * hsh[jj];
+        hsh[jj - 1] = tmp;
+    }
+    const double tmp = sqrt(hsh[i - 1] * hsh[i - 1] + hsh[i] * hsh[i]);
+    const double ci = hsh[i - 1] / tmp;
+    const double si = hsh[i] / tmp;
+    hsh[i - 1] = tmp;
+    hsh[i] = 0.0;
+    A.c[i - 1] = ci;
+    A.sn[i - 1] = si;
+    double* Hc = A.H + (size_t)(i - 1) * (sD + 1);
+    for (int jj = 0; jj <= i; ++jj) Hc[jj] = hsh[jj];
+    const double ei = A.err[i - 1];
+    const double eNew = -si * ei;
+    A.err[i] = eNew;
+    A.err[i - 1] = ci * ei;
+    if (fabs(eNew) < st->eps) {
+        st->suc = 1;
+        st->stop = 1;
+        st->iconv = i;
+    } else if (i >= sD) {
+        st->stop = 1;
+        st->iconv = sD;
+    }
+}
+
 // Back substitution of the cycle (GMRES.f:369-376).  The block stages the
 // upper triangle of H in LDS (packed by columns) when it fits, then one
 // thread runs the reference's row-oriented substitution.
@@ -682,6 +799,57 @@ __global__ void k_sum_to(int m, const double* __restrict__ partial, double* __re
     if (threadIdx.x == 0) *out = s;
 }
 
+GraphCache::~GraphCache()
+{
+    clear();
+    if (cap) (void)hipStreamDestroy(cap);
+}
+
+void GraphCache::clear()
+{
+    for (auto& x : e) (void)hipGraphExecDestroy(x.exec);
+    e.clear();
+    sig.clear();
+}
+
+// Replays the graph stored under `key`, capturing `enqueue(stream)` into it
+// first if needed.  Every buffer the captured launches use must be allocated
+// before the capture (no allocation happens while a stream captures).
+template <class F>
+static int graph_run(GraphCache& gc, int key, hipStream_t s, F&& enqueue)
+{
+    for (auto& x : gc.e)
+        if (x.key == key) {
+            SV_HIP(hipGraphLaunch(x.exec, s));
+            return SVFSI_OK;
+        }
+    if (!gc.cap) SV_HIP(hipStreamCreateWithFlags(&gc.cap, hipStreamNonBlocking));
+    SV_HIP(hipStreamBeginCapture(gc.cap, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(gc.cap);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(gc.cap, &g);
+    if (rc != SVFSI_OK || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != SVFSI_OK) return rc;
+        SV_HIP(ec);
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SV_HIP(ei);
+    gc.e.push_back({key, ex});
+    SV_HIP(hipGraphLaunch(ex, s));
+    return SVFSI_OK;
+}
+
+// Graph replay is on unless SVFSI_GRAPHS=0; it is off while SpMV profiling
+// is on (profiled launches carry events that a capture cannot hold).
+static bool graphs_enabled()
+{
+    const char* v = getenv("SVFSI_GRAPHS");
+    return !(v && v[0] == '0') && !prof().on;
+}
+
 // Generic operator: y = A x on the device (FP64 or FP32 matrix, plain or
 // transposed), optionally gated by the device stop flag.
 template <typename BT> struct GmOp;
@@ -698,6 +866,9 @@ template <> struct GmOp<float> {
         return launch_spmv_f32(*L, dof, K, x, y, s, stop);
     }
 };
+
+static bool op_transposed(const GmOp<double>& op) { return op.transpose; }
+static bool op_transposed(const GmOp<float>&) { return false; }
 
 // Restarted GMRES(sD) on n = nNo*dof unknowns.  R: RHS in, solution out
 // (GMRESV overwrites R with X, GMRES.f:383).  Fills suc/itr/iNorm/fNorm/dB.
@@ -755,6 +926,17 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
     SV_TRY(poll.ensure(256));
     GmState hs{};
     const int chunk = 8;
+    const bool useGraphs = graphs_enabled();
+    if (useGraphs) {
+        const std::vector<uintptr_t> sig = {
+            (uintptr_t)n, (uintptr_t)dof, (uintptr_t)sD, (uintptr_t)op.K, (uintptr_t)sizeof(BT),
+            (uintptr_t)op_transposed(op), (uintptr_t)basis, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.hess.p,
+            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)L.trans.p};
+        if (sig != w.gmGraphs.sig) {
+            w.gmGraphs.clear();
+            w.gmGraphs.sig = sig;
+        }
+    }
     for (int l = 1; l <= ls.mItr; ++l) {
         ls.dB = ls.fNorm;
         ls.itr += 1;
@@ -767,14 +949,25 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
         bool stopped = false;
         for (int i0 = 1; i0 <= sD && !stopped; i0 += chunk) {
             const int i1 = (i0 + chunk - 1 < sD) ? i0 + chunk - 1 : sD;
-            for (int i = i0; i <= i1; ++i) {
-                SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, s, stopFlag));
-                hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
-                                   (const BT*)basis, part, (const GmState*)st);
-                hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, sD, (int)nbr, (const double*)part, st, A, 0);
-                hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, s, n, i, sD, basis, (const GmState*)st, A);
-            }
-            SV_HIP(hipGetLastError());
+            // Arnoldi steps i0..i1: the same launches every restart cycle,
+            // replayed as one graph per chunk
+            auto steps = [&, i0, i1](hipStream_t cs) -> int {
+                for (int i = i0; i <= i1; ++i) {
+                    SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, cs, stopFlag));
+                    hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, cs,
+                                       n, i, (const BT*)basis, part, (const GmState*)st);
+                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, sD, (int)nbr, (const double*)part,
+                                       st, A, 0);
+                    hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, cs, n, i, sD, basis,
+                                       (const GmState*)st, A);
+                }
+                SV_HIP(hipGetLastError());
+                return SVFSI_OK;
+            };
+            if (useGraphs)
+                SV_TRY(graph_run(w.gmGraphs, i0, s, steps));
+            else
+                SV_TRY(steps(s));
             const int q = posted & 1;
             SV_TRY(poll.post(q, st, sizeof(GmState), s));
             ++posted;
@@ -887,8 +1080,8 @@ static int gmres_inner_dev(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& bas
         SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, s, &st->stop));
         hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
                            (const BT*)basis, part, (const GmState*)st);
-        hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, sD, (int)nbr, (const double*)part, st, A, 0);
-        hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, s, n, i, sD, basis, (const GmState*)st, A);
+        hipLaunchKernelGGL(k_gm_rotupdate<BT>, dim3((unsigned)nbr), dim3(256), 0, s, n, i, sD, (int)nbr,
+                           (const double*)part, basis, st, A);
     }
     {
         const size_t tri = (size_t)sD * (sD + 1) / 2;
@@ -995,6 +1188,30 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
         return SVFSI_OK;
     }
     SV_TRY(w.state.ensure(256));
+    // inner-cycle buffers (allocated here so that no allocation happens
+    // while a graph is being captured)
+    {
+        const size_t nbi = (n + (size_t)256 * VecE<float>::E - 1) / ((size_t)256 * VecE<float>::E);
+        SV_TRY(w.basis32.ensure((size_t)(mi + 1) * n * sizeof(float)));
+        const size_t pi = ((size_t)(mi + 2) * nbi > gv ? (size_t)(mi + 2) * nbi : gv) + 16;
+        SV_TRY(w.partial.ensure(pi * sizeof(double)));
+        SV_TRY(w.hess.ensure(gm_arrays_doubles(mi) * sizeof(double)));
+    }
+    const bool useGraphs = graphs_enabled();
+    if (useGraphs) {
+        double tinv = tin;
+        uintptr_t tbits;
+        std::memcpy(&tbits, &tinv, sizeof(tbits));
+        const std::vector<uintptr_t> sig = {
+            (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)K32,
+            (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
+            (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
+            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p};
+        if (sig != w.graphs.sig) {
+            w.graphs.clear();
+            w.graphs.sig = sig;
+        }
+    }
     hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, st, eps);
     hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, w.state.as<GmState>(), 0.0);
     PinnedPoll& poll = w.poll;
@@ -1010,21 +1227,33 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
         outerMv += 1;
         int posted = 0;
         for (int i = 1; i <= mo; ++i) {
-            float* zi = Z + (size_t)(i - 1) * n;
-            hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, s, n, (const double*)(Q + (size_t)(i - 1) * n), zi);
-            SV_TRY(gmres_inner_dev<float>(L, op32, dof, zi, w.basis32, mi, tin, &st->stop, s));
-            hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, s, n, (const float*)zi, t64);
-            SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, s, &st->stop));
+            // one outer step: inner FP32 cycle on z_i = M^-1 q_i, then the
+            // FP64 product and the CGS2 Arnoldi step; the same launches
+            // every cycle, so they replay as one graph per i
+            auto step = [&, i](hipStream_t cs) -> int {
+                float* zi = Z + (size_t)(i - 1) * n;
+                hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, cs, n,
+                                   (const double*)(Q + (size_t)(i - 1) * n), zi);
+                SV_TRY(gmres_inner_dev<float>(L, op32, dof, zi, w.basis32, mi, tin, &st->stop, cs));
+                hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, cs, n, (const float*)zi, t64);
+                SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, cs, &st->stop));
+                // outer Arnoldi: CGS2 (two classical passes)
+                for (int pass = 1; pass <= 2; ++pass) {
+                    hipLaunchKernelGGL(k_gm_mdot<double>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0,
+                                       cs, n, i, (const double*)Q, part, (const GmState*)st);
+                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, mo, (int)nbr, (const double*)part,
+                                       st, A, pass);
+                    hipLaunchKernelGGL(k_gm_update<double>, dim3(gv), dim3(256), 0, cs, n, i, mo, Q,
+                                       (const GmState*)st, A);
+                }
+                SV_HIP(hipGetLastError());
+                return SVFSI_OK;
+            };
+            if (useGraphs)
+                SV_TRY(graph_run(w.graphs, i, s, step));
+            else
+                SV_TRY(step(s));
             outerMv += 1;
-            // outer Arnoldi: CGS2 (two classical passes)
-            for (int pass = 1; pass <= 2; ++pass) {
-                hipLaunchKernelGGL(k_gm_mdot<double>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n,
-                                   i, (const double*)Q, part, (const GmState*)st);
-                hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, s, i, mo, (int)nbr, (const double*)part, st, A,
-                                   pass);
-                hipLaunchKernelGGL(k_gm_update<double>, dim3(gv), dim3(256), 0, s, n, i, mo, Q, (const GmState*)st, A);
-            }
-            SV_HIP(hipGetLastError());
             // lagged poll: the state of step i-1 is read while step i runs
             const int q = posted & 1;
             SV_TRY(poll.post(q, st, sizeof(GmState), s));

@@ -142,7 +142,7 @@ def test_adjoint_fd_1m_elements(gpu):
 
     from adjoint_fn import LelasSolve
 
-    prob, adj, z, u_obs = _setup(55, 1e-10)
+    prob, adj, z, u_obs = _setup(55, 1e-12)
     print("nEl", prob.mesh.nEl, "nNo", prob.nNo, flush=True)
     zr = z.clone().requires_grad_(True)
     u = LelasSolve.apply(zr, adj, E0)

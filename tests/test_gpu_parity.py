@@ -272,6 +272,32 @@ def test_mixed_reaches_fp64_tolerance(gpu, orc):
     assert relerr(xg, xo) < 1e-4
 
 
+@pytest.mark.parametrize("mixed", [True, False])
+def test_graph_replays_match_direct_launches(gpu, monkeypatch, mixed):
+    """The GMRES solves replay fixed kernel sequences as HIP graphs (one per
+    outer step of the mixed solve, one per chunk of Arnoldi steps of the FP64
+    solve).  Several replays, with eager work and synchronisation between
+    them, must reproduce the directly launched solve bit for bit."""
+    import torch
+
+    prob = problems.fluid_cylinder(h=0.5)
+    st = NewtonStep(prob)
+    monkeypatch.setenv("SVFSI_GRAPHS", "0")
+    st.assemble()
+    ls0 = st.solve(mixed=mixed)
+    torch.cuda.synchronize()
+    x0 = st.R.cpu().numpy().copy()
+    monkeypatch.setenv("SVFSI_GRAPHS", "1")
+    for rep in range(3):
+        st.assemble()
+        ls = st.solve(mixed=mixed)
+        torch.cuda.synchronize()
+        say("graph replay", mixed, rep, ls.RI.itr, ls.RI.cM)
+        assert (ls.RI.itr, ls.RI.cM, ls.RI.suc) == (ls0.RI.itr, ls0.RI.cM, ls0.RI.suc)
+        assert np.array_equal(st.R.cpu().numpy(), x0)
+        assert float((torch.arange(4096, device=gpu, dtype=torch.float64) * 0.5).sum().item()) == 4096 * 4095 / 4
+
+
 def test_lelas_dkde(gpu, orc):
     import torch
 
