@@ -255,13 +255,31 @@ int svfsi_lelas_dkde_dot_dev(void* mesh, const svfsi_eq_params* p,
                              const double* dlam, const double* du,
                              double* dg, void* stream);
 
-/* Transposed solve K^T lam = rhs with the same scaling and Dirichlet mask as
- * fsils_gpu_solve_dev (W K^T W, GMRES or CG by ls->LS_type).  dVal is the
- * UNSCALED assembled matrix and is not modified; drhs(dof,nNo) in,
- * dlam(dof,nNo) out. */
+/* Element sensitivity of the FLUID (VMS, tet4) tangent and residual to the
+ * per-element viscosity: dg[e] = lam_e^T (dR_e/dmu_e - dK_e/dmu_e du_e),
+ * the element term of dL/dmu_e when K du = R and K^T lam = dL/du.  Exact
+ * derivative (forward-mode dual numbers through the element routine of
+ * svfsi_assemble_dev, including the mu dependence of tau_M, tau_C, tau_B).
+ * dAg, dYg: the state of the assembly (tDof,nNo); dmu_el: per-element
+ * viscosity or NULL (uniform p->mu); dlam, ddu: (4,nNo); dg: (nEl). */
+int svfsi_fluid_dmu_dot_dev(void* mesh, const svfsi_eq_params* p, int tDof,
+                            const double* dAg, const double* dYg,
+                            const double* dmu_el, const double* dlam,
+                            const double* ddu, double* dg, void* stream);
+
+/* Transposed solve K^T lam = rhs with the scaling and Dirichlet mask of
+ * fsils_gpu_solve_dev: the values of K^T are formed on the (structurally
+ * symmetric) pattern, whose diagonal and hence W are those of K, and the
+ * system (W K W)^T y = W rhs is solved like a forward one (GMRES or CG by
+ * ls->LS_type; the _mixed_ variant uses the mixed-precision solver of
+ * fsils_gpu_solve_mixed_dev).  dVal is the UNSCALED assembled matrix and is
+ * not modified; drhs(dof,nNo) in, dlam(dof,nNo) out. */
 int fsils_gpu_solve_t_dev(void* lhs, FSILS_lsType* ls, int dof,
                           const double* dVal, const double* drhs,
                           double* dlam, const int* incL, void* stream);
+int fsils_gpu_solve_t_mixed_dev(void* lhs, FSILS_lsType* ls, int dof,
+                                const double* dVal, const double* drhs,
+                                double* dlam, const int* incL, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Profiling hook (HIP events around the SpMV launches of the solvers)     */

@@ -52,7 +52,8 @@ _lib = None
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        if not SO.exists() or SO.stat().st_mtime < SRC.stat().st_mtime:
+        srcs = [SRC, HERE / "svfsi_oracle.h", HERE / "fluid_elem.inc"]
+        if not SO.exists() or SO.stat().st_mtime < max(p.stat().st_mtime for p in srcs):
             build()
         _lib = C.CDLL(str(SO))
         d = C.c_double
@@ -195,6 +196,18 @@ def fsils_solve(rowPtr, colPtr, dof, mask, settings: dict, R, Val):
     if rc != 0:
         raise RuntimeError(f"orc_fsils_solve rc={rc}")
     return x, V, s
+
+
+def fluid_dmu_dot(prob, lam, du) -> np.ndarray:
+    """g[e] = lam_e . (dR_e/dmu_e - dK_e/dmu_e du_e) (complex step)."""
+    m = prob.mesh
+    g = np.zeros(m.nEl)
+    eq = eq_struct(prob.eq)
+    mu = None if prob.prop_el is None else P(f64(prob.prop_el))
+    Ag, Yg = f64(prob.Ag), f64(prob.Yg)
+    lib().orc_fluid_dmu_dot(m.nEl, P(i32(m.ien)), P(f64(m.x)), P(Ag), P(Yg), Ag.shape[1], C.byref(eq), mu,
+                            P(f64(lam)), P(f64(du)), P(g))
+    return g
 
 
 def lelas_dkde_dot(prob, lam, u) -> np.ndarray:

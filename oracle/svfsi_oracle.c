@@ -15,6 +15,7 @@
 #include <string.h>
 #include <float.h>
 #include <time.h>
+#include <complex.h>
 
 #define EPS DBL_EPSILON
 
@@ -345,172 +346,17 @@ void orc_construct_lelas(int nNo, int nEl, const int* IEN, const double* x,
 /* FLUID.f                                                                */
 /* ===================================================================== */
 
-/* Kinematics shared by FLUID3D_M and FLUID3D_C (FLUID.f:228-412 and
- * 847-1000), Newtonian viscosity (GETVISCOSITY viscType_Const,
- * FLUID.f:1700-1703: mu_x = mu_g = 0). */
-typedef struct {
-    double ud[3], u[3], ux[3][3], d2u2[3], divU, p, px[3];
-    double es[3][3], tauM, up[3];
-} fl_kin;
-
-static void fluid_kin(int eNoN, const double ks[3][3], const double* N,
-                      const double (*Nx)[3], const double (*Nxx)[6],
-                      const double* al, const double* yl, int tDof,
-                      const orc_eq* eq, double mu, fl_kin* k)
-{
-    const double ctM = 1.0, ctC = 36.0, rho = eq->rho;
-    memset(k, 0, sizeof(*k));
-    for (int i = 0; i < 3; ++i) k->ud[i] = -eq->fb[i];
-    double uxx[3][3][3] = {{{0}}}; /* uxx[i][c][j] = d2 u_c / dx_i dx_j */
-    for (int a = 0; a < eNoN; ++a) {
-        const double* A = al + a * tDof;
-        const double* Y = yl + a * tDof;
-        for (int c = 0; c < 3; ++c) {
-            k->ud[c] += N[a] * A[c];
-            k->u[c] += N[a] * Y[c];
-            for (int i = 0; i < 3; ++i) k->ux[i][c] += Nx[a][i] * Y[c];
-            uxx[0][c][0] += Nxx[a][0] * Y[c];
-            uxx[1][c][1] += Nxx[a][1] * Y[c];
-            uxx[2][c][2] += Nxx[a][2] * Y[c];
-            uxx[1][c][0] += Nxx[a][3] * Y[c];
-            uxx[2][c][1] += Nxx[a][4] * Y[c];
-            uxx[0][c][2] += Nxx[a][5] * Y[c];
-        }
-    }
-    for (int c = 0; c < 3; ++c) {
-        uxx[0][c][1] = uxx[1][c][0];
-        uxx[1][c][2] = uxx[2][c][1];
-        uxx[2][c][0] = uxx[0][c][2];
-        k->d2u2[c] = uxx[0][c][0] + uxx[1][c][1] + uxx[2][c][2];
-    }
-    k->divU = k->ux[0][0] + k->ux[1][1] + k->ux[2][2];
-    for (int a = 0; a < eNoN; ++a) {
-        const double* Y = yl + a * tDof;
-        k->p += N[a] * Y[3];
-        for (int i = 0; i < 3; ++i) k->px[i] += Nx[a][i] * Y[3];
-    }
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) k->es[i][j] = k->ux[i][j] + k->ux[j][i];
-
-    double kT = 4.0 * (ctM / eq->dt) * (ctM / eq->dt);
-    double kU = 0.0, kS = 0.0;
-    for (int j = 0; j < 3; ++j)
-        for (int i = 0; i < 3; ++i) {
-            kU += k->u[i] * k->u[j] * ks[i][j];
-            kS += ks[i][j] * ks[i][j];
-        }
-    kS = ctC * kS * (mu / rho) * (mu / rho);
-    k->tauM = 1.0 / (rho * sqrt(kT + kU + kS));
-    for (int c = 0; c < 3; ++c) {
-        double rV = k->ud[c] + k->u[0] * k->ux[0][c] + k->u[1] * k->ux[1][c] + k->u[2] * k->ux[2][c];
-        double rS = mu * k->d2u2[c];
-        k->up[c] = -k->tauM * (rho * rV + k->px[c] - rS);
-    }
-}
-
-/* FLUID3D_M (svFSI/FLUID.f:192-560), vmsFlag = .TRUE. (nFs = 1). */
-void orc_fluid3d_m(int eNoN, double w, const double ks[3][3], const double* N,
-                   const double (*Nx)[3], const double (*Nxx)[6],
-                   const double* al, const double* yl, int tDof,
-                   const orc_eq* eq, double mu, double* lR, double* lK)
-{
-    const int dof = 4, dd = 16;
-    const double rho = eq->rho;
-    double T1c = eq->af * eq->gam * eq->dt;
-    double amd = eq->am / T1c, wl = w * T1c, wr = w * rho;
-    fl_kin k;
-    fluid_kin(eNoN, ks, N, Nx, Nxx, al, yl, tDof, eq, mu, &k);
-    double tauM = k.tauM;
-    double tauC = 1.0 / (tauM * (ks[0][0] + ks[1][1] + ks[2][2]));
-    double tauB = 0.0;
-    for (int j = 0; j < 3; ++j)
-        for (int i = 0; i < 3; ++i) tauB += k.up[i] * k.up[j] * ks[i][j];
-    if (orc_iszero(tauB)) tauB = EPS;
-    tauB = rho / sqrt(tauB);
-    double ua[3], pa = k.p - tauC * k.divU;
-    for (int c = 0; c < 3; ++c) ua[c] = k.u[c] + k.up[c];
-    double rV[3], rM[3][3];
-    for (int c = 0; c < 3; ++c)
-        rV[c] = tauB * (k.up[0] * k.ux[0][c] + k.up[1] * k.ux[1][c] + k.up[2] * k.ux[2][c]);
-    for (int c = 0; c < 3; ++c)
-        for (int i = 0; i < 3; ++i)
-            rM[i][c] = mu * k.es[i][c] - rho * k.up[c] * ua[i] + rV[c] * k.up[i] - (i == c ? pa : 0.0);
-    for (int c = 0; c < 3; ++c)
-        rV[c] = k.ud[c] + ua[0] * k.ux[0][c] + ua[1] * k.ux[1][c] + ua[2] * k.ux[2][c];
-
-    double uNx[32], upNx[32], uaNx[32], T1a[32];
-    for (int a = 0; a < eNoN; ++a) {
-        for (int c = 0; c < 3; ++c)
-            lR[a * dof + c] += wr * N[a] * rV[c]
-                + w * (Nx[a][0] * rM[0][c] + Nx[a][1] * rM[1][c] + Nx[a][2] * rM[2][c]);
-        uNx[a] = k.u[0] * Nx[a][0] + k.u[1] * Nx[a][1] + k.u[2] * Nx[a][2];
-        upNx[a] = k.up[0] * Nx[a][0] + k.up[1] * Nx[a][1] + k.up[2] * Nx[a][2];
-        uaNx[a] = uNx[a] + upNx[a];
-        /* updu(i,c,a) = delta_ic * T1a (mu_x = mu_g = 0) */
-        T1a[a] = -rho * uNx[a] + mu * (Nxx[a][0] + Nxx[a][1] + Nxx[a][2]);
-    }
-    for (int b = 0; b < eNoN; ++b)
-        for (int a = 0; a < eNoN; ++a) {
-            double* kk = lK + ((size_t)b * eNoN + a) * dd;
-            double NxNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
-            double T1 = mu * NxNx + rho * amd * N[b] * (N[a] + rho * tauM * uaNx[a])
-                      + rho * N[a] * (uNx[b] + upNx[b]) + tauB * upNx[a] * upNx[b];
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) {
-                    double updu = (r == c) ? T1a[b] : 0.0;
-                    double T2 = mu * Nx[a][c] * Nx[b][r] + tauC * Nx[a][r] * Nx[b][c]
-                              - rho * tauM * uaNx[a] * updu;
-                    kk[r * dof + c] += wl * (r == c ? T2 + T1 : T2);
-                }
-        }
-    for (int b = 0; b < eNoN; ++b)
-        for (int a = 0; a < eNoN; ++a) {
-            double* kk = lK + ((size_t)b * eNoN + a) * dd;
-            double T1 = rho * tauM * uaNx[a];
-            for (int r = 0; r < 3; ++r)
-                kk[r * dof + 3] -= wl * (Nx[a][r] * N[b] - Nx[b][r] * T1);
-        }
-}
-
-/* FLUID3D_C (svFSI/FLUID.f:813-1084), vmsFlag = .TRUE. */
-void orc_fluid3d_c(int eNoN, double w, const double ks[3][3], const double* N,
-                   const double (*Nx)[3], const double (*Nxx)[6],
-                   const double* al, const double* yl, int tDof,
-                   const orc_eq* eq, double mu, double* lR, double* lK)
-{
-    const int dof = 4, dd = 16;
-    const double rho = eq->rho;
-    double T1c = eq->af * eq->gam * eq->dt;
-    double amd = eq->am / T1c, wl = w * T1c;
-    fl_kin k;
-    fluid_kin(eNoN, ks, N, Nx, Nxx, al, yl, tDof, eq, mu, &k);
-    double tauM = k.tauM;
-    double T1a[32];
-    for (int a = 0; a < eNoN; ++a) {
-        double uNx = k.u[0] * Nx[a][0] + k.u[1] * Nx[a][1] + k.u[2] * Nx[a][2];
-        T1a[a] = -rho * uNx + mu * (Nxx[a][0] + Nxx[a][1] + Nxx[a][2]);
-    }
-    for (int a = 0; a < eNoN; ++a) {
-        double upNx = k.up[0] * Nx[a][0] + k.up[1] * Nx[a][1] + k.up[2] * Nx[a][2];
-        lR[a * dof + 3] += w * (N[a] * k.divU - upNx);
-    }
-    for (int b = 0; b < eNoN; ++b) {
-        double T1 = rho * amd * N[b];
-        for (int a = 0; a < eNoN; ++a) {
-            double* kk = lK + ((size_t)b * eNoN + a) * dd;
-            for (int c = 0; c < 3; ++c) {
-                /* T2 = sum_i Nqx(i,a) (updu(c,i,b) - delta_ic T1) */
-                double T2 = Nx[a][c] * (T1a[b] - T1);
-                kk[12 + c] += wl * (N[a] * Nx[b][c] - tauM * T2);
-            }
-        }
-    }
-    for (int b = 0; b < eNoN; ++b)
-        for (int a = 0; a < eNoN; ++a) {
-            double NxNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
-            lK[((size_t)b * eNoN + a) * dd + 15] += wl * tauM * NxNx;
-        }
-}
+#define FREAL double
+#define FSQRT sqrt
+#define FRE(x) (x)
+#define FN(n) n
+#define FSTATIC
+#include "fluid_elem.inc"
+#undef FREAL
+#undef FSQRT
+#undef FRE
+#undef FN
+#undef FSTATIC
 
 /* CONSTRUCT_FLUID (svFSI/FLUID.f:40-190), tet4 VMS (nFs = 1): both
  * function spaces are the mesh tet4 basis (GETTHOODFS, FS.f:257-286);
@@ -554,6 +400,73 @@ void orc_construct_fluid(int nNo, int nEl, const int* IEN, const double* x,
                           (const double (*)[3])Nx, (const double (*)[6])Nxx,
                           al, yl, 4, eq, mu, lR, lK);
         orc_doassem(eNoN, ptr, dof, lK, lR, rowPtr, colPtr, Val, R);
+    }
+}
+
+/* Complex-step instantiation of FLUID3D_M/C: with mu = mu_e + i h the
+ * imaginary parts of lK, lR are h dK/dmu, h dR/dmu to rounding (no
+ * subtractive cancellation), exact through tau_M, tau_C and tau_B. */
+#define FREAL double complex
+#define FSQRT csqrt
+#define FRE(x) creal(x)
+#define FN(n) n##_cs
+#define FSTATIC static
+#include "fluid_elem.inc"
+#undef FREAL
+#undef FSQRT
+#undef FRE
+#undef FN
+#undef FSTATIC
+
+/* Viscosity sensitivity of one linearised step (no reference counterpart):
+ * g[e] = lam_e . (dR_e/dmu_e - dK_e/dmu_e du_e), the element term of
+ * dL/dmu_e when K du = R and K^T lam = dL/du; element loop of
+ * orc_construct_fluid with the derivative by complex step. */
+void orc_fluid_dmu_dot(int nEl, const int* IEN, const double* x, const double* Ag,
+                       const double* Yg, int tDof, const orc_eq* eq, const double* mu_el,
+                       const double* lam, const double* du, double* g)
+{
+    const int eNoN = 4, dof = 4;
+    double wg[4], xi[4][3], Ng[4][4], Nxi[4][3];
+    orc_rule_tet4(wg, xi);
+    for (int q = 0; q < 4; ++q) orc_shape_tet4(xi[q], Ng[q], Nxi);
+    double Nxx[4][6];
+    memset(Nxx, 0, sizeof(Nxx));
+    double xl[4][3], al[16], yl[16], Nx[4][3], ks[3][3], Jac = 0.0;
+    double complex lR[16], lK[16 * 16];
+    int ptr[4];
+    for (int e = 0; e < nEl; ++e) {
+        for (int a = 0; a < eNoN; ++a) {
+            int Ac = IEN[(size_t)e * eNoN + a];
+            ptr[a] = Ac;
+            for (int i = 0; i < 3; ++i) xl[a][i] = x[(size_t)Ac * 3 + i];
+            for (int i = 0; i < 4; ++i) {
+                al[a * 4 + i] = Ag[(size_t)Ac * tDof + i];
+                yl[a * 4 + i] = Yg[(size_t)Ac * tDof + i];
+            }
+        }
+        for (int q = 0; q < 16; ++q) lR[q] = 0.0;
+        for (int q = 0; q < 256; ++q) lK[q] = 0.0;
+        const double mu0 = mu_el ? mu_el[e] : eq->mu;
+        const double h = 1e-30 * (mu0 != 0.0 ? fabs(mu0) : 1.0);
+        const double complex mu = mu0 + h * I;
+        orc_gnn(eNoN, (const double (*)[3])Nxi, (const double (*)[3])xl, Nx, &Jac, ks);
+        for (int q = 0; q < 4; ++q)
+            orc_fluid3d_m_cs(eNoN, wg[q] * Jac, (const double (*)[3])ks, Ng[q], (const double (*)[3])Nx,
+                             (const double (*)[6])Nxx, al, yl, 4, eq, mu, lR, lK);
+        for (int q = 0; q < 4; ++q)
+            orc_fluid3d_c_cs(eNoN, wg[q] * Jac, (const double (*)[3])ks, Ng[q], (const double (*)[3])Nx,
+                             (const double (*)[6])Nxx, al, yl, 4, eq, mu, lR, lK);
+        double s = 0.0;
+        for (int a = 0; a < eNoN; ++a)
+            for (int r = 0; r < dof; ++r) {
+                double t = cimag(lR[a * dof + r]) / h;
+                for (int b = 0; b < eNoN; ++b)
+                    for (int c = 0; c < dof; ++c)
+                        t -= cimag(lK[((size_t)b * eNoN + a) * 16 + r * dof + c]) / h * du[(size_t)ptr[b] * dof + c];
+                s += lam[(size_t)ptr[a] * dof + r] * t;
+            }
+        g[e] = s;
     }
 }
 

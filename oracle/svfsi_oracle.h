@@ -103,6 +103,11 @@ void orc_construct_fluid(int nNo, int nEl, const int* IEN, const double* x,
                          const orc_eq* eq, const double* mu_el,
                          const int* rowPtr, const int* colPtr,
                          double* Val, double* R);
+/* Viscosity sensitivity (no reference counterpart): g[e] = lam_e .
+ * (dR_e/dmu_e - dK_e/dmu_e du_e), derivative by complex step. */
+void orc_fluid_dmu_dot(int nEl, const int* IEN, const double* x, const double* Ag,
+                       const double* Yg, int tDof, const orc_eq* eq, const double* mu_el,
+                       const double* lam, const double* du, double* g);
 void orc_construct_struct(int nNo, int nEl, int eNoN, const int* IEN,
                           const double* x, const double* Ag, const double* Yg,
                           const double* Dg, int tDof, const orc_eq* eq,

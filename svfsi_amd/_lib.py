@@ -81,6 +81,7 @@ SIGNATURES = {
     "fsils_gpu_solve_dev": [_P, C.POINTER(FSILS_lsType), _I, _P, _P, _I, _IP, _DP, _P],
     "fsils_gpu_solve_mixed_dev": [_P, C.POINTER(FSILS_lsType), _I, _P, _P, _I, _IP, _DP, _P],
     "fsils_gpu_solve_t_dev": [_P, C.POINTER(FSILS_lsType), _I, _P, _P, _P, _IP, _P],
+    "fsils_gpu_solve_t_mixed_dev": [_P, C.POINTER(FSILS_lsType), _I, _P, _P, _P, _IP, _P],
     "svfsi_spmv_dev": [_P, _I, _P, _P, _P, _P],
     "svfsi_spmv_f32_dev": [_P, _I, _P, _P, _P, _P],
     "svfsi_spmv_t_dev": [_P, _I, _P, _P, _P, _P],
@@ -96,6 +97,7 @@ SIGNATURES = {
     "svfsi_assemble_traction_dev": [_P, _I, _I, _DP, _P, _P],
     "svfsi_assemble_backflow_dev": [_P, _I, C.POINTER(EqParams), _I, _P, _P, _P, _P],
     "svfsi_lelas_dkde_dot_dev": [_P, C.POINTER(EqParams), _P, _P, _P, _P],
+    "svfsi_fluid_dmu_dot_dev": [_P, C.POINTER(EqParams), _I, _P, _P, _P, _P, _P, _P, _P],
     "svfsi_prof_enable": [_I],
     "svfsi_prof_get": [_I, C.POINTER(C.c_longlong), _DP],
     "svfsi_prof_reset": [],

@@ -81,6 +81,12 @@ class DeviceMesh:
         call("svfsi_lelas_dkde_dot_dev", self.handle, C.byref(p), dev_ptr(lam), dev_ptr(u), dev_ptr(g),
              stream or cur_stream())
 
+    def fluid_dmu_dot(self, eq: dict, Ag, Yg, mu_el, lam, du, g, stream=None) -> None:
+        """g[e] = lam_e . (dR_e/dmu_e - dK_e/dmu_e du_e) (exact element derivative)."""
+        p = eq_params(eq)
+        call("svfsi_fluid_dmu_dot_dev", self.handle, C.byref(p), Yg.shape[1], dev_ptr(Ag), dev_ptr(Yg),
+             _opt(mu_el), dev_ptr(lam), dev_ptr(du), dev_ptr(g), stream or cur_stream())
+
     def free(self) -> None:
         if self.handle:
             call("svfsi_mesh_free", self.handle)

@@ -471,8 +471,8 @@ int fsils_gpu_solve(void* lhs, FSILS_lsType* ls, int dof, double* Ri, double* Va
     return SVFSI_OK;
 }
 
-int fsils_gpu_solve_t_dev(void* lhs, FSILS_lsType* ls, int dof, const double* dVal, const double* drhs,
-                          double* dlam, const int* incL, void* stream)
+static int solve_t_impl(void* lhs, FSILS_lsType* ls, int dof, const double* dVal, const double* drhs,
+                        double* dlam, const int* incL, void* stream, int mode)
 {
     SV_ARG(lhs && ls && dVal && drhs && dlam, "NULL argument");
     SV_TRY(require_device());
@@ -482,25 +482,24 @@ int fsils_gpu_solve_t_dev(void* lhs, FSILS_lsType* ls, int dof, const double* dV
     SV_ARG(L.symmetric, "pattern is not structurally symmetric: no transposed solve");
     const size_t n = (size_t)L.nNo * dof, nV = (size_t)L.nnz * dof * dof;
     Work& w = L.ws;
-    DBuf vcopy;
-    SV_TRY(vcopy.ensure(nV * sizeof(double)));
-    SV_TRY(w.W.ensure(n * sizeof(double)));
-    SV_HIP(hipMemcpyAsync(vcopy.p, dVal, nV * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SV_TRY(w.valT.ensure(nV * sizeof(double)));
+    // K^T on the same pattern; its diagonal (hence W and the Dirichlet
+    // mask) is that of K, so the scaled system is (W K W)^T
+    SV_TRY(launch_transpose_blocks(L, dof, dVal, w.valT.as<double>(), s));
     SV_HIP(hipMemcpyAsync(dlam, drhs, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    // W from diag(K) = diag(K^T): the same scaling as the forward solve, so
-    // (W K W)^T = W K^T W is solved with the transposed product.
-    SV_TRY(precond_diag(L, dof, vcopy.as<double>(), dlam, w.W.as<double>(), nullptr, incL, s));
-    int rc;
-    if (ls->LS_type == LS_TYPE_GMRES) rc = gmres_solve(L, ls->RI, dof, vcopy.as<double>(), dlam, s, true);
-    else if (ls->LS_type == LS_TYPE_CG) rc = cg_solve(L, ls->RI, dof, vcopy.as<double>(), dlam, s);
-    else {
-        set_error("LS_type %d not supported", ls->LS_type);
-        return SVFSI_ERR_NOTSUP;
-    }
-    if (rc) return rc;
-    SV_TRY(launch_scale_vec(n, w.W.as<double>(), dlam, s));
-    SV_HIP(hipStreamSynchronize(s));
-    return SVFSI_OK;
+    return solve_impl(L, ls, dof, dlam, w.valT.as<double>(), PRECOND_FSILS, incL, nullptr, s, mode);
+}
+
+int fsils_gpu_solve_t_dev(void* lhs, FSILS_lsType* ls, int dof, const double* dVal, const double* drhs,
+                          double* dlam, const int* incL, void* stream)
+{
+    return solve_t_impl(lhs, ls, dof, dVal, drhs, dlam, incL, stream, 0);
+}
+
+int fsils_gpu_solve_t_mixed_dev(void* lhs, FSILS_lsType* ls, int dof, const double* dVal, const double* drhs,
+                                double* dlam, const int* incL, void* stream)
+{
+    return solve_t_impl(lhs, ls, dof, dVal, drhs, dlam, incL, stream, 1);
 }
 
 int svfsi_spmv_dev(void* lhs, int dof, const double* dK, const double* dU, double* dKU, void* stream)
@@ -734,6 +733,18 @@ int svfsi_lelas_dkde_dot_dev(void* mesh, const svfsi_eq_params* p, const double*
     Mesh& M = *static_cast<Mesh*>(mesh);
     SV_ARG(M.eNoN == 4, "tet4 only");
     return launch_lelas_dkde(M, to_asm(*p), make_tet4_rule(), dlam, du, dg, static_cast<hipStream_t>(stream));
+}
+
+int svfsi_fluid_dmu_dot_dev(void* mesh, const svfsi_eq_params* p, int tDof, const double* dAg, const double* dYg,
+                            const double* dmu_el, const double* dlam, const double* ddu, double* dg, void* stream)
+{
+    SV_ARG(mesh && p && dAg && dYg && dlam && ddu && dg, "NULL argument");
+    SV_TRY(require_device());
+    Mesh& M = *static_cast<Mesh*>(mesh);
+    SV_ARG(M.eNoN == 4, "tet4 only");
+    SV_ARG(tDof >= 4, "FLUID needs tDof >= 4");
+    return launch_fluid_dmu(M, to_asm(*p), make_tet4_rule(), tDof, dAg, dYg, dmu_el, dlam, ddu, dg,
+                            static_cast<hipStream_t>(stream));
 }
 
 int svfsi_lhsa(int nNo, int nEl, int eNoN, const int* IEN, int* nnz, int* rowPtr, int* colPtr)

@@ -173,6 +173,223 @@ __device__ __forceinline__ bool iszero_d(double a)
     return (aa / nrm) < 10.0 * eps;
 }
 
+// Forward-mode dual number: value and derivative with respect to one
+// parameter.  The fluid element routine below is written once over a
+// scalar type T; T = double assembles, T = Dual with mu = (mu_e, 1) gives the
+// exact derivative of the element tangent and residual with respect to the
+// element viscosity.
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ Dual mkd(double v) { return {v, 0.0}; }
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+__device__ __forceinline__ Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+__device__ __forceinline__ Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b)
+{
+    const double q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+__device__ __forceinline__ Dual operator/(double a, Dual b)
+{
+    const double q = a / b.v;
+    return {q, -q * b.d / b.v};
+}
+__device__ __forceinline__ Dual dsqrt(Dual a)
+{
+    const double r = sqrt(a.v);
+    return {r, 0.5 * a.d / r};
+}
+__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ double val_of(double a) { return a; }
+__device__ __forceinline__ double val_of(Dual a) { return a.v; }
+template <typename T> __device__ __forceinline__ T scal(double v);
+template <> __device__ __forceinline__ double scal<double>(double v) { return v; }
+template <> __device__ __forceinline__ Dual scal<Dual>(double v) { return mkd(v); }
+
+// Element data of one tet4 shared by the 16 (a,b) threads of an element.
+struct FluidElem {
+    int nd[4];
+    double Nx[4][3], ks[3][3], Jac;
+    double yl[4][4], al[4][3];
+    double ux[3][3], px[3], divU;
+};
+
+__device__ __forceinline__ void fluid_elem(int p, const int* __restrict__ ien, const double* __restrict__ x,
+                                           int tDof, const double* __restrict__ Ag,
+                                           const double* __restrict__ Yg, FluidElem& E)
+{
+    double xl[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        E.nd[q] = ien[(size_t)p * 4 + q];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) xl[q][i] = x[(size_t)E.nd[q] * 3 + i];
+    }
+    gnn_tet4(xl, E.Nx, E.Jac, E.ks);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) E.yl[q][c] = Yg[(size_t)E.nd[q] * tDof + c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) E.al[q][c] = Ag[(size_t)E.nd[q] * tDof + c];
+    }
+    // element-constant kinematics: ux(i,c) = sum Nx(i,q) y(c,q), px, divU
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) E.ux[i][c] = 0.0;
+        E.px[i] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) E.ux[i][c] += E.Nx[q][i] * E.yl[q][c];
+        }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) E.px[i] += E.Nx[q][i] * E.yl[q][3];
+    E.divU = E.ux[0][0] + E.ux[1][1] + E.ux[2][2];
+}
+
+// FLUID3D_M + FLUID3D_C (svFSI/FLUID.f:192-560, 813-1084) for the (a,b)
+// block of one tet4 element, Newtonian viscosity mu, second derivatives
+// zero: k (4x4 block, row-major) and, when want_r, the residual of node a.
+template <typename T>
+__device__ __forceinline__ void fluid_pair(const FluidElem& E, T mu, const AsmParams& P, const Tet4Rule& rule,
+                                           int a, int b, bool want_r, T k[16], T r[4])
+{
+    const double rho = P.rho;
+    const double ctM = 1.0, ctC = 36.0;
+    const double T1c = P.af * P.gam * P.dt;
+    const double amd = P.am / T1c;
+    double kS0 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) kS0 += E.ks[i][j] * E.ks[i][j];
+    const T kS = ctC * kS0 * (mu / rho) * (mu / rho);
+    const double kT = 4.0 * (ctM / P.dt) * (ctM / P.dt);
+    const double trK = E.ks[0][0] + E.ks[1][1] + E.ks[2][2];
+    const double NxNx = E.Nx[a][0] * E.Nx[b][0] + E.Nx[a][1] * E.Nx[b][1] + E.Nx[a][2] * E.Nx[b][2];
+
+#pragma unroll
+    for (int q = 0; q < 16; ++q) k[q] = scal<T>(0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[q] = scal<T>(0.0);
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+        const double w = rule.w[g] * E.Jac;
+        const double wl = w * T1c;
+        const double wr = w * rho;
+        double ud[3] = {-P.fb[0], -P.fb[1], -P.fb[2]};
+        double u[3] = {0, 0, 0};
+        double pp = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double Nq = rule.N[g][q];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ud[c] += Nq * E.al[q][c];
+                u[c] += Nq * E.yl[q][c];
+            }
+            pp += Nq * E.yl[q][3];
+        }
+        double kU = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) kU += u[i] * u[j] * E.ks[i][j];
+        const T tauM = 1.0 / (rho * dsqrt(kT + kU + kS));
+        T up[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double rV = ud[c] + u[0] * E.ux[0][c] + u[1] * E.ux[1][c] + u[2] * E.ux[2][c];
+            up[c] = -tauM * (rho * rV + E.px[c] - mu * 0.0);
+        }
+        const T tauC = 1.0 / (tauM * trK);
+        T tauB = scal<T>(0.0);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tauB = tauB + up[i] * up[j] * E.ks[i][j];
+        if (iszero_d(val_of(tauB))) tauB = scal<T>(2.220446049250313e-16);
+        tauB = rho / dsqrt(tauB);
+        const T ua[3] = {u[0] + up[0], u[1] + up[1], u[2] + up[2]};
+
+        const double Na = rule.N[g][a], Nb = rule.N[g][b];
+        const double uNxa = u[0] * E.Nx[a][0] + u[1] * E.Nx[a][1] + u[2] * E.Nx[a][2];
+        const double uNxb = u[0] * E.Nx[b][0] + u[1] * E.Nx[b][1] + u[2] * E.Nx[b][2];
+        const T upNxa = up[0] * E.Nx[a][0] + up[1] * E.Nx[a][1] + up[2] * E.Nx[a][2];
+        const T upNxb = up[0] * E.Nx[b][0] + up[1] * E.Nx[b][1] + up[2] * E.Nx[b][2];
+        const T uaNxa = uNxa + upNxa;
+        const double T1b = -rho * uNxb;   // updu(i,c,b) = delta_ic T1b
+
+        // momentum u-u blocks (FLUID.f:494-545)
+        const T T1 = mu * NxNx + rho * amd * Nb * (Na + rho * tauM * uaNxa) + rho * Na * (uNxb + upNxb)
+                   + tauB * upNxa * upNxb;
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double updu = (rr == c) ? T1b : 0.0;
+                const T T2 = mu * E.Nx[a][c] * E.Nx[b][rr] + tauC * E.Nx[a][rr] * E.Nx[b][c]
+                           - rho * tauM * uaNxa * updu;
+                k[rr * 4 + c] = k[rr * 4 + c] + wl * ((rr == c) ? T2 + T1 : T2);
+            }
+        // momentum u-p blocks (FLUID.f:548-557)
+        {
+            const T T1p = rho * tauM * uaNxa;
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) k[rr * 4 + 3] = k[rr * 4 + 3] - wl * (E.Nx[a][rr] * Nb - E.Nx[b][rr] * T1p);
+        }
+        // continuity p-u and p-p blocks (FLUID.f:1052-1081)
+        {
+            const double T1q = rho * amd * Nb;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double T2 = E.Nx[a][c] * (T1b - T1q);
+                k[12 + c] = k[12 + c] + wl * (Na * E.Nx[b][c] - tauM * T2);
+            }
+            k[15] = k[15] + wl * tauM * NxNx;
+        }
+        if (want_r) {
+            // momentum residual (FLUID.f:429-465) and continuity (1046-1049)
+            const T pa = pp - tauC * E.divU;
+            T rVb[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                rVb[c] = tauB * (up[0] * E.ux[0][c] + up[1] * E.ux[1][c] + up[2] * E.ux[2][c]);
+            T rM[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const double es = E.ux[i][c] + E.ux[c][i];
+                    rM[i][c] = mu * es - rho * up[c] * ua[i] + rVb[c] * up[i] - (i == c ? pa : scal<T>(0.0));
+                }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T rV = ud[c] + ua[0] * E.ux[0][c] + ua[1] * E.ux[1][c] + ua[2] * E.ux[2][c];
+                r[c] = r[c] + (wr * Na * rV
+                               + w * (E.Nx[a][0] * rM[0][c] + E.Nx[a][1] * rM[1][c] + E.Nx[a][2] * rM[2][c]));
+            }
+            r[3] = r[3] + w * (Na * E.divU - upNxa);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const int* __restrict__ ien,
                                                    const int* __restrict__ elemId,
                                                    const int* __restrict__ emap,
@@ -188,166 +405,56 @@ __global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const i
     const int p = begin + (t >> 4);
     const int ab = t & 15;
     const int a = ab >> 2, b = ab & 3;
-    int nd[4];
-    double xl[4][3];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        nd[q] = ien[(size_t)p * 4 + q];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) xl[q][i] = x[(size_t)nd[q] * 3 + i];
-    }
-    double Nx[4][3], ks[3][3], Jac;
-    gnn_tet4(xl, Nx, Jac, ks);
+    FluidElem E;
+    fluid_elem(p, ien, x, tDof, Ag, Yg, E);
     const double mu = muEl ? muEl[elemId[p]] : P.mu;
-    const double rho = P.rho;
-    const double ctM = 1.0, ctC = 36.0;
-    const double T1c = P.af * P.gam * P.dt;
-    const double amd = P.am / T1c;
-
-    // element-constant kinematics: ux(i,c) = sum Nx(i,q) y(c,q), px, divU, es
-    double yl[4][4], al[4][3];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) yl[q][c] = Yg[(size_t)nd[q] * tDof + c];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) al[q][c] = Ag[(size_t)nd[q] * tDof + c];
-    }
-    double ux[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    double px[3] = {0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) ux[i][c] += Nx[q][i] * yl[q][c];
-        }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) px[i] += Nx[q][i] * yl[q][3];
-    const double divU = ux[0][0] + ux[1][1] + ux[2][2];
-    double kS = 0.0;
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) kS += ks[i][j] * ks[i][j];
-    kS = ctC * kS * (mu / rho) * (mu / rho);
-    const double kT = 4.0 * (ctM / P.dt) * (ctM / P.dt);
-    const double trK = ks[0][0] + ks[1][1] + ks[2][2];
-    const double NxNx = Nx[a][0] * Nx[b][0] + Nx[a][1] * Nx[b][1] + Nx[a][2] * Nx[b][2];
-
-    double k[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) k[q] = 0.0;
-    double r[4] = {0, 0, 0, 0};
-#pragma unroll 1
-    for (int g = 0; g < 4; ++g) {
-        const double w = rule.w[g] * Jac;
-        const double wl = w * T1c;
-        const double wr = w * rho;
-        double ud[3] = {-P.fb[0], -P.fb[1], -P.fb[2]};
-        double u[3] = {0, 0, 0};
-        double pp = 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double Nq = rule.N[g][q];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ud[c] += Nq * al[q][c];
-                u[c] += Nq * yl[q][c];
-            }
-            pp += Nq * yl[q][3];
-        }
-        double kU = 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) kU += u[i] * u[j] * ks[i][j];
-        const double tauM = 1.0 / (rho * sqrt(kT + kU + kS));
-        double up[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double rV = ud[c] + u[0] * ux[0][c] + u[1] * ux[1][c] + u[2] * ux[2][c];
-            up[c] = -tauM * (rho * rV + px[c] - mu * 0.0);
-        }
-        const double tauC = 1.0 / (tauM * trK);
-        double tauB = 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) tauB += up[i] * up[j] * ks[i][j];
-        if (iszero_d(tauB)) tauB = 2.220446049250313e-16;
-        tauB = rho / sqrt(tauB);
-        const double ua[3] = {u[0] + up[0], u[1] + up[1], u[2] + up[2]};
-
-        const double Na = rule.N[g][a], Nb = rule.N[g][b];
-        const double uNxa = u[0] * Nx[a][0] + u[1] * Nx[a][1] + u[2] * Nx[a][2];
-        const double uNxb = u[0] * Nx[b][0] + u[1] * Nx[b][1] + u[2] * Nx[b][2];
-        const double upNxa = up[0] * Nx[a][0] + up[1] * Nx[a][1] + up[2] * Nx[a][2];
-        const double upNxb = up[0] * Nx[b][0] + up[1] * Nx[b][1] + up[2] * Nx[b][2];
-        const double uaNxa = uNxa + upNxa;
-        const double T1b = -rho * uNxb;   // updu(i,c,b) = delta_ic T1b
-
-        // momentum u-u blocks (FLUID.f:494-545)
-        const double T1 = mu * NxNx + rho * amd * Nb * (Na + rho * tauM * uaNxa)
-                        + rho * Na * (uNxb + upNxb) + tauB * upNxa * upNxb;
-#pragma unroll
-        for (int rr = 0; rr < 3; ++rr)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double updu = (rr == c) ? T1b : 0.0;
-                const double T2 = mu * Nx[a][c] * Nx[b][rr] + tauC * Nx[a][rr] * Nx[b][c]
-                                - rho * tauM * uaNxa * updu;
-                k[rr * 4 + c] += wl * ((rr == c) ? T2 + T1 : T2);
-            }
-        // momentum u-p blocks (FLUID.f:548-557)
-        {
-            const double T1p = rho * tauM * uaNxa;
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) k[rr * 4 + 3] -= wl * (Nx[a][rr] * Nb - Nx[b][rr] * T1p);
-        }
-        // continuity p-u and p-p blocks (FLUID.f:1052-1081)
-        {
-            const double T1q = rho * amd * Nb;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double T2 = Nx[a][c] * (T1b - T1q);
-                k[12 + c] += wl * (Na * Nx[b][c] - tauM * T2);
-            }
-            k[15] += wl * tauM * NxNx;
-        }
-        if (b == 0) {
-            // momentum residual (FLUID.f:429-465) and continuity (1046-1049)
-            const double pa = pp - tauC * divU;
-            double rVb[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                rVb[c] = tauB * (up[0] * ux[0][c] + up[1] * ux[1][c] + up[2] * ux[2][c]);
-            double rM[3][3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    const double es = ux[i][c] + ux[c][i];
-                    rM[i][c] = mu * es - rho * up[c] * ua[i] + rVb[c] * up[i] - (i == c ? pa : 0.0);
-                }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double rV = ud[c] + ua[0] * ux[0][c] + ua[1] * ux[1][c] + ua[2] * ux[2][c];
-                r[c] += wr * Na * rV + w * (Nx[a][0] * rM[0][c] + Nx[a][1] * rM[1][c] + Nx[a][2] * rM[2][c]);
-            }
-            r[3] += w * (Na * divU - upNxa);
-        }
-    }
+    double k[16], r[4];
+    fluid_pair<double>(E, mu, P, rule, a, b, b == 0, k, r);
     double* vb = val + (size_t)emap[(size_t)p * 16 + ab] * 16;
 #pragma unroll
     for (int q = 0; q < 16; ++q) vb[q] += k[q];
     if (b == 0) {
-        double* Rr = R + (size_t)nd[a] * 4;
+        double* Rr = R + (size_t)E.nd[a] * 4;
 #pragma unroll
         for (int c = 0; c < 4; ++c) Rr[c] += r[c];
     }
+}
+
+// Viscosity sensitivity of the fluid element (no reference counterpart):
+// g_e = lam_e^T (dR_e/dmu_e - dK_e/dmu_e du_e), the element term of
+// dL/dmu_e when K du = R and K^T lam = dL/du.  Sixteen lanes per element
+// (one per (a,b) block) evaluate the dual-number element routine; the
+// element sum is a fixed-order shuffle reduction over the 16 lanes.
+__global__ __launch_bounds__(256) void k_fluid_dmu(int nEl, const int* __restrict__ ien,
+                                                   const int* __restrict__ elemId, const double* __restrict__ x,
+                                                   int tDof, const double* __restrict__ Ag,
+                                                   const double* __restrict__ Yg, const double* __restrict__ muEl,
+                                                   AsmParams P, Tet4Rule rule, const double* __restrict__ lam,
+                                                   const double* __restrict__ du, double* __restrict__ g)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nEl * 16) return;
+    const int p = t >> 4;
+    const int ab = t & 15;
+    const int a = ab >> 2, b = ab & 3;
+    FluidElem E;
+    fluid_elem(p, ien, x, tDof, Ag, Yg, E);
+    const Dual mu = {muEl ? muEl[elemId[p]] : P.mu, 1.0};
+    Dual k[16], r[4];
+    fluid_pair<Dual>(E, mu, P, rule, a, b, b == 0, k, r);
+    const double* la = lam + (size_t)E.nd[a] * 4;
+    const double* ub = du + (size_t)E.nd[b] * 4;
+    double s = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        double kd = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) kd += k[rr * 4 + c].d * ub[c];
+        s += la[rr] * ((b == 0 ? r[rr].d : 0.0) - kd);
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (ab == 0) g[elemId[p]] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -794,6 +901,18 @@ int launch_face_backflow(const Mesh& M, const Mesh::FaceSet& F, const AsmParams&
                            F.order.as<int>(), F.fien.as<int>(), F.inner.as<int>(), F.fmap.as<int>(),
                            M.x.as<double>(), tDof, Yg, P, rule, val, R);
     }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_fluid_dmu(const Mesh& M, const AsmParams& P, const Tet4Rule& rule, int tDof, const double* Ag,
+                     const double* Yg, const double* muEl, const double* lam, const double* du, double* g,
+                     hipStream_t s)
+{
+    if (M.nEl == 0) return SVFSI_OK;
+    const unsigned grid = (unsigned)(((size_t)M.nEl * 16 + 255) / 256);
+    hipLaunchKernelGGL(k_fluid_dmu, dim3(grid), dim3(256), 0, s, M.nEl, M.ien.as<int>(), M.elemId.as<int>(),
+                       M.x.as<double>(), tDof, Ag, Yg, muEl, P, rule, lam, du, g);
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }

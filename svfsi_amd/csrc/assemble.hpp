@@ -43,6 +43,9 @@ int launch_face_backflow(const Mesh& M, const Mesh::FaceSet& F, const AsmParams&
                          hipStream_t s);
 int launch_lelas_dkde(const Mesh& M, const AsmParams& P, const Tet4Rule& rule, const double* lam,
                       const double* u, double* g, hipStream_t s);
+int launch_fluid_dmu(const Mesh& M, const AsmParams& P, const Tet4Rule& rule, int tDof, const double* Ag,
+                     const double* Yg, const double* muEl, const double* lam, const double* du, double* g,
+                     hipStream_t s);
 
 } // namespace svfsi
 

@@ -114,6 +114,7 @@ struct Work {
     DBuf partial, state, hess;
     DBuf zdir, opartial, ostate, ohess; // flexible GMRES: FP32 directions, outer state
     DBuf mask;
+    DBuf valT; // K^T values for transposed solves
     int maskDof = -1;
     std::vector<int> maskInc;
     int maskVersion = -1;
@@ -175,6 +176,7 @@ int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
                     float* KU, hipStream_t s, const int* stop = nullptr);
 int launch_spmv_t(const Lhs& L, int dof, const double* K, const double* U,
                   double* KU, hipStream_t s, const int* stop = nullptr);
+int launch_transpose_blocks(const Lhs& L, int dof, const double* V, double* VT, hipStream_t s);
 int launch_dot(int n, const double* U, const double* V, double* partial,
                double* result, hipStream_t s);
 int build_mask(Lhs& L, int dof, const int* incL, hipStream_t s);

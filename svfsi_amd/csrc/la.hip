@@ -410,4 +410,40 @@ int precond_diag(Lhs& L, int dof, double* Val, double* R, double* W, float* Val3
     return SVFSI_OK;
 }
 
+// Values of K^T on the (structurally symmetric) pattern of K:
+// VT[k](i,j) = V[trans[k]](j,i).
+template <int DOF>
+__global__ __launch_bounds__(256) void k_transpose_blocks(size_t nEnt, const int* __restrict__ trans,
+                                                          const double* __restrict__ V, double* __restrict__ VT)
+{
+    constexpr int DD = DOF * DOF;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < nEnt; q += (size_t)gridDim.x * 256) {
+        const size_t blk = q / DD;
+        const int e = (int)(q - blk * DD);
+        const int i = e / DOF, j = e - (e / DOF) * DOF;
+        VT[q] = V[(size_t)trans[blk] * DD + j * DOF + i];
+    }
+}
+
+int launch_transpose_blocks(const Lhs& L, int dof, const double* V, double* VT, hipStream_t s)
+{
+    if (!L.symmetric) {
+        set_error("transpose needs a structurally symmetric pattern");
+        return SVFSI_ERR_ARG;
+    }
+    const size_t nEnt = (size_t)L.nnz * dof * dof;
+    size_t g = (nEnt + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    const int* tr = L.trans.as<int>();
+    switch (dof) {
+    case 2: hipLaunchKernelGGL(k_transpose_blocks<2>, dim3((unsigned)g), dim3(256), 0, s, nEnt, tr, V, VT); break;
+    case 3: hipLaunchKernelGGL(k_transpose_blocks<3>, dim3((unsigned)g), dim3(256), 0, s, nEnt, tr, V, VT); break;
+    case 4: hipLaunchKernelGGL(k_transpose_blocks<4>, dim3((unsigned)g), dim3(256), 0, s, nEnt, tr, V, VT); break;
+    default: set_error("transpose: dof %d not supported (2..4)", dof); return SVFSI_ERR_NOTSUP;
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
 } // namespace svfsi

@@ -177,13 +177,16 @@ def fsils_solve(lhs: Lhs, ls: FSILS_lsType, dof: int, Ri, Val, prec: int = PRECO
     call(name, lhs.handle, C.byref(ls), dof, dev_ptr(Ri), dev_ptr(Val), prec, incp, resp, s)
 
 
-def fsils_solve_t(lhs: Lhs, ls: FSILS_lsType, dof: int, Val, rhs, lam, incL=None, stream=None) -> None:
+def fsils_solve_t(lhs: Lhs, ls: FSILS_lsType, dof: int, Val, rhs, lam, incL=None, stream=None,
+                  mixed: bool = False) -> None:
     """Transposed solve (W K^T W) y = W rhs, lam = W y, with the forward
     solve's scaling and Dirichlet mask.  Val is the unscaled matrix and is
-    not modified.  Device tensors only."""
+    not modified.  Device tensors only; mixed=True uses the mixed-precision
+    solver."""
     inc = None if incL is None else _i32(incL)
     s = cur_stream() if stream is None else stream
-    call("fsils_gpu_solve_t_dev", lhs.handle, C.byref(ls), dof, dev_ptr(Val), dev_ptr(rhs), dev_ptr(lam),
+    name = "fsils_gpu_solve_t_mixed_dev" if mixed else "fsils_gpu_solve_t_dev"
+    call(name, lhs.handle, C.byref(ls), dof, dev_ptr(Val), dev_ptr(rhs), dev_ptr(lam),
          None if inc is None else _iptr(inc), s)
 
 

@@ -23,3 +23,21 @@ class LelasSolve(torch.autograd.Function):
         E, u = ctx.saved_tensors
         gE, _, _ = ctx.adj.grad_E(u, gu)
         return gE * E, None, None
+
+
+class FluidViscositySolve(torch.autograd.Function):
+    """z -> du = K(mu)^-1 R(mu), mu = mu0 exp(z) (one linearised VMS step)."""
+
+    @staticmethod
+    def forward(ctx, z, adj, mu0: float):
+        mu = (mu0 * torch.exp(z)).contiguous()
+        du, _ = adj.forward(mu)
+        ctx.save_for_backward(mu, du)
+        ctx.adj = adj
+        return du
+
+    @staticmethod
+    def backward(ctx, gdu):
+        mu, du = ctx.saved_tensors
+        g, _, _ = ctx.adj.grad_mu(mu, du, gdu)
+        return g * mu, None, None

@@ -1,0 +1,188 @@
+"""Viscosity adjoint of one linearised VMS fluid step: K(mu) du = R(mu),
+mu_e = mu0 exp(z_e), loss L = c . du, the outlet flux of du weighted by
+1 - r^2.
+
+CPU: the oracle's complex-step element sensitivity (orc_fluid_dmu_dot) is
+checked against central differences of the oracle's assembled K and R, and
+the dense adjoint built on it against central differences of the dense
+solve.
+GPU: the dual-number kernel (svfsi_fluid_dmu_dot_dev) against the oracle;
+the autograd wrapper over the C ABI against the oracle's FP64 CPU adjoint
+and central differences (relative step 1e-4 in mu) on 20 random elements
+among the most sensitive ones."""
+import numpy as np
+import pytest
+
+from svfsi_amd import problems
+
+MU0 = 0.04
+
+
+def outlet_flux_weights(prob):
+    """c with c . du = integral over the outlet of (1 - r^2) du . n (tri3,
+    integral of N_a: area / 3; outlet normal +z).  The weight measures the
+    profile's shape: the plain outlet flux of an increment is pinned by
+    continuity (the inlet and wall increments are zero) and is nearly
+    independent of the viscosity."""
+    m = prob.mesh
+    c = np.zeros((m.nNo, 4))
+    wr = 1.0 - (m.x[:, 0] ** 2 + m.x[:, 1] ** 2)
+    for fien, _ in prob.bflow_faces:
+        p = m.x[fien]
+        area = 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
+        for k in range(3):
+            np.add.at(c[:, 2], fien[:, k], area / 3.0 * wr[fien[:, k]])
+    c[prob.dir_mask() == 0.0] = 0.0
+    return c
+
+
+def fluid_problem(h, seed=5):
+    prob = problems.fluid_cylinder(h=h)
+    rng = np.random.default_rng(seed)
+    prob.Ag[:, :3] = rng.normal(size=(prob.nNo, 3))
+    prob.prop_el = MU0 * np.exp(0.1 * rng.normal(size=prob.mesh.nEl))
+    return prob
+
+
+def dense(Val, rp, cp, nNo, dof):
+    K = np.zeros((nNo * dof, nNo * dof))
+    for i in range(nNo):
+        for k in range(rp[i], rp[i + 1]):
+            K[i * dof:(i + 1) * dof, cp[k] * dof:(cp[k] + 1) * dof] = Val[k].reshape(dof, dof)
+    return K
+
+
+def test_oracle_dmu_matches_fd_of_assembly(orc):
+    prob = fluid_problem(0.6)
+    m = prob.mesh
+    rp, cp = orc.lhsa(m.nNo, m.ien)
+    rng = np.random.default_rng(1)
+    lam = rng.normal(size=(m.nNo, 4))
+    du = rng.normal(size=(m.nNo, 4))
+    g = orc.fluid_dmu_dot(prob, lam, du)
+    mu = prob.prop_el.copy()
+    for e in rng.choice(m.nEl, 6, replace=False):
+        h = 1e-4 * mu[e]  # O(h^2) truncation ~1e-8; rounding of |R| ~ 1e3 stays below 1e-7
+        out = []
+        for s in (1, -1):
+            prob.prop_el = mu.copy()
+            prob.prop_el[e] += s * h
+            V, R = orc.assemble(prob, rp, cp)
+            Kd = dense(V, rp, cp, m.nNo, 4)
+            out.append(R.ravel() - Kd @ du.ravel())
+        fd = lam.ravel() @ (out[0] - out[1]) / (2 * h)
+        assert abs(fd - g[e]) <= 1e-6 * abs(g[e]) + 1e-9 * np.abs(g).max(), (e, fd, g[e])
+    prob.prop_el = mu
+
+
+def test_oracle_dense_adjoint_vs_fd(orc):
+    prob = fluid_problem(0.5)
+    m = prob.mesh
+    rp, cp = orc.lhsa(m.nNo, m.ien)
+    free = prob.dir_mask().ravel() == 1.0
+    c = outlet_flux_weights(prob).ravel()
+    mu = prob.prop_el.copy()
+
+    def solve(mm):
+        prob.prop_el = mm
+        V, R = orc.assemble(prob, rp, cp)
+        K = dense(V, rp, cp, m.nNo, 4)
+        x = np.zeros(m.nNo * 4)
+        x[free] = np.linalg.solve(K[np.ix_(free, free)], R.ravel()[free])
+        return x, K
+
+    x, K = solve(mu)
+    lam = np.zeros_like(x)
+    lam[free] = np.linalg.solve(K[np.ix_(free, free)].T, c[free])
+    prob.prop_el = mu
+    g = orc.fluid_dmu_dot(prob, lam.reshape(-1, 4), x.reshape(-1, 4))
+    # the dense FD of L (|L| ~ 4e4, K's condition ~ 4e8) resolves only the
+    # sensitive elements: the 5 largest |g|
+    for e in np.argsort(-np.abs(g))[:5]:
+        h = 1e-3 * mu[e]
+        L = []
+        for s in (1, -1):
+            mm = mu.copy()
+            mm[e] += s * h
+            L.append(c @ solve(mm)[0])
+        fd = (L[0] - L[1]) / (2 * h)
+        assert abs(fd - g[e]) <= 1e-4 * abs(g[e]), (e, fd, g[e])
+
+
+@pytest.mark.gpu
+def test_fluid_dmu_kernel_parity(gpu, orc):
+    import torch
+
+    from svfsi_amd.step import NewtonStep
+
+    prob = fluid_problem(0.4)
+    st = NewtonStep(prob)
+    rng = np.random.default_rng(3)
+    lam = rng.normal(size=(prob.nNo, 4))
+    du = rng.normal(size=(prob.nNo, 4))
+    dl, dd = (torch.from_numpy(a).to(gpu) for a in (lam, du))
+    g = torch.zeros(prob.mesh.nEl, dtype=torch.float64, device=gpu)
+    st.dmesh.fluid_dmu_dot(prob.eq, st.Ag, st.Yg, st.prop, dl, dd, g)
+    torch.cuda.synchronize()
+    go = orc.fluid_dmu_dot(prob, lam, du)
+    gg = g.cpu().numpy()
+    assert np.abs(gg - go).max() <= 1e-12 * np.abs(go).max()
+
+
+def _fd_check(adj, z, du, c, g, elems, h=1e-4):
+    """Central differences of L = c . du(z +- h e_k), du(z +- h e_k) from
+    correction-form solves: L+ - L- = c . (d+ - d-)."""
+    import torch
+
+    adj.prob.ls = dict(adj.prob.ls, relTol=1e-12, mItr=5000)
+    errs = []
+    for e in elems:
+        d = []
+        for s in (1.0, -1.0):
+            zz = z.clone()
+            zz[e] += s * h
+            d.append(adj.solve_correction(MU0 * torch.exp(zz), du) - du)
+        fd = torch.sum(c * (d[0] - d[1])).item() / (2 * h)
+        errs.append(abs(fd - g[e].item()) / max(abs(fd), 1e-300))
+        print("fd", int(e), fd, g[e].item(), errs[-1], flush=True)
+    return max(errs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mixed", [False, True])
+def test_viscosity_adjoint_vs_cpu_adjoint_and_fd(gpu, orc, mixed):
+    import torch
+
+    from adjoint_fn import FluidViscositySolve
+    from svfsi_amd.adjoint import FluidViscosityAdjoint
+
+    prob = fluid_problem(0.5)
+    prob.ls = dict(prob.ls, relTol=1e-12, mItr=5000)
+    m = prob.mesh
+    adj = FluidViscosityAdjoint(prob, mixed=mixed)
+    z = torch.tensor(np.log(prob.prop_el / MU0), device=gpu)
+    c = torch.tensor(outlet_flux_weights(prob), device=gpu)
+    zr = z.clone().requires_grad_(True)
+    du = FluidViscositySolve.apply(zr, adj, MU0)
+    L = torch.sum(c * du)
+    L.backward()
+    g = zr.grad.detach()
+    # FP64 CPU adjoint: oracle forward and transposed GMRES, complex-step element terms
+    rp, cp = orc.lhsa(m.nNo, m.ien)
+    V, R = orc.assemble(prob, rp, cp)
+    mask = prob.dir_mask()
+    x_o, _, so = orc.fsils_solve(rp, cp, 4, mask, prob.ls, R, V)
+    from test_adjoint import transpose_blocks
+
+    lam_o, _, sl = orc.fsils_solve(rp, cp, 4, mask, prob.ls, c.cpu().numpy(), transpose_blocks(V, rp, cp, m.nNo, 4))
+    assert so.suc == 1 and sl.suc == 1
+    g_o = orc.fluid_dmu_dot(prob, lam_o, x_o) * prob.prop_el
+    gn = g.cpu().numpy()
+    err = np.abs(gn - g_o).max() / np.abs(g_o).max()
+    print("cpu adjoint relerr", err, flush=True)
+    assert err <= 1e-7
+    # 20 random elements among the 200 most sensitive: elsewhere dL/dz is
+    # below the resolution of a difference of solves of this system
+    top = np.argsort(-np.abs(gn))[:200]
+    elems = np.random.default_rng(13).choice(top, 20, replace=False)
+    assert _fd_check(adj, z, du.detach(), c, g, elems) < 1e-5
