@@ -59,9 +59,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["fluid", "struct", "lelas"], default="fluid",
-                    help="fluid: VMS cylinder ~200k tet4 (the metric's configuration); struct: neo-Hookean tet10 "
-                         "cube 40^3 x 6, 4-point rule; lelas: linear elasticity cube 20^3 x 6 tet4 with CG")
+    ap.add_argument("--workload", choices=["fluid", "fluid-large", "struct", "lelas"], default="fluid",
+                    help="fluid: VMS cylinder ~200k tet4 (the metric's configuration); fluid-large: the same physics "
+                         "on ~3.2M tet4 with GMRES(100); struct: neo-Hookean tet10 cube 40^3 x 6, 4-point rule; "
+                         "lelas: linear elasticity cube 20^3 x 6 tet4 with CG")
     ap.add_argument("--n", type=int, default=0, help="cube cells per side for struct/lelas (default 40/20)")
     ap.add_argument("--h", type=float, default=0.11, help="cylinder lattice spacing (0.11 -> ~200k tets)")
     ap.add_argument("--sD", type=int, default=0, help="GMRES restart length (default: 50 fluid, 100 struct)")
@@ -100,10 +101,13 @@ def main() -> int:
 
     t0 = time.time()
     if not args.sD:
-        args.sD = 100 if args.workload == "struct" else 50
-    if args.workload == "fluid":
-        prob = problems.fluid_cylinder(h=args.h, sD=args.sD)
-        wl = "FLUID3D VMS tet4 cylinder r=1 L=10, one Newton step, GMRES(%d) to 1e-8" % args.sD
+        args.sD = 100 if args.workload in ("struct", "fluid-large") else 50
+    if args.workload in ("fluid", "fluid-large"):
+        h = 0.042 if args.workload == "fluid-large" else args.h
+        prob = problems.fluid_cylinder(h=h, sD=args.sD)
+        if args.workload == "fluid-large":
+            prob.ls = dict(prob.ls, mItr=2000)
+        wl = "FLUID3D VMS tet4 cylinder r=1 L=10 (h=%g), one Newton step, GMRES(%d) to 1e-8" % (h, args.sD)
     elif args.workload == "struct":
         n = args.n or 40
         prob = problems.struct_cube(n, quad=1, sD=args.sD)

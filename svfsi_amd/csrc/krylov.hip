@@ -632,11 +632,24 @@ __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, i
         if (jj <= i && b0 == 0) hsh[jj] = t;
     }
     __syncthreads();
+    // Pythagorean h(i+1,i) = sqrt|<w,w> - sum h_j^2|: the sum over j < i by
+    // the first wavefront (fixed order), so no thread walks the column alone
+    __shared__ double cs_s[2][kMaxSD];
     for (int jj = threadIdx.x; jj < i; jj += 256) hc[jj] = -hsh[jj];
+    if (blockIdx.x == 0)
+        for (int jj = threadIdx.x; jj < i - 1; jj += 256) {
+            cs_s[0][jj] = A.c[jj];
+            cs_s[1][jj] = A.sn[jj];
+        }
+    if (threadIdx.x < kWave) {
+        double q = 0.0;
+        for (int jj = threadIdx.x; jj < i; jj += kWave) q += hsh[jj] * hsh[jj];
+        for (int o = kWave >> 1; o > 0; o >>= 1) q += __shfl_xor(q, o, kWave);
+        if (threadIdx.x == 0) inv_s = q;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        double hii = hsh[i];
-        for (int jj = 0; jj < i; ++jj) hii = hii - hsh[jj] * hsh[jj];
-        hii = sqrt(fabs(hii));
+        const double hii = sqrt(fabs(hsh[i] - inv_s));
         hsh[i] = hii;
         inv_s = 1.0 / hii;
     }
@@ -647,19 +660,14 @@ __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, i
         double v[E];
         loadE(w, q, n, v);
         int jj = 0;
-        for (; jj + 4 <= i; jj += 4) {
-            double u0[E], u1[E], u2[E], u3[E];
-            loadE(basis + (size_t)jj * n, q, n, u0);
-            loadE(basis + (size_t)(jj + 1) * n, q, n, u1);
-            loadE(basis + (size_t)(jj + 2) * n, q, n, u2);
-            loadE(basis + (size_t)(jj + 3) * n, q, n, u3);
+        for (; jj + 8 <= i; jj += 8) {   // eight loads in flight, same order
+            double u[8][E];
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                v[e] = v[e] + hc[jj] * u0[e];
-                v[e] = v[e] + hc[jj + 1] * u1[e];
-                v[e] = v[e] + hc[jj + 2] * u2[e];
-                v[e] = v[e] + hc[jj + 3] * u3[e];
-            }
+            for (int m = 0; m < 8; ++m) loadE(basis + (size_t)(jj + m) * n, q, n, u[m]);
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+#pragma unroll
+                for (int m = 0; m < 8; ++m) v[e] = v[e] + hc[jj + m] * u[m][e];
         }
         for (; jj < i; ++jj) {
             double u[E];
@@ -674,7 +682,7 @@ __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, i
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     st->iters += 1;
     for (int jj = 1; jj <= i - 1; ++jj) {
-        const double c = A.c[jj - 1], sn = A.sn[jj - 1];
+        const double c = cs_s[0][jj - 1], sn = cs_s[1][jj - 1];
         const double tmp = c * hsh[jj - 1] + sn * hsh[jj];
         hsh[jj] = -sn * hsh[jj - 1] + c * hsh[jj];
         hsh[jj - 1] = tmp;
