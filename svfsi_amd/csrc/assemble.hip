@@ -632,7 +632,7 @@ __device__ __forceinline__ double det3d(const double A[3][3])
 }
 
 template <int NG, int EPW>
-__global__ __launch_bounds__(64) void k_asm_struct(int begin, int count, const int* __restrict__ ien,
+__global__ __launch_bounds__(256) void k_asm_struct(int begin, int count, const int* __restrict__ ien,
                                                    const int* __restrict__ emap, const double* __restrict__ x,
                                                    int tDof, const double* __restrict__ Ag,
                                                    const double* __restrict__ Dg, AsmParams P,
@@ -640,6 +640,7 @@ __global__ __launch_bounds__(64) void k_asm_struct(int begin, int count, const i
                                                    double* __restrict__ R)
 {
     __shared__ double rec[EPW * NG][SR_SZ];
+    __shared__ double dbm[EPW * NG][10][18]; // Dm B_b per (element, Gauss point, node b)
     const int lane = threadIdx.x;
     const int e0 = begin + blockIdx.x * EPW;
     const int eEnd = begin + count;
@@ -649,22 +650,19 @@ __global__ __launch_bounds__(64) void k_asm_struct(int begin, int count, const i
         double* rc = rec[lane];
         if (e < eEnd) {
             const double* rg = rule + g * kTet10Rec;
-            double xl[10][3];
-            int nd[10];
-#pragma unroll
-            for (int a = 0; a < 10; ++a) {
-                nd[a] = ien[(size_t)e * 10 + a];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) xl[a][i] = x[(size_t)nd[a] * 3 + i];
-            }
-            // GNN (NN.f:1469-1565)
+            // GNN (NN.f:1469-1565), streaming the nodes to keep registers low
             double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 2
+            for (int a = 0; a < 10; ++a) {
+                const int na = ien[(size_t)e * 10 + a];
+                double xa[3];
 #pragma unroll
-            for (int a = 0; a < 10; ++a)
+                for (int i = 0; i < 3; ++i) xa[i] = x[(size_t)na * 3 + i];
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) A[i][j] += xl[a][i] * rg[11 + a * 3 + j];
+                    for (int j = 0; j < 3; ++j) A[i][j] += xa[i] * rg[11 + a * 3 + j];
+            }
             const double J = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0]
                            + A[0][2] * A[1][0] * A[2][1] - A[0][0] * A[1][2] * A[2][1]
                            - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
@@ -678,24 +676,26 @@ __global__ __launch_bounds__(64) void k_asm_struct(int begin, int count, const i
             B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / J;
             B[2][1] = (A[2][0] * A[0][1] - A[2][1] * A[0][0]) / J;
             B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / J;
-            double Nx[10][3];
-#pragma unroll
-            for (int a = 0; a < 10; ++a)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    Nx[a][j] = rg[11 + a * 3] * B[0][j] + rg[11 + a * 3 + 1] * B[1][j] + rg[11 + a * 3 + 2] * B[2][j];
-            // F, inertia (STRUCT.f:195-235)
+            // Nx (to LDS), F and inertia (STRUCT.f:195-235)
             double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
             double ud[3] = {-P.rho * P.fb[0], -P.rho * P.fb[1], -P.rho * P.fb[2]};
-#pragma unroll
+#pragma unroll 2
             for (int a = 0; a < 10; ++a) {
+                const int na = ien[(size_t)e * 10 + a];
+                double Nxa[3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    Nxa[j] = rg[11 + a * 3] * B[0][j] + rg[11 + a * 3 + 1] * B[1][j] + rg[11 + a * 3 + 2] * B[2][j];
+                    rc[SR_NX + a * 3 + j] = Nxa[j];
+                }
                 const double Na = rg[1 + a];
+                rc[SR_N + a] = Na;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const double Dai = Dg[(size_t)nd[a] * tDof + i];
-                    ud[i] += Na * (P.rho * Ag[(size_t)nd[a] * tDof + i]);
+                    const double Dai = Dg[(size_t)na * tDof + i];
+                    ud[i] += Na * (P.rho * Ag[(size_t)na * tDof + i]);
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) F[i][j] += Nx[a][j] * Dai;
+                    for (int j = 0; j < 3; ++j) F[i][j] += Nxa[j] * Dai;
                 }
             }
             // GETPK2CC, neo-Hookean + GETSVOLP
@@ -764,25 +764,48 @@ __global__ __launch_bounds__(64) void k_asm_struct(int begin, int count, const i
                 for (int j = 0; j < 3; ++j) rc[SR_P + i * 3 + j] = F[i][0] * S[0][j] + F[i][1] * S[1][j] + F[i][2] * S[2][j];
             rc[SR_W] = rg[0] * J;
 #pragma unroll
-            for (int a = 0; a < 10; ++a) {
-                rc[SR_N + a] = rg[1 + a];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) rc[SR_NX + a * 3 + j] = Nx[a][j];
-            }
-#pragma unroll
             for (int i = 0; i < 3; ++i) rc[SR_UD + i] = ud[i];
         }
     }
     __syncthreads();
+    // Dm B_b, shared by the ten blocks (a, b) of column b
+    for (int idx = lane; idx < EPW * NG * 10; idx += 256) {
+        const int kg = idx / 10, b = idx - (idx / 10) * 10;
+        if (e0 + kg / NG >= eEnd) continue;
+        const double* rc = rec[kg];
+        const double* Nxb = rc + SR_NX + b * 3;
+        const double* Fg = rc + SR_F;
+        double Bb[6][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Bb[0][c] = Nxb[0] * Fg[c * 3 + 0];
+            Bb[1][c] = Nxb[1] * Fg[c * 3 + 1];
+            Bb[2][c] = Nxb[2] * Fg[c * 3 + 2];
+            Bb[3][c] = Nxb[0] * Fg[c * 3 + 1] + Fg[c * 3 + 0] * Nxb[1];
+            Bb[4][c] = Nxb[1] * Fg[c * 3 + 2] + Fg[c * 3 + 1] * Nxb[2];
+            Bb[5][c] = Nxb[2] * Fg[c * 3 + 0] + Fg[c * 3 + 2] * Nxb[0];
+        }
+#pragma unroll
+        for (int rr = 0; rr < 6; ++rr)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double sacc = 0.0;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) sacc += rc[SR_DM + rr * 6 + q] * Bb[q][c];
+                dbm[kg][b][rr * 3 + c] = sacc;
+            }
+    }
+    __syncthreads();
     const double afu = P.af * P.beta * P.dt * P.dt;
     const double amd = P.am * P.rho;
-    for (int pidx = lane; pidx < EPW * 100; pidx += 64) {
+    for (int pidx = lane; pidx < EPW * 100; pidx += 256) {
         const int k = pidx / 100, ab = pidx - (pidx / 100) * 100;
         const int a = ab / 10, b = ab - (ab / 10) * 10;
         const int e = e0 + k;
         if (e >= eEnd) continue;
         double kk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         double r[3] = {0, 0, 0};
+#pragma unroll 1
         for (int g = 0; g < NG; ++g) {
             const double* rc = rec[k * NG + g];
             const double w = rc[SR_W];
@@ -796,7 +819,7 @@ __global__ __launch_bounds__(64) void k_asm_struct(int begin, int count, const i
 #pragma unroll
                 for (int i = 0; i < 3; ++i) NxSNx += Nxa[i] * Sg[i * 3 + j] * Nxb[j];
             const double T1 = amd * rc[SR_N + a] * rc[SR_N + b] + afu * NxSNx;
-            double Ba[6][3], Bb[6][3];
+            double Ba[6][3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 Ba[0][c] = Nxa[0] * Fg[c * 3 + 0];
@@ -805,30 +828,15 @@ __global__ __launch_bounds__(64) void k_asm_struct(int begin, int count, const i
                 Ba[3][c] = Nxa[0] * Fg[c * 3 + 1] + Fg[c * 3 + 0] * Nxa[1];
                 Ba[4][c] = Nxa[1] * Fg[c * 3 + 2] + Fg[c * 3 + 1] * Nxa[2];
                 Ba[5][c] = Nxa[2] * Fg[c * 3 + 0] + Fg[c * 3 + 2] * Nxa[0];
-                Bb[0][c] = Nxb[0] * Fg[c * 3 + 0];
-                Bb[1][c] = Nxb[1] * Fg[c * 3 + 1];
-                Bb[2][c] = Nxb[2] * Fg[c * 3 + 2];
-                Bb[3][c] = Nxb[0] * Fg[c * 3 + 1] + Fg[c * 3 + 0] * Nxb[1];
-                Bb[4][c] = Nxb[1] * Fg[c * 3 + 2] + Fg[c * 3 + 1] * Nxb[2];
-                Bb[5][c] = Nxb[2] * Fg[c * 3 + 0] + Fg[c * 3 + 2] * Nxb[0];
             }
-            double DBm[6][3];
-#pragma unroll
-            for (int rr = 0; rr < 6; ++rr)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    double sacc = 0.0;
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) sacc += rc[SR_DM + rr * 6 + q] * Bb[q][c];
-                    DBm[rr][c] = sacc;
-                }
+            const double* DBm = dbm[k * NG + g][b];
 #pragma unroll
             for (int rr = 0; rr < 3; ++rr)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     double BmDBm = 0.0;
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) BmDBm += Ba[q][rr] * DBm[q][c];
+                    for (int q = 0; q < 6; ++q) BmDBm += Ba[q][rr] * DBm[q * 3 + c];
                     kk[rr * 3 + c] += w * ((rr == c ? T1 : 0.0) + afu * BmDBm);
                 }
             if (b == 0) {
@@ -936,11 +944,11 @@ int launch_asm_struct(const Mesh& M, const AsmParams& P, int nG, const double* r
         if (count == 0) continue;
         if (nG == 4) {
             constexpr int EPW = 8;
-            hipLaunchKernelGGL((k_asm_struct<4, EPW>), dim3((count + EPW - 1) / EPW), dim3(64), 0, s, begin, count,
+            hipLaunchKernelGGL((k_asm_struct<4, EPW>), dim3((count + EPW - 1) / EPW), dim3(256), 0, s, begin, count,
                                M.ien.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, P, rule, val, R);
         } else if (nG == 15) {
             constexpr int EPW = 2;
-            hipLaunchKernelGGL((k_asm_struct<15, EPW>), dim3((count + EPW - 1) / EPW), dim3(64), 0, s, begin, count,
+            hipLaunchKernelGGL((k_asm_struct<15, EPW>), dim3((count + EPW - 1) / EPW), dim3(256), 0, s, begin, count,
                                M.ien.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, P, rule, val, R);
         } else {
             set_error("STRUCT: nG %d not supported", nG);
