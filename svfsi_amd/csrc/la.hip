@@ -76,26 +76,41 @@ __global__ __launch_bounds__(256) void k_bspmv(int nNo, const int* __restrict__ 
 // values: 32 B FP64 / 16 B FP32) plus the 4-vector U(:,col) with vector
 // loads; no cross-lane reduction.  16 block rows per wavefront, two blocks
 // in flight per lane.
+// NT: streaming (nontemporal) loads of the matrix values, for matrices that
+// do not fit the last-level cache; otherwise ordinary loads, so that a
+// matrix reused across Krylov steps can stay resident in the 256 MB MALL.
+template <bool NT>
 __device__ __forceinline__ double dot4_nt(const double* __restrict__ v, const double* __restrict__ x)
 {
     typedef double d2 __attribute__((ext_vector_type(2)));
-    const d2 a = __builtin_nontemporal_load(reinterpret_cast<const d2*>(v));
-    const d2 b = __builtin_nontemporal_load(reinterpret_cast<const d2*>(v) + 1);
+    d2 a, b;
+    if constexpr (NT) {
+        a = __builtin_nontemporal_load(reinterpret_cast<const d2*>(v));
+        b = __builtin_nontemporal_load(reinterpret_cast<const d2*>(v) + 1);
+    } else {
+        a = *reinterpret_cast<const d2*>(v);
+        b = *(reinterpret_cast<const d2*>(v) + 1);
+    }
     const d2 xa = *reinterpret_cast<const d2*>(x);
     const d2 xb = *(reinterpret_cast<const d2*>(x) + 1);
     return a.x * xa.x + a.y * xa.y + b.x * xb.x + b.y * xb.y;
 }
 
+template <bool NT>
 __device__ __forceinline__ double dot4_nt(const float* __restrict__ v, const float* __restrict__ x)
 {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    const f4 a = __builtin_nontemporal_load(reinterpret_cast<const f4*>(v));
+    f4 a;
+    if constexpr (NT)
+        a = __builtin_nontemporal_load(reinterpret_cast<const f4*>(v));
+    else
+        a = *reinterpret_cast<const f4*>(v);
     const f4 xa = *reinterpret_cast<const f4*>(x);
     return (double)a.x * (double)xa.x + (double)a.y * (double)xa.y + (double)a.z * (double)xa.z
          + (double)a.w * (double)xa.w;
 }
 
-template <typename VT, int H>
+template <typename VT, int H, bool NT>
 __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__ rowPtr,
                                                 const int* __restrict__ colPtr,
                                                 const VT* __restrict__ val,
@@ -116,10 +131,10 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
         int k = rowPtr[row] + h;
         for (; k + H < e; k += 2 * H) {
             const int c0 = colPtr[k], c1 = colPtr[k + H];
-            a0 += dot4_nt(val + (size_t)k * 16 + i * 4, x + (size_t)c0 * 4);
-            a1 += dot4_nt(val + (size_t)(k + H) * 16 + i * 4, x + (size_t)c1 * 4);
+            a0 += dot4_nt<NT>(val + (size_t)k * 16 + i * 4, x + (size_t)c0 * 4);
+            a1 += dot4_nt<NT>(val + (size_t)(k + H) * 16 + i * 4, x + (size_t)c1 * 4);
         }
-        if (k < e) a0 += dot4_nt(val + (size_t)k * 16 + i * 4, x + (size_t)colPtr[k] * 4);
+        if (k < e) a0 += dot4_nt<NT>(val + (size_t)k * 16 + i * 4, x + (size_t)colPtr[k] * 4);
     }
     double acc = a0 + a1;
 #pragma unroll
@@ -220,13 +235,26 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
                 const int v = e ? atoi(e) : 4;
                 return (v == 1 || v == 2) ? v : 4;
             }();
+            // streaming loads only when the values exceed half the MALL
+            // (tuning knob SVFSI_SPMV_NT = 0 / 1 overrides)
+            static const int ntMode = [] {
+                const char* e = getenv("SVFSI_SPMV_NT");
+                return e ? atoi(e) : -1;
+            }();
+            const bool nt = ntMode >= 0 ? ntMode != 0 : (size_t)L.nnz * 16 * sizeof(VT) > ((size_t)128 << 20);
             const int g4 = (int)(((size_t)L.nNo * 4 * H + 255) / 256);
-            if (H == 1)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 1>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+            if (H == 1 && nt)
+                SV_LAUNCH(ps, (k_bspmv4<VT, 1, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+            else if (H == 1)
+                SV_LAUNCH(ps, (k_bspmv4<VT, 1, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+            else if (H == 2 && nt)
+                SV_LAUNCH(ps, (k_bspmv4<VT, 2, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
             else if (H == 2)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 2>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 2, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+            else if (nt)
+                SV_LAUNCH(ps, (k_bspmv4<VT, 4, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
             else
-                SV_LAUNCH(ps, (k_bspmv4<VT, 4>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 4, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
             SV_HIP(hipGetLastError());
             return SVFSI_OK;
         }
