@@ -109,6 +109,7 @@ struct GraphCache {
 struct Work {
     GraphCache graphs;   // mixed solve: one graph per outer step
     GraphCache gmGraphs; // FP64 GMRES: one graph per chunk of Arnoldi steps
+    GraphCache cgGraphs; // CG: one graph for a chunk of iterations
     PinnedPoll poll;
     DBuf W, X, R0, tmp, basis, basis32, Z32, val32, x32, y32;
     DBuf partial, state, hess;

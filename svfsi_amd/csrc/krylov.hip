@@ -33,9 +33,23 @@ struct CgState {
     int done, suc, itr, mItr;
     unsigned int counter;
     int pad;
+    double e[2]; // |R|^2 of the current and the next iteration, by iteration parity
 };
 
-// K P for the rows of this block, <P, K P>; last block: alpha = errO / <P,KP>.
+// Every reduction ends at a kernel boundary (no last-block fences, which on
+// a multi-XCD part write back L2): a kernel writes per-block partials and
+// each block of the next kernel sums them in the same fixed order, so all
+// blocks hold bitwise the same scalar.
+
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int m, double* lds)
+{
+    double t = 0.0;
+    for (int q = threadIdx.x; q < m; q += 256) t += part[q];
+    return block_sum<256>(t, lds);
+}
+
+// K P for the block rows of this block (grid-stride) and per-block
+// partials of <P, K P>.
 template <int DOF>
 __global__ __launch_bounds__(256) void k_cg_spmv_dot(int nNo, const int* __restrict__ rowPtr,
                                                      const int* __restrict__ colPtr,
@@ -43,73 +57,69 @@ __global__ __launch_bounds__(256) void k_cg_spmv_dot(int nNo, const int* __restr
                                                      const double* __restrict__ p,
                                                      double* __restrict__ kp,
                                                      double* __restrict__ partial,
-                                                     CgState* __restrict__ st)
+                                                     const CgState* __restrict__ st)
 {
     if (st->done) return;
     constexpr int DD = DOF * DOF;
     constexpr int G = kWave / DD;
     __shared__ double lds[4];
-    __shared__ int flag;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x >> 6;
     const int g = lane / DD;
     const int r = lane - g * DD;
     const int i = r / DOF;
     const int j = r - i * DOF;
-    const int row = (blockIdx.x * 4 + wave) * G + g;
-    const bool active = (g < G) && (row < nNo);
-    double acc = 0.0;
-    if (active) {
-        int k = rowPtr[row];
-        const int e = rowPtr[row + 1];
-        double a0 = 0.0, a1 = 0.0;
-        for (; k + 1 < e; k += 2) {
-            const int c0 = colPtr[k], c1 = colPtr[k + 1];
-            a0 += __builtin_nontemporal_load(val + (size_t)k * DD + r) * p[(size_t)c0 * DOF + j];
-            a1 += __builtin_nontemporal_load(val + (size_t)(k + 1) * DD + r) * p[(size_t)c1 * DOF + j];
-        }
-        if (k < e) a0 += __builtin_nontemporal_load(val + (size_t)k * DD + r) * p[(size_t)colPtr[k] * DOF + j];
-        acc = a0 + a1;
-    }
-    if constexpr (DOF == 4) {
-        acc += __shfl_xor(acc, 1, kWave);
-        acc += __shfl_xor(acc, 2, kWave);
-    } else if constexpr (DOF == 2) {
-        acc += __shfl_xor(acc, 1, kWave);
-    } else {
-        double t = 0.0;
-#pragma unroll
-        for (int q = 1; q < DOF; ++q) t += __shfl_down(acc, q, kWave);
-        acc += t;
-    }
     double contrib = 0.0;
-    if (active && j == 0) {
-        kp[(size_t)row * DOF + i] = acc;
-        contrib = p[(size_t)row * DOF + i] * acc;
+    for (int rb = blockIdx.x; rb * 4 * G < nNo; rb += gridDim.x) {
+        const int row = (rb * 4 + wave) * G + g;
+        const bool active = (g < G) && (row < nNo);
+        double acc = 0.0;
+        if (active) {
+            int k = rowPtr[row];
+            const int e = rowPtr[row + 1];
+            double a0 = 0.0, a1 = 0.0;
+            for (; k + 1 < e; k += 2) {
+                const int c0 = colPtr[k], c1 = colPtr[k + 1];
+                a0 += val[(size_t)k * DD + r] * p[(size_t)c0 * DOF + j];
+                a1 += val[(size_t)(k + 1) * DD + r] * p[(size_t)c1 * DOF + j];
+            }
+            if (k < e) a0 += val[(size_t)k * DD + r] * p[(size_t)colPtr[k] * DOF + j];
+            acc = a0 + a1;
+        }
+        if constexpr (DOF == 4) {
+            acc += __shfl_xor(acc, 1, kWave);
+            acc += __shfl_xor(acc, 2, kWave);
+        } else if constexpr (DOF == 2) {
+            acc += __shfl_xor(acc, 1, kWave);
+        } else {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 1; q < DOF; ++q) t += __shfl_down(acc, q, kWave);
+            acc += t;
+        }
+        if (active && j == 0) {
+            kp[(size_t)row * DOF + i] = acc;
+            contrib += p[(size_t)row * DOF + i] * acc;
+        }
     }
-    double s = block_sum<256>(contrib, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
-    if (last_block_done(&st->counter, &flag)) {
-        double t = 0.0;
-        for (int q = threadIdx.x; q < (int)gridDim.x; q += 256) t += load_agent(partial + q);
-        t = block_sum<256>(t, lds);
-        if (threadIdx.x == 0) st->alpha = st->err / t;
-    }
+    const double sb = block_sum<256>(contrib, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sb;
 }
 
-// X += alpha P ; R -= alpha KP ; err = |R|^2 ; convergence test of the next
-// loop top (CGRAD.f:205-215).
+// alpha = |R|^2 / <P, K P> ; X += alpha P ; R -= alpha KP ; partials of |R|^2
+// (CGRAD.f:216-226).  par: parity of the iteration.
 __global__ __launch_bounds__(256) void k_cg_update(size_t n, double* __restrict__ x,
                                                    double* __restrict__ rr,
                                                    const double* __restrict__ p,
                                                    const double* __restrict__ kp,
-                                                   double* __restrict__ partial,
-                                                   CgState* __restrict__ st)
+                                                   const double* __restrict__ part1, int m1,
+                                                   double* __restrict__ part2,
+                                                   CgState* __restrict__ st, int par)
 {
     if (st->done) return;
     __shared__ double lds[4];
-    __shared__ int flag;
-    const double alpha = st->alpha;
+    const double pkp = sum_partials(part1, m1, lds);
+    const double alpha = st->e[par] / pkp;
     double s = 0.0;
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
         x[q] = x[q] + alpha * p[q];
@@ -118,38 +128,47 @@ __global__ __launch_bounds__(256) void k_cg_update(size_t n, double* __restrict_
         s += rv * rv;
     }
     s = block_sum<256>(s, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
-    if (last_block_done(&st->counter, &flag)) {
-        double t = 0.0;
-        for (int q = threadIdx.x; q < (int)gridDim.x; q += 256) t += load_agent(partial + q);
-        t = block_sum<256>(t, lds);
-        if (threadIdx.x == 0) {
-            // FSILS_NORMV returns sqrt(sum); CGRADV squares it again.
-            const double nrm = sqrt(t);
-            const double err = nrm * nrm;
-            st->errO = st->err;
-            st->err = err;
-            st->itr += 1;
-            if (st->itr >= st->mItr) {
-                st->done = 1;             // loop exhausted: suc stays false
-            } else if (err < st->eps) {
-                st->done = 1;
-                st->suc = 1;
-            }
-        }
+    if (threadIdx.x == 0) {
+        part2[blockIdx.x] = s;
+        if (blockIdx.x == 0) st->alpha = alpha;
     }
 }
 
-// P = (P + (errO/err) R) * (err/errO)  (OMPSUMV then OMPMULV, CGRAD.f:227-229)
+// err = |R|^2 (FSILS_NORMV squared again by CGRADV), the convergence test of
+// the next loop top (CGRAD.f:205-215) by block 0, and
+// P = (P + (errO/err) R) * (err/errO)  (OMPSUMV then OMPMULV, CGRAD.f:227-229).
+// Block 0 may set the done flag while later blocks start; they then skip the
+// P update, which is not used after the end.
 __global__ __launch_bounds__(256) void k_cg_pupdate(size_t n, double* __restrict__ p,
                                                     const double* __restrict__ rr,
-                                                    const CgState* __restrict__ st)
+                                                    const double* __restrict__ part2, int m2,
+                                                    CgState* __restrict__ st, int par)
 {
-    if (st->done) return;
-    const double r1 = st->errO / st->err;
-    const double r2 = st->err / st->errO;
+    __shared__ double lds[4];
+    __shared__ int done_s;
+    if (threadIdx.x == 0) done_s = *(volatile int*)&st->done;
+    __syncthreads();
+    if (done_s) return;
+    const double t = sum_partials(part2, m2, lds);
+    const double nrm = sqrt(t);
+    const double err = nrm * nrm;
+    const double errO = st->e[par];
+    const double r1 = errO / err;
+    const double r2 = err / errO;
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
         p[q] = (p[q] + r1 * rr[q]) * r2;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->errO = errO;
+        st->err = err;
+        st->e[par ^ 1] = err;
+        st->itr += 1;
+        if (st->itr >= st->mItr) {
+            st->done = 1;             // loop exhausted: suc stays false
+        } else if (err < st->eps) {
+            st->done = 1;
+            st->suc = 1;
+        }
+    }
 }
 
 static unsigned vec_grid(size_t n)
@@ -194,30 +213,89 @@ int PinnedPoll::wait(int q)
     return SVFSI_OK;
 }
 
+GraphCache::~GraphCache()
+{
+    clear();
+    if (cap) (void)hipStreamDestroy(cap);
+}
+
+void GraphCache::clear()
+{
+    for (auto& x : e) (void)hipGraphExecDestroy(x.exec);
+    e.clear();
+    sig.clear();
+}
+
+// Replays the graph stored under `key`, capturing `enqueue(stream)` into it
+// first if needed.  Every buffer the captured launches use must be allocated
+// before the capture (no allocation happens while a stream captures).
+template <class F>
+static int graph_run(GraphCache& gc, int key, hipStream_t s, F&& enqueue)
+{
+    for (auto& x : gc.e)
+        if (x.key == key) {
+            SV_HIP(hipGraphLaunch(x.exec, s));
+            return SVFSI_OK;
+        }
+    if (!gc.cap) SV_HIP(hipStreamCreateWithFlags(&gc.cap, hipStreamNonBlocking));
+    SV_HIP(hipStreamBeginCapture(gc.cap, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(gc.cap);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(gc.cap, &g);
+    if (rc != SVFSI_OK || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != SVFSI_OK) return rc;
+        SV_HIP(ec);
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SV_HIP(ei);
+    gc.e.push_back({key, ex});
+    SV_HIP(hipGraphLaunch(ex, s));
+    return SVFSI_OK;
+}
+
+// Graph replay is on unless SVFSI_GRAPHS=0; it is off while SpMV profiling
+// is on (profiled launches carry events that a capture cannot hold).
+static bool graphs_enabled()
+{
+    const char* v = getenv("SVFSI_GRAPHS");
+    return !(v && v[0] == '0') && !prof().on;
+}
+
 int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s)
 {
+    if (dof < 2 || dof > 4) {
+        set_error("CG: dof %d not supported", dof);
+        return SVFSI_ERR_NOTSUP;
+    }
     const size_t n = (size_t)L.nNo * dof;
     Work& w = L.ws;
-    const int gs = (L.nNo + 4 * (kWave / (dof * dof)) - 1) / (4 * (kWave / (dof * dof)));
-    const unsigned gv = vec_grid(n);
+    const int rowsPerBlock = 4 * (kWave / (dof * dof));
+    int gs = (L.nNo + rowsPerBlock - 1) / rowsPerBlock;
+    if (gs > 1024) gs = 1024;
+    if (gs < 1) gs = 1;
+    unsigned gv = vec_grid(n);
+    if (gv > 512) gv = 512;
     SV_TRY(w.X.ensure(n * sizeof(double)));
     SV_TRY(w.tmp.ensure(2 * n * sizeof(double)));
-    // partials of the widest reduction (SpMV grid, vector grid, or the
-    // launch_dot grid of 512) followed by the scalar result slot
-    size_t nPart = (size_t)(gs > (int)gv ? gs : gv);
-    if (nPart < 512) nPart = 512;
-    SV_TRY(w.partial.ensure((nPart + 16) * sizeof(double)));
+    // partials: SpMV grid (<= 1024), vector grid (<= 512), launch_dot (512),
+    // then the scalar result slot
+    const size_t nPart = 1024 + 512 + 16;
+    SV_TRY(w.partial.ensure(nPart * sizeof(double)));
     SV_TRY(w.state.ensure(sizeof(CgState)));
     double* X = w.X.as<double>();
     double* P = w.tmp.as<double>();
     double* KP = P + n;
-    double* part = w.partial.as<double>();
-    double* scal = part + nPart + 8;
+    double* part1 = w.partial.as<double>();
+    double* part2 = part1 + 1024;
+    double* scal = part1 + nPart - 8;
     CgState* st = w.state.as<CgState>();
 
     ls.callD = wall_seconds();
     ls.suc = 0;
-    SV_TRY(launch_dot((int)n, R, R, part, scal, s));
+    SV_TRY(launch_dot((int)n, R, R, part1, scal, s));
     double rr = 0.0;
     SV_HIP(hipMemcpyAsync(&rr, scal, sizeof(double), hipMemcpyDeviceToHost, s));
     SV_HIP(hipStreamSynchronize(s));
@@ -227,6 +305,7 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
     CgState h0{};
     h0.errO = ls.iNorm * ls.iNorm;
     h0.err = h0.errO;
+    h0.e[0] = h0.err;
     h0.eps = eps;
     h0.mItr = ls.mItr;
     h0.itr = 0;
@@ -240,25 +319,45 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
 
     PinnedPoll& poll = w.poll;
     SV_TRY(poll.ensure(256));
-    const int chunk = 16;
+    constexpr int chunk = 16; // even: iteration parity = position in the chunk
+    // a chunk of iterations: the same launches every time, one graph
+    auto iters = [&](hipStream_t cs) -> int {
+        for (int it = 0; it < chunk; ++it) {
+            const int par = it & 1;
+            switch (dof) {
+            case 2: hipLaunchKernelGGL(k_cg_spmv_dot<2>, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
+            case 3: hipLaunchKernelGGL(k_cg_spmv_dot<3>, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
+            default: hipLaunchKernelGGL(k_cg_spmv_dot<4>, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
+            }
+            hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, cs, n, X, R, (const double*)P, (const double*)KP,
+                               (const double*)part1, gs, part2, st, par);
+            hipLaunchKernelGGL(k_cg_pupdate, dim3(gv), dim3(256), 0, cs, n, P, (const double*)R,
+                               (const double*)part2, (int)gv, st, par);
+        }
+        SV_HIP(hipGetLastError());
+        return SVFSI_OK;
+    };
+    const bool useGraphs = graphs_enabled();
+    if (useGraphs) {
+        const std::vector<uintptr_t> sig = {(uintptr_t)n, (uintptr_t)dof, (uintptr_t)K, (uintptr_t)R, (uintptr_t)X,
+                                            (uintptr_t)P, (uintptr_t)part1, (uintptr_t)st, (uintptr_t)L.rowPtr.p,
+                                            (uintptr_t)L.colPtr.p};
+        if (sig != w.cgGraphs.sig) {
+            w.cgGraphs.clear();
+            w.cgGraphs.sig = sig;
+        }
+    }
     int posted = 0;
     CgState hs = h0;
     bool finished = h0.done != 0;
     int launched = 0;
     while (!finished) {
         const int q = posted & 1;
-        for (int it = 0; it < chunk; ++it) {
-            switch (dof) {
-            case 2: hipLaunchKernelGGL(k_cg_spmv_dot<2>, dim3(gs), dim3(256), 0, s, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, P, KP, part, st); break;
-            case 3: hipLaunchKernelGGL(k_cg_spmv_dot<3>, dim3(gs), dim3(256), 0, s, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, P, KP, part, st); break;
-            case 4: hipLaunchKernelGGL(k_cg_spmv_dot<4>, dim3(gs), dim3(256), 0, s, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, P, KP, part, st); break;
-            default: set_error("CG: dof %d not supported", dof); return SVFSI_ERR_NOTSUP;
-            }
-            hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, s, n, X, R, P, KP, part, st);
-            hipLaunchKernelGGL(k_cg_pupdate, dim3(gv), dim3(256), 0, s, n, P, R, (const CgState*)st);
-        }
+        if (useGraphs)
+            SV_TRY(graph_run(w.cgGraphs, 0, s, iters));
+        else
+            SV_TRY(iters(s));
         launched += chunk;
-        SV_HIP(hipGetLastError());
         SV_TRY(poll.post(q, st, sizeof(CgState), s));
         ++posted;
         // check the previous chunk while this one runs
@@ -805,57 +904,6 @@ __global__ void k_sum_to(int m, const double* __restrict__ partial, double* __re
     for (int q = threadIdx.x; q < m; q += 256) s += partial[q];
     s = block_sum<256>(s, lds);
     if (threadIdx.x == 0) *out = s;
-}
-
-GraphCache::~GraphCache()
-{
-    clear();
-    if (cap) (void)hipStreamDestroy(cap);
-}
-
-void GraphCache::clear()
-{
-    for (auto& x : e) (void)hipGraphExecDestroy(x.exec);
-    e.clear();
-    sig.clear();
-}
-
-// Replays the graph stored under `key`, capturing `enqueue(stream)` into it
-// first if needed.  Every buffer the captured launches use must be allocated
-// before the capture (no allocation happens while a stream captures).
-template <class F>
-static int graph_run(GraphCache& gc, int key, hipStream_t s, F&& enqueue)
-{
-    for (auto& x : gc.e)
-        if (x.key == key) {
-            SV_HIP(hipGraphLaunch(x.exec, s));
-            return SVFSI_OK;
-        }
-    if (!gc.cap) SV_HIP(hipStreamCreateWithFlags(&gc.cap, hipStreamNonBlocking));
-    SV_HIP(hipStreamBeginCapture(gc.cap, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue(gc.cap);
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(gc.cap, &g);
-    if (rc != SVFSI_OK || ec != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        if (rc != SVFSI_OK) return rc;
-        SV_HIP(ec);
-    }
-    hipGraphExec_t ex = nullptr;
-    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    SV_HIP(ei);
-    gc.e.push_back({key, ex});
-    SV_HIP(hipGraphLaunch(ex, s));
-    return SVFSI_OK;
-}
-
-// Graph replay is on unless SVFSI_GRAPHS=0; it is off while SpMV profiling
-// is on (profiled launches carry events that a capture cannot hold).
-static bool graphs_enabled()
-{
-    const char* v = getenv("SVFSI_GRAPHS");
-    return !(v && v[0] == '0') && !prof().on;
 }
 
 // Generic operator: y = A x on the device (FP64 or FP32 matrix, plain or

@@ -272,15 +272,16 @@ def test_mixed_reaches_fp64_tolerance(gpu, orc):
     assert relerr(xg, xo) < 1e-4
 
 
-@pytest.mark.parametrize("mixed", [True, False])
-def test_graph_replays_match_direct_launches(gpu, monkeypatch, mixed):
-    """The GMRES solves replay fixed kernel sequences as HIP graphs (one per
-    outer step of the mixed solve, one per chunk of Arnoldi steps of the FP64
-    solve).  Several replays, with eager work and synchronisation between
-    them, must reproduce the directly launched solve bit for bit."""
+@pytest.mark.parametrize("mixed,cg", [(True, False), (False, False), (False, True)])
+def test_graph_replays_match_direct_launches(gpu, monkeypatch, mixed, cg):
+    """The solvers replay fixed kernel sequences as HIP graphs (one per outer
+    step of the mixed solve, one per chunk of Arnoldi steps of the FP64
+    GMRES, one per chunk of CG iterations).  Several replays, with eager work
+    and synchronisation between them, must reproduce the directly launched
+    solve bit for bit."""
     import torch
 
-    prob = problems.fluid_cylinder(h=0.5)
+    prob = problems.lelas_cube(8) if cg else problems.fluid_cylinder(h=0.5)
     st = NewtonStep(prob)
     monkeypatch.setenv("SVFSI_GRAPHS", "0")
     st.assemble()
