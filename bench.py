@@ -49,7 +49,7 @@ def measured_traffic(nnzb: int, kernel: str):
     except (OSError, ValueError):
         return None
     for r in rec.get("kernels", []):
-        if r.get("kernel") == kernel and r.get("nnzb") == nnzb:
+        if r.get("kernel", "").startswith(kernel) and r.get("nnzb") == nnzb:
             return r.get("traffic_bytes")
     return None
 
@@ -240,8 +240,8 @@ def main() -> int:
     speed = fp64_ms / mix_ms if (fp64_ms and mix_ms > 0) else None
     roof = None
     if gbs32 is not None:
-        kname = "k_bspmv4<float, 4>" if dof == 4 else f"k_bspmv<{dof}, float, float>"
-        roof = {"kernel": kname + " (FP32 block SpMV, inner GMRES)",
+        kname = "k_bspmv4<float, 4" if dof == 4 else f"k_bspmv<{dof}, float, float"
+        roof = {"kernel": kname + ">" + " (FP32 block SpMV, inner GMRES)",
                 "bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(nnzb, kname),
                 "bytes_per_launch": b32, "avg_launch_us": round(t32 / n32 * 1e3, 2), "launches": n32,

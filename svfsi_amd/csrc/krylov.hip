@@ -701,7 +701,6 @@ __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, i
     if (threadIdx.x == 0) stop_s = *(volatile int*)&st->stop;
     __syncthreads();
     if (stop_s) return;
-    constexpr int E = VecE<BT>::E;
     __shared__ double hsh[kMaxSD + 2];
     __shared__ double hc[kMaxSD + 1];
     __shared__ double inv_s;
@@ -754,29 +753,29 @@ __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, i
     }
     __syncthreads();
     const double inv = inv_s;
+    // one element per lane (coalesced 4- or 8-byte loads across the wave):
+    // four times the blocks of a vector-load layout, so the latency-bound
+    // sweep has a wave on every SIMD; the basis loads go out 16 at a time
     BT* w = basis + (size_t)i * n;
-    for (size_t q = ((size_t)blockIdx.x * 256 + threadIdx.x) * E; q < n; q += (size_t)gridDim.x * 256 * E) {
-        double v[E];
-        loadE(w, q, n, v);
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        double v = (double)w[q];
         int jj = 0;
-        for (; jj + 8 <= i; jj += 8) {   // eight loads in flight, same order
-            double u[8][E];
+        for (; jj + 16 <= i; jj += 16) {
+            BT u[16];
 #pragma unroll
-            for (int m = 0; m < 8; ++m) loadE(basis + (size_t)(jj + m) * n, q, n, u[m]);
+            for (int m = 0; m < 16; ++m) u[m] = basis[(size_t)(jj + m) * n + q];
 #pragma unroll
-            for (int e = 0; e < E; ++e)
-#pragma unroll
-                for (int m = 0; m < 8; ++m) v[e] = v[e] + hc[jj + m] * u[m][e];
+            for (int m = 0; m < 16; ++m) v = v + hc[jj + m] * (double)u[m];
         }
-        for (; jj < i; ++jj) {
-            double u[E];
-            loadE(basis + (size_t)jj * n, q, n, u);
+        for (; jj + 4 <= i; jj += 4) {
+            BT u[4];
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = v[e] + hc[jj] * u[e];
+            for (int m = 0; m < 4; ++m) u[m] = basis[(size_t)(jj + m) * n + q];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) v = v + hc[jj + m] * (double)u[m];
         }
-#pragma unroll
-        for (int e = 0; e < E; ++e)
-            if (q + e < n) w[q + e] = (BT)(v[e] * inv);
+        for (; jj < i; ++jj) v = v + hc[jj] * (double)basis[(size_t)jj * n + q];
+        w[q] = (BT)(v * inv);
     }
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     st->iters += 1;
@@ -1136,7 +1135,7 @@ static int gmres_inner_dev(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& bas
         SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, s, &st->stop));
         hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
                            (const BT*)basis, part, (const GmState*)st);
-        hipLaunchKernelGGL(k_gm_rotupdate<BT>, dim3((unsigned)nbr), dim3(256), 0, s, n, i, sD, (int)nbr,
+        hipLaunchKernelGGL(k_gm_rotupdate<BT>, dim3(gv), dim3(256), 0, s, n, i, sD, (int)nbr,
                            (const double*)part, basis, st, A);
     }
     {

@@ -84,14 +84,18 @@ def _setup(n, relTol):
     return prob, adj, z, u_obs
 
 
-def _fd_check(adj, z, u, u_obs, g, elems, h=1e-6):
+def _fd_check(adj, z, u, u_obs, g, elems, h=1e-6, rtol=1e-5, atol_frac=1e-6):
     """Central differences of L(z +- h e_k), u(z +- h e_k) from
     correction-form solves and L+ - L- = 0.5 (d+ - d-) . (2u + d+ + d- - 2 u_obs)
-    so that the difference is formed without cancellation."""
+    so that the difference is formed without cancellation.  Returns the
+    largest |fd - g_e| / (rtol |fd| + atol), atol = atol_frac max|g|: the FP64
+    forward and adjoint solves resolve g to ~1e-7 of its largest entry, which
+    bounds the relative agreement on elements with a small gradient."""
     import torch
 
     adj.prob.ls = dict(adj.prob.ls, relTol=1e-12)
-    errs = []
+    atol = atol_frac * g.abs().max().item()
+    worst = 0.0
     for e in elems:
         d = []
         for s in (1.0, -1.0):
@@ -101,9 +105,11 @@ def _fd_check(adj, z, u, u_obs, g, elems, h=1e-6):
             d.append(up - u)
         dl = 0.5 * torch.sum((d[0] - d[1]) * (2 * u + d[0] + d[1] - 2 * u_obs)).item()
         fd = dl / (2 * h)
-        errs.append(abs(fd - g[e].item()) / max(abs(fd), 1e-300))
-        print("fd", int(e), fd, g[e].item(), errs[-1], flush=True)
-    return max(errs)
+        ge = g[e].item()
+        ratio = abs(fd - ge) / (rtol * abs(fd) + atol)
+        worst = max(worst, ratio)
+        print("fd", int(e), fd, ge, abs(fd - ge) / max(abs(fd), 1e-300), ratio, flush=True)
+    return worst
 
 
 @pytest.mark.gpu
@@ -133,7 +139,7 @@ def test_adjoint_matches_cpu_adjoint_and_fd(gpu, orc):
     assert np.abs(gn - g_o).max() <= 1e-7 * np.abs(g_o).max()
     assert np.abs(u.detach().cpu().numpy() - u_o).max() <= 1e-9 * np.abs(u_o).max()
     elems = np.random.default_rng(11).choice(m.nEl, 20, replace=False)
-    assert _fd_check(adj, z, u.detach(), u_obs, g, elems) < 1e-5
+    assert _fd_check(adj, z, u.detach(), u_obs, g, elems) <= 1.0
 
 
 @pytest.mark.gpu
@@ -150,4 +156,4 @@ def test_adjoint_fd_1m_elements(gpu):
     L.backward()
     g = zr.grad.detach()
     elems = np.random.default_rng(12).choice(prob.mesh.nEl, 20, replace=False)
-    assert _fd_check(adj, z, u.detach(), u_obs, g, elems) < 1e-5
+    assert _fd_check(adj, z, u.detach(), u_obs, g, elems) <= 1.0

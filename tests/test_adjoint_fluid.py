@@ -8,8 +8,9 @@ the dense adjoint built on it against central differences of the dense
 solve.
 GPU: the dual-number kernel (svfsi_fluid_dmu_dot_dev) against the oracle;
 the autograd wrapper over the C ABI against the oracle's FP64 CPU adjoint
-and central differences (relative step 1e-4 in mu) on 20 random elements
-among the most sensitive ones."""
+and central differences (relative step 1e-4 in mu) on 20 random elements,
+to 1e-5 relative plus an absolute floor of 5e-6 of the largest gradient
+entry (the resolution of solves of this system, condition ~4e8)."""
 import numpy as np
 import pytest
 
@@ -129,13 +130,17 @@ def test_fluid_dmu_kernel_parity(gpu, orc):
     assert np.abs(gg - go).max() <= 1e-12 * np.abs(go).max()
 
 
-def _fd_check(adj, z, du, c, g, elems, h=1e-4):
+def _fd_check(adj, z, du, c, g, elems, h=1e-4, rtol=1e-5, atol_frac=5e-6):
     """Central differences of L = c . du(z +- h e_k), du(z +- h e_k) from
-    correction-form solves: L+ - L- = c . (d+ - d-)."""
+    correction-form solves: L+ - L- = c . (d+ - d-).  Returns the largest
+    |fd - g_e| / (rtol |fd| + atol), atol = atol_frac max|g|: with K's
+    condition number ~4e8 a correction solved to 1e-12 carries a relative
+    error up to ~4e-4, which bounds the agreement on the smaller entries."""
     import torch
 
     adj.prob.ls = dict(adj.prob.ls, relTol=1e-12, mItr=5000)
-    errs = []
+    atol = atol_frac * g.abs().max().item()
+    worst = 0.0
     for e in elems:
         d = []
         for s in (1.0, -1.0):
@@ -143,9 +148,11 @@ def _fd_check(adj, z, du, c, g, elems, h=1e-4):
             zz[e] += s * h
             d.append(adj.solve_correction(MU0 * torch.exp(zz), du) - du)
         fd = torch.sum(c * (d[0] - d[1])).item() / (2 * h)
-        errs.append(abs(fd - g[e].item()) / max(abs(fd), 1e-300))
-        print("fd", int(e), fd, g[e].item(), errs[-1], flush=True)
-    return max(errs)
+        ge = g[e].item()
+        ratio = abs(fd - ge) / (rtol * abs(fd) + atol)
+        worst = max(worst, ratio)
+        print("fd", int(e), fd, ge, abs(fd - ge) / max(abs(fd), 1e-300), ratio, flush=True)
+    return worst
 
 
 @pytest.mark.gpu
@@ -181,8 +188,5 @@ def test_viscosity_adjoint_vs_cpu_adjoint_and_fd(gpu, orc, mixed):
     err = np.abs(gn - g_o).max() / np.abs(g_o).max()
     print("cpu adjoint relerr", err, flush=True)
     assert err <= 1e-7
-    # 20 random elements among the 200 most sensitive: elsewhere dL/dz is
-    # below the resolution of a difference of solves of this system
-    top = np.argsort(-np.abs(gn))[:200]
-    elems = np.random.default_rng(13).choice(top, 20, replace=False)
-    assert _fd_check(adj, z, du.detach(), c, g, elems) < 1e-5
+    elems = np.random.default_rng(13).choice(m.nEl, 20, replace=False)
+    assert _fd_check(adj, z, du.detach(), c, g, elems) <= 1.0
