@@ -57,4 +57,39 @@ __device__ __forceinline__ bool last_block_done(unsigned int* counter, int* lds_
     return last;
 }
 
+// Row i of a row-major 4x4 block times a 4-vector (dof-4 SpMV helpers).
+// NT: streaming (nontemporal) loads of the matrix values, for matrices that
+// do not fit the last-level cache; otherwise ordinary loads, so that a
+// matrix reused across Krylov steps can stay resident in the 256 MB MALL.
+template <bool NT>
+__device__ __forceinline__ double dot4_nt(const double* __restrict__ v, const double* __restrict__ x)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    d2 a, b;
+    if constexpr (NT) {
+        a = __builtin_nontemporal_load(reinterpret_cast<const d2*>(v));
+        b = __builtin_nontemporal_load(reinterpret_cast<const d2*>(v) + 1);
+    } else {
+        a = *reinterpret_cast<const d2*>(v);
+        b = *(reinterpret_cast<const d2*>(v) + 1);
+    }
+    const d2 xa = *reinterpret_cast<const d2*>(x);
+    const d2 xb = *(reinterpret_cast<const d2*>(x) + 1);
+    return a.x * xa.x + a.y * xa.y + b.x * xb.x + b.y * xb.y;
+}
+
+template <bool NT>
+__device__ __forceinline__ double dot4_nt(const float* __restrict__ v, const float* __restrict__ x)
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 a;
+    if constexpr (NT)
+        a = __builtin_nontemporal_load(reinterpret_cast<const f4*>(v));
+    else
+        a = *reinterpret_cast<const f4*>(v);
+    const f4 xa = *reinterpret_cast<const f4*>(x);
+    return (double)a.x * (double)xa.x + (double)a.y * (double)xa.y + (double)a.z * (double)xa.z
+         + (double)a.w * (double)xa.w;
+}
+
 } // namespace svfsi

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace svfsi {
 
@@ -690,7 +691,9 @@ __global__ __launch_bounds__(256) void k_gm_update(size_t n, int i, int sD, BT* 
 // (u_{i+1} - sum_j h(j,i) u_j) / h(i+1,i); block 0 then applies the Givens
 // rotations and the convergence test.  Once block 0 has set the stop flag a
 // block that starts later skips its rows: u_{i+1} is not used after a stop.
-template <typename BT>
+// PRE: the coefficients were already summed into A.hraw by k_gm_hsum (the
+// fused SpMV + dot path); otherwise every block sums the multi-dot partials.
+template <typename BT, bool PRE = false>
 __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, int nb,
                                                       const double* __restrict__ partial, BT* __restrict__ basis,
                                                       GmState* __restrict__ st, GmArrays A)
@@ -704,30 +707,34 @@ __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, i
     __shared__ double hsh[kMaxSD + 2];
     __shared__ double hc[kMaxSD + 1];
     __shared__ double inv_s;
-    int npow = 1;
-    while (npow < i + 1) npow <<= 1;
-    int T = 256 / npow;
-    if (T > 64) T = 64;
-    if (T < 1) T = 1;
-    const int per = 256 / T;
-    for (int jj0 = 0; jj0 <= i; jj0 += per) {
-        const int jj = jj0 + threadIdx.x / T;
-        const int b0 = threadIdx.x % T;
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-        if (jj <= i) {
-            const double* pp = partial + (size_t)jj * nb;
-            int b = b0;
-            for (; b + 3 * T < nb; b += 4 * T) {
-                t0 += pp[b];
-                t1 += pp[b + T];
-                t2 += pp[b + 2 * T];
-                t3 += pp[b + 3 * T];
+    if constexpr (PRE) {
+        for (int jj = threadIdx.x; jj <= i; jj += 256) hsh[jj] = A.hraw[jj];
+    } else {
+        int npow = 1;
+        while (npow < i + 1) npow <<= 1;
+        int T = 256 / npow;
+        if (T > 64) T = 64;
+        if (T < 1) T = 1;
+        const int per = 256 / T;
+        for (int jj0 = 0; jj0 <= i; jj0 += per) {
+            const int jj = jj0 + threadIdx.x / T;
+            const int b0 = threadIdx.x % T;
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+            if (jj <= i) {
+                const double* pp = partial + (size_t)jj * nb;
+                int b = b0;
+                for (; b + 3 * T < nb; b += 4 * T) {
+                    t0 += pp[b];
+                    t1 += pp[b + T];
+                    t2 += pp[b + 2 * T];
+                    t3 += pp[b + 3 * T];
+                }
+                for (; b < nb; b += T) t0 += pp[b];
             }
-            for (; b < nb; b += T) t0 += pp[b];
+            double t = (t0 + t1) + (t2 + t3);
+            for (int o = T >> 1; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+            if (jj <= i && b0 == 0) hsh[jj] = t;
         }
-        double t = (t0 + t1) + (t2 + t3);
-        for (int o = T >> 1; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
-        if (jj <= i && b0 == 0) hsh[jj] = t;
     }
     __syncthreads();
     // Pythagorean h(i+1,i) = sqrt|<w,w> - sum h_j^2|: the sum over j < i by
@@ -806,6 +813,78 @@ __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, i
         st->stop = 1;
         st->iconv = sD;
     }
+}
+
+// Inner cycle, dof 4, FP32: the SpMV w = K u_{i-1} (written to basis slot
+// i) and, in the same pass, per-block partials of <u_j, w> for j < i and of
+// <w, w> (slot i), so the step needs no separate multi-dot sweep.  Lane
+// layout as k_bspmv4<float, 4>: 16 lanes per block row, lane (h, c) with
+// c the output component; after the row sum every lane holds w(row, c), and
+// lane (h, c) forms the products of the coefficients j = h, h + 4, ...
+// Supports i + 1 <= 32 coefficients.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_gm_spmv_dots4(int nNo, const int* __restrict__ rowPtr,
+                                                       const int* __restrict__ colPtr,
+                                                       const float* __restrict__ val, float* __restrict__ basis,
+                                                       size_t n, int i, double* __restrict__ partial, int nb,
+                                                       const GmState* __restrict__ st)
+{
+    if (st->stop) return;
+    __shared__ double red[4][32];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int row = t >> 4;
+    const int l = t & 15;
+    const int h = l >> 2, c = l & 3;
+    const float* x = basis + (size_t)(i - 1) * n;
+    double a0 = 0.0, a1 = 0.0;
+    if (row < nNo) {
+        const int e = rowPtr[row + 1];
+        int k = rowPtr[row] + h;
+        for (; k + 4 < e; k += 8) {
+            const int c0 = colPtr[k], c1 = colPtr[k + 4];
+            a0 += dot4_nt<NT>(val + (size_t)k * 16 + c * 4, x + (size_t)c0 * 4);
+            a1 += dot4_nt<NT>(val + (size_t)(k + 4) * 16 + c * 4, x + (size_t)c1 * 4);
+        }
+        if (k < e) a0 += dot4_nt<NT>(val + (size_t)k * 16 + c * 4, x + (size_t)colPtr[k] * 4);
+    }
+    double acc = a0 + a1;
+    acc += __shfl_xor(acc, 4, kWave);
+    acc += __shfl_xor(acc, 8, kWave);
+    const float wv = (float)acc;
+    const size_t q = (size_t)row * 4 + c;
+    if (row < nNo && h == 0) basis[(size_t)i * n + q] = wv;
+    const double wd = (double)wv;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int j = h + 4 * m;
+        double pv = 0.0;
+        if (row < nNo && j <= i) pv = (j == i ? wd : (double)basis[(size_t)j * n + q]) * wd;
+        // sum over the four components and the four rows of the wavefront
+        pv += __shfl_xor(pv, 1, kWave);
+        pv += __shfl_xor(pv, 2, kWave);
+        pv += __shfl_xor(pv, 16, kWave);
+        pv += __shfl_xor(pv, 32, kWave);
+        if ((lane & 0x33) == 0 && j <= i) red[wave][j] = pv;
+    }
+    __syncthreads();
+    if (threadIdx.x <= i)
+        partial[(size_t)threadIdx.x * nb + blockIdx.x] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
+// h(j,i), j <= i: the per-block partials of k_gm_spmv_dots4 summed in block
+// order, one block per coefficient, into A.hraw.
+__global__ __launch_bounds__(256) void k_gm_hsum(int i, int nb, const double* __restrict__ partial,
+                                                 const GmState* __restrict__ st, GmArrays A)
+{
+    if (st->stop) return;
+    __shared__ double lds[4];
+    const double* pp = partial + (size_t)blockIdx.x * nb;
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) t += pp[b];
+    t = block_sum<256>(t, lds);
+    if (threadIdx.x == 0) A.hraw[blockIdx.x] = t;
 }
 
 // Back substitution of the cycle (GMRES.f:369-376).  The block stages the
@@ -1110,6 +1189,25 @@ __global__ __launch_bounds__(256) void k_gm_xsolve(size_t n, BT* __restrict__ R,
     }
 }
 
+// Partial-sum slots of the inner cycle: the multi-dot (row chunks of 256
+// lanes x 4 elements) or the fused SpMV + dot (16 rows per block), and the
+// vector grid of the norm.
+static size_t inner_partials(int nNo, size_t n, int sD, size_t nbr, unsigned gv)
+{
+    size_t m = (size_t)(sD + 2) * nbr;
+    const size_t f = (size_t)(sD + 2) * (size_t)((nNo + 15) / 16);
+    if (f > m) m = f;
+    if (gv > m) m = gv;
+    return m + 16;
+}
+
+// The fused SpMV + multi-dot inner step is on unless SVFSI_FUSED_INNER=0.
+static bool fused_inner_enabled()
+{
+    const char* v = getenv("SVFSI_FUSED_INNER");
+    return !(v && v[0] == '0');
+}
+
 // One cycle of GMRES(sD) on R (RHS in, approximate solution out), enqueued
 // without any host synchronisation.  Uses the workspace's inner state.
 template <typename BT>
@@ -1121,7 +1219,7 @@ static int gmres_inner_dev(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& bas
     const size_t nbr = (n + (size_t)256 * VecE<BT>::E - 1) / ((size_t)256 * VecE<BT>::E);
     const unsigned gv = vec_grid(n);
     SV_TRY(basisBuf.ensure((size_t)(sD + 1) * n * sizeof(BT)));
-    const size_t partN = ((size_t)(sD + 2) * nbr > gv ? (size_t)(sD + 2) * nbr : gv) + 16;
+    const size_t partN = inner_partials(L.nNo, n, sD, nbr, gv);
     SV_TRY(w.partial.ensure(partN * sizeof(double)));
     SV_TRY(w.hess.ensure(gm_arrays_doubles(sD) * sizeof(double)));
     BT* basis = basisBuf.as<BT>();
@@ -1131,7 +1229,31 @@ static int gmres_inner_dev(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& bas
     hipLaunchKernelGGL(k_normsq_partial<BT>, dim3(gv), dim3(256), 0, s, n, (const BT*)R, part);
     hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(256), 0, s, (int)gv, (const double*)part, st, A, relTol, gate);
     hipLaunchKernelGGL(k_gm_scale_copy<BT>, dim3(gv), dim3(256), 0, s, n, (const BT*)R, basis, (const GmState*)st, A);
-    for (int i = 1; i <= sD; ++i) {
+    bool fused = false;
+    if constexpr (std::is_same<BT, float>::value) {
+        if (dof == 4 && sD + 1 <= 32 && fused_inner_enabled()) {
+            fused = true;
+            // fused SpMV + multi-dot, coefficient sums, update: three
+            // launches per Arnoldi step
+            const int nbS = (L.nNo + 15) / 16;
+            const bool nt = (size_t)L.nnz * 16 * sizeof(float) > ((size_t)128 << 20);
+            for (int i = 1; i <= sD; ++i) {
+                if (nt)
+                    hipLaunchKernelGGL(k_gm_spmv_dots4<true>, dim3((unsigned)nbS), dim3(256), 0, s, L.nNo,
+                                       L.rowPtr.as<int>(), L.colPtr.as<int>(), op.K, basis, n, i, part, nbS,
+                                       (const GmState*)st);
+                else
+                    hipLaunchKernelGGL(k_gm_spmv_dots4<false>, dim3((unsigned)nbS), dim3(256), 0, s, L.nNo,
+                                       L.rowPtr.as<int>(), L.colPtr.as<int>(), op.K, basis, n, i, part, nbS,
+                                       (const GmState*)st);
+                hipLaunchKernelGGL(k_gm_hsum, dim3((unsigned)(i + 1)), dim3(256), 0, s, i, nbS, (const double*)part,
+                                   (const GmState*)st, A);
+                hipLaunchKernelGGL((k_gm_rotupdate<BT, true>), dim3(gv), dim3(256), 0, s, n, i, sD, nbS,
+                                   (const double*)part, basis, st, A);
+            }
+        }
+    }
+    for (int i = 1; i <= sD && !fused; ++i) {
         SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, s, &st->stop));
         hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, s, n, i,
                            (const BT*)basis, part, (const GmState*)st);
@@ -1248,8 +1370,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     {
         const size_t nbi = (n + (size_t)256 * VecE<float>::E - 1) / ((size_t)256 * VecE<float>::E);
         SV_TRY(w.basis32.ensure((size_t)(mi + 1) * n * sizeof(float)));
-        const size_t pi = ((size_t)(mi + 2) * nbi > gv ? (size_t)(mi + 2) * nbi : gv) + 16;
-        SV_TRY(w.partial.ensure(pi * sizeof(double)));
+        SV_TRY(w.partial.ensure(inner_partials(L.nNo, n, mi, nbi, gv) * sizeof(double)));
         SV_TRY(w.hess.ensure(gm_arrays_doubles(mi) * sizeof(double)));
     }
     const bool useGraphs = graphs_enabled();
@@ -1261,7 +1382,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
             (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)K32,
             (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
             (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
-            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p};
+            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled()};
         if (sig != w.graphs.sig) {
             w.graphs.clear();
             w.graphs.sig = sig;
