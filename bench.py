@@ -153,9 +153,11 @@ def main() -> int:
         L.svfsi_prof_enable(0)
         cnt = [C.c_longlong(), C.c_longlong()]
         tot = [C.c_double(), C.c_double()]
+        byt = [C.c_double(), C.c_double()]
         for kk in (0, 1):
             L.svfsi_prof_get(kk, C.byref(cnt[kk]), C.byref(tot[kk]))
-        return (b - a) * 1e3 / k, lss, [(cnt[q].value, tot[q].value) for q in (0, 1)]
+            L.svfsi_prof_get_bytes(kk, C.byref(byt[kk]))
+        return (b - a) * 1e3 / k, lss, [(cnt[q].value, tot[q].value, byt[q].value) for q in (0, 1)]
 
     # phase split (outside the timed loops): assembly alone
     torch.cuda.synchronize()
@@ -166,7 +168,7 @@ def main() -> int:
     asm_ms = (time.perf_counter() - a) * 1e3 / max(1, args.steps)
 
     if args.mixed_only:
-        fp64_ms, ls64, prof64 = None, None, [(0, 0.0), (0, 0.0)]
+        fp64_ms, ls64, prof64 = None, None, [(0, 0.0, 0.0), (0, 0.0, 0.0)]
     else:
         fp64_ms, ls64, _ = timed(False, args.steps, False)
         _, _, prof64 = timed(False, 1, True)
@@ -177,11 +179,13 @@ def main() -> int:
     # Launches queued after an inner cycle has converged exit at once; the
     # achieved rate charges their time to the launches that did the work
     # (inner SpMVs = cM minus the outer FP64 products).
-    n32, t32 = profmx[1]
+    # The library accumulates each profiled launch's algorithmic bytes (for
+    # the fused inner step: SpMV + the basis vectors its multi-dot reads).
+    n32, t32, by32 = profmx[1]
     n64m = profmx[0][0]
     n32_full = max(1, min(n32, lsprof[-1].RI.cM - n64m)) if n32 else 0
-    n64, t64 = prof64[0]
-    b32 = spmv_bytes(nnzb, m.nNo, dof, 4)
+    n64, t64, by64 = prof64[0]
+    b32 = by32 / n32 if n32 else spmv_bytes(nnzb, m.nNo, dof, 4)
     b64 = spmv_bytes(nnzb, m.nNo, dof, 8)
     gbs32 = b32 * n32_full / (t32 * 1e-3) / 1e9 if n32 else None
     gbs64 = b64 / (t64 / n64 * 1e-3) / 1e9 if n64 else None
@@ -240,11 +244,17 @@ def main() -> int:
     speed = fp64_ms / mix_ms if (fp64_ms and mix_ms > 0) else None
     roof = None
     if gbs32 is not None:
-        kname = "k_bspmv4<float, 4" if dof == 4 else f"k_bspmv<{dof}, float, float"
-        roof = {"kernel": kname + ">" + " (FP32 block SpMV, inner GMRES)",
+        fused = dof == 4 and os.environ.get("SVFSI_FUSED_INNER", "1") != "0"
+        if fused:
+            nt = "true" if nnzb * 64 > (128 << 20) else "false"
+            kname, what = f"k_gm_spmv_dots4<{nt}", "FP32 block SpMV fused with the inner multi-dot"
+        else:
+            kname = "k_bspmv4<float, 4" if dof == 4 else f"k_bspmv<{dof}, float, float"
+            what = "FP32 block SpMV, inner GMRES"
+        roof = {"kernel": f"{kname}> ({what})",
                 "bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(nnzb, kname),
-                "bytes_per_launch": b32, "avg_launch_us": round(t32 / n32 * 1e3, 2), "launches": n32,
+                "bytes_per_launch": round(b32), "avg_launch_us": round(t32 / n32 * 1e3, 2), "launches": n32,
                 "full_work_launches": n32_full, "timing": "events attached to each dispatch (hipExtLaunchKernelGGL)"}
     out = {
         "metric": "Assembly+GMRES time per Newton step at 1e-8 residual; FP64 vs mixed; SpMV GB/s",

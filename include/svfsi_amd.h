@@ -285,8 +285,13 @@ int fsils_gpu_solve_t_mixed_dev(void* lhs, FSILS_lsType* ls, int dof,
 /* Profiling hook (HIP events around the SpMV launches of the solvers)     */
 /* ---------------------------------------------------------------------- */
 int svfsi_prof_enable(int on);
-/* kernel: 0 = FP64 SpMV, 1 = FP32 SpMV.  Returns launches and total ms. */
+/* kernel: 0 = FP64 SpMV, 1 = FP32 SpMV (including the inner solver's SpMV
+ * fused with its multi-dot).  Returns launches and total ms. */
 int svfsi_prof_get(int kernel, long long* count, double* total_ms);
+/* Algorithmic bytes of the profiled launches of `kernel` (values, indices,
+ * row pointers, x and y once; for the fused inner step also the basis
+ * vectors its multi-dot reads and its partial sums). */
+int svfsi_prof_get_bytes(int kernel, double* bytes);
 int svfsi_prof_reset(void);
 
 #ifdef __cplusplus

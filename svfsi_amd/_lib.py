@@ -100,6 +100,7 @@ SIGNATURES = {
     "svfsi_fluid_dmu_dot_dev": [_P, C.POINTER(EqParams), _I, _P, _P, _P, _P, _P, _P, _P],
     "svfsi_prof_enable": [_I],
     "svfsi_prof_get": [_I, C.POINTER(C.c_longlong), _DP],
+    "svfsi_prof_get_bytes": [_I, C.POINTER(C.c_double)],
     "svfsi_prof_reset": [],
 }
 

@@ -39,6 +39,7 @@ void prof_resolve()
         (void)hipEventElapsedTime(&ms, e.a, e.b);
         P.count[e.k] += 1;
         P.ms[e.k] += ms;
+        P.bytes[e.k] += e.bytes;
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
@@ -812,12 +813,21 @@ int svfsi_prof_get(int kernel, long long* count, double* total_ms)
     return SVFSI_OK;
 }
 
+int svfsi_prof_get_bytes(int kernel, double* bytes)
+{
+    SV_ARG(kernel == 0 || kernel == 1, "kernel must be 0 or 1");
+    prof_resolve();
+    if (bytes) *bytes = prof().bytes[kernel];
+    return SVFSI_OK;
+}
+
 int svfsi_prof_reset(void)
 {
     prof_resolve();
     Prof& P = prof();
     P.count[0] = P.count[1] = 0;
     P.ms[0] = P.ms[1] = 0.0;
+    P.bytes[0] = P.bytes[1] = 0.0;
     return SVFSI_OK;
 }
 

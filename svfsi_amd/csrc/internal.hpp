@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <string>
@@ -160,15 +161,45 @@ struct Mesh {
 struct ProfPending {
     hipEvent_t a, b;
     int k;
+    double bytes;
 };
 struct Prof {
     bool on = false;
     long long count[2] = {0, 0};
     double ms[2] = {0.0, 0.0};
+    double bytes[2] = {0.0, 0.0}; // algorithmic bytes of the timed launches
     std::vector<ProfPending> pending;
 };
 Prof& prof();
 void prof_resolve();
+
+// Kernel-accurate timing of a profiled launch: the events are attached to
+// the dispatch itself (hipExtLaunchKernelGGL), so they bracket the kernel's
+// execution and not the launch gaps around it.  Resolved lazily by
+// prof_resolve() so that profiling adds no host synchronisation.
+struct ProfScope {
+    hipEvent_t a = nullptr, b = nullptr;
+    int k;
+    double bytes;
+    explicit ProfScope(int kernel, double algBytes = 0.0) : k(kernel), bytes(algBytes) {
+        if (prof().on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+        }
+    }
+    ~ProfScope() {
+        if (a) prof().pending.push_back({a, b, k, bytes});
+    }
+};
+
+#define SV_LAUNCH(ps, kern, grid, block, shm, st, ...)                                           \
+    do {                                                                                          \
+        if ((ps).a)                                                                               \
+            hipExtLaunchKernelGGL(kern, grid, block, shm, st, (ps).a, (ps).b, 0, __VA_ARGS__);    \
+        else                                                                                      \
+            hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);                          \
+    } while (0)
+
 
 // ---- launchers (device pointers; stream may be null) ----
 int launch_spmv(const Lhs& L, int dof, const double* K, const double* U,

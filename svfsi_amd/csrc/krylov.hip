@@ -1237,15 +1237,18 @@ static int gmres_inner_dev(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& bas
             // launches per Arnoldi step
             const int nbS = (L.nNo + 15) / 16;
             const bool nt = (size_t)L.nnz * 16 * sizeof(float) > ((size_t)128 << 20);
+            const double spmvBytes = (double)L.nnz * 64.0 + 4.0 * L.nnz + 4.0 * (L.nNo + 1) + 2.0 * n * 4.0;
             for (int i = 1; i <= sD; ++i) {
+                // SpMV bytes + the i basis vectors the dots read + partials
+                ProfScope ps(1, spmvBytes + (double)i * n * 4.0 + (double)(i + 1) * nbS * 8.0);
                 if (nt)
-                    hipLaunchKernelGGL(k_gm_spmv_dots4<true>, dim3((unsigned)nbS), dim3(256), 0, s, L.nNo,
-                                       L.rowPtr.as<int>(), L.colPtr.as<int>(), op.K, basis, n, i, part, nbS,
-                                       (const GmState*)st);
+                    SV_LAUNCH(ps, k_gm_spmv_dots4<true>, dim3((unsigned)nbS), dim3(256), 0, s, L.nNo,
+                              L.rowPtr.as<int>(), L.colPtr.as<int>(), op.K, basis, n, i, part, nbS,
+                              (const GmState*)st);
                 else
-                    hipLaunchKernelGGL(k_gm_spmv_dots4<false>, dim3((unsigned)nbS), dim3(256), 0, s, L.nNo,
-                                       L.rowPtr.as<int>(), L.colPtr.as<int>(), op.K, basis, n, i, part, nbS,
-                                       (const GmState*)st);
+                    SV_LAUNCH(ps, k_gm_spmv_dots4<false>, dim3((unsigned)nbS), dim3(256), 0, s, L.nNo,
+                              L.rowPtr.as<int>(), L.colPtr.as<int>(), op.K, basis, n, i, part, nbS,
+                              (const GmState*)st);
                 hipLaunchKernelGGL(k_gm_hsum, dim3((unsigned)(i + 1)), dim3(256), 0, s, i, nbS, (const double*)part,
                                    (const GmState*)st, A);
                 hipLaunchKernelGGL((k_gm_rotupdate<BT, true>), dim3(gv), dim3(256), 0, s, n, i, sD, nbS,

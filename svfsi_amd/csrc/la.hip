@@ -158,38 +158,14 @@ static inline int spmv_grid(int nNo, int dof)
     return (nNo + rowsPerBlock - 1) / rowsPerBlock;
 }
 
-// Kernel-accurate timing of a profiled launch: the events are attached to
-// the dispatch itself (hipExtLaunchKernelGGL), so they bracket the kernel's
-// execution and not the launch gaps around it.  Resolved lazily by
-// prof_resolve() so that profiling adds no host synchronisation.
-struct ProfScope {
-    hipEvent_t a = nullptr, b = nullptr;
-    int k;
-    explicit ProfScope(int kernel) : k(kernel) {
-        if (prof().on) {
-            (void)hipEventCreate(&a);
-            (void)hipEventCreate(&b);
-        }
-    }
-    ~ProfScope() {
-        if (a) prof().pending.push_back({a, b, k});
-    }
-};
-
-#define SV_LAUNCH(ps, kern, grid, block, shm, st, ...)                                           \
-    do {                                                                                          \
-        if ((ps).a)                                                                               \
-            hipExtLaunchKernelGGL(kern, grid, block, shm, st, (ps).a, (ps).b, 0, __VA_ARGS__);    \
-        else                                                                                      \
-            hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);                          \
-    } while (0)
-
 template <typename VT, typename XT>
 static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU,
                          hipStream_t s, int profKernel, const int* stop)
 {
     if (L.nNo == 0) return SVFSI_OK;
-    ProfScope ps(profKernel);
+    // algorithmic bytes: values, column indices, row pointers, x once, y once
+    ProfScope ps(profKernel, (double)L.nnz * dof * dof * sizeof(VT) + 4.0 * L.nnz + 4.0 * (L.nNo + 1) +
+                                 2.0 * L.nNo * dof * sizeof(XT));
     const int* rp = L.rowPtr.as<int>();
     const int* cp = L.colPtr.as<int>();
     if constexpr (std::is_same<VT, XT>::value) {
