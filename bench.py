@@ -244,7 +244,7 @@ def main() -> int:
     speed = fp64_ms / mix_ms if (fp64_ms and mix_ms > 0) else None
     roof = None
     if gbs32 is not None:
-        fused = dof == 4 and os.environ.get("SVFSI_FUSED_INNER", "1") != "0"
+        fused = dof == 4 and os.environ.get("SVFSI_FUSED_INNER", "0") == "1"
         if fused:
             nt = "true" if nnzb * 64 > (128 << 20) else "false"
             kname, what = f"k_gm_spmv_dots4<{nt}", "FP32 block SpMV fused with the inner multi-dot"

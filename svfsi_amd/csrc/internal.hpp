@@ -117,6 +117,7 @@ struct Work {
     DBuf zdir, opartial, ostate, ohess; // flexible GMRES: FP32 directions, outer state
     DBuf mask;
     DBuf valT; // K^T values for transposed solves
+    DBuf dinv32; // inverse diagonal blocks (FP32) of the inner block-Jacobi
     int maskDof = -1;
     std::vector<int> maskInc;
     int maskVersion = -1;
@@ -205,7 +206,11 @@ struct ProfScope {
 int launch_spmv(const Lhs& L, int dof, const double* K, const double* U,
                 double* KU, hipStream_t s, const int* stop = nullptr);
 int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
-                    float* KU, hipStream_t s, const int* stop = nullptr);
+                    float* KU, hipStream_t s, const int* stop = nullptr,
+                    const float* dinv = nullptr);
+int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s);
+int launch_blockdiag_apply(const Lhs& L, int dof, const float* dinv, float* x, hipStream_t s,
+                           const int* stop);
 int launch_spmv_t(const Lhs& L, int dof, const double* K, const double* U,
                   double* KU, hipStream_t s, const int* stop = nullptr);
 int launch_transpose_blocks(const Lhs& L, int dof, const double* V, double* VT, hipStream_t s);

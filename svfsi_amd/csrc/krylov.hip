@@ -996,8 +996,9 @@ template <> struct GmOp<double> {
 };
 template <> struct GmOp<float> {
     const Lhs* L; int dof; const float* K;
+    const float* dinv; // optional block-Jacobi: the operator is D^-1 K
     int apply(const float* x, float* y, hipStream_t s, const int* stop) const {
-        return launch_spmv_f32(*L, dof, K, x, y, s, stop);
+        return launch_spmv_f32(*L, dof, K, x, y, s, stop, dinv);
     }
 };
 
@@ -1201,11 +1202,13 @@ static size_t inner_partials(int nNo, size_t n, int sD, size_t nbr, unsigned gv)
     return m + 16;
 }
 
-// The fused SpMV + multi-dot inner step is on unless SVFSI_FUSED_INNER=0.
+// The fused SpMV + multi-dot inner step is off unless SVFSI_FUSED_INNER=1:
+// measured on the 200k-tet cylinder it is slower (14.1 us + 4.1 us for the
+// coefficient sums) than the SpMV and multi-dot it replaces (8.9 + 6.2 us).
 static bool fused_inner_enabled()
 {
     const char* v = getenv("SVFSI_FUSED_INNER");
-    return !(v && v[0] == '0');
+    return v && v[0] == '1';
 }
 
 // One cycle of GMRES(sD) on R (RHS in, approximate solution out), enqueued
@@ -1231,7 +1234,7 @@ static int gmres_inner_dev(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& bas
     hipLaunchKernelGGL(k_gm_scale_copy<BT>, dim3(gv), dim3(256), 0, s, n, (const BT*)R, basis, (const GmState*)st, A);
     bool fused = false;
     if constexpr (std::is_same<BT, float>::value) {
-        if (dof == 4 && sD + 1 <= 32 && fused_inner_enabled()) {
+        if (dof == 4 && sD + 1 <= 32 && fused_inner_enabled() && !op.dinv) {
             fused = true;
             // fused SpMV + multi-dot, coefficient sums, update: three
             // launches per Arnoldi step
@@ -1347,7 +1350,17 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     double* scal = part + partN - 8;
     GmState* st = w.ostate.as<GmState>();
     GmArrays A = gm_carve(w.ohess.as<double>(), mo);
-    GmOp<float> op32{&L, dof, K32};
+    // block-Jacobi inside the FP32 inner cycle (dof 4; SVFSI_INNER_BJ=0 turns
+    // it off): the inner GMRES solves D^-1 K z = D^-1 q, D the 4x4 diagonal
+    // blocks of the scaled matrix
+    const bool bj = dof == 4 && env_int("SVFSI_INNER_BJ", 1) != 0;
+    const float* dinv = nullptr;
+    if (bj) {
+        SV_TRY(w.dinv32.ensure((size_t)L.nNo * 16 * sizeof(float)));
+        SV_TRY(launch_blockdiag_inv(L, dof, K, w.dinv32.as<float>(), s));
+        dinv = w.dinv32.as<float>();
+    }
+    GmOp<float> op32{&L, dof, K32, dinv};
 
     ls.callD = wall_seconds();
     ls.suc = 0;
@@ -1385,7 +1398,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
             (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)K32,
             (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
             (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
-            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled()};
+            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv};
         if (sig != w.graphs.sig) {
             w.graphs.clear();
             w.graphs.sig = sig;
@@ -1413,6 +1426,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
                 float* zi = Z + (size_t)(i - 1) * n;
                 hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, cs, n,
                                    (const double*)(Q + (size_t)(i - 1) * n), zi);
+                if (dinv) SV_TRY(launch_blockdiag_apply(L, dof, dinv, zi, cs, &st->stop));
                 SV_TRY(gmres_inner_dev<float>(L, op32, dof, zi, w.basis32, mi, tin, &st->stop, cs));
                 hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, cs, n, (const float*)zi, t64);
                 SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, cs, &st->stop));

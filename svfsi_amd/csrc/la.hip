@@ -76,13 +76,16 @@ __global__ __launch_bounds__(256) void k_bspmv(int nNo, const int* __restrict__ 
 // values: 32 B FP64 / 16 B FP32) plus the 4-vector U(:,col) with vector
 // loads; no cross-lane reduction.  16 block rows per wavefront, two blocks
 // in flight per lane.
+// dinv (optional): per-node inverse diagonal blocks (row-major 4x4); the
+// output is then D^-1 (K x) (block-Jacobi left preconditioning).
 template <typename VT, int H, bool NT>
 __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__ rowPtr,
                                                 const int* __restrict__ colPtr,
                                                 const VT* __restrict__ val,
                                                 const VT* __restrict__ x,
                                                 VT* __restrict__ y,
-                                                const int* __restrict__ stop)
+                                                const int* __restrict__ stop,
+                                                const VT* __restrict__ dinv)
 {
     // 4*H lanes per block row: lane (h, i) takes blocks k = s + h, s + h + H,
     // ... and output component i; the H partial sums meet by lane shuffles.
@@ -105,7 +108,118 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
     double acc = a0 + a1;
 #pragma unroll
     for (int o = 4; o < 4 * H; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+    if (dinv) {
+        // every lane holds (K x)(row, i); gather the row's four components
+        const int base = (threadIdx.x & (kWave - 1)) & ~3;
+        double kx[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) kx[m] = __shfl(acc, base + m, kWave);
+        double v = 0.0;
+        if (row < nNo) {
+            const VT* d = dinv + (size_t)row * 16 + i * 4;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) v += (double)d[m] * kx[m];
+        }
+        acc = v;
+    }
     if (row < nNo && h == 0) y[(size_t)row * 4 + i] = (VT)acc;
+}
+
+// Inverse of each node's diagonal block (dof 4, Gauss-Jordan with partial
+// pivoting in FP64), stored FP32 row-major.  Rows and columns zeroed by the
+// Dirichlet mask get a unit diagonal first.
+__global__ __launch_bounds__(256) void k_blockdiag_inv4(int nNo, const int* __restrict__ diagPtr,
+                                                        const double* __restrict__ val, float* __restrict__ dinv)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nNo) return;
+    double a[4][8];
+    const double* b = val + (size_t)diagPtr[r] * 16;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a[p][q] = b[p * 4 + q];
+            a[p][4 + q] = (p == q) ? 1.0 : 0.0;
+        }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        bool zero = true;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) zero = zero && a[p][q] == 0.0 && a[q][p] == 0.0;
+        if (zero) a[p][p] = 1.0;
+    }
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        double best = fabs(a[c][c]);
+#pragma unroll
+        for (int p = c + 1; p < 4; ++p)
+            if (fabs(a[p][c]) > best) { best = fabs(a[p][c]); piv = p; }
+        if (best == 0.0) { ok = false; break; }
+        if (piv != c)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { const double t = a[c][q]; a[c][q] = a[piv][q]; a[piv][q] = t; }
+        const double inv = 1.0 / a[c][c];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[c][q] *= inv;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+            if (p != c) {
+                const double f = a[p][c];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) a[p][q] -= f * a[c][q];
+            }
+    }
+    float* o = dinv + (size_t)r * 16;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[p * 4 + q] = ok ? (float)a[p][4 + q] : (p == q ? 1.0f : 0.0f);
+}
+
+// x = D^-1 x per node (dof 4, FP32, in place).
+__global__ __launch_bounds__(256) void k_blockdiag_apply4(int nNo, const float* __restrict__ dinv,
+                                                          float* __restrict__ x, const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nNo) return;
+    float xi[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) xi[m] = x[(size_t)r * 4 + m];
+    const float* d = dinv + (size_t)r * 16;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        double v = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) v += (double)d[p * 4 + m] * (double)xi[m];
+        x[(size_t)r * 4 + p] = (float)v;
+    }
+}
+
+int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s)
+{
+    if (dof != 4) {
+        set_error("block-Jacobi: dof %d not supported (4)", dof);
+        return SVFSI_ERR_NOTSUP;
+    }
+    hipLaunchKernelGGL(k_blockdiag_inv4, dim3((L.nNo + 255) / 256), dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val,
+                       dinv);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_blockdiag_apply(const Lhs& L, int dof, const float* dinv, float* x, hipStream_t s, const int* stop)
+{
+    if (dof != 4) {
+        set_error("block-Jacobi: dof %d not supported (4)", dof);
+        return SVFSI_ERR_NOTSUP;
+    }
+    hipLaunchKernelGGL(k_blockdiag_apply4, dim3((L.nNo + 255) / 256), dim3(256), 0, s, L.nNo, dinv, x, stop);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
 }
 
 // Transpose product: row i of K^T gathers block (col,i) through the
@@ -160,7 +274,7 @@ static inline int spmv_grid(int nNo, int dof)
 
 template <typename VT, typename XT>
 static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU,
-                         hipStream_t s, int profKernel, const int* stop)
+                         hipStream_t s, int profKernel, const int* stop, const VT* dinv = nullptr)
 {
     if (L.nNo == 0) return SVFSI_OK;
     // algorithmic bytes: values, column indices, row pointers, x once, y once
@@ -186,17 +300,17 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
             const bool nt = ntMode >= 0 ? ntMode != 0 : (size_t)L.nnz * 16 * sizeof(VT) > ((size_t)128 << 20);
             const int g4 = (int)(((size_t)L.nNo * 4 * H + 255) / 256);
             if (H == 1 && nt)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 1, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 1, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
             else if (H == 1)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 1, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 1, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
             else if (H == 2 && nt)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 2, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 2, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
             else if (H == 2)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 2, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 2, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
             else if (nt)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 4, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 4, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
             else
-                SV_LAUNCH(ps, (k_bspmv4<VT, 4, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 4, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
             SV_HIP(hipGetLastError());
             return SVFSI_OK;
         }
@@ -219,9 +333,13 @@ int launch_spmv(const Lhs& L, int dof, const double* K, const double* U, double*
 }
 
 int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U, float* KU,
-                    hipStream_t s, const int* stop)
+                    hipStream_t s, const int* stop, const float* dinv)
 {
-    return spmv_dispatch<float, float>(L, dof, K, U, KU, s, 1, stop);
+    if (dinv && dof != 4) {
+        set_error("block-Jacobi SpMV: dof %d not supported (4)", dof);
+        return SVFSI_ERR_NOTSUP;
+    }
+    return spmv_dispatch<float, float>(L, dof, K, U, KU, s, 1, stop, dinv);
 }
 
 int launch_spmv_t(const Lhs& L, int dof, const double* K, const double* U, double* KU,
