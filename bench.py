@@ -38,6 +38,11 @@ def spmv_bytes(nnzb: int, nNo: int, dof: int, s: int) -> int:
     return nnzb * dof * dof * s + 4 * nnzb + 4 * (nNo + 1) + 2 * nNo * dof * s
 
 
+def log(*a) -> None:
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print("#", *a, file=sys.stderr, flush=True)
+
+
 def measured_traffic(nnzb: int, kernel: str):
     """HBM bytes per launch of `kernel` from the PMC summary in profiles/
     (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy kernel of known
@@ -129,10 +134,12 @@ def main() -> int:
         return ls
 
     # warmup both paths
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
         if not args.mixed_only:
-            run(False)
-        run(True)
+            ls = run(False)
+            log(f"warmup {w} fp64: itr {ls.RI.itr} suc {ls.RI.suc}")
+        ls = run(True)
+        log(f"warmup {w} mixed: outer {ls.RI.itr} spmv {ls.RI.cM} suc {ls.RI.suc}")
     torch.cuda.synchronize()
 
     def timed(mixed: bool, k: int, profile: bool):
@@ -167,13 +174,18 @@ def main() -> int:
     torch.cuda.synchronize()
     asm_ms = (time.perf_counter() - a) * 1e3 / max(1, args.steps)
 
+    log(f"assembly {asm_ms:.3f} ms")
     if args.mixed_only:
         fp64_ms, ls64, prof64 = None, None, [(0, 0.0, 0.0), (0, 0.0, 0.0)]
     else:
         fp64_ms, ls64, _ = timed(False, args.steps, False)
+        log(f"fp64 {fp64_ms:.3f} ms/step")
         _, _, prof64 = timed(False, 1, True)
+        log("fp64 profiled run done")
     mix_ms, lsmx, _ = timed(True, args.steps, False)
+    log(f"mixed {mix_ms:.3f} ms/step")
     _, lsprof, profmx = timed(True, 1, True)
+    log("mixed profiled run done")
 
     # SpMV roofline: FP32 SpMV of the mixed path (the inner solver's kernel).
     # Launches queued after an inner cycle has converged exit at once; the
@@ -204,9 +216,11 @@ def main() -> int:
         # CPU leg, bounded sample: the full FP64 oracle assembly, its diagonal
         # scaling, and `cycles` restart cycles of GMRESV; the per-iteration
         # time is scaled to the FP64 iteration count of the same step.
+        log("cpu leg: oracle assembly")
         c0 = time.perf_counter()
         Vo, Ro = orc.assemble(prob, st.rowPtr - 1, st.colPtr - 1)
         c1 = time.perf_counter()
+        log(f"cpu leg: assembly {c1 - c0:.1f} s; solver sample")
         Vs, Rs, W = orc.precond_diag(st.rowPtr - 1, st.colPtr - 1, dof, Vo, Ro, mask)
         c2 = time.perf_counter()
         smp = dict(prob.ls)
@@ -225,6 +239,7 @@ def main() -> int:
         cpu_ms = ((c1 - c0) + (c2 - c1) + per_it * itr64) * 1e3
         acc["assembly_rel_err_vs_cpu"] = float(np.abs(V0 - Vo).max() / np.abs(Vo).max())
         acc["residual_rel_err_vs_cpu"] = float(np.abs(R0 - Ro).max() / np.abs(Ro).max())
+        log(f"cpu leg: {so.itr} iterations in {c3 - c2:.1f} s; accuracy checks")
         # FP64 scaled residual of each GPU solution, recomputed on the CPU
         for name, mixed in (("fp64", False), ("mixed", True)):
             st.assemble()

@@ -278,8 +278,10 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
 {
     if (L.nNo == 0) return SVFSI_OK;
     // algorithmic bytes: values, column indices, row pointers, x once, y once
+    // (+ the inverse diagonal blocks of a block-Jacobi epilogue)
     ProfScope ps(profKernel, (double)L.nnz * dof * dof * sizeof(VT) + 4.0 * L.nnz + 4.0 * (L.nNo + 1) +
-                                 2.0 * L.nNo * dof * sizeof(XT));
+                                 2.0 * L.nNo * dof * sizeof(XT) +
+                                 (dinv ? (double)L.nNo * dof * dof * sizeof(VT) : 0.0));
     const int* rp = L.rowPtr.as<int>();
     const int* cp = L.colPtr.as<int>();
     if constexpr (std::is_same<VT, XT>::value) {
