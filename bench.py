@@ -72,6 +72,8 @@ def main() -> int:
     ap.add_argument("--h", type=float, default=0.11, help="cylinder lattice spacing (0.11 -> ~200k tets)")
     ap.add_argument("--sD", type=int, default=0, help="GMRES restart length (default: 50 fluid, 100 struct)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--mitr", type=int, default=0,
+                    help="cap the solver's restart count (profiling and sweeps; 0 keeps the workload's setting)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (calibrates PMC byte counters)")
     ap.add_argument("--mixed-only", action="store_true",
@@ -121,6 +123,8 @@ def main() -> int:
         n = args.n or 20
         prob = problems.lelas_cube(n)
         wl = f"LELAS3D tet4 cube {n}^3 x 6, one assembly + CG to 1e-8"
+    if args.mitr:
+        prob.ls = dict(prob.ls, mItr=args.mitr)
     st = NewtonStep(prob)
     t_setup = time.time() - t0
     m = prob.mesh

@@ -107,6 +107,58 @@ __global__ __launch_bounds__(256) void k_cg_spmv_dot(int nNo, const int* __restr
     if (threadIdx.x == 0) partial[blockIdx.x] = sb;
 }
 
+// dof 2 / 3: the value-streaming layout of the SpMV (LPR lanes per block
+// row reading the row's contiguous values in coalesced strides), grid-stride
+// over rows, per-block partials of <P, K P>.
+template <int DOF, int LPR>
+__global__ __launch_bounds__(256) void k_cg_spmv_dot_vs(int nNo, const int* __restrict__ rowPtr,
+                                                        const int* __restrict__ colPtr,
+                                                        const double* __restrict__ val,
+                                                        const double* __restrict__ p,
+                                                        double* __restrict__ kp,
+                                                        double* __restrict__ partial,
+                                                        const CgState* __restrict__ st)
+{
+    if (st->done) return;
+    constexpr int DD = DOF * DOF;
+    constexpr int RPB = 256 / LPR;
+    __shared__ double lds[4];
+    const int l = threadIdx.x % LPR;
+    double contrib = 0.0;
+    for (int rb = blockIdx.x; rb * RPB < nNo; rb += gridDim.x) {
+        const int row = rb * RPB + threadIdx.x / LPR;
+        double acc[DOF];
+#pragma unroll
+        for (int i = 0; i < DOF; ++i) acc[i] = 0.0;
+        if (row < nNo) {
+            const int k0 = rowPtr[row];
+            const int nv = (rowPtr[row + 1] - k0) * DD;
+            const double* vr = val + (size_t)k0 * DD;
+            const int* cr = colPtr + k0;
+            for (int v = l; v < nv; v += LPR) {
+                const int k = v / DD, e = v - k * DD;
+                const double a = vr[v] * p[(size_t)cr[k] * DOF + (e % DOF)];
+#pragma unroll
+                for (int i = 0; i < DOF; ++i) acc[i] += (e / DOF == i) ? a : 0.0;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < DOF; ++i)
+#pragma unroll
+            for (int o = LPR >> 1; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, kWave);
+        if (row < nNo && l < DOF) {
+            double out = acc[0];
+#pragma unroll
+            for (int i = 1; i < DOF; ++i)
+                if (l == i) out = acc[i];
+            kp[(size_t)row * DOF + l] = out;
+            contrib += p[(size_t)row * DOF + l] * out;
+        }
+    }
+    const double sb = block_sum<256>(contrib, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sb;
+}
+
 // alpha = |R|^2 / <P, K P> ; X += alpha P ; R -= alpha KP ; partials of |R|^2
 // (CGRAD.f:216-226).  par: parity of the iteration.
 __global__ __launch_bounds__(256) void k_cg_update(size_t n, double* __restrict__ x,
@@ -273,7 +325,8 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
     }
     const size_t n = (size_t)L.nNo * dof;
     Work& w = L.ws;
-    const int rowsPerBlock = 4 * (kWave / (dof * dof));
+    const bool vs = dof <= 3;  // value-streaming SpMV for dof 2 / 3
+    const int rowsPerBlock = vs ? 256 / 16 : 4 * (kWave / (dof * dof));
     int gs = (L.nNo + rowsPerBlock - 1) / rowsPerBlock;
     if (gs > 1024) gs = 1024;
     if (gs < 1) gs = 1;
@@ -326,8 +379,8 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
         for (int it = 0; it < chunk; ++it) {
             const int par = it & 1;
             switch (dof) {
-            case 2: hipLaunchKernelGGL(k_cg_spmv_dot<2>, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
-            case 3: hipLaunchKernelGGL(k_cg_spmv_dot<3>, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
+            case 2: hipLaunchKernelGGL((k_cg_spmv_dot_vs<2, 16>), dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
+            case 3: hipLaunchKernelGGL((k_cg_spmv_dot_vs<3, 16>), dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
             default: hipLaunchKernelGGL(k_cg_spmv_dot<4>, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
             }
             hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, cs, n, X, R, (const double*)P, (const double*)KP,

@@ -71,6 +71,67 @@ __global__ __launch_bounds__(256) void k_bspmv(int nNo, const int* __restrict__ 
     if (active && j == 0) y[(size_t)row * DOF + i] = (XT)acc;
 }
 
+// Value-streaming block SpMV for dof 2 and 3, whose dof x dof blocks are
+// not 16-byte multiples: LPR lanes own a block row and read its values (one
+// contiguous array of nb*dof^2 entries) in coalesced strides of LPR values;
+// the lane holding value v adds val[v] x(col(v/dd), (v % dd) % dof) to output
+// (v % dd) / dof.  The three (or two) partial sums then meet by shuffles.
+template <int DOF, typename VT, typename XT, int LPR, bool NT>
+__global__ __launch_bounds__(256) void k_bspmv_vs(int nNo, const int* __restrict__ rowPtr,
+                                                  const int* __restrict__ colPtr,
+                                                  const VT* __restrict__ val,
+                                                  const XT* __restrict__ x,
+                                                  XT* __restrict__ y,
+                                                  const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    constexpr int DD = DOF * DOF;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int row = t / LPR;
+    const int l = t - row * LPR;
+    double acc[DOF];
+#pragma unroll
+    for (int i = 0; i < DOF; ++i) acc[i] = 0.0;
+    if (row < nNo) {
+        const int k0 = rowPtr[row];
+        const int nv = (rowPtr[row + 1] - k0) * DD;
+        const VT* vr = val + (size_t)k0 * DD;
+        const int* cr = colPtr + k0;
+        int v = l;
+        for (; v + LPR < nv; v += 2 * LPR) {
+            const int va = v, vb = v + LPR;
+            const double a = NT ? (double)__builtin_nontemporal_load(vr + va) : (double)vr[va];
+            const double b = NT ? (double)__builtin_nontemporal_load(vr + vb) : (double)vr[vb];
+            const int ka = va / DD, ea = va - ka * DD, kb = vb / DD, eb = vb - kb * DD;
+            const double xa = (double)x[(size_t)cr[ka] * DOF + (ea % DOF)];
+            const double xb = (double)x[(size_t)cr[kb] * DOF + (eb % DOF)];
+#pragma unroll
+            for (int i = 0; i < DOF; ++i) {
+                acc[i] += (ea / DOF == i) ? a * xa : 0.0;
+                acc[i] += (eb / DOF == i) ? b * xb : 0.0;
+            }
+        }
+        if (v < nv) {
+            const double a = NT ? (double)__builtin_nontemporal_load(vr + v) : (double)vr[v];
+            const int ka = v / DD, ea = v - ka * DD;
+            const double xa = (double)x[(size_t)cr[ka] * DOF + (ea % DOF)];
+#pragma unroll
+            for (int i = 0; i < DOF; ++i) acc[i] += (ea / DOF == i) ? a * xa : 0.0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DOF; ++i)
+#pragma unroll
+        for (int o = LPR >> 1; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, kWave);
+    if (row < nNo && l < DOF) {
+        double out = acc[0];
+#pragma unroll
+        for (int i = 1; i < DOF; ++i)
+            if (l == i) out = acc[i];
+        y[(size_t)row * DOF + l] = (XT)out;
+    }
+}
+
 // dof = 4 specialisation: 4 lanes per block row, lane i owns output
 // component i and streams row i of every block of the row (4 contiguous
 // values: 32 B FP64 / 16 B FP32) plus the 4-vector U(:,col) with vector
@@ -313,6 +374,36 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
                 SV_LAUNCH(ps, (k_bspmv4<VT, 4, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
             else
                 SV_LAUNCH(ps, (k_bspmv4<VT, 4, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
+            SV_HIP(hipGetLastError());
+            return SVFSI_OK;
+        }
+    }
+    if (dof == 2 || dof == 3) {
+        // value-streaming layout (tuning knob SVFSI_SPMV_LPR = 8 / 16 / 32;
+        // 0 selects the block layout below)
+        static const int lpr = [] {
+            const char* e = getenv("SVFSI_SPMV_LPR");
+            const int v = e ? atoi(e) : 16;
+            return (v == 0 || v == 8 || v == 32) ? v : 16;
+        }();
+        const bool nt = (size_t)L.nnz * dof * dof * sizeof(VT) > ((size_t)128 << 20);
+        if (lpr) {
+            const unsigned gvs = (unsigned)(((size_t)L.nNo * lpr + 255) / 256);
+#define SV_VS(D, P)                                                                                           \
+    do {                                                                                                      \
+        if (nt)                                                                                               \
+            SV_LAUNCH(ps, (k_bspmv_vs<D, VT, XT, P, true>), dim3(gvs), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, \
+                      stop);                                                                                  \
+        else                                                                                                  \
+            SV_LAUNCH(ps, (k_bspmv_vs<D, VT, XT, P, false>), dim3(gvs), dim3(256), 0, s, L.nNo, rp, cp, K, U,    \
+                      KU, stop);                                                                              \
+    } while (0)
+            if (dof == 2) {
+                if (lpr == 8) SV_VS(2, 8); else if (lpr == 32) SV_VS(2, 32); else SV_VS(2, 16);
+            } else {
+                if (lpr == 8) SV_VS(3, 8); else if (lpr == 32) SV_VS(3, 32); else SV_VS(3, 16);
+            }
+#undef SV_VS
             SV_HIP(hipGetLastError());
             return SVFSI_OK;
         }
