@@ -54,7 +54,7 @@ def measured_traffic(nnzb: int, kernel: str):
     except (OSError, ValueError):
         return None
     for r in rec.get("kernels", []):
-        if r.get("kernel", "").startswith(kernel) and r.get("nnzb") == nnzb:
+        if r.get("kernel", "").startswith(kernel.split("*")[0]) and r.get("nnzb") == nnzb:
             return r.get("traffic_bytes")
     return None
 
@@ -264,7 +264,13 @@ def main() -> int:
     roof = None
     if gbs32 is not None:
         fused = dof == 4 and os.environ.get("SVFSI_FUSED_INNER", "0") == "1"
-        if fused:
+        poly = dof == 4 and os.environ.get("SVFSI_INNER_POLY", "1") != "0"
+        if poly:
+            nt = "true" if nnzb * 64 > (128 << 20) else "false"
+            kname = f"k_poly4<*, {nt}"
+            what = ("polynomial preconditioner steps: FP32 block SpMV with the block-Jacobi and vector "
+                    "epilogues, modes 0/1/2 together, plus the setup Arnoldi SpMVs")
+        elif fused:
             nt = "true" if nnzb * 64 > (128 << 20) else "false"
             kname, what = f"k_gm_spmv_dots4<{nt}", "FP32 block SpMV fused with the inner multi-dot"
         else:

@@ -281,6 +281,15 @@ int fsils_gpu_solve_t_mixed_dev(void* lhs, FSILS_lsType* ls, int dof,
                                 const double* dVal, const double* drhs,
                                 double* dlam, const int* incL, void* stream);
 
+/* Host utility of the polynomial inner preconditioner (no reference
+ * counterpart): the harmonic Ritz values of an Arnoldi Hessenberg Hu
+ * ((d+1) x d, column-major, leading dimension ld), Leja-ordered and grouped
+ * into factors of 1 - a p(a): kind[s] = 2: 1 - coef[2s] a + coef[2s+1] a^2
+ * (a conjugate pair or two real roots), kind[s] = 1: 1 - coef[2s] a.
+ * Returns the number of factors (<= maxFactors) or a negative code. */
+int svfsi_poly_factors(int d, int ld, const double* Hu, double* coef,
+                       int* kind, int maxFactors);
+
 /* ---------------------------------------------------------------------- */
 /* Profiling hook (HIP events around the SpMV launches of the solvers)     */
 /* ---------------------------------------------------------------------- */

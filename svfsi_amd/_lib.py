@@ -98,6 +98,7 @@ SIGNATURES = {
     "svfsi_assemble_backflow_dev": [_P, _I, C.POINTER(EqParams), _I, _P, _P, _P, _P],
     "svfsi_lelas_dkde_dot_dev": [_P, C.POINTER(EqParams), _P, _P, _P, _P],
     "svfsi_fluid_dmu_dot_dev": [_P, C.POINTER(EqParams), _I, _P, _P, _P, _P, _P, _P, _P],
+    "svfsi_poly_factors": [_I, _I, _DP, _DP, _IP, _I],
     "svfsi_prof_enable": [_I],
     "svfsi_prof_get": [_I, C.POINTER(C.c_longlong), _DP],
     "svfsi_prof_get_bytes": [_I, C.POINTER(C.c_double)],

@@ -748,6 +748,22 @@ int svfsi_fluid_dmu_dot_dev(void* mesh, const svfsi_eq_params* p, int tDof, cons
                             static_cast<hipStream_t>(stream));
 }
 
+int svfsi_poly_factors(int d, int ld, const double* Hu, double* coef, int* kind, int maxFactors)
+{
+    SV_ARG(d >= 1 && ld >= d + 1 && Hu && coef && kind && maxFactors >= 1, "bad argument");
+    std::vector<double> c;
+    std::vector<int> k;
+    const int nf = poly_factors(d, ld, Hu, c, k);
+    if (nf < 0) {
+        set_error("no usable harmonic Ritz values");
+        return SVFSI_ERR_STATE;
+    }
+    SV_ARG(nf <= maxFactors, "%d factors exceed maxFactors %d", nf, maxFactors);
+    std::copy(c.begin(), c.end(), coef);
+    std::copy(k.begin(), k.end(), kind);
+    return nf;
+}
+
 int svfsi_lhsa(int nNo, int nEl, int eNoN, const int* IEN, int* nnz, int* rowPtr, int* colPtr)
 {
     SV_ARG(nNo > 0 && nEl > 0 && eNoN > 0 && IEN && nnz && rowPtr, "bad argument");

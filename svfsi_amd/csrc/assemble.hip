@@ -17,6 +17,12 @@
 
 #include <cmath>
 
+// The element routines are evaluated without contracting multiply-adds into
+// FMAs, in the oracle's operation order: Delaunay meshes contain sliver
+// elements whose Jacobian inversion amplifies any difference in rounding, and
+// the FP64 assembly is to reproduce the CPU restatement to ~1e-12 on them too.
+#pragma clang fp contract(off)
+
 namespace svfsi {
 
 // GNN for an affine tet4 (svFSI/NN.f:1469-1565): Nxi of TET4 are constant

@@ -118,6 +118,7 @@ struct Work {
     DBuf mask;
     DBuf valT; // K^T values for transposed solves
     DBuf dinv32; // inverse diagonal blocks (FP32) of the inner block-Jacobi
+    DBuf polyCoef, polyVec; // polynomial preconditioner: factors, work vectors
     int maskDof = -1;
     std::vector<int> maskInc;
     int maskVersion = -1;
@@ -209,6 +210,12 @@ int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
                     float* KU, hipStream_t s, const int* stop = nullptr,
                     const float* dinv = nullptr);
 int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s);
+int launch_poly4(const Lhs& L, int mode, const float* K, const float* dinv, const float* x, const float* prodOld,
+                 float* w, float* y, float* out, const double* coef, int step, hipStream_t s, const int* stop);
+int launch_poly_last(size_t n, const float* prod, float* y, const double* coef, int step, hipStream_t s,
+                     const int* stop);
+int launch_rand_masked(size_t n, unsigned seed, const double* W, float* v, hipStream_t s);
+int poly_factors(int d, int ld, const double* Hu, std::vector<double>& coef, std::vector<int>& kind);
 int launch_blockdiag_apply(const Lhs& L, int dof, const float* dinv, float* x, hipStream_t s,
                            const int* stop);
 int launch_spmv_t(const Lhs& L, int dof, const double* K, const double* U,

@@ -455,9 +455,13 @@ struct GmArrays {
     double* err;   // sD+1
     double* y;     // sD
     double* hraw;  // sD+2: CGS coefficients of the current step, then 1/h(i+1,i)
+    double* Hu;    // (sD+1, sD): the unrotated Arnoldi Hessenberg (inner cycle)
 };
 
-static size_t gm_arrays_doubles(int sD) { return (size_t)(sD + 1) * sD + 3 * sD + (sD + 1) + (sD + 3); }
+static size_t gm_arrays_doubles(int sD)
+{
+    return 2 * (size_t)(sD + 1) * sD + 3 * sD + (sD + 1) + (sD + 3);
+}
 
 static GmArrays gm_carve(double* base, int sD)
 {
@@ -468,6 +472,7 @@ static GmArrays gm_carve(double* base, int sD)
     a.err = a.sn + sD;
     a.y = a.err + (sD + 1);
     a.hraw = a.y + sD;
+    a.Hu = a.hraw + (sD + 3);
     return a;
 }
 
@@ -839,6 +844,10 @@ __global__ __launch_bounds__(256) void k_gm_rotupdate(size_t n, int i, int sD, i
     }
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     st->iters += 1;
+    {
+        double* Hc = A.Hu + (size_t)(i - 1) * (sD + 1); // unrotated column i
+        for (int jj = 0; jj <= i; ++jj) Hc[jj] = hsh[jj];
+    }
     for (int jj = 1; jj <= i - 1; ++jj) {
         const double c = cs_s[0][jj - 1], sn = cs_s[1][jj - 1];
         const double tmp = c * hsh[jj - 1] + sn * hsh[jj];
@@ -1463,6 +1472,61 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     SV_TRY(poll.ensure(256));
     GmState hs{};
     int outerMv = 0;
+
+    // Polynomial inner preconditioner (dof 4; SVFSI_INNER_POLY=0 keeps the
+    // inner GMRES cycle): d Arnoldi steps of the FP32 inner operator B from
+    // a seeded random start give the harmonic Ritz values, and every outer
+    // step then applies the fixed GMRES(d) residual polynomial of B, one
+    // SpMV per root and no inner dot products.
+    std::vector<int> kinds;
+    int polySpmv = 0;
+    long long polySetup = 0;
+    float *P0 = nullptr, *P1 = nullptr, *Wb = nullptr;
+    const double* coefD = nullptr;
+    bool usePoly = dof == 4 && env_int("SVFSI_INNER_POLY", 1) != 0;
+    if (usePoly) {
+        const int d = env_int("SVFSI_POLY_DEG", 30);
+        if (d < 2 || d > kMaxSD) {
+            set_error("SVFSI_POLY_DEG must be in 2..%d", kMaxSD);
+            return SVFSI_ERR_ARG;
+        }
+        SV_TRY(w.polyVec.ensure(3 * n * sizeof(float)));
+        P0 = w.polyVec.as<float>();
+        P1 = P0 + n;
+        Wb = P1 + n;
+        SV_TRY(w.basis32.ensure((size_t)(std::max(mi, d) + 1) * n * sizeof(float)));
+        {
+            const size_t nbi = (n + (size_t)256 * VecE<float>::E - 1) / ((size_t)256 * VecE<float>::E);
+            SV_TRY(w.partial.ensure(inner_partials(L.nNo, n, std::max(mi, d), nbi, gv) * sizeof(double)));
+            SV_TRY(w.hess.ensure(gm_arrays_doubles(std::max(mi, d)) * sizeof(double)));
+        }
+        SV_TRY(launch_rand_masked(n, 0x5f3759dfu, w.W.as<double>(), P0, s));
+        SV_TRY(gmres_inner_dev<float>(L, op32, dof, P0, w.basis32, d, 0.0, nullptr, s));
+        GmArrays Ad = gm_carve(w.hess.as<double>(), d);
+        std::vector<double> Hu((size_t)(d + 1) * d);
+        SV_HIP(hipMemcpyAsync(Hu.data(), Ad.Hu, Hu.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        GmState ins{};
+        SV_HIP(hipMemcpyAsync(&ins, w.state.p, sizeof(GmState), hipMemcpyDeviceToHost, s));
+        SV_HIP(hipStreamSynchronize(s));
+        polySetup = ins.iters;
+        std::vector<double> coef;
+        const int nf = ins.iconv == d ? poly_factors(d, d + 1, Hu.data(), coef, kinds) : -1;
+        if (nf > 0) {
+            SV_TRY(w.polyCoef.ensure(coef.size() * sizeof(double)));
+            SV_HIP(hipMemcpyAsync(w.polyCoef.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  s));
+            coefD = w.polyCoef.as<double>();
+            for (int f = 0; f < nf; ++f) {
+                const bool last = f == nf - 1;
+                if (kinds[f] == 1) polySpmv += last ? 0 : 1;
+                else polySpmv += 1 + ((!last && coef[2 * f + 1] != 0.0) ? 1 : 0);
+            }
+        } else {
+            usePoly = false; // no usable roots: the inner GMRES cycle
+        }
+        hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, w.state.as<GmState>(), 0.0);
+    }
+    long long polySteps = 0;
     for (int l = 1; l <= ls.mItr; ++l) {
         ls.dB = ls.fNorm;
         ls.itr += 1;
@@ -1477,10 +1541,41 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
             // every cycle, so they replay as one graph per i
             auto step = [&, i](hipStream_t cs) -> int {
                 float* zi = Z + (size_t)(i - 1) * n;
-                hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, cs, n,
-                                   (const double*)(Q + (size_t)(i - 1) * n), zi);
-                if (dinv) SV_TRY(launch_blockdiag_apply(L, dof, dinv, zi, cs, &st->stop));
-                SV_TRY(gmres_inner_dev<float>(L, op32, dof, zi, w.basis32, mi, tin, &st->stop, cs));
+                if (usePoly) {
+                    // z_i = p(B) D^-1 q_i, factor by factor
+                    hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, cs, n,
+                                       (const double*)(Q + (size_t)(i - 1) * n), P0);
+                    if (dinv) SV_TRY(launch_blockdiag_apply(L, dof, dinv, P0, cs, &st->stop));
+                    hipLaunchKernelGGL(k_set_zero<float>, dim3(gv), dim3(256), 0, cs, n, zi);
+                    float* cur = P0;
+                    float* nxt = P1;
+                    const int nf = (int)kinds.size();
+                    for (int f = 0; f < nf; ++f) {
+                        const bool last = f == nf - 1;
+                        if (kinds[f] == 1) {
+                            if (last) {
+                                SV_TRY(launch_poly_last(n, cur, zi, coefD, f, cs, &st->stop));
+                            } else {
+                                SV_TRY(launch_poly4(L, 0, K32, dinv, cur, nullptr, nullptr, zi, nxt, coefD, f, cs,
+                                                    &st->stop));
+                                std::swap(cur, nxt);
+                            }
+                        } else {
+                            SV_TRY(launch_poly4(L, 1, K32, dinv, cur, nullptr, Wb, zi, nullptr, coefD, f, cs,
+                                                &st->stop));
+                            if (!last) {
+                                SV_TRY(launch_poly4(L, 2, K32, dinv, Wb, cur, nullptr, nullptr, nxt, coefD, f, cs,
+                                                    &st->stop));
+                                std::swap(cur, nxt);
+                            }
+                        }
+                    }
+                } else {
+                    hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, cs, n,
+                                       (const double*)(Q + (size_t)(i - 1) * n), zi);
+                    if (dinv) SV_TRY(launch_blockdiag_apply(L, dof, dinv, zi, cs, &st->stop));
+                    SV_TRY(gmres_inner_dev<float>(L, op32, dof, zi, w.basis32, mi, tin, &st->stop, cs));
+                }
                 hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, cs, n, (const float*)zi, t64);
                 SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, cs, &st->stop));
                 // outer Arnoldi: CGS2 (two classical passes)
@@ -1495,11 +1590,14 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
                 SV_HIP(hipGetLastError());
                 return SVFSI_OK;
             };
-            if (useGraphs)
+            // the polynomial steps read their coefficients from device memory
+            // but their launch sequence depends on the roots: launched directly
+            if (useGraphs && !usePoly)
                 SV_TRY(graph_run(w.graphs, i, s, step));
             else
                 SV_TRY(step(s));
             outerMv += 1;
+            if (usePoly) ++polySteps;
             // lagged poll: the state of step i-1 is read while step i runs
             const int q = posted & 1;
             SV_TRY(poll.post(q, st, sizeof(GmState), s));
@@ -1530,7 +1628,10 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
         GmState in{};
         SV_HIP(hipMemcpyAsync(&in, w.state.p, sizeof(GmState), hipMemcpyDeviceToHost, s));
         SV_HIP(hipStreamSynchronize(s));
-        ls.cM = in.iters + outerMv;
+        // SpMVs: inner GMRES steps (or the polynomial's setup Arnoldi steps
+        // plus its products), and the outer FP64 products (an outer step
+        // launched after convergence exits at once but is counted)
+        ls.cM = usePoly ? (int)(polySetup + polySteps * polySpmv + outerMv) : in.iters + outerMv;
     }
     SV_HIP(hipMemcpyAsync(R, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     SV_HIP(hipStreamSynchronize(s));

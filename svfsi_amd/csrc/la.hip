@@ -186,6 +186,120 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
     if (row < nNo && h == 0) y[(size_t)row * 4 + i] = (VT)acc;
 }
 
+// ---------------------------------------------------------------------------
+// Polynomial preconditioner steps (dof 4, FP32 vectors, FP64 arithmetic).
+// p(B) v with 1 - a p(a) = prod_s f_s(a) is accumulated factor by factor:
+// prod <- f_s(B) prod, y <- y + (the matching part of p).  For a quadratic
+// factor 1 - c1 a + c2 a^2:
+//   mode 1: w = B prod;           y += c1 prod - c2 w
+//   mode 2: prod' = prod - c1 w + c2 B w     (the SpMV is skipped if c2 = 0)
+// For a linear factor 1 - c1 a:
+//   mode 0: prod' = prod - c1 B prod;  y += c1 prod
+// B = D^-1 K when dinv is given (block-Jacobi), else K.  coef is a device
+// array (c1, c2 per factor), so the launches do not depend on the values.
+// ---------------------------------------------------------------------------
+template <int MODE, bool NT>
+__global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ rowPtr,
+                                               const int* __restrict__ colPtr, const float* __restrict__ val,
+                                               const float* __restrict__ dinv, const float* __restrict__ x,
+                                               const float* __restrict__ prodOld, float* __restrict__ w,
+                                               float* __restrict__ y, float* __restrict__ out,
+                                               const double* __restrict__ coef, int step,
+                                               const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    constexpr int H = 4;
+    const double c1 = coef[2 * step], c2 = coef[2 * step + 1];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / (4 * H);
+    const int l = t - row * 4 * H;
+    const int h = l >> 2, i = l & 3;
+    const bool skipSpmv = (MODE == 2) && c2 == 0.0;
+    double a0 = 0.0, a1 = 0.0;
+    if (row < nNo && !skipSpmv) {
+        const int e = rowPtr[row + 1];
+        int k = rowPtr[row] + h;
+        for (; k + H < e; k += 2 * H) {
+            const int cA = colPtr[k], cB = colPtr[k + H];
+            a0 += dot4_nt<NT>(val + (size_t)k * 16 + i * 4, x + (size_t)cA * 4);
+            a1 += dot4_nt<NT>(val + (size_t)(k + H) * 16 + i * 4, x + (size_t)cB * 4);
+        }
+        if (k < e) a0 += dot4_nt<NT>(val + (size_t)k * 16 + i * 4, x + (size_t)colPtr[k] * 4);
+    }
+    double acc = a0 + a1;
+#pragma unroll
+    for (int o = 4; o < 4 * H; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+    if (dinv && !skipSpmv) {
+        const int base = (threadIdx.x & (kWave - 1)) & ~3;
+        double kx[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) kx[m] = __shfl(acc, base + m, kWave);
+        double v = 0.0;
+        if (row < nNo) {
+            const float* d = dinv + (size_t)row * 16 + i * 4;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) v += (double)d[m] * kx[m];
+        }
+        acc = v;
+    }
+    if (row >= nNo || h != 0) return;
+    const size_t q = (size_t)row * 4 + i;
+    if constexpr (MODE == 0) {
+        const double pv = (double)x[q];
+        y[q] = (float)((double)y[q] + c1 * pv);
+        out[q] = (float)(pv - c1 * acc);
+    } else if constexpr (MODE == 1) {
+        const double pv = (double)x[q];
+        w[q] = (float)acc;
+        y[q] = (float)((double)y[q] + c1 * pv - c2 * acc);
+    } else {
+        out[q] = (float)((double)prodOld[q] - c1 * (double)x[q] + c2 * acc);
+    }
+}
+
+// y += c1 prod (the last, linear factor: no product needed)
+__global__ __launch_bounds__(256) void k_poly_last(size_t n, const float* __restrict__ prod, float* __restrict__ y,
+                                                   const double* __restrict__ coef, int step,
+                                                   const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const double c1 = coef[2 * step];
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        y[q] = (float)((double)y[q] + c1 * (double)prod[q]);
+}
+
+int launch_poly4(const Lhs& L, int mode, const float* K, const float* dinv, const float* x, const float* prodOld,
+                 float* w, float* y, float* out, const double* coef, int step, hipStream_t s, const int* stop)
+{
+    const bool nt = (size_t)L.nnz * 16 * sizeof(float) > ((size_t)128 << 20);
+    const unsigned g = (unsigned)(((size_t)L.nNo * 16 + 255) / 256);
+    const int* rp = L.rowPtr.as<int>();
+    const int* cp = L.colPtr.as<int>();
+    // algorithmic bytes: matrix, indices, x, the vectors the mode touches
+    const double base = (double)L.nnz * 64.0 + 4.0 * L.nnz + 4.0 * (L.nNo + 1) + (dinv ? L.nNo * 64.0 : 0.0);
+    const double vec = (double)L.nNo * 16.0;
+    ProfScope ps(1, base + vec * (mode == 0 ? 4 : mode == 1 ? 4 : 3));
+#define SV_POLY(M, N)                                                                                           \
+    SV_LAUNCH(ps, (k_poly4<M, N>), dim3(g), dim3(256), 0, s, L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, \
+              step, stop)
+    if (mode == 0) { if (nt) SV_POLY(0, true); else SV_POLY(0, false); }
+    else if (mode == 1) { if (nt) SV_POLY(1, true); else SV_POLY(1, false); }
+    else { if (nt) SV_POLY(2, true); else SV_POLY(2, false); }
+#undef SV_POLY
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_poly_last(size_t n, const float* prod, float* y, const double* coef, int step, hipStream_t s,
+                     const int* stop)
+{
+    size_t g = (n + 255) / 256;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(k_poly_last, dim3((unsigned)g), dim3(256), 0, s, n, prod, y, coef, step, stop);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
 // Inverse of each node's diagonal block (dof 4, Gauss-Jordan with partial
 // pivoting in FP64), stored FP32 row-major.  Rows and columns zeroed by the
 // Dirichlet mask get a unit diagonal first.
@@ -258,6 +372,32 @@ __global__ __launch_bounds__(256) void k_blockdiag_apply4(int nNo, const float* 
         for (int m = 0; m < 4; ++m) v += (double)d[p * 4 + m] * (double)xi[m];
         x[(size_t)r * 4 + p] = (float)v;
     }
+}
+
+// Seeded pseudo-random start vector in (-0.5, 0.5), zero where W = 0 (the
+// Dirichlet-masked unknowns, whose rows and columns are zero).
+__global__ __launch_bounds__(256) void k_rand_masked(size_t n, unsigned seed, const double* __restrict__ W,
+                                                     float* __restrict__ v)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        unsigned x = (unsigned)q * 2654435761u ^ seed;
+        x ^= x >> 16;
+        x *= 0x7feb352du;
+        x ^= x >> 15;
+        x *= 0x846ca68bu;
+        x ^= x >> 16;
+        const float r = (float)(x >> 8) * (1.0f / 16777216.0f) - 0.5f;
+        v[q] = (W[q] != 0.0) ? r : 0.0f;
+    }
+}
+
+int launch_rand_masked(size_t n, unsigned seed, const double* W, float* v, hipStream_t s)
+{
+    size_t g = (n + 255) / 256;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(k_rand_masked, dim3((unsigned)g), dim3(256), 0, s, n, seed, W, v);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
 }
 
 int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s)

@@ -179,6 +179,17 @@ def test_struct_assembly_parity(gpu, orc, quad):
     assert np.array_equal(st.R.cpu().numpy(), Rg)
 
 
+def test_fluid_assembly_parity_benchmark_mesh(gpu, orc):
+    """The ~200k-tet benchmark cylinder, slivers included: FP64 assembly
+    within 1e-12 of the oracle (max entry) and bitwise reproducible."""
+    prob = problems.fluid_cylinder(h=0.11)
+    say("fluid assembly (benchmark mesh)", prob.nNo, prob.mesh.nEl)
+    st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
+    say("relerr", relerr(Vg, Vo), relerr(Rg, Ro))
+    assert relerr(Vg, Vo) < 1e-12
+    assert relerr(Rg, Ro) < 1e-12
+
+
 def _solve_pair(prob, orc, mixed=False):
     st = NewtonStep(prob)
     st.assemble()
@@ -272,11 +283,38 @@ def test_mixed_reaches_fp64_tolerance(gpu, orc):
     assert relerr(xg, xo) < 1e-4
 
 
+@pytest.mark.parametrize("bj", ["0", "1"])
+def test_polynomial_inner_preconditioner(gpu, orc, monkeypatch, bj):
+    """Mixed solve with the GMRES(30) residual polynomial as the inner
+    preconditioner (with and without block-Jacobi): reaches the FP64
+    tolerance, and repeats bit for bit."""
+    import torch
+
+    monkeypatch.setenv("SVFSI_INNER_POLY", "1")
+    monkeypatch.setenv("SVFSI_INNER_BJ", bj)
+    prob = problems.fluid_cylinder(h=0.4)
+    st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc, mixed=True)
+    say("poly bj", bj, "outer", ls.RI.itr, "SpMVs", ls.RI.cM)
+    assert ls.RI.suc == 1
+    mask = prob.dir_mask()
+    _, Rs, W = orc.precond_diag(st.rowPtr - 1, st.colPtr - 1, prob.dof, V0, R0, mask)
+    y = np.where(W != 0, xg / np.where(W != 0, W, 1), 0.0)
+    r = Rs - orc.spmv(st.rowPtr - 1, st.colPtr - 1, prob.dof, Vs, y)
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(Rs) * 1.0001
+    assert relerr(xg, xo) < 1e-4
+    st.assemble()
+    ls2 = st.solve(mixed=True)
+    torch.cuda.synchronize()
+    assert (ls2.RI.itr, ls2.RI.cM) == (ls.RI.itr, ls.RI.cM)
+    assert np.array_equal(st.R.cpu().numpy(), xg)
+
+
 def test_inner_block_jacobi_reduces_inner_iterations(gpu, orc, monkeypatch):
     """The FP32 inner cycle of the mixed solve is left-preconditioned by the
     inverse 4x4 node blocks.  Both variants must reach the FP64 tolerance;
     the block-Jacobi one with fewer inner SpMVs."""
     prob = problems.fluid_cylinder(h=0.4)
+    monkeypatch.setenv("SVFSI_INNER_POLY", "0")
     res = {}
     for bj in ("0", "1"):
         monkeypatch.setenv("SVFSI_INNER_BJ", bj)
@@ -303,6 +341,9 @@ def test_graph_replays_match_direct_launches(gpu, monkeypatch, mixed, cg):
 
     prob = problems.lelas_cube(8) if cg else problems.fluid_cylinder(h=0.5)
     st = NewtonStep(prob)
+    # the graph-captured inner step is the GMRES cycle (the polynomial
+    # inner steps are launched directly)
+    monkeypatch.setenv("SVFSI_INNER_POLY", "0")
     monkeypatch.setenv("SVFSI_GRAPHS", "0")
     st.assemble()
     ls0 = st.solve(mixed=mixed)
