@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"mfma64": 78.6, "mfma32": 157.3}  # dense FP64 / FP32 matrix peaks
 
 
 def spmv_bytes(nnzb: int, nNo: int, dof: int, s: int) -> int:
@@ -88,7 +89,7 @@ def main() -> int:
     import torch
 
     from svfsi_amd import problems
-    from svfsi_amd._lib import lib
+    from svfsi_amd._lib import SVFSI_STRUCT_MFMA32, SVFSI_STRUCT_MFMA64, SVFSI_STRUCT_VALU, lib
     from svfsi_amd.step import NewtonStep
 
     if not torch.cuda.is_available():
@@ -179,6 +180,44 @@ def main() -> int:
     asm_ms = (time.perf_counter() - a) * 1e3 / max(1, args.steps)
 
     log(f"assembly {asm_ms:.3f} ms")
+    # STRUCT: each element kernel on its own (vector units, FP64 MFMA, FP32
+    # MFMA), with the contraction kernels' event timing and MFMA rate
+    asm_kern = {}
+    struct_kernels = (("valu", SVFSI_STRUCT_VALU), ("mfma64", SVFSI_STRUCT_MFMA64),
+                      ("mfma32", SVFSI_STRUCT_MFMA32))
+    base_quad = prob.quad
+    if args.workload == "struct":
+        import ctypes as C
+
+        L = lib()
+        for name, kern in struct_kernels:
+            prob.quad = (base_quad & 1) | kern
+            st.assemble()
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            for _ in range(max(1, args.steps)):
+                st.assemble()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - a) * 1e3 / max(1, args.steps)
+            L.svfsi_prof_reset()
+            L.svfsi_prof_enable(1)
+            st.assemble()
+            torch.cuda.synchronize()
+            L.svfsi_prof_enable(0)
+            cnt, tms, fl = C.c_longlong(), C.c_double(), C.c_double()
+            L.svfsi_prof_get(2, C.byref(cnt), C.byref(tms))
+            L.svfsi_prof_get_bytes(2, C.byref(fl))
+            pc, pms = C.c_longlong(), C.c_double()
+            L.svfsi_prof_get(3, C.byref(pc), C.byref(pms))
+            asm_kern[name] = {"ms": round(ms, 3), "element_kernels_ms": round(tms.value, 3),
+                              "element_launches": cnt.value, "points_kernel_ms": round(pms.value, 3),
+                              "contraction_gflop": round(fl.value / 1e9, 2)}
+            if name != "valu" and tms.value > 0:
+                tf = fl.value / (tms.value * 1e-3) / 1e12
+                asm_kern[name]["mfma_tflops"] = round(tf, 2)
+                asm_kern[name]["mfma_frac"] = round(tf / MFMA_PEAK_TFLOPS[name], 4)
+            log(f"assembly[{name}] {ms:.3f} ms (element kernels {tms.value:.3f} ms, points {pms.value:.3f} ms)")
+        prob.quad = base_quad
     if args.mixed_only:
         fp64_ms, ls64, prof64 = None, None, [(0, 0.0, 0.0), (0, 0.0, 0.0)]
     else:
@@ -243,6 +282,14 @@ def main() -> int:
         cpu_ms = ((c1 - c0) + (c2 - c1) + per_it * itr64) * 1e3
         acc["assembly_rel_err_vs_cpu"] = float(np.abs(V0 - Vo).max() / np.abs(Vo).max())
         acc["residual_rel_err_vs_cpu"] = float(np.abs(R0 - Ro).max() / np.abs(Ro).max())
+        for name, kern in struct_kernels if asm_kern else ():
+            prob.quad = (base_quad & 1) | kern
+            st.assemble()
+            torch.cuda.synchronize()
+            Vk = st.Val.cpu().numpy()
+            asm_kern[name]["rel_err_vs_cpu"] = float(np.abs(Vk - Vo).max() / np.abs(Vo).max())
+            del Vk
+        prob.quad = base_quad
         log(f"cpu leg: {so.itr} iterations in {c3 - c2:.1f} s; accuracy checks")
         # FP64 scaled residual of each GPU solution, recomputed on the CPU
         for name, mixed in (("fp64", False), ("mixed", True)):
@@ -260,14 +307,23 @@ def main() -> int:
                        f"scaled to the {itr64} iterations of the FP64 step"),
         }
 
+    # assembly against HBM: Val written once, R, IEN, x and the state arrays
+    # read once (zero-fill and face terms included in the timed call)
+    k_state = 3 if args.workload == "struct" else 2
+    asm_bytes = (nnzb * dof * dof * 8 + m.nNo * dof * 8 + 4 * m.nEl * m.eNoN + 24 * m.nNo
+                 + k_state * m.nNo * st.Ag.shape[1] * 8)
+    asm_gbs = asm_bytes / (asm_ms * 1e-3) / 1e9
+    asm_roof = {"algorithmic_bytes": asm_bytes, "achieved_gbs": round(asm_gbs, 1),
+                "frac_of_hbm_peak": round(asm_gbs / HBM_PEAK_GBS, 4),
+                "note": "whole assembly call (zero-fill, element kernels, face terms) over the unique bytes"}
     speed = fp64_ms / mix_ms if (fp64_ms and mix_ms > 0) else None
     roof = None
     if gbs32 is not None:
         fused = dof == 4 and os.environ.get("SVFSI_FUSED_INNER", "0") == "1"
-        poly = dof == 4 and os.environ.get("SVFSI_INNER_POLY", "1") != "0"
+        poly = dof in (3, 4) and os.environ.get("SVFSI_INNER_POLY", "1") != "0"
         if poly:
-            nt = "true" if nnzb * 64 > (128 << 20) else "false"
-            kname = f"k_poly4<*, {nt}"
+            nt = "true" if nnzb * dof * dof * 4 > (128 << 20) else "false"
+            kname = f"k_poly{dof}<*, {nt}"
             what = ("polynomial preconditioner steps: FP32 block SpMV with the block-Jacobi and vector "
                     "epilogues, modes 0/1/2 together, plus the setup Arnoldi SpMVs")
         elif fused:
@@ -297,6 +353,8 @@ def main() -> int:
         "config": {"workload": wl, "nNo": m.nNo, "nEl": m.nEl, "nnzb": nnzb, "dof": dof, "colors": st.ncolors},
         "fp64_ms_per_step": None if fp64_ms is None else round(fp64_ms, 3),
         "assembly_ms": round(asm_ms, 3),
+        "assembly_roofline": asm_roof,
+        **({"assembly_kernels": asm_kern} if asm_kern else {}),
         "speedup_mixed_vs_fp64": None if speed is None else round(speed, 3),
         "fp64_gmres_itr": None if ls64 is None else ls64[-1].RI.itr,
         "mixed_outer_itr": lsmx[-1].RI.itr,

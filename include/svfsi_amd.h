@@ -204,6 +204,17 @@ typedef struct {
 #define SVFSI_PHYS_LELAS   2  /* CONSTRUCT_LELAS, LELAS.f:38-128 (tet4) */
 #define SVFSI_PHYS_STRUCT  3  /* CONSTRUCT_dSOLID, STRUCT.f:39-162 (nHK) */
 
+/* STRUCT (tet10) element kernel, ORed into svfsi_assemble_dev's quad.  No
+ * counterpart in the reference (STRUCT3D is one FP64 code path there):
+ *   VALU   -- FP64 vector units throughout;
+ *   MFMA64 -- B^T Dm B on FP64 matrix cores (the default);
+ *   MFMA32 -- B^T Dm B on FP32 matrix cores with FP32 accumulation (the
+ *             rest, and the accumulation into Val, stays FP64). */
+#define SVFSI_STRUCT_AUTO   0x000
+#define SVFSI_STRUCT_VALU   0x100
+#define SVFSI_STRUCT_MFMA64 0x200
+#define SVFSI_STRUCT_MFMA32 0x300
+
 /* Device-resident mesh bound to an lhs pattern: coordinates, connectivity,
  * a deterministic element colouring and the per-element DOASSEM scatter map
  * (the column search of LHSA.f:282-292, done once).  IEN(eNoN,nEl) 1-based
@@ -226,7 +237,8 @@ int svfsi_zero_dev(void* mesh, int dof, double* dVal, double* dR, void* stream);
  * (accumulates; call svfsi_zero_dev first).  State arrays Ag,Yg,Dg are
  * device (tDof,nNo).  prop_el: optional per-element property (LELAS E_e,
  * FLUID mu_e, NULL = uniform).  quad: 0 = the reference rule (tet4 4-pt,
- * tet10 15-pt), 1 = tet10 4-point rule. */
+ * tet10 15-pt), 1 = tet10 4-point rule; for STRUCT, OR one of the
+ * SVFSI_STRUCT_* kernel selectors below into quad (default: MFMA64). */
 int svfsi_assemble_dev(void* mesh, int phys, const svfsi_eq_params* p,
                        int tDof, const double* dAg, const double* dYg,
                        const double* dDg, const double* dprop_el, int quad,
@@ -295,11 +307,14 @@ int svfsi_poly_factors(int d, int ld, const double* Hu, double* coef,
 /* ---------------------------------------------------------------------- */
 int svfsi_prof_enable(int on);
 /* kernel: 0 = FP64 SpMV, 1 = FP32 SpMV (including the inner solver's SpMV
- * fused with its multi-dot).  Returns launches and total ms. */
+ * fused with its multi-dot, and the polynomial preconditioner's steps),
+ * 2 = STRUCT element kernels (one per colour), 3 = STRUCT material-point
+ * kernel.  Returns launches and total ms. */
 int svfsi_prof_get(int kernel, long long* count, double* total_ms);
 /* Algorithmic bytes of the profiled launches of `kernel` (values, indices,
  * row pointers, x and y once; for the fused inner step also the basis
- * vectors its multi-dot reads and its partial sums). */
+ * vectors its multi-dot reads and its partial sums).  For kernel 2 the
+ * flops of the B^T Dm B contraction (2 * 30 * 30 * 6 nG per element). */
 int svfsi_prof_get_bytes(int kernel, double* bytes);
 int svfsi_prof_reset(void);
 

@@ -123,7 +123,7 @@ def assemble(prob, rowPtr: np.ndarray, colPtr: np.ndarray) -> tuple[np.ndarray, 
                               P(rowPtr), P(colPtr), P(Val), P(R))
     elif prob.phys == 3:
         L.orc_construct_struct(m.nNo, m.nEl, m.eNoN, P(ien), P(x), P(Ag), P(Yg), P(Dg), Ag.shape[1],
-                               C.byref(eq), prob.quad, P(rowPtr), P(colPtr), P(Val), P(R))
+                               C.byref(eq), prob.quad & 1, P(rowPtr), P(colPtr), P(Val), P(R))
     for fien, fel, h in prob.trac_faces:
         fi, fe, hh = i32(fien), i32(fel), f64(h)
         L.orc_traction_tri3(len(fi), P(fi), P(fe), P(ien), m.eNoN, P(x), P(hh), dof, P(rowPtr), P(colPtr),

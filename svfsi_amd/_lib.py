@@ -24,6 +24,12 @@ SVFSI_PHYS_FLUID = 1
 SVFSI_PHYS_LELAS = 2
 SVFSI_PHYS_STRUCT = 3
 
+# STRUCT element kernel, ORed into the assembly's quad argument
+SVFSI_STRUCT_AUTO = 0x000
+SVFSI_STRUCT_VALU = 0x100
+SVFSI_STRUCT_MFMA64 = 0x200
+SVFSI_STRUCT_MFMA32 = 0x300
+
 SVFSI_OK = 0
 SVFSI_ERR_ARG = -1
 SVFSI_ERR_HIP = -2

@@ -585,10 +585,29 @@ int svfsi_mesh_create(void** mesh, void* lhs, int nNo, int nEl, int eNoN, const 
                 emap[((size_t)p * eNoN + a) * eNoN + b] = k;
             }
     }
+    // gather lists, filled in element-id order
+    const int NN = eNoN * eNoN;
+    std::vector<int> slot(nEl);
+    for (int p = 0; p < nEl; ++p) slot[order[p]] = p;
+    std::vector<int> gptr((size_t)L.nnz + 1, 0), rptr((size_t)nNo + 1, 0);
+    for (int k : emap) gptr[(size_t)k + 1]++;
+    for (int nd : sien) rptr[(size_t)nd + 1]++;
+    for (size_t j = 0; j < (size_t)L.nnz; ++j) gptr[j + 1] += gptr[j];
+    for (size_t i = 0; i < (size_t)nNo; ++i) rptr[i + 1] += rptr[i];
+    std::vector<int> gslot(emap.size()), rslot(sien.size());
+    {
+        std::vector<int> gpos(gptr.begin(), gptr.end() - 1), rpos(rptr.begin(), rptr.end() - 1);
+        for (int e = 0; e < nEl; ++e) {
+            const int p = slot[e];
+            for (int ab = 0; ab < NN; ++ab) gslot[(size_t)p * NN + ab] = gpos[emap[(size_t)p * NN + ab]]++;
+            for (int a = 0; a < eNoN; ++a) rslot[(size_t)p * eNoN + a] = rpos[sien[(size_t)p * eNoN + a]]++;
+        }
+    }
     std::vector<double> xv(x, x + (size_t)3 * nNo);
     int rc = SVFSI_OK;
     if ((rc = upload(M->ien, sien)) || (rc = upload(M->elemId, order)) || (rc = upload(M->emap, emap)) ||
-        (rc = upload(M->x, xv))) {
+        (rc = upload(M->x, xv)) || (rc = upload(M->gPtr, gptr)) || (rc = upload(M->gSlot, gslot)) ||
+        (rc = upload(M->rPtr, rptr)) || (rc = upload(M->rSlot, rslot))) {
         delete M;
         return rc;
     }
@@ -687,7 +706,9 @@ int svfsi_assemble_dev(void* mesh, int phys, const svfsi_eq_params* p, int tDof,
     } else if (phys == SVFSI_PHYS_STRUCT) {
         SV_ARG(M.eNoN == 10, "STRUCT assembly supports tet10 only");
         SV_ARG(tDof >= 3 && dAg && dDg, "STRUCT needs Ag, Dg with tDof >= 3");
-        SV_ARG(quad == 0 || quad == 1, "quad must be 0 (15-point) or 1 (4-point)");
+        const int kern = quad & 0x300;
+        SV_ARG((quad & ~0x301) == 0, "quad must be 0 (15-point) or 1 (4-point), ORed with SVFSI_STRUCT_*");
+        quad &= 1;
         if (!M.rule10nG[quad]) {
             std::vector<double> tab;
             const int nG = make_tet10_rule(quad, tab);
@@ -695,7 +716,7 @@ int svfsi_assemble_dev(void* mesh, int phys, const svfsi_eq_params* p, int tDof,
             M.rule10nG[quad] = nG;
         }
         return launch_asm_struct(M, to_asm(*p), M.rule10nG[quad], M.rule10[quad].as<double>(), tDof, dAg, dDg,
-                                 dVal, dR, s);
+                                 dVal, dR, kern, s);
     } else {
         set_error("phys %d not supported", phys);
         return SVFSI_ERR_NOTSUP;
@@ -822,7 +843,7 @@ int svfsi_prof_enable(int on)
 
 int svfsi_prof_get(int kernel, long long* count, double* total_ms)
 {
-    SV_ARG(kernel == 0 || kernel == 1, "kernel must be 0 or 1");
+    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..3");
     prof_resolve();
     if (count) *count = prof().count[kernel];
     if (total_ms) *total_ms = prof().ms[kernel];
@@ -831,7 +852,7 @@ int svfsi_prof_get(int kernel, long long* count, double* total_ms)
 
 int svfsi_prof_get_bytes(int kernel, double* bytes)
 {
-    SV_ARG(kernel == 0 || kernel == 1, "kernel must be 0 or 1");
+    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..3");
     prof_resolve();
     if (bytes) *bytes = prof().bytes[kernel];
     return SVFSI_OK;
@@ -841,9 +862,11 @@ int svfsi_prof_reset(void)
 {
     prof_resolve();
     Prof& P = prof();
-    P.count[0] = P.count[1] = 0;
-    P.ms[0] = P.ms[1] = 0.0;
-    P.bytes[0] = P.bytes[1] = 0.0;
+    for (int k = 0; k < Prof::kSlots; ++k) {
+        P.count[k] = 0;
+        P.ms[k] = 0.0;
+        P.bytes[k] = 0.0;
+    }
     return SVFSI_OK;
 }
 

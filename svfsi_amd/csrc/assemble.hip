@@ -76,7 +76,9 @@ __global__ __launch_bounds__(256) void k_asm_lelas(int begin, int count, const i
                                                    const double* __restrict__ Dg,
                                                    const double* __restrict__ Eel, AsmParams P,
                                                    Tet4Rule rule, double* __restrict__ val,
-                                                   double* __restrict__ R)
+                                                   double* __restrict__ R, double* __restrict__ ke,
+                                                   double* __restrict__ re, const int* __restrict__ gslot,
+                                                   const int* __restrict__ rslot)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count * 16) return;
@@ -116,9 +118,15 @@ __global__ __launch_bounds__(256) void k_asm_lelas(int begin, int count, const i
                 k[i * 3 + j] += wl * v;
             }
     }
-    double* vb = val + (size_t)emap[(size_t)p * 16 + ab] * 9;
+    if (ke) {
+        double* kb = ke + (size_t)gslot[(size_t)p * 16 + ab] * 9;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) vb[q] += k[q];
+        for (int q = 0; q < 9; ++q) kb[q] = k[q];
+    } else {
+        double* vb = val + (size_t)emap[(size_t)p * 16 + ab] * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) vb[q] += k[q];
+    }
 
     if (b != 0) return;
     // residual row a (LELAS.f:161-205 and 208-215)
@@ -156,6 +164,13 @@ __global__ __launch_bounds__(256) void k_asm_lelas(int begin, int count, const i
         r[0] += w * (rho * Na * ud[0] + Nx[a][0] * S[0] + Nx[a][1] * S[3] + Nx[a][2] * S[5]);
         r[1] += w * (rho * Na * ud[1] + Nx[a][0] * S[3] + Nx[a][1] * S[1] + Nx[a][2] * S[4]);
         r[2] += w * (rho * Na * ud[2] + Nx[a][0] * S[5] + Nx[a][1] * S[4] + Nx[a][2] * S[2]);
+    }
+    if (re) {
+        double* rr = re + (size_t)rslot[(size_t)p * 4 + a] * 3;
+        rr[0] = r[0];
+        rr[1] = r[1];
+        rr[2] = r[2];
+        return;
     }
     double* Rr = R + (size_t)nd[a] * 3;
     Rr[0] += r[0];
@@ -404,7 +419,9 @@ __global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const i
                                                    const double* __restrict__ Yg,
                                                    const double* __restrict__ muEl, AsmParams P,
                                                    Tet4Rule rule, double* __restrict__ val,
-                                                   double* __restrict__ R)
+                                                   double* __restrict__ R, double* __restrict__ ke,
+                                                   double* __restrict__ re, const int* __restrict__ gslot,
+                                                   const int* __restrict__ rslot)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count * 16) return;
@@ -416,6 +433,17 @@ __global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const i
     const double mu = muEl ? muEl[elemId[p]] : P.mu;
     double k[16], r[4];
     fluid_pair<double>(E, mu, P, rule, a, b, b == 0, k, r);
+    if (ke) {
+        double* kb = ke + (size_t)gslot[(size_t)p * 16 + ab] * 16;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) kb[q] = k[q];
+        if (b == 0) {
+            double* rr = re + (size_t)rslot[(size_t)p * 4 + a] * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) rr[c] = r[c];
+        }
+        return;
+    }
     double* vb = val + (size_t)emap[(size_t)p * 16 + ab] * 16;
 #pragma unroll
     for (int q = 0; q < 16; ++q) vb[q] += k[q];
@@ -627,14 +655,143 @@ __global__ __launch_bounds__(256) void k_lelas_dkde(int nEl, const int* __restri
 // reference does for that block, and add it to Val through the scatter map;
 // the b = 0 lane also builds the residual row of node a.
 // ---------------------------------------------------------------------------
-// record layout (doubles)
-constexpr int SR_W = 0, SR_N = 1, SR_NX = 11, SR_S = 41, SR_DM = 50, SR_P = 86, SR_UD = 95, SR_F = 98, SR_SZ = 107;
+// Gauss-point record layout (doubles).  The shape functions come last: the
+// material-point records kept in HBM (SR_HBM doubles each) leave them out,
+// since N depends only on the rule.
+constexpr int SR_W = 0, SR_NX = 1, SR_S = 31, SR_DM = 40, SR_P = 76, SR_UD = 85, SR_F = 88, SR_HBM = 97,
+              SR_N = 97, SR_SZ = 107;
 
 __device__ __forceinline__ double det3d(const double A[3][3])
 {
     return A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1])
          - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
          + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+}
+
+// Material point (element e, Gauss point with rule record rg) of STRUCT3D:
+// GNN, F, the neo-Hookean S and Voigt Dm (GETPK2CC), P = F S and the inertia
+// term, written to the record rc (layout SR_*; N only when WITH_N).
+template <bool WITH_N>
+__device__ __forceinline__ void struct_point(int e, const int* __restrict__ ien, const double* __restrict__ x,
+                                             int tDof, const double* __restrict__ Ag,
+                                             const double* __restrict__ Dg, const AsmParams& P,
+                                             const double* __restrict__ rg, double* rc)
+{
+    // GNN (NN.f:1469-1565), streaming the nodes to keep registers low
+    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 2
+    for (int a = 0; a < 10; ++a) {
+        const int na = ien[(size_t)e * 10 + a];
+        double xa[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) xa[i] = x[(size_t)na * 3 + i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) A[i][j] += xa[i] * rg[11 + a * 3 + j];
+    }
+    const double J = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0]
+                   + A[0][2] * A[1][0] * A[2][1] - A[0][0] * A[1][2] * A[2][1]
+                   - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
+    double B[3][3];
+    B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / J;
+    B[0][1] = (A[2][1] * A[0][2] - A[2][2] * A[0][1]) / J;
+    B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / J;
+    B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / J;
+    B[1][1] = (A[2][2] * A[0][0] - A[2][0] * A[0][2]) / J;
+    B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / J;
+    B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / J;
+    B[2][1] = (A[2][0] * A[0][1] - A[2][1] * A[0][0]) / J;
+    B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / J;
+    // Nx (to LDS), F and inertia (STRUCT.f:195-235)
+    double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double ud[3] = {-P.rho * P.fb[0], -P.rho * P.fb[1], -P.rho * P.fb[2]};
+#pragma unroll 2
+    for (int a = 0; a < 10; ++a) {
+        const int na = ien[(size_t)e * 10 + a];
+        double Nxa[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Nxa[j] = rg[11 + a * 3] * B[0][j] + rg[11 + a * 3 + 1] * B[1][j] + rg[11 + a * 3 + 2] * B[2][j];
+            rc[SR_NX + a * 3 + j] = Nxa[j];
+        }
+        const double Na = rg[1 + a];
+        if constexpr (WITH_N) rc[SR_N + a] = Na;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double Dai = Dg[(size_t)na * tDof + i];
+            ud[i] += Na * (P.rho * Ag[(size_t)na * tDof + i]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) F[i][j] += Nxa[j] * Dai;
+        }
+    }
+    // GETPK2CC, neo-Hookean + GETSVOLP
+    const double nd3 = 3.0;
+    const double Jf = det3d(F);
+    const double J2d = pow(Jf, -2.0 / nd3);
+    double C[3][3], Ci[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+    {
+        const double d = det3d(C);
+        Ci[0][0] = (C[1][1] * C[2][2] - C[1][2] * C[2][1]) / d;
+        Ci[0][1] = (C[0][2] * C[2][1] - C[0][1] * C[2][2]) / d;
+        Ci[0][2] = (C[0][1] * C[1][2] - C[0][2] * C[1][1]) / d;
+        Ci[1][0] = (C[1][2] * C[2][0] - C[1][0] * C[2][2]) / d;
+        Ci[1][1] = (C[0][0] * C[2][2] - C[0][2] * C[2][0]) / d;
+        Ci[1][2] = (C[0][2] * C[1][0] - C[0][0] * C[1][2]) / d;
+        Ci[2][0] = (C[1][0] * C[2][1] - C[1][1] * C[2][0]) / d;
+        Ci[2][1] = (C[0][1] * C[2][0] - C[0][0] * C[2][1]) / d;
+        Ci[2][2] = (C[0][0] * C[1][1] - C[0][1] * C[1][0]) / d;
+    }
+    double p = 0.0, pl = 0.0;
+    const double Kp = P.Kpen;
+    if (!iszero_d(Kp)) {
+        if (P.volType == 1) { p = Kp * (Jf - 1.0); pl = Kp * (2.0 * Jf - 1.0); }
+        else if (P.volType == 3) { p = Kp * (1.0 - 1.0 / Jf); pl = Kp; }
+        else { p = 0.5 * Kp * (Jf - 1.0 / Jf); pl = Kp * Jf; }
+    }
+    const double g1 = 2.0 * P.C10;
+    double CSb = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) CSb += C[i][j] * (i == j ? g1 : 0.0);
+    const double r1 = J2d * CSb / nd3;
+    double Sd[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Sd[i][j] = J2d * (i == j ? g1 : 0.0) - r1 * Ci[i][j];
+    const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {2, 0}};
+#pragma unroll
+    for (int aa = 0; aa < 6; ++aa)
+#pragma unroll
+        for (int bb = 0; bb < 6; ++bb) {
+            const int i = vi[aa][0], j = vi[aa][1], kk = vi[bb][0], l = vi[bb][1];
+            const double dy1 = Ci[i][j] * Sd[kk][l] + Sd[i][j] * Ci[kk][l];
+            const double sym = 0.5 * (Ci[i][kk] * Ci[j][l] + Ci[i][l] * Ci[j][kk]);
+            const double dyc = Ci[i][j] * Ci[kk][l];
+            rc[SR_DM + aa * 6 + bb] = (-2.0 / nd3) * dy1 + 2.0 * (r1 - p * Jf) * sym + (pl * Jf - 2.0 * r1 / nd3) * dyc;
+        }
+    double S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            S[i][j] = Sd[i][j] + p * Jf * Ci[i][j];
+            rc[SR_S + i * 3 + j] = S[i][j];
+            rc[SR_F + i * 3 + j] = F[i][j];
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) rc[SR_P + i * 3 + j] = F[i][0] * S[0][j] + F[i][1] * S[1][j] + F[i][2] * S[2][j];
+    rc[SR_W] = rg[0] * J;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rc[SR_UD + i] = ud[i];
 }
 
 template <int NG, int EPW>
@@ -654,124 +811,7 @@ __global__ __launch_bounds__(256) void k_asm_struct(int begin, int count, const 
         const int k = lane / NG, g = lane - (lane / NG) * NG;
         const int e = e0 + k;
         double* rc = rec[lane];
-        if (e < eEnd) {
-            const double* rg = rule + g * kTet10Rec;
-            // GNN (NN.f:1469-1565), streaming the nodes to keep registers low
-            double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll 2
-            for (int a = 0; a < 10; ++a) {
-                const int na = ien[(size_t)e * 10 + a];
-                double xa[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) xa[i] = x[(size_t)na * 3 + i];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) A[i][j] += xa[i] * rg[11 + a * 3 + j];
-            }
-            const double J = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0]
-                           + A[0][2] * A[1][0] * A[2][1] - A[0][0] * A[1][2] * A[2][1]
-                           - A[0][1] * A[1][0] * A[2][2] - A[0][2] * A[1][1] * A[2][0];
-            double B[3][3];
-            B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / J;
-            B[0][1] = (A[2][1] * A[0][2] - A[2][2] * A[0][1]) / J;
-            B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / J;
-            B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / J;
-            B[1][1] = (A[2][2] * A[0][0] - A[2][0] * A[0][2]) / J;
-            B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / J;
-            B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / J;
-            B[2][1] = (A[2][0] * A[0][1] - A[2][1] * A[0][0]) / J;
-            B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / J;
-            // Nx (to LDS), F and inertia (STRUCT.f:195-235)
-            double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-            double ud[3] = {-P.rho * P.fb[0], -P.rho * P.fb[1], -P.rho * P.fb[2]};
-#pragma unroll 2
-            for (int a = 0; a < 10; ++a) {
-                const int na = ien[(size_t)e * 10 + a];
-                double Nxa[3];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    Nxa[j] = rg[11 + a * 3] * B[0][j] + rg[11 + a * 3 + 1] * B[1][j] + rg[11 + a * 3 + 2] * B[2][j];
-                    rc[SR_NX + a * 3 + j] = Nxa[j];
-                }
-                const double Na = rg[1 + a];
-                rc[SR_N + a] = Na;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    const double Dai = Dg[(size_t)na * tDof + i];
-                    ud[i] += Na * (P.rho * Ag[(size_t)na * tDof + i]);
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) F[i][j] += Nxa[j] * Dai;
-                }
-            }
-            // GETPK2CC, neo-Hookean + GETSVOLP
-            const double nd3 = 3.0;
-            const double Jf = det3d(F);
-            const double J2d = pow(Jf, -2.0 / nd3);
-            double C[3][3], Ci[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
-            {
-                const double d = det3d(C);
-                Ci[0][0] = (C[1][1] * C[2][2] - C[1][2] * C[2][1]) / d;
-                Ci[0][1] = (C[0][2] * C[2][1] - C[0][1] * C[2][2]) / d;
-                Ci[0][2] = (C[0][1] * C[1][2] - C[0][2] * C[1][1]) / d;
-                Ci[1][0] = (C[1][2] * C[2][0] - C[1][0] * C[2][2]) / d;
-                Ci[1][1] = (C[0][0] * C[2][2] - C[0][2] * C[2][0]) / d;
-                Ci[1][2] = (C[0][2] * C[1][0] - C[0][0] * C[1][2]) / d;
-                Ci[2][0] = (C[1][0] * C[2][1] - C[1][1] * C[2][0]) / d;
-                Ci[2][1] = (C[0][1] * C[2][0] - C[0][0] * C[2][1]) / d;
-                Ci[2][2] = (C[0][0] * C[1][1] - C[0][1] * C[1][0]) / d;
-            }
-            double p = 0.0, pl = 0.0;
-            const double Kp = P.Kpen;
-            if (!iszero_d(Kp)) {
-                if (P.volType == 1) { p = Kp * (Jf - 1.0); pl = Kp * (2.0 * Jf - 1.0); }
-                else if (P.volType == 3) { p = Kp * (1.0 - 1.0 / Jf); pl = Kp; }
-                else { p = 0.5 * Kp * (Jf - 1.0 / Jf); pl = Kp * Jf; }
-            }
-            const double g1 = 2.0 * P.C10;
-            double CSb = 0.0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) CSb += C[i][j] * (i == j ? g1 : 0.0);
-            const double r1 = J2d * CSb / nd3;
-            double Sd[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) Sd[i][j] = J2d * (i == j ? g1 : 0.0) - r1 * Ci[i][j];
-            const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {2, 0}};
-#pragma unroll
-            for (int aa = 0; aa < 6; ++aa)
-#pragma unroll
-                for (int bb = 0; bb < 6; ++bb) {
-                    const int i = vi[aa][0], j = vi[aa][1], kk = vi[bb][0], l = vi[bb][1];
-                    const double dy1 = Ci[i][j] * Sd[kk][l] + Sd[i][j] * Ci[kk][l];
-                    const double sym = 0.5 * (Ci[i][kk] * Ci[j][l] + Ci[i][l] * Ci[j][kk]);
-                    const double dyc = Ci[i][j] * Ci[kk][l];
-                    rc[SR_DM + aa * 6 + bb] = (-2.0 / nd3) * dy1 + 2.0 * (r1 - p * Jf) * sym + (pl * Jf - 2.0 * r1 / nd3) * dyc;
-                }
-            double S[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    S[i][j] = Sd[i][j] + p * Jf * Ci[i][j];
-                    rc[SR_S + i * 3 + j] = S[i][j];
-                    rc[SR_F + i * 3 + j] = F[i][j];
-                }
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) rc[SR_P + i * 3 + j] = F[i][0] * S[0][j] + F[i][1] * S[1][j] + F[i][2] * S[2][j];
-            rc[SR_W] = rg[0] * J;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) rc[SR_UD + i] = ud[i];
-        }
+        if (e < eEnd) struct_point<true>(e, ien, x, tDof, Ag, Dg, P, rule + g * kTet10Rec, rc);
     }
     __syncthreads();
     // Dm B_b, shared by the ten blocks (a, b) of column b
@@ -865,12 +905,289 @@ __global__ __launch_bounds__(256) void k_asm_struct(int begin, int count, const 
 }
 
 // ---------------------------------------------------------------------------
+// STRUCT3D with the B^T Dm B contraction on the matrix cores.
+//
+// k_struct_points evaluates every material point (element, Gauss point) of
+// the mesh in one launch, a lane each, and stores the record (SR_HBM doubles,
+// colour-sorted element order) in HBM.  k_asm_struct_mfma then assembles one
+// colour: per element the material stiffness
+//   K_mat[(a,i),(b,j)] = afu sum_g w_g (B_a,g^T Dm_g B_b,g)_ij
+// is the product A^T C of two (6 nG) x 30 matrices, A[(g,q),(a,i)] =
+// B_a,g[q][i] and C[(g,q),(b,j)] = afu w_g (Dm_g B_b,g)[q][j] (STRUCT.f:
+// 295-319, 348-493), padded to 32 columns and a multiple of 4 rows.  Each
+// of its four 16 x 16 output tiles is a chain of 16x16x4 MFMA steps (FP64,
+// or FP32 with FP32 accumulation).  The geometric and inertia term
+// delta_ij w_g (amd N_a N_b + afu Nx_a.S.Nx_b) and the residual stay on the
+// FP64 vector units in the scatter pass.
+// ---------------------------------------------------------------------------
+template <int NG>
+__global__ __launch_bounds__(256) void k_struct_points(int nEl, const int* __restrict__ ien,
+                                                      const double* __restrict__ x, int tDof,
+                                                      const double* __restrict__ Ag,
+                                                      const double* __restrict__ Dg, AsmParams P,
+                                                      const double* __restrict__ rule, double* __restrict__ rec)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (size_t)nEl * NG) return;
+    const int e = (int)(t / NG), g = (int)(t - (size_t)e * NG);
+    struct_point<false>(e, ien, x, tDof, Ag, Dg, P, rule + g * kTet10Rec, rec + t * SR_HBM);
+}
+
+// One 16x16x4 MFMA step.  Operands: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16].  The output item r of lane l is D[row(l, r)][l % 16];
+// the row order differs between the FP64 and FP32 instructions (measured
+// on gfx950 with random operands).
+template <typename MT> struct MfmaTile;
+template <> struct MfmaTile<double> {
+    typedef double V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V step(double a, double b, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <> struct MfmaTile<float> {
+    typedef float V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V step(float a, float b, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <int NG, typename MT, int EPW>
+__global__ __launch_bounds__(256) void k_asm_struct_mfma(int begin, int count, const int* __restrict__ ien,
+                                                        const int* __restrict__ emap, AsmParams P,
+                                                        const double* __restrict__ rule,
+                                                        const double* __restrict__ rec, double* __restrict__ val,
+                                                        double* __restrict__ R, double* __restrict__ ke,
+                                                        double* __restrict__ re, const int* __restrict__ gslot,
+                                                        const int* __restrict__ rslot)
+{
+    constexpr int KP = (6 * NG + 3) / 4 * 4;   // contraction length, padded to the MFMA depth
+    constexpr int OPN = 2 * EPW * KP * 32 > EPW * 1024 ? 2 * EPW * KP * 32 : EPW * 1024;
+    constexpr int NTASK = EPW * 4;             // (element, output tile) pairs
+    constexpr int TPW = (NTASK + 3) / 4;       // per wave
+    typedef typename MfmaTile<MT>::V Acc;
+    __shared__ double rs[EPW * NG * SR_HBM];
+    __shared__ MT op[OPN];
+    MT* Aop = op;                    // [EPW][KP][32]
+    MT* Cop = op + EPW * KP * 32;    // [EPW][KP][32]
+    MT* Kout = op;                   // [EPW][32][32], after the products
+    const int tid = threadIdx.x;
+    const int e0 = begin + blockIdx.x * EPW;
+    const int nE = min(EPW, begin + count - e0);
+    {
+        const double* src = rec + (size_t)e0 * NG * SR_HBM;
+        for (int i = tid; i < nE * NG * SR_HBM; i += 256) rs[i] = src[i];
+    }
+    // zero padding: columns 30, 31 of every row, and rows 6 NG .. KP-1
+    for (int i = tid; i < EPW * KP * 2; i += 256) {
+        const int row = i >> 1, c = 30 + (i & 1);
+        Aop[row * 32 + c] = (MT)0;
+        Cop[row * 32 + c] = (MT)0;
+    }
+    if constexpr (KP > 6 * NG) {
+        for (int i = tid; i < EPW * (KP - 6 * NG) * 32; i += 256) {
+            const int e = i / ((KP - 6 * NG) * 32), r = i - e * ((KP - 6 * NG) * 32);
+            Aop[((size_t)e * KP + 6 * NG) * 32 + r] = (MT)0;
+            Cop[((size_t)e * KP + 6 * NG) * 32 + r] = (MT)0;
+        }
+    }
+    __syncthreads();
+    const double afu = P.af * P.beta * P.dt * P.dt;
+    // operands: B_b (rows (g, q), columns (b, j)) and afu w Dm B_b
+    for (int t = tid; t < nE * NG * 10; t += 256) {
+        const int e = t / (NG * 10), r = t - e * (NG * 10);
+        const int g = r / 10, b = r - g * 10;
+        const double* rc = rs + (e * NG + g) * SR_HBM;
+        const double* Nxb = rc + SR_NX + b * 3;
+        const double* Fg = rc + SR_F;
+        double Bb[6][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Bb[0][c] = Nxb[0] * Fg[c * 3 + 0];
+            Bb[1][c] = Nxb[1] * Fg[c * 3 + 1];
+            Bb[2][c] = Nxb[2] * Fg[c * 3 + 2];
+            Bb[3][c] = Nxb[0] * Fg[c * 3 + 1] + Fg[c * 3 + 0] * Nxb[1];
+            Bb[4][c] = Nxb[1] * Fg[c * 3 + 2] + Fg[c * 3 + 1] * Nxb[2];
+            Bb[5][c] = Nxb[2] * Fg[c * 3 + 0] + Fg[c * 3 + 2] * Nxb[0];
+        }
+        const double sc = afu * rc[SR_W];
+        MT* Ar = Aop + ((size_t)e * KP + g * 6) * 32 + b * 3;
+        MT* Cr = Cop + ((size_t)e * KP + g * 6) * 32 + b * 3;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double d = 0.0;
+#pragma unroll
+                for (int m = 0; m < 6; ++m) d += rc[SR_DM + q * 6 + m] * Bb[m][c];
+                Ar[q * 32 + c] = (MT)Bb[q][c];
+                Cr[q * 32 + c] = (MT)(sc * d);
+            }
+    }
+    __syncthreads();
+    // products: wave w takes the (element, tile) tasks w, w + 4, ...
+    const int wave = tid >> 6, l = tid & 63;
+    Acc acc[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        acc[u] = Acc{0, 0, 0, 0};
+        const int task = wave + 4 * u;
+        const int e = task >> 2;
+        if (task < NTASK && e < nE) { // uniform across the wave
+            const int ti = (task >> 1) & 1, tj = task & 1;
+            const MT* Ab = Aop + (size_t)e * KP * 32 + (l >> 4) * 32 + 16 * ti + (l & 15);
+            const MT* Cb = Cop + (size_t)e * KP * 32 + (l >> 4) * 32 + 16 * tj + (l & 15);
+#pragma unroll 6
+            for (int s = 0; s < KP / 4; ++s) acc[u] = MfmaTile<MT>::step(Ab[s * 128], Cb[s * 128], acc[u]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        const int task = wave + 4 * u;
+        const int e = task >> 2;
+        if (task < NTASK && e < nE) {
+            const int ti = (task >> 1) & 1, tj = task & 1;
+            MT* Kt = Kout + e * 1024 + 16 * ti * 32 + 16 * tj + (l & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Kt[MfmaTile<MT>::row(l, r) * 32] = acc[u][r];
+        }
+    }
+    __syncthreads();
+    // scatter: geometric + inertia term and the residual in FP64, then Val
+    const double amd = P.am * P.rho;
+    for (int t = tid; t < nE * 100; t += 256) {
+        const int e = t / 100, ab = t - (t / 100) * 100;
+        const int a = ab / 10, b = ab - (ab / 10) * 10;
+        double T1 = 0.0;
+        double r[3] = {0, 0, 0};
+#pragma unroll 1
+        for (int g = 0; g < NG; ++g) {
+            const double* rc = rs + (e * NG + g) * SR_HBM;
+            const double* rg = rule + g * kTet10Rec;
+            const double w = rc[SR_W];
+            const double* Nxa = rc + SR_NX + a * 3;
+            const double* Nxb = rc + SR_NX + b * 3;
+            const double* Sg = rc + SR_S;
+            double NxSNx = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) NxSNx += Nxa[i] * Sg[i * 3 + j] * Nxb[j];
+            const double Na = rg[1 + a];
+            T1 += w * (amd * Na * rg[1 + b] + afu * NxSNx);
+            if (b == 0) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    r[i] += w * (Na * rc[SR_UD + i] + Nxa[0] * rc[SR_P + i * 3] + Nxa[1] * rc[SR_P + i * 3 + 1]
+                                 + Nxa[2] * rc[SR_P + i * 3 + 2]);
+            }
+        }
+        const MT* Kb = Kout + e * 1024 + (3 * a) * 32 + 3 * b;
+        if (ke) {
+            double* kb = ke + (size_t)gslot[(size_t)(e0 + e) * 100 + ab] * 9;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) kb[i * 3 + j] = (double)Kb[i * 32 + j] + (i == j ? T1 : 0.0);
+            if (b == 0) {
+                double* rr = re + (size_t)rslot[(size_t)(e0 + e) * 10 + a] * 3;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) rr[i] = r[i];
+            }
+            continue;
+        }
+        double* vb = val + (size_t)emap[(size_t)(e0 + e) * 100 + ab] * 9;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) vb[i * 3 + j] += (double)Kb[i * 32 + j] + (i == j ? T1 : 0.0);
+        if (b == 0) {
+            double* Rr = R + (size_t)ien[(size_t)(e0 + e) * 10 + a] * 3;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Rr[i] += r[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gather assembly: Val(:, j) += the sum of the element blocks that land on
+// nnz block j, R(:, i) += the sum of node i's element residual rows.  The
+// element kernels stored those contributions contiguously, in element-id
+// order (DOASSEM's element loop, LHSA.f:266-298), so each sum is a streaming
+// read, deterministic, and associates like the reference's.  DD consecutive
+// lanes share one block.
+// ---------------------------------------------------------------------------
+template <int DD>
+__global__ __launch_bounds__(256) void k_gather(size_t nItems, const int* __restrict__ ptr,
+                                               const double* __restrict__ src, double* __restrict__ dst)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t j = t / DD;
+    if (j >= nItems) return;
+    const int q = (int)(t - j * DD);
+    const int k1 = ptr[j + 1];
+    double sum = 0.0;
+    for (int k = ptr[j]; k < k1; ++k) sum += src[(size_t)k * DD + q];
+    dst[j * DD + q] += sum;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-int launch_asm_tet4(const Mesh& M, int phys, const AsmParams& P, const Tet4Rule& rule, int tDof,
-                    const double* Ag, const double* Yg, const double* Dg, const double* prop,
-                    double* val, double* R, hipStream_t s)
+// Gather assembly: one launch over all elements stores the element blocks
+// at their gather positions, then k_gather sums them into Val and R.  The
+// default for tet4, whose 16- and 9-double blocks; TET10 (STRUCT) keeps the
+// coloured scatter, which measured faster there (5.0 vs 5.8 ms on the 40^3
+// tet10 cube: its 72-byte blocks straddle cache lines, so the scattered
+// stores of the element blocks cost more than the coloured read-modify-
+// write).  SVFSI_ASM_GATHER = 0 / 1 overrides.
+static bool gather_enabled(int eNoN)
 {
+    const char* v = getenv("SVFSI_ASM_GATHER");
+    if (v && (v[0] == '0' || v[0] == '1')) return v[0] == '1';
+    return eNoN == 4;
+}
+
+static int launch_gather(Mesh& M, int dof, double* val, double* R, hipStream_t s)
+{
+    const size_t nb = (size_t)M.lhs->nnz, nr = (size_t)M.nNo;
+    const double* ke = M.ke.as<double>();
+    const double* re = M.re.as<double>();
+    auto grid = [](size_t items, int dd) { return dim3((unsigned)((items * dd + 255) / 256)); };
+    if (dof == 3) {
+        hipLaunchKernelGGL(k_gather<9>, grid(nb, 9), dim3(256), 0, s, nb, M.gPtr.as<int>(), ke, val);
+        hipLaunchKernelGGL(k_gather<3>, grid(nr, 3), dim3(256), 0, s, nr, M.rPtr.as<int>(), re, R);
+    } else {
+        hipLaunchKernelGGL(k_gather<16>, grid(nb, 16), dim3(256), 0, s, nb, M.gPtr.as<int>(), ke, val);
+        hipLaunchKernelGGL(k_gather<4>, grid(nr, 4), dim3(256), 0, s, nr, M.rPtr.as<int>(), re, R);
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_asm_tet4(Mesh& M, int phys, const AsmParams& P, const Tet4Rule& rule, int tDof, const double* Ag,
+                    const double* Yg, const double* Dg, const double* prop, double* val, double* R, hipStream_t s)
+{
+    const int dof = phys == SVFSI_PHYS_LELAS ? 3 : 4;
+    if (gather_enabled(M.eNoN)) {
+        SV_TRY(M.ke.ensure((size_t)M.nEl * 16 * dof * dof * sizeof(double)));
+        SV_TRY(M.re.ensure((size_t)M.nEl * 4 * dof * sizeof(double)));
+        const unsigned grid = (unsigned)(((size_t)M.nEl * 16 + 255) / 256);
+        if (phys == SVFSI_PHYS_LELAS)
+            hipLaunchKernelGGL(k_asm_lelas, dim3(grid), dim3(256), 0, s, 0, M.nEl, M.ien.as<int>(),
+                               M.elemId.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, prop, P, rule,
+                               val, R, M.ke.as<double>(), M.re.as<double>(), M.gSlot.as<int>(), M.rSlot.as<int>());
+        else
+            hipLaunchKernelGGL(k_asm_fluid, dim3(grid), dim3(256), 0, s, 0, M.nEl, M.ien.as<int>(),
+                               M.elemId.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Yg, prop, P, rule,
+                               val, R, M.ke.as<double>(), M.re.as<double>(), M.gSlot.as<int>(), M.rSlot.as<int>());
+        SV_HIP(hipGetLastError());
+        return launch_gather(M, dof, val, R, s);
+    }
     const int nC = (int)M.colorOff.size() - 1;
     for (int c = 0; c < nC; ++c) {
         const int begin = M.colorOff[c], count = M.colorOff[c + 1] - begin;
@@ -879,11 +1196,11 @@ int launch_asm_tet4(const Mesh& M, int phys, const AsmParams& P, const Tet4Rule&
         if (phys == SVFSI_PHYS_LELAS)
             hipLaunchKernelGGL(k_asm_lelas, dim3(grid), dim3(256), 0, s, begin, count, M.ien.as<int>(),
                                M.elemId.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, prop,
-                               P, rule, val, R);
+                               P, rule, val, R, nullptr, nullptr, nullptr, nullptr);
         else
             hipLaunchKernelGGL(k_asm_fluid, dim3(grid), dim3(256), 0, s, begin, count, M.ien.as<int>(),
                                M.elemId.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Yg, prop,
-                               P, rule, val, R);
+                               P, rule, val, R, nullptr, nullptr, nullptr, nullptr);
     }
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
@@ -941,24 +1258,80 @@ int launch_lelas_dkde(const Mesh& M, const AsmParams& P, const Tet4Rule& rule, c
     return SVFSI_OK;
 }
 
-int launch_asm_struct(const Mesh& M, const AsmParams& P, int nG, const double* rule, int tDof,
-                      const double* Ag, const double* Dg, double* val, double* R, hipStream_t s)
+int launch_asm_struct(Mesh& M, const AsmParams& P, int nG, const double* rule, int tDof, const double* Ag,
+                      const double* Dg, double* val, double* R, int kern, hipStream_t s)
 {
+    if (nG != 4 && nG != 15) {
+        set_error("STRUCT: nG %d not supported", nG);
+        return SVFSI_ERR_NOTSUP;
+    }
+    if (kern == SVFSI_STRUCT_AUTO) kern = SVFSI_STRUCT_MFMA64;
+    if (kern != SVFSI_STRUCT_VALU && kern != SVFSI_STRUCT_MFMA64 && kern != SVFSI_STRUCT_MFMA32) {
+        set_error("STRUCT: unknown kernel selector %d", kern);
+        return SVFSI_ERR_ARG;
+    }
     const int nC = (int)M.colorOff.size() - 1;
+    if (kern != SVFSI_STRUCT_VALU) {
+        const size_t nPts = (size_t)M.nEl * nG;
+        SV_TRY(M.srec.ensure(nPts * SR_HBM * sizeof(double)));
+        double* rec = M.srec.as<double>();
+        const unsigned gp = (unsigned)((nPts + 255) / 256);
+        {
+            ProfScope ps(3, (double)nPts * SR_HBM * 8.0);
+            if (nG == 4)
+                SV_LAUNCH(ps, (k_struct_points<4>), dim3(gp), dim3(256), 0, s, M.nEl, M.ien.as<int>(),
+                          M.x.as<double>(), tDof, Ag, Dg, P, rule, rec);
+            else
+                SV_LAUNCH(ps, (k_struct_points<15>), dim3(gp), dim3(256), 0, s, M.nEl, M.ien.as<int>(),
+                          M.x.as<double>(), tDof, Ag, Dg, P, rule, rec);
+        }
+        SV_HIP(hipGetLastError());
+        constexpr int EPW4 = 4, EPW15 = 2;
+        const bool gather = gather_enabled(M.eNoN);
+        double* ke = nullptr;
+        double* re = nullptr;
+        if (gather) {
+            SV_TRY(M.ke.ensure((size_t)M.nEl * 100 * 9 * sizeof(double)));
+            SV_TRY(M.re.ensure((size_t)M.nEl * 10 * 3 * sizeof(double)));
+            ke = M.ke.as<double>();
+            re = M.re.as<double>();
+        }
+        // gather: one launch over all elements; scatter: one per colour
+        const int nL = gather ? 1 : nC;
+        for (int c = 0; c < nL; ++c) {
+            const int begin = gather ? 0 : M.colorOff[c];
+            const int count = gather ? M.nEl : M.colorOff[c + 1] - begin;
+            if (count == 0) continue;
+            const int epw = nG == 4 ? EPW4 : EPW15;
+            const dim3 grid((count + epw - 1) / epw);
+            ProfScope ps(2, 2.0 * 30 * 30 * 6 * nG * (double)count);
+#define SV_SMFMA(NGV, MT, EPWV)                                                                                     \
+    SV_LAUNCH(ps, (k_asm_struct_mfma<NGV, MT, EPWV>), grid, dim3(256), 0, s, begin, count, M.ien.as<int>(),   \
+              M.emap.as<int>(), P, rule, rec, val, R, ke, re, M.gSlot.as<int>(), M.rSlot.as<int>())
+            if (nG == 4) {
+                if (kern == SVFSI_STRUCT_MFMA64) SV_SMFMA(4, double, EPW4);
+                else SV_SMFMA(4, float, EPW4);
+            } else {
+                if (kern == SVFSI_STRUCT_MFMA64) SV_SMFMA(15, double, EPW15);
+                else SV_SMFMA(15, float, EPW15);
+            }
+#undef SV_SMFMA
+        }
+        SV_HIP(hipGetLastError());
+        return gather ? launch_gather(M, 3, val, R, s) : SVFSI_OK;
+    }
     for (int c = 0; c < nC; ++c) {
         const int begin = M.colorOff[c], count = M.colorOff[c + 1] - begin;
         if (count == 0) continue;
+        ProfScope ps(2, 2.0 * 30 * 30 * 6 * nG * (double)count);
         if (nG == 4) {
             constexpr int EPW = 8;
-            hipLaunchKernelGGL((k_asm_struct<4, EPW>), dim3((count + EPW - 1) / EPW), dim3(256), 0, s, begin, count,
-                               M.ien.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, P, rule, val, R);
-        } else if (nG == 15) {
-            constexpr int EPW = 2;
-            hipLaunchKernelGGL((k_asm_struct<15, EPW>), dim3((count + EPW - 1) / EPW), dim3(256), 0, s, begin, count,
-                               M.ien.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, P, rule, val, R);
+            SV_LAUNCH(ps, (k_asm_struct<4, EPW>), dim3((count + EPW - 1) / EPW), dim3(256), 0, s, begin, count,
+                      M.ien.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, P, rule, val, R);
         } else {
-            set_error("STRUCT: nG %d not supported", nG);
-            return SVFSI_ERR_NOTSUP;
+            constexpr int EPW = 2;
+            SV_LAUNCH(ps, (k_asm_struct<15, EPW>), dim3((count + EPW - 1) / EPW), dim3(256), 0, s, begin, count,
+                      M.ien.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Dg, P, rule, val, R);
         }
     }
     SV_HIP(hipGetLastError());

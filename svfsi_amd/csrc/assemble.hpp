@@ -33,7 +33,7 @@ Tet4Rule make_tet4_rule();
 Tri3Rule make_tri3_rule();
 AsmParams to_asm(const svfsi_eq_params& p);
 
-int launch_asm_tet4(const Mesh& M, int phys, const AsmParams& P, const Tet4Rule& rule, int tDof,
+int launch_asm_tet4(Mesh& M, int phys, const AsmParams& P, const Tet4Rule& rule, int tDof,
                     const double* Ag, const double* Yg, const double* Dg, const double* prop,
                     double* val, double* R, hipStream_t s);
 int launch_face_traction(const Mesh& M, const Mesh::FaceSet& F, int dof, const double* h,
@@ -55,7 +55,9 @@ namespace svfsi {
 // 674-715): per Gauss point w, N(10), Nxi(10,3) -> 41 doubles.
 constexpr int kTet10Rec = 41;
 int make_tet10_rule(int quad, std::vector<double>& tab); // returns nG
-int launch_asm_struct(const Mesh& M, const AsmParams& P, int nG, const double* rule, int tDof,
-                      const double* Ag, const double* Dg, double* val, double* R, hipStream_t s);
+// kern: one of SVFSI_STRUCT_*.  The MFMA kernels keep material-point
+// records in M.srec.
+int launch_asm_struct(Mesh& M, const AsmParams& P, int nG, const double* rule, int tDof,
+                      const double* Ag, const double* Dg, double* val, double* R, int kern, hipStream_t s);
 
 } // namespace svfsi

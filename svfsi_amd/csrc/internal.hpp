@@ -154,6 +154,14 @@ struct Mesh {
     std::vector<FaceSet*> faceSets;
     DBuf rule10[2];  // TET10 quadrature tables (15-point, 4-point), built on first use
     int rule10nG[2] = {0, 0};
+    DBuf srec;       // TET10 material-point records (nEl x nG x 97 doubles), grown on demand
+    // Gather assembly: element block (slot p, a, b) is stored at position
+    // gSlot[p*eNoN^2 + a*eNoN + b] of ke, so that the contributions to nnz
+    // block j are ke positions gPtr[j] .. gPtr[j+1]-1, in element-id order
+    // (the order of DOASSEM's element loop); likewise element row (p, a) at
+    // rSlot[p*eNoN + a] of re, node i's rows at rPtr[i] .. rPtr[i+1]-1.
+    DBuf gPtr, gSlot, rPtr, rSlot;
+    DBuf ke, re;     // element blocks / element residual rows, grown on demand
     ~Mesh() {
         for (auto* f : faceSets) delete f;
     }
@@ -167,9 +175,12 @@ struct ProfPending {
 };
 struct Prof {
     bool on = false;
-    long long count[2] = {0, 0};
-    double ms[2] = {0.0, 0.0};
-    double bytes[2] = {0.0, 0.0}; // algorithmic bytes of the timed launches
+    // slots: 0 FP64 SpMV, 1 FP32 SpMV (inner solver), 2 STRUCT element
+    // kernels, 3 STRUCT material-point kernel
+    static constexpr int kSlots = 4;
+    long long count[kSlots] = {0, 0, 0, 0};
+    double ms[kSlots] = {0.0, 0.0, 0.0, 0.0};
+    double bytes[kSlots] = {0.0, 0.0, 0.0, 0.0}; // algorithmic bytes (slot 2: MFMA flops) of the timed launches
     std::vector<ProfPending> pending;
 };
 Prof& prof();
@@ -210,8 +221,9 @@ int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
                     float* KU, hipStream_t s, const int* stop = nullptr,
                     const float* dinv = nullptr);
 int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s);
-int launch_poly4(const Lhs& L, int mode, const float* K, const float* dinv, const float* x, const float* prodOld,
-                 float* w, float* y, float* out, const double* coef, int step, hipStream_t s, const int* stop);
+int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* dinv, const float* x,
+                const float* prodOld, float* w, float* y, float* out, const double* coef, int step, hipStream_t s,
+                const int* stop);
 int launch_poly_last(size_t n, const float* prod, float* y, const double* coef, int step, hipStream_t s,
                      const int* stop);
 int launch_rand_masked(size_t n, unsigned seed, const double* W, float* v, hipStream_t s);

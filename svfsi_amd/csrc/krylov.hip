@@ -1387,7 +1387,9 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
 {
     const size_t n = (size_t)L.nNo * dof;
     Work& w = L.ws;
-    const int mo = env_int("SVFSI_FGMRES_M", 30);           // outer Krylov dimension
+    // outer Krylov dimension: the caller's restart length (dimKry), 30 when
+    // none is set (CG requests); SVFSI_FGMRES_M overrides
+    const int mo = env_int("SVFSI_FGMRES_M", ls.sD > 0 ? std::min(ls.sD, kMaxSD) : 30);
     const double tin = env_dbl("SVFSI_FGMRES_INNER_TOL", 0.1);
     const int mi = env_int("SVFSI_FGMRES_INNER_M", 30);     // inner Krylov dimension
     if (mo > kMaxSD || mi > kMaxSD) {
@@ -1412,13 +1414,15 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     double* scal = part + partN - 8;
     GmState* st = w.ostate.as<GmState>();
     GmArrays A = gm_carve(w.ohess.as<double>(), mo);
-    // block-Jacobi inside the FP32 inner cycle (dof 4; SVFSI_INNER_BJ=0 turns
-    // it off): the inner GMRES solves D^-1 K z = D^-1 q, D the 4x4 diagonal
-    // blocks of the scaled matrix
-    const bool bj = dof == 4 && env_int("SVFSI_INNER_BJ", 1) != 0;
+    // block-Jacobi inside the FP32 inner work: the inner operator is D^-1 K,
+    // D the dof x dof diagonal blocks of the scaled matrix.  On by default for
+    // dof 4 (fewer inner SpMVs on the fluid); off for dof 3, where it measured
+    // more outer steps on the elasticity cube (SVFSI_INNER_BJ = 0 / 1
+    // overrides)
+    const bool bj = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_BJ", dof == 4 ? 1 : 0) != 0;
     const float* dinv = nullptr;
     if (bj) {
-        SV_TRY(w.dinv32.ensure((size_t)L.nNo * 16 * sizeof(float)));
+        SV_TRY(w.dinv32.ensure((size_t)L.nNo * dof * dof * sizeof(float)));
         SV_TRY(launch_blockdiag_inv(L, dof, K, w.dinv32.as<float>(), s));
         dinv = w.dinv32.as<float>();
     }
@@ -1451,21 +1455,6 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
         SV_TRY(w.partial.ensure(inner_partials(L.nNo, n, mi, nbi, gv) * sizeof(double)));
         SV_TRY(w.hess.ensure(gm_arrays_doubles(mi) * sizeof(double)));
     }
-    const bool useGraphs = graphs_enabled();
-    if (useGraphs) {
-        double tinv = tin;
-        uintptr_t tbits;
-        std::memcpy(&tbits, &tinv, sizeof(tbits));
-        const std::vector<uintptr_t> sig = {
-            (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)K32,
-            (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
-            (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
-            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv};
-        if (sig != w.graphs.sig) {
-            w.graphs.clear();
-            w.graphs.sig = sig;
-        }
-    }
     hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, st, eps);
     hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, w.state.as<GmState>(), 0.0);
     PinnedPoll& poll = w.poll;
@@ -1473,7 +1462,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     GmState hs{};
     int outerMv = 0;
 
-    // Polynomial inner preconditioner (dof 4; SVFSI_INNER_POLY=0 keeps the
+    // Polynomial inner preconditioner (dof 3, 4; SVFSI_INNER_POLY=0 keeps the
     // inner GMRES cycle): d Arnoldi steps of the FP32 inner operator B from
     // a seeded random start give the harmonic Ritz values, and every outer
     // step then applies the fixed GMRES(d) residual polynomial of B, one
@@ -1483,7 +1472,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     long long polySetup = 0;
     float *P0 = nullptr, *P1 = nullptr, *Wb = nullptr;
     const double* coefD = nullptr;
-    bool usePoly = dof == 4 && env_int("SVFSI_INNER_POLY", 1) != 0;
+    bool usePoly = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", 1) != 0;
     if (usePoly) {
         const int d = env_int("SVFSI_POLY_DEG", 30);
         if (d < 2 || d > kMaxSD) {
@@ -1526,6 +1515,25 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
         }
         hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, w.state.as<GmState>(), 0.0);
     }
+    // graph signature: every pointer and size baked into the step launches.
+    // The polynomial steps are launched directly: their ~30 SpMV kernels of
+    // ~9 us keep the queue full, and replaying them as graphs measured
+    // 0.25 ms slower per Newton step on the 200k-tet cylinder.
+    const bool useGraphs = graphs_enabled() && !usePoly;
+    if (useGraphs) {
+        double tinv = tin;
+        uintptr_t tbits;
+        std::memcpy(&tbits, &tinv, sizeof(tbits));
+        const std::vector<uintptr_t> sig = {
+            (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)K32,
+            (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
+            (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
+            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv};
+        if (sig != w.graphs.sig) {
+            w.graphs.clear();
+            w.graphs.sig = sig;
+        }
+    }
     long long polySteps = 0;
     for (int l = 1; l <= ls.mItr; ++l) {
         ls.dB = ls.fNorm;
@@ -1556,15 +1564,15 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
                             if (last) {
                                 SV_TRY(launch_poly_last(n, cur, zi, coefD, f, cs, &st->stop));
                             } else {
-                                SV_TRY(launch_poly4(L, 0, K32, dinv, cur, nullptr, nullptr, zi, nxt, coefD, f, cs,
+                                SV_TRY(launch_poly(L, dof, 0, K32, dinv, cur, nullptr, nullptr, zi, nxt, coefD, f, cs,
                                                     &st->stop));
                                 std::swap(cur, nxt);
                             }
                         } else {
-                            SV_TRY(launch_poly4(L, 1, K32, dinv, cur, nullptr, Wb, zi, nullptr, coefD, f, cs,
+                            SV_TRY(launch_poly(L, dof, 1, K32, dinv, cur, nullptr, Wb, zi, nullptr, coefD, f, cs,
                                                 &st->stop));
                             if (!last) {
-                                SV_TRY(launch_poly4(L, 2, K32, dinv, Wb, cur, nullptr, nullptr, nxt, coefD, f, cs,
+                                SV_TRY(launch_poly(L, dof, 2, K32, dinv, Wb, cur, nullptr, nullptr, nxt, coefD, f, cs,
                                                     &st->stop));
                                 std::swap(cur, nxt);
                             }
@@ -1590,9 +1598,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
                 SV_HIP(hipGetLastError());
                 return SVFSI_OK;
             };
-            // the polynomial steps read their coefficients from device memory
-            // but their launch sequence depends on the roots: launched directly
-            if (useGraphs && !usePoly)
+            if (useGraphs)
                 SV_TRY(graph_run(w.graphs, i, s, step));
             else
                 SV_TRY(step(s));

@@ -82,7 +82,8 @@ __global__ __launch_bounds__(256) void k_bspmv_vs(int nNo, const int* __restrict
                                                   const VT* __restrict__ val,
                                                   const XT* __restrict__ x,
                                                   XT* __restrict__ y,
-                                                  const int* __restrict__ stop)
+                                                  const int* __restrict__ stop,
+                                                  const VT* __restrict__ dinv = nullptr)
 {
     if (stop && *stop) return;
     constexpr int DD = DOF * DOF;
@@ -123,6 +124,17 @@ __global__ __launch_bounds__(256) void k_bspmv_vs(int nNo, const int* __restrict
     for (int i = 0; i < DOF; ++i)
 #pragma unroll
         for (int o = LPR >> 1; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, kWave);
+    if (dinv && row < nNo) { // block-Jacobi epilogue: acc <- D^-1 acc
+        double v[DOF];
+#pragma unroll
+        for (int i = 0; i < DOF; ++i) {
+            v[i] = 0.0;
+#pragma unroll
+            for (int m = 0; m < DOF; ++m) v[i] += (double)dinv[(size_t)row * DD + i * DOF + m] * acc[m];
+        }
+#pragma unroll
+        for (int i = 0; i < DOF; ++i) acc[i] = v[i];
+    }
     if (row < nNo && l < DOF) {
         double out = acc[0];
 #pragma unroll
@@ -257,6 +269,62 @@ __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ 
     }
 }
 
+// dof 3: the same factor steps on the value-streaming SpMV layout (LPR lanes
+// per block row stream the row's values; D^-1 applied after the reduction).
+template <int MODE, bool NT>
+__global__ __launch_bounds__(256) void k_poly3(int nNo, const int* __restrict__ rowPtr,
+                                               const int* __restrict__ colPtr, const float* __restrict__ val,
+                                               const float* __restrict__ dinv, const float* __restrict__ x,
+                                               const float* __restrict__ prodOld, float* __restrict__ w,
+                                               float* __restrict__ y, float* __restrict__ out,
+                                               const double* __restrict__ coef, int step,
+                                               const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    constexpr int DOF = 3, DD = 9, LPR = 16;
+    const double c1 = coef[2 * step], c2 = coef[2 * step + 1];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int row = t / LPR;
+    const int l = t - row * LPR;
+    const bool skipSpmv = (MODE == 2) && c2 == 0.0;
+    double acc[DOF] = {0.0, 0.0, 0.0};
+    if (row < nNo && !skipSpmv) {
+        const int k0 = rowPtr[row];
+        const int nv = (rowPtr[row + 1] - k0) * DD;
+        const float* vr = val + (size_t)k0 * DD;
+        const int* cr = colPtr + k0;
+        for (int v = l; v < nv; v += LPR) {
+            const double a = NT ? (double)__builtin_nontemporal_load(vr + v) : (double)vr[v];
+            const int ka = v / DD, ea = v - ka * DD;
+            const double xa = (double)x[(size_t)cr[ka] * DOF + (ea % DOF)];
+#pragma unroll
+            for (int i = 0; i < DOF; ++i) acc[i] += (ea / DOF == i) ? a * xa : 0.0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DOF; ++i)
+#pragma unroll
+        for (int o = LPR >> 1; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, kWave);
+    if (row >= nNo || l >= DOF) return;
+    double a = l == 0 ? acc[0] : (l == 1 ? acc[1] : acc[2]);
+    if (dinv && !skipSpmv) {
+        const float* d = dinv + (size_t)row * DD + l * DOF;
+        a = (double)d[0] * acc[0] + (double)d[1] * acc[1] + (double)d[2] * acc[2];
+    }
+    const size_t q = (size_t)row * DOF + l;
+    if constexpr (MODE == 0) {
+        const double pv = (double)x[q];
+        y[q] = (float)((double)y[q] + c1 * pv);
+        out[q] = (float)(pv - c1 * a);
+    } else if constexpr (MODE == 1) {
+        const double pv = (double)x[q];
+        w[q] = (float)a;
+        y[q] = (float)((double)y[q] + c1 * pv - c2 * a);
+    } else {
+        out[q] = (float)((double)prodOld[q] - c1 * (double)x[q] + c2 * a);
+    }
+}
+
 // y += c1 prod (the last, linear factor: no product needed)
 __global__ __launch_bounds__(256) void k_poly_last(size_t n, const float* __restrict__ prod, float* __restrict__ y,
                                                    const double* __restrict__ coef, int step,
@@ -268,23 +336,35 @@ __global__ __launch_bounds__(256) void k_poly_last(size_t n, const float* __rest
         y[q] = (float)((double)y[q] + c1 * (double)prod[q]);
 }
 
-int launch_poly4(const Lhs& L, int mode, const float* K, const float* dinv, const float* x, const float* prodOld,
-                 float* w, float* y, float* out, const double* coef, int step, hipStream_t s, const int* stop)
+int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* dinv, const float* x,
+                const float* prodOld, float* w, float* y, float* out, const double* coef, int step, hipStream_t s,
+                const int* stop)
 {
-    const bool nt = (size_t)L.nnz * 16 * sizeof(float) > ((size_t)128 << 20);
-    const unsigned g = (unsigned)(((size_t)L.nNo * 16 + 255) / 256);
+    if (dof != 3 && dof != 4) {
+        set_error("polynomial preconditioner: dof %d not supported (3, 4)", dof);
+        return SVFSI_ERR_NOTSUP;
+    }
+    const int dd = dof * dof;
+    const bool nt = (size_t)L.nnz * dd * sizeof(float) > ((size_t)128 << 20);
     const int* rp = L.rowPtr.as<int>();
     const int* cp = L.colPtr.as<int>();
     // algorithmic bytes: matrix, indices, x, the vectors the mode touches
-    const double base = (double)L.nnz * 64.0 + 4.0 * L.nnz + 4.0 * (L.nNo + 1) + (dinv ? L.nNo * 64.0 : 0.0);
-    const double vec = (double)L.nNo * 16.0;
+    const double base = (double)L.nnz * dd * 4.0 + 4.0 * L.nnz + 4.0 * (L.nNo + 1) + (dinv ? L.nNo * dd * 4.0 : 0.0);
+    const double vec = (double)L.nNo * dof * 4.0;
     ProfScope ps(1, base + vec * (mode == 0 ? 4 : mode == 1 ? 4 : 3));
-#define SV_POLY(M, N)                                                                                           \
-    SV_LAUNCH(ps, (k_poly4<M, N>), dim3(g), dim3(256), 0, s, L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, \
+    const unsigned g = (unsigned)(((size_t)L.nNo * 16 + 255) / 256); // 16 lanes per block row, both layouts
+#define SV_POLY(KERN, M, N)                                                                                     \
+    SV_LAUNCH(ps, (KERN<M, N>), dim3(g), dim3(256), 0, s, L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, \
               step, stop)
-    if (mode == 0) { if (nt) SV_POLY(0, true); else SV_POLY(0, false); }
-    else if (mode == 1) { if (nt) SV_POLY(1, true); else SV_POLY(1, false); }
-    else { if (nt) SV_POLY(2, true); else SV_POLY(2, false); }
+    if (dof == 4) {
+        if (mode == 0) { if (nt) SV_POLY(k_poly4, 0, true); else SV_POLY(k_poly4, 0, false); }
+        else if (mode == 1) { if (nt) SV_POLY(k_poly4, 1, true); else SV_POLY(k_poly4, 1, false); }
+        else { if (nt) SV_POLY(k_poly4, 2, true); else SV_POLY(k_poly4, 2, false); }
+    } else {
+        if (mode == 0) { if (nt) SV_POLY(k_poly3, 0, true); else SV_POLY(k_poly3, 0, false); }
+        else if (mode == 1) { if (nt) SV_POLY(k_poly3, 1, true); else SV_POLY(k_poly3, 1, false); }
+        else { if (nt) SV_POLY(k_poly3, 2, true); else SV_POLY(k_poly3, 2, false); }
+    }
 #undef SV_POLY
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
@@ -303,74 +383,77 @@ int launch_poly_last(size_t n, const float* prod, float* y, const double* coef, 
 // Inverse of each node's diagonal block (dof 4, Gauss-Jordan with partial
 // pivoting in FP64), stored FP32 row-major.  Rows and columns zeroed by the
 // Dirichlet mask get a unit diagonal first.
-__global__ __launch_bounds__(256) void k_blockdiag_inv4(int nNo, const int* __restrict__ diagPtr,
-                                                        const double* __restrict__ val, float* __restrict__ dinv)
+template <int DOF>
+__global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __restrict__ diagPtr,
+                                                       const double* __restrict__ val, float* __restrict__ dinv)
 {
+    constexpr int DD = DOF * DOF;
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= nNo) return;
-    double a[4][8];
-    const double* b = val + (size_t)diagPtr[r] * 16;
+    double a[DOF][2 * DOF];
+    const double* b = val + (size_t)diagPtr[r] * DD;
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+    for (int p = 0; p < DOF; ++p)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            a[p][q] = b[p * 4 + q];
-            a[p][4 + q] = (p == q) ? 1.0 : 0.0;
+        for (int q = 0; q < DOF; ++q) {
+            a[p][q] = b[p * DOF + q];
+            a[p][DOF + q] = (p == q) ? 1.0 : 0.0;
         }
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
+    for (int p = 0; p < DOF; ++p) {
         bool zero = true;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) zero = zero && a[p][q] == 0.0 && a[q][p] == 0.0;
+        for (int q = 0; q < DOF; ++q) zero = zero && a[p][q] == 0.0 && a[q][p] == 0.0;
         if (zero) a[p][p] = 1.0;
     }
     bool ok = true;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < DOF; ++c) {
         int piv = c;
         double best = fabs(a[c][c]);
 #pragma unroll
-        for (int p = c + 1; p < 4; ++p)
+        for (int p = c + 1; p < DOF; ++p)
             if (fabs(a[p][c]) > best) { best = fabs(a[p][c]); piv = p; }
         if (best == 0.0) { ok = false; break; }
         if (piv != c)
 #pragma unroll
-            for (int q = 0; q < 8; ++q) { const double t = a[c][q]; a[c][q] = a[piv][q]; a[piv][q] = t; }
+            for (int q = 0; q < 2 * DOF; ++q) { const double t = a[c][q]; a[c][q] = a[piv][q]; a[piv][q] = t; }
         const double inv = 1.0 / a[c][c];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a[c][q] *= inv;
+        for (int q = 0; q < 2 * DOF; ++q) a[c][q] *= inv;
 #pragma unroll
-        for (int p = 0; p < 4; ++p)
+        for (int p = 0; p < DOF; ++p)
             if (p != c) {
                 const double f = a[p][c];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) a[p][q] -= f * a[c][q];
+                for (int q = 0; q < 2 * DOF; ++q) a[p][q] -= f * a[c][q];
             }
     }
-    float* o = dinv + (size_t)r * 16;
+    float* o = dinv + (size_t)r * DD;
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+    for (int p = 0; p < DOF; ++p)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) o[p * 4 + q] = ok ? (float)a[p][4 + q] : (p == q ? 1.0f : 0.0f);
+        for (int q = 0; q < DOF; ++q) o[p * DOF + q] = ok ? (float)a[p][DOF + q] : (p == q ? 1.0f : 0.0f);
 }
 
-// x = D^-1 x per node (dof 4, FP32, in place).
-__global__ __launch_bounds__(256) void k_blockdiag_apply4(int nNo, const float* __restrict__ dinv,
-                                                          float* __restrict__ x, const int* __restrict__ stop)
+// x = D^-1 x per node (FP32, in place).
+template <int DOF>
+__global__ __launch_bounds__(256) void k_blockdiag_apply(int nNo, const float* __restrict__ dinv,
+                                                         float* __restrict__ x, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= nNo) return;
-    float xi[4];
+    float xi[DOF];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) xi[m] = x[(size_t)r * 4 + m];
-    const float* d = dinv + (size_t)r * 16;
+    for (int m = 0; m < DOF; ++m) xi[m] = x[(size_t)r * DOF + m];
+    const float* d = dinv + (size_t)r * DOF * DOF;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
+    for (int p = 0; p < DOF; ++p) {
         double v = 0.0;
 #pragma unroll
-        for (int m = 0; m < 4; ++m) v += (double)d[p * 4 + m] * (double)xi[m];
-        x[(size_t)r * 4 + p] = (float)v;
+        for (int m = 0; m < DOF; ++m) v += (double)d[p * DOF + m] * (double)xi[m];
+        x[(size_t)r * DOF + p] = (float)v;
     }
 }
 
@@ -402,23 +485,30 @@ int launch_rand_masked(size_t n, unsigned seed, const double* W, float* v, hipSt
 
 int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s)
 {
-    if (dof != 4) {
-        set_error("block-Jacobi: dof %d not supported (4)", dof);
+    const dim3 g((L.nNo + 255) / 256);
+    if (dof == 4)
+        hipLaunchKernelGGL(k_blockdiag_inv<4>, g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
+    else if (dof == 3)
+        hipLaunchKernelGGL(k_blockdiag_inv<3>, g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
+    else {
+        set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
     }
-    hipLaunchKernelGGL(k_blockdiag_inv4, dim3((L.nNo + 255) / 256), dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val,
-                       dinv);
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }
 
 int launch_blockdiag_apply(const Lhs& L, int dof, const float* dinv, float* x, hipStream_t s, const int* stop)
 {
-    if (dof != 4) {
-        set_error("block-Jacobi: dof %d not supported (4)", dof);
+    const dim3 g((L.nNo + 255) / 256);
+    if (dof == 4)
+        hipLaunchKernelGGL(k_blockdiag_apply<4>, g, dim3(256), 0, s, L.nNo, dinv, x, stop);
+    else if (dof == 3)
+        hipLaunchKernelGGL(k_blockdiag_apply<3>, g, dim3(256), 0, s, L.nNo, dinv, x, stop);
+    else {
+        set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
     }
-    hipLaunchKernelGGL(k_blockdiag_apply4, dim3((L.nNo + 255) / 256), dim3(256), 0, s, L.nNo, dinv, x, stop);
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }
@@ -533,10 +623,10 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
     do {                                                                                                      \
         if (nt)                                                                                               \
             SV_LAUNCH(ps, (k_bspmv_vs<D, VT, XT, P, true>), dim3(gvs), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, \
-                      stop);                                                                                  \
+                      stop, dinv);                                                                            \
         else                                                                                                  \
             SV_LAUNCH(ps, (k_bspmv_vs<D, VT, XT, P, false>), dim3(gvs), dim3(256), 0, s, L.nNo, rp, cp, K, U,    \
-                      KU, stop);                                                                              \
+                      KU, stop, dinv);                                                                        \
     } while (0)
             if (dof == 2) {
                 if (lpr == 8) SV_VS(2, 8); else if (lpr == 32) SV_VS(2, 32); else SV_VS(2, 16);
@@ -547,6 +637,10 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
             SV_HIP(hipGetLastError());
             return SVFSI_OK;
         }
+    }
+    if (dinv) {
+        set_error("block-Jacobi SpMV: needs the value-streaming layout for dof %d", dof);
+        return SVFSI_ERR_NOTSUP;
     }
     const int grid = spmv_grid(L.nNo, dof);
     switch (dof) {
@@ -568,8 +662,8 @@ int launch_spmv(const Lhs& L, int dof, const double* K, const double* U, double*
 int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U, float* KU,
                     hipStream_t s, const int* stop, const float* dinv)
 {
-    if (dinv && dof != 4) {
-        set_error("block-Jacobi SpMV: dof %d not supported (4)", dof);
+    if (dinv && dof != 4 && dof != 3) {
+        set_error("block-Jacobi SpMV: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
     }
     return spmv_dispatch<float, float>(L, dof, K, U, KU, s, 1, stop, dinv);

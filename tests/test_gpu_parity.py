@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from svfsi_amd import fsils, mesh, problems
+from svfsi_amd._lib import SVFSI_STRUCT_MFMA32, SVFSI_STRUCT_MFMA64, SVFSI_STRUCT_VALU
 from svfsi_amd.assembly import DeviceMesh
 from svfsi_amd.step import NewtonStep
 
@@ -162,13 +163,16 @@ def test_fluid_assembly_parity(gpu, orc):
     assert relerr(Rg, Ro) < 1e-12
 
 
+@pytest.mark.parametrize("kern", [SVFSI_STRUCT_VALU, SVFSI_STRUCT_MFMA64])
 @pytest.mark.parametrize("quad", [0, 1])
-def test_struct_assembly_parity(gpu, orc, quad):
+def test_struct_assembly_parity(gpu, orc, quad, kern):
+    """STRUCT tet10 FP64 assembly (vector-unit and FP64-MFMA kernels) within
+    1e-12 of the oracle and bitwise reproducible."""
     import torch
 
-    prob = problems.struct_cube(3, quad=quad)
+    prob = problems.struct_cube(3, quad=quad | kern)
     prob.Ag = np.random.default_rng(5).normal(size=prob.Ag.shape)
-    say("struct assembly", prob.nNo, prob.mesh.nEl, "quad", quad)
+    say("struct assembly", prob.nNo, prob.mesh.nEl, "quad", quad, "kernel", hex(kern))
     st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
     say("struct relerr", relerr(Vg, Vo), relerr(Rg, Ro))
     assert relerr(Vg, Vo) < 1e-12
@@ -177,6 +181,36 @@ def test_struct_assembly_parity(gpu, orc, quad):
     torch.cuda.synchronize()
     assert np.array_equal(st.Val.cpu().numpy(), Vg)
     assert np.array_equal(st.R.cpu().numpy(), Rg)
+
+
+@pytest.mark.parametrize("which", ["fluid", "lelas", "struct"])
+def test_assembly_other_scatter_path(gpu, orc, monkeypatch, which):
+    """Each element kernel also runs with the other DOASSEM strategy (tet4:
+    the coloured read-modify-write; tet10: the gather); FP64 results within
+    1e-12 of the oracle."""
+    monkeypatch.setenv("SVFSI_ASM_GATHER", "1" if which == "struct" else "0")
+    prob = {"fluid": lambda: problems.fluid_cylinder(h=0.4), "lelas": lambda: problems.lelas_cube(6),
+            "struct": lambda: problems.struct_cube(3, quad=1 | SVFSI_STRUCT_MFMA64)}[which]()
+    prob.Ag = np.random.default_rng(5).normal(size=prob.Ag.shape)
+    st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
+    say(which, "other scatter path relerr", relerr(Vg, Vo), relerr(Rg, Ro))
+    assert relerr(Vg, Vo) < 1e-12
+    assert relerr(Rg, Ro) < 1e-12
+
+
+@pytest.mark.parametrize("quad", [0, 1])
+def test_struct_assembly_mfma32(gpu, orc, quad):
+    """B^T Dm B on FP32 matrix cores with FP32 accumulation: the assembled
+    matrix stays within 1e-6 of the FP64 oracle (max entry; FP32 rounding of
+    ~6 nG products per entry), the residual is unchanged (FP64), and the
+    sparsity and the FP64 kernel's bits are untouched."""
+    prob = problems.struct_cube(3, quad=quad | SVFSI_STRUCT_MFMA32)
+    prob.Ag = np.random.default_rng(5).normal(size=prob.Ag.shape)
+    st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
+    e = relerr(Vg, Vo)
+    say("struct mfma32 relerr", e, relerr(Rg, Ro))
+    assert 1e-12 < e < 1e-6
+    assert relerr(Rg, Ro) < 1e-12
 
 
 def test_fluid_assembly_parity_benchmark_mesh(gpu, orc):
@@ -284,15 +318,17 @@ def test_mixed_reaches_fp64_tolerance(gpu, orc):
 
 
 @pytest.mark.parametrize("bj", ["0", "1"])
-def test_polynomial_inner_preconditioner(gpu, orc, monkeypatch, bj):
+@pytest.mark.parametrize("which", ["fluid", "lelas", "struct"])
+def test_polynomial_inner_preconditioner(gpu, orc, monkeypatch, bj, which):
     """Mixed solve with the GMRES(30) residual polynomial as the inner
-    preconditioner (with and without block-Jacobi): reaches the FP64
-    tolerance, and repeats bit for bit."""
+    preconditioner (with and without block-Jacobi; dof 4 and dof 3): reaches
+    the FP64 tolerance, and repeats bit for bit."""
     import torch
 
     monkeypatch.setenv("SVFSI_INNER_POLY", "1")
     monkeypatch.setenv("SVFSI_INNER_BJ", bj)
-    prob = problems.fluid_cylinder(h=0.4)
+    prob = {"fluid": lambda: problems.fluid_cylinder(h=0.4), "lelas": lambda: problems.lelas_cube(6),
+            "struct": lambda: problems.struct_cube(3, quad=1)}[which]()
     st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc, mixed=True)
     say("poly bj", bj, "outer", ls.RI.itr, "SpMVs", ls.RI.cM)
     assert ls.RI.suc == 1
@@ -330,20 +366,20 @@ def test_inner_block_jacobi_reduces_inner_iterations(gpu, orc, monkeypatch):
     assert res["1"] < res["0"], res
 
 
-@pytest.mark.parametrize("mixed,cg", [(True, False), (False, False), (False, True)])
-def test_graph_replays_match_direct_launches(gpu, monkeypatch, mixed, cg):
+@pytest.mark.parametrize("mixed,cg,poly", [(True, False, "0"), (True, False, "1"), (True, True, "1"),
+                                           (False, False, "1"), (False, True, "1")])
+def test_graph_replays_match_direct_launches(gpu, monkeypatch, mixed, cg, poly):
     """The solvers replay fixed kernel sequences as HIP graphs (one per outer
-    step of the mixed solve, one per chunk of Arnoldi steps of the FP64
-    GMRES, one per chunk of CG iterations).  Several replays, with eager work
-    and synchronisation between them, must reproduce the directly launched
-    solve bit for bit."""
+    step of the mixed solve with the inner GMRES cycle, one per chunk of
+    Arnoldi steps of the FP64 GMRES, one per chunk of CG iterations; the
+    polynomial steps are launched directly either way).  Several replays,
+    with eager work and synchronisation between them, must reproduce the
+    directly launched solve bit for bit."""
     import torch
 
     prob = problems.lelas_cube(8) if cg else problems.fluid_cylinder(h=0.5)
     st = NewtonStep(prob)
-    # the graph-captured inner step is the GMRES cycle (the polynomial
-    # inner steps are launched directly)
-    monkeypatch.setenv("SVFSI_INNER_POLY", "0")
+    monkeypatch.setenv("SVFSI_INNER_POLY", poly)
     monkeypatch.setenv("SVFSI_GRAPHS", "0")
     st.assemble()
     ls0 = st.solve(mixed=mixed)
