@@ -25,6 +25,8 @@
 
 namespace svfsi {
 
+
+
 // GNN for an affine tet4 (svFSI/NN.f:1469-1565): Nxi of TET4 are constant
 // (NN.f:663-671): dN1 = e1, dN2 = e2, dN3 = e3, dN4 = -(1,1,1).
 __device__ __forceinline__ void gnn_tet4(const double xl[4][3], double Nx[4][3], double& Jac,
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(256) void k_asm_lelas(int begin, int count, const i
         rr[2] = r[2];
         return;
     }
-    double* Rr = R + (size_t)nd[a] * 3;
+    double* Rr = R + (size_t)ien[(size_t)p * 4 + a] * 3;
     Rr[0] += r[0];
     Rr[1] += r[1];
     Rr[2] += r[2];
@@ -284,17 +286,20 @@ __device__ __forceinline__ void fluid_elem(int p, const int* __restrict__ ien, c
     E.divU = E.ux[0][0] + E.ux[1][1] + E.ux[2][2];
 }
 
-// FLUID3D_M + FLUID3D_C (svFSI/FLUID.f:192-560, 813-1084) for the (a,b)
-// block of one tet4 element, Newtonian viscosity mu, second derivatives
-// zero: k (4x4 block, row-major) and, when want_r, the residual of node a.
+// Gauss-point quantities of FLUID3D_M/C shared by all (a,b) blocks of a
+// tet4 element: u, u_dot (with body force), p, the VMS parameters tau_M,
+// tau_C, tau_B and the fine-scale velocity u' (svFSI/FLUID.f:220-412).
+template <typename T> struct FluidGP {
+    double u[3], ud[3], pp;
+    T tauM, tauC, tauB, up[3];
+};
+
 template <typename T>
-__device__ __forceinline__ void fluid_pair(const FluidElem& E, T mu, const AsmParams& P, const Tet4Rule& rule,
-                                           int a, int b, bool want_r, T k[16], T r[4])
+__device__ __forceinline__ void fluid_gp(const FluidElem& E, T mu, const AsmParams& P, const Tet4Rule& rule, int g,
+                                         FluidGP<T>& G)
 {
     const double rho = P.rho;
     const double ctM = 1.0, ctC = 36.0;
-    const double T1c = P.af * P.gam * P.dt;
-    const double amd = P.am / T1c;
     double kS0 = 0.0;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
@@ -303,111 +308,186 @@ __device__ __forceinline__ void fluid_pair(const FluidElem& E, T mu, const AsmPa
     const T kS = ctC * kS0 * (mu / rho) * (mu / rho);
     const double kT = 4.0 * (ctM / P.dt) * (ctM / P.dt);
     const double trK = E.ks[0][0] + E.ks[1][1] + E.ks[2][2];
-    const double NxNx = E.Nx[a][0] * E.Nx[b][0] + E.Nx[a][1] * E.Nx[b][1] + E.Nx[a][2] * E.Nx[b][2];
+    double ud[3] = {-P.fb[0], -P.fb[1], -P.fb[2]};
+    double u[3] = {0, 0, 0};
+    double pp = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double Nq = rule.N[g][q];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ud[c] += Nq * E.al[q][c];
+            u[c] += Nq * E.yl[q][c];
+        }
+        pp += Nq * E.yl[q][3];
+    }
+    double kU = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) kU += u[i] * u[j] * E.ks[i][j];
+    const T tauM = 1.0 / (rho * dsqrt(kT + kU + kS));
+    T up[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double rV = ud[c] + u[0] * E.ux[0][c] + u[1] * E.ux[1][c] + u[2] * E.ux[2][c];
+        up[c] = -tauM * (rho * rV + E.px[c] - mu * 0.0);
+    }
+    const T tauC = 1.0 / (tauM * trK);
+    T tauB = scal<T>(0.0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tauB = tauB + up[i] * up[j] * E.ks[i][j];
+    if (iszero_d(val_of(tauB))) tauB = scal<T>(2.220446049250313e-16);
+    tauB = rho / dsqrt(tauB);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        G.u[c] = u[c];
+        G.ud[c] = ud[c];
+        G.up[c] = up[c];
+    }
+    G.pp = pp;
+    G.tauM = tauM;
+    G.tauC = tauC;
+    G.tauB = tauB;
+}
 
+// FLUID3D_M + FLUID3D_C (svFSI/FLUID.f:192-560, 813-1084), the Gauss point
+// g contribution to the (a,b) block of one tet4 element, Newtonian
+// viscosity mu, second derivatives zero: k (4x4 block, row-major) and, when
+// want_r, the residual of node a.
+// Rows a and b of the element arrays, selected without dynamic indexing
+// (which would place the element record in scratch memory).
+struct FluidAB {
+    double Nxa[3], Nxb[3];
+    double Na[4], Nb[4]; // N_a, N_b at the four Gauss points
+};
+
+__device__ __forceinline__ void fluid_ab(const FluidElem& E, const Tet4Rule& rule, int a, int b, FluidAB& S)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        S.Nxa[i] = 0.0;
+        S.Nxb[i] = 0.0;
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        S.Na[g] = 0.0;
+        S.Nb[g] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (q == a) S.Nxa[i] = E.Nx[q][i];
+            if (q == b) S.Nxb[i] = E.Nx[q][i];
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            if (q == a) S.Na[g] = rule.N[g][q];
+            if (q == b) S.Nb[g] = rule.N[g][q];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void fluid_pair_gp(const FluidElem& E, T mu, const AsmParams& P, const Tet4Rule& rule,
+                                              int g, const FluidGP<T>& G, const FluidAB& S, bool want_r, T k[16],
+                                              T r[4])
+{
+    const double* Nxa = S.Nxa;
+    const double* Nxb = S.Nxb;
+    const double rho = P.rho;
+    const double T1c = P.af * P.gam * P.dt;
+    const double amd = P.am / T1c;
+    const double NxNx = Nxa[0] * Nxb[0] + Nxa[1] * Nxb[1] + Nxa[2] * Nxb[2];
+    const double w = rule.w[g] * E.Jac;
+    const double wl = w * T1c;
+    const double wr = w * rho;
+    const double* u = G.u;
+    const double* ud = G.ud;
+    const T tauM = G.tauM, tauC = G.tauC, tauB = G.tauB;
+    const T* up = G.up;
+    const T ua[3] = {u[0] + up[0], u[1] + up[1], u[2] + up[2]};
+
+    const double Na = S.Na[g], Nb = S.Nb[g];
+    const double uNxa = u[0] * Nxa[0] + u[1] * Nxa[1] + u[2] * Nxa[2];
+    const double uNxb = u[0] * Nxb[0] + u[1] * Nxb[1] + u[2] * Nxb[2];
+    const T upNxa = up[0] * Nxa[0] + up[1] * Nxa[1] + up[2] * Nxa[2];
+    const T upNxb = up[0] * Nxb[0] + up[1] * Nxb[1] + up[2] * Nxb[2];
+    const T uaNxa = uNxa + upNxa;
+    const double T1b = -rho * uNxb;   // updu(i,c,b) = delta_ic T1b
+
+    // momentum u-u blocks (FLUID.f:494-545)
+    const T T1 = mu * NxNx + rho * amd * Nb * (Na + rho * tauM * uaNxa) + rho * Na * (uNxb + upNxb)
+               + tauB * upNxa * upNxb;
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double updu = (rr == c) ? T1b : 0.0;
+            const T T2 = mu * Nxa[c] * Nxb[rr] + tauC * Nxa[rr] * Nxb[c]
+                       - rho * tauM * uaNxa * updu;
+            k[rr * 4 + c] = k[rr * 4 + c] + wl * ((rr == c) ? T2 + T1 : T2);
+        }
+    // momentum u-p blocks (FLUID.f:548-557)
+    {
+        const T T1p = rho * tauM * uaNxa;
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) k[rr * 4 + 3] = k[rr * 4 + 3] - wl * (Nxa[rr] * Nb - Nxb[rr] * T1p);
+    }
+    // continuity p-u and p-p blocks (FLUID.f:1052-1081)
+    {
+        const double T1q = rho * amd * Nb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double T2 = Nxa[c] * (T1b - T1q);
+            k[12 + c] = k[12 + c] + wl * (Na * Nxb[c] - tauM * T2);
+        }
+        k[15] = k[15] + wl * tauM * NxNx;
+    }
+    if (want_r) {
+        // momentum residual (FLUID.f:429-465) and continuity (1046-1049)
+        const T pa = G.pp - tauC * E.divU;
+        T rVb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            rVb[c] = tauB * (up[0] * E.ux[0][c] + up[1] * E.ux[1][c] + up[2] * E.ux[2][c]);
+        T rM[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double es = E.ux[i][c] + E.ux[c][i];
+                rM[i][c] = mu * es - rho * up[c] * ua[i] + rVb[c] * up[i] - (i == c ? pa : scal<T>(0.0));
+            }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T rV = ud[c] + ua[0] * E.ux[0][c] + ua[1] * E.ux[1][c] + ua[2] * E.ux[2][c];
+            r[c] = r[c] + (wr * Na * rV
+                           + w * (Nxa[0] * rM[0][c] + Nxa[1] * rM[1][c] + Nxa[2] * rM[2][c]));
+        }
+        r[3] = r[3] + w * (Na * E.divU - upNxa);
+    }
+}
+
+// The whole (a,b) block: the four Gauss points in order.
+template <typename T>
+__device__ __forceinline__ void fluid_pair(const FluidElem& E, T mu, const AsmParams& P, const Tet4Rule& rule,
+                                           int a, int b, bool want_r, T k[16], T r[4])
+{
+    FluidAB S;
+    fluid_ab(E, rule, a, b, S);
 #pragma unroll
     for (int q = 0; q < 16; ++q) k[q] = scal<T>(0.0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) r[q] = scal<T>(0.0);
-#pragma unroll 1
+#pragma unroll
     for (int g = 0; g < 4; ++g) {
-        const double w = rule.w[g] * E.Jac;
-        const double wl = w * T1c;
-        const double wr = w * rho;
-        double ud[3] = {-P.fb[0], -P.fb[1], -P.fb[2]};
-        double u[3] = {0, 0, 0};
-        double pp = 0.0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double Nq = rule.N[g][q];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ud[c] += Nq * E.al[q][c];
-                u[c] += Nq * E.yl[q][c];
-            }
-            pp += Nq * E.yl[q][3];
-        }
-        double kU = 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) kU += u[i] * u[j] * E.ks[i][j];
-        const T tauM = 1.0 / (rho * dsqrt(kT + kU + kS));
-        T up[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double rV = ud[c] + u[0] * E.ux[0][c] + u[1] * E.ux[1][c] + u[2] * E.ux[2][c];
-            up[c] = -tauM * (rho * rV + E.px[c] - mu * 0.0);
-        }
-        const T tauC = 1.0 / (tauM * trK);
-        T tauB = scal<T>(0.0);
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) tauB = tauB + up[i] * up[j] * E.ks[i][j];
-        if (iszero_d(val_of(tauB))) tauB = scal<T>(2.220446049250313e-16);
-        tauB = rho / dsqrt(tauB);
-        const T ua[3] = {u[0] + up[0], u[1] + up[1], u[2] + up[2]};
-
-        const double Na = rule.N[g][a], Nb = rule.N[g][b];
-        const double uNxa = u[0] * E.Nx[a][0] + u[1] * E.Nx[a][1] + u[2] * E.Nx[a][2];
-        const double uNxb = u[0] * E.Nx[b][0] + u[1] * E.Nx[b][1] + u[2] * E.Nx[b][2];
-        const T upNxa = up[0] * E.Nx[a][0] + up[1] * E.Nx[a][1] + up[2] * E.Nx[a][2];
-        const T upNxb = up[0] * E.Nx[b][0] + up[1] * E.Nx[b][1] + up[2] * E.Nx[b][2];
-        const T uaNxa = uNxa + upNxa;
-        const double T1b = -rho * uNxb;   // updu(i,c,b) = delta_ic T1b
-
-        // momentum u-u blocks (FLUID.f:494-545)
-        const T T1 = mu * NxNx + rho * amd * Nb * (Na + rho * tauM * uaNxa) + rho * Na * (uNxb + upNxb)
-                   + tauB * upNxa * upNxb;
-#pragma unroll
-        for (int rr = 0; rr < 3; ++rr)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double updu = (rr == c) ? T1b : 0.0;
-                const T T2 = mu * E.Nx[a][c] * E.Nx[b][rr] + tauC * E.Nx[a][rr] * E.Nx[b][c]
-                           - rho * tauM * uaNxa * updu;
-                k[rr * 4 + c] = k[rr * 4 + c] + wl * ((rr == c) ? T2 + T1 : T2);
-            }
-        // momentum u-p blocks (FLUID.f:548-557)
-        {
-            const T T1p = rho * tauM * uaNxa;
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) k[rr * 4 + 3] = k[rr * 4 + 3] - wl * (E.Nx[a][rr] * Nb - E.Nx[b][rr] * T1p);
-        }
-        // continuity p-u and p-p blocks (FLUID.f:1052-1081)
-        {
-            const double T1q = rho * amd * Nb;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double T2 = E.Nx[a][c] * (T1b - T1q);
-                k[12 + c] = k[12 + c] + wl * (Na * E.Nx[b][c] - tauM * T2);
-            }
-            k[15] = k[15] + wl * tauM * NxNx;
-        }
-        if (want_r) {
-            // momentum residual (FLUID.f:429-465) and continuity (1046-1049)
-            const T pa = pp - tauC * E.divU;
-            T rVb[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                rVb[c] = tauB * (up[0] * E.ux[0][c] + up[1] * E.ux[1][c] + up[2] * E.ux[2][c]);
-            T rM[3][3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    const double es = E.ux[i][c] + E.ux[c][i];
-                    rM[i][c] = mu * es - rho * up[c] * ua[i] + rVb[c] * up[i] - (i == c ? pa : scal<T>(0.0));
-                }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T rV = ud[c] + ua[0] * E.ux[0][c] + ua[1] * E.ux[1][c] + ua[2] * E.ux[2][c];
-                r[c] = r[c] + (wr * Na * rV
-                               + w * (E.Nx[a][0] * rM[0][c] + E.Nx[a][1] * rM[1][c] + E.Nx[a][2] * rM[2][c]));
-            }
-            r[3] = r[3] + w * (Na * E.divU - upNxa);
-        }
+        FluidGP<T> G;
+        fluid_gp(E, mu, P, rule, g, G);
+        fluid_pair_gp(E, mu, P, rule, g, G, S, want_r, k, r);
     }
 }
 
@@ -432,7 +512,42 @@ __global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const i
     fluid_elem(p, ien, x, tDof, Ag, Yg, E);
     const double mu = muEl ? muEl[elemId[p]] : P.mu;
     double k[16], r[4];
-    fluid_pair<double>(E, mu, P, rule, a, b, b == 0, k, r);
+    // the 16 lanes of an element (consecutive lanes of one wave; count * 16
+    // threads, so a group is active or inactive as a whole) evaluate the
+    // Gauss-point quantities once per point, lane (ab & 3) for point ab & 3,
+    // and fetch each point's values from its lane
+    FluidGP<double> mine;
+    {
+        // the point this lane evaluates, selected without dynamic indexing
+        FluidGP<double> G;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            if (g == (ab & 3)) fluid_gp<double>(E, mu, P, rule, g, mine);
+        (void)G;
+    }
+    FluidAB S;
+    fluid_ab(E, rule, a, b, S);
+    const int base = (threadIdx.x & (kWave - 1)) & ~15;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) k[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[q] = 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        FluidGP<double> G;
+        const int src = base + g;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            G.u[c] = __shfl(mine.u[c], src, kWave);
+            G.ud[c] = __shfl(mine.ud[c], src, kWave);
+            G.up[c] = __shfl(mine.up[c], src, kWave);
+        }
+        G.pp = __shfl(mine.pp, src, kWave);
+        G.tauM = __shfl(mine.tauM, src, kWave);
+        G.tauC = __shfl(mine.tauC, src, kWave);
+        G.tauB = __shfl(mine.tauB, src, kWave);
+        fluid_pair_gp<double>(E, mu, P, rule, g, G, S, b == 0, k, r);
+    }
     if (ke) {
         double* kb = ke + (size_t)gslot[(size_t)p * 16 + ab] * 16;
 #pragma unroll
@@ -448,7 +563,10 @@ __global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const i
 #pragma unroll
     for (int q = 0; q < 16; ++q) vb[q] += k[q];
     if (b == 0) {
-        double* Rr = R + (size_t)E.nd[a] * 4;
+        // the node index reloaded from ien: indexing the per-thread E.nd with
+        // the lane-dependent a would place the whole element record in
+        // scratch memory
+        double* Rr = R + (size_t)ien[(size_t)p * 4 + a] * 4;
 #pragma unroll
         for (int c = 0; c < 4; ++c) Rr[c] += r[c];
     }
@@ -476,8 +594,8 @@ __global__ __launch_bounds__(256) void k_fluid_dmu(int nEl, const int* __restric
     const Dual mu = {muEl ? muEl[elemId[p]] : P.mu, 1.0};
     Dual k[16], r[4];
     fluid_pair<Dual>(E, mu, P, rule, a, b, b == 0, k, r);
-    const double* la = lam + (size_t)E.nd[a] * 4;
-    const double* ub = du + (size_t)E.nd[b] * 4;
+    const double* la = lam + (size_t)ien[(size_t)p * 4 + a] * 4;
+    const double* ub = du + (size_t)ien[(size_t)p * 4 + b] * 4;
     double s = 0.0;
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {

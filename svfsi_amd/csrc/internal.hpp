@@ -221,6 +221,7 @@ int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
                     float* KU, hipStream_t s, const int* stop = nullptr,
                     const float* dinv = nullptr);
 int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s);
+int launch_bjscale(const Lhs& L, int dof, const double* K, const float* dinv, float* B, hipStream_t s);
 int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* dinv, const float* x,
                 const float* prodOld, float* w, float* y, float* out, const double* coef, int step, hipStream_t s,
                 const int* stop);
@@ -245,7 +246,7 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
 int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
                 double* R, hipStream_t s, bool transpose);
 int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
-                const float* K32, double* R, hipStream_t s);
+                float* K32, double* R, hipStream_t s);
 
 double wall_seconds();
 

@@ -1382,7 +1382,7 @@ static double env_dbl(const char* k, double d)
 // a loose relative tolerance; the outer Krylov space keeps what restarted
 // inner cycles would lose.  Convergence is GMRESV's FP64 test
 // |W(R - K x)| < max(absTol, relTol |W R|) on the outer estimate.
-int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const float* K32, double* R,
+int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K32, double* R,
                 hipStream_t s)
 {
     const size_t n = (size_t)L.nNo * dof;
@@ -1391,6 +1391,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     // none is set (CG requests); SVFSI_FGMRES_M overrides
     const int mo = env_int("SVFSI_FGMRES_M", ls.sD > 0 ? std::min(ls.sD, kMaxSD) : 30);
     const double tin = env_dbl("SVFSI_FGMRES_INNER_TOL", 0.1);
+    const int cgsPasses = env_int("SVFSI_OUTER_CGS", 2) == 1 ? 1 : 2;
     const int mi = env_int("SVFSI_FGMRES_INNER_M", 30);     // inner Krylov dimension
     if (mo > kMaxSD || mi > kMaxSD) {
         set_error("FGMRES: Krylov dimension exceeds %d", kMaxSD);
@@ -1414,19 +1415,28 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     double* scal = part + partN - 8;
     GmState* st = w.ostate.as<GmState>();
     GmArrays A = gm_carve(w.ohess.as<double>(), mo);
-    // block-Jacobi inside the FP32 inner work: the inner operator is D^-1 K,
-    // D the dof x dof diagonal blocks of the scaled matrix.  On by default for
-    // dof 4 (fewer inner SpMVs on the fluid); off for dof 3, where it measured
-    // more outer steps on the elasticity cube (SVFSI_INNER_BJ = 0 / 1
-    // overrides)
+    // Inner work: the polynomial preconditioner (dof 3, 4; default) or one
+    // FP32 GMRES cycle (SVFSI_INNER_POLY=0).  Block-Jacobi (inner operator
+    // D^-1 K, D the dof x dof diagonal blocks of the scaled matrix) is on by
+    // default for dof 4: on the fluid cylinders it cut the inner GMRES
+    // cycle's SpMVs (467 -> 405) and the polynomial's outer steps on the
+    // 3.2M-tet cylinder (72 -> 54).  Off for dof 3, where it added outer
+    // steps on the elasticity cube (16 -> 22).  SVFSI_INNER_BJ = 0 / 1
+    // overrides.
+    const bool polyWanted = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", 1) != 0;
     const bool bj = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_BJ", dof == 4 ? 1 : 0) != 0;
+    // With block Jacobi the FP32 inner matrix is replaced by B = D^-1 K (K32
+    // is overwritten; formed once per solve from the FP64 matrix), so the
+    // inner products carry no epilogue, and each inner right-hand side is
+    // multiplied by D^-1.
     const float* dinv = nullptr;
     if (bj) {
         SV_TRY(w.dinv32.ensure((size_t)L.nNo * dof * dof * sizeof(float)));
         SV_TRY(launch_blockdiag_inv(L, dof, K, w.dinv32.as<float>(), s));
+        SV_TRY(launch_bjscale(L, dof, K, w.dinv32.as<float>(), K32, s));
         dinv = w.dinv32.as<float>();
     }
-    GmOp<float> op32{&L, dof, K32, dinv};
+    GmOp<float> op32{&L, dof, K32, nullptr};
 
     ls.callD = wall_seconds();
     ls.suc = 0;
@@ -1472,7 +1482,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
     long long polySetup = 0;
     float *P0 = nullptr, *P1 = nullptr, *Wb = nullptr;
     const double* coefD = nullptr;
-    bool usePoly = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", 1) != 0;
+    bool usePoly = polyWanted;
     if (usePoly) {
         const int d = env_int("SVFSI_POLY_DEG", 30);
         if (d < 2 || d > kMaxSD) {
@@ -1528,7 +1538,8 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
             (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)K32,
             (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
             (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
-            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv};
+            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv,
+            (uintptr_t)cgsPasses};
         if (sig != w.graphs.sig) {
             w.graphs.clear();
             w.graphs.sig = sig;
@@ -1564,15 +1575,15 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
                             if (last) {
                                 SV_TRY(launch_poly_last(n, cur, zi, coefD, f, cs, &st->stop));
                             } else {
-                                SV_TRY(launch_poly(L, dof, 0, K32, dinv, cur, nullptr, nullptr, zi, nxt, coefD, f, cs,
+                                SV_TRY(launch_poly(L, dof, 0, K32, nullptr, cur, nullptr, nullptr, zi, nxt, coefD, f, cs,
                                                     &st->stop));
                                 std::swap(cur, nxt);
                             }
                         } else {
-                            SV_TRY(launch_poly(L, dof, 1, K32, dinv, cur, nullptr, Wb, zi, nullptr, coefD, f, cs,
+                            SV_TRY(launch_poly(L, dof, 1, K32, nullptr, cur, nullptr, Wb, zi, nullptr, coefD, f, cs,
                                                 &st->stop));
                             if (!last) {
-                                SV_TRY(launch_poly(L, dof, 2, K32, dinv, Wb, cur, nullptr, nullptr, nxt, coefD, f, cs,
+                                SV_TRY(launch_poly(L, dof, 2, K32, nullptr, Wb, cur, nullptr, nullptr, nxt, coefD, f, cs,
                                                     &st->stop));
                                 std::swap(cur, nxt);
                             }
@@ -1586,8 +1597,10 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, const flo
                 }
                 hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, cs, n, (const float*)zi, t64);
                 SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, cs, &st->stop));
-                // outer Arnoldi: CGS2 (two classical passes)
-                for (int pass = 1; pass <= 2; ++pass) {
+                // outer Arnoldi: classical Gram-Schmidt, two passes (CGS2) by
+                // default; SVFSI_OUTER_CGS=1 runs the reference's single pass
+                for (int pp = 1; pp <= cgsPasses; ++pp) {
+                    const int pass = cgsPasses == 1 ? 0 : pp;
                     hipLaunchKernelGGL(k_gm_mdot<double>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0,
                                        cs, n, i, (const double*)Q, part, (const GmState*)st);
                     hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, mo, (int)nbr, (const double*)part,

@@ -415,9 +415,13 @@ __global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __res
         for (int p = c + 1; p < DOF; ++p)
             if (fabs(a[p][c]) > best) { best = fabs(a[p][c]); piv = p; }
         if (best == 0.0) { ok = false; break; }
-        if (piv != c)
+        // row swap with compile-time row indices (a lane-dependent index
+        // would move the matrix to scratch memory)
 #pragma unroll
-            for (int q = 0; q < 2 * DOF; ++q) { const double t = a[c][q]; a[c][q] = a[piv][q]; a[piv][q] = t; }
+        for (int p = c + 1; p < DOF; ++p)
+            if (p == piv)
+#pragma unroll
+                for (int q = 0; q < 2 * DOF; ++q) { const double t = a[c][q]; a[c][q] = a[p][q]; a[p][q] = t; }
         const double inv = 1.0 / a[c][c];
 #pragma unroll
         for (int q = 0; q < 2 * DOF; ++q) a[c][q] *= inv;
@@ -434,6 +438,28 @@ __global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __res
     for (int p = 0; p < DOF; ++p)
 #pragma unroll
         for (int q = 0; q < DOF; ++q) o[p * DOF + q] = ok ? (float)a[p][DOF + q] : (p == q ? 1.0f : 0.0f);
+}
+
+// B = D^-1 K in FP32 (block Jacobi folded into the inner matrix): block k of
+// block row i becomes D_i^-1 K_k, formed in FP64 from the FP64 scaled matrix.
+// DD lanes per block, one output entry each.
+template <int DOF>
+__global__ __launch_bounds__(256) void k_bjscale(size_t nnz, const int* __restrict__ blockRow,
+                                                 const double* __restrict__ K, const float* __restrict__ dinv,
+                                                 float* __restrict__ B)
+{
+    constexpr int DD = DOF * DOF;
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t k = t / DD;
+    if (k >= nnz) return;
+    const int e = (int)(t - k * DD);
+    const int r = e / DOF, c = e - (e / DOF) * DOF;
+    const float* d = dinv + (size_t)blockRow[k] * DD + r * DOF;
+    const double* kb = K + k * DD;
+    double v = 0.0;
+#pragma unroll
+    for (int m = 0; m < DOF; ++m) v += (double)d[m] * kb[m * DOF + c];
+    B[k * DD + e] = (float)v;
 }
 
 // x = D^-1 x per node (FP32, in place).
@@ -490,6 +516,22 @@ int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, 
         hipLaunchKernelGGL(k_blockdiag_inv<4>, g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
     else if (dof == 3)
         hipLaunchKernelGGL(k_blockdiag_inv<3>, g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
+    else {
+        set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
+        return SVFSI_ERR_NOTSUP;
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_bjscale(const Lhs& L, int dof, const double* K, const float* dinv, float* B, hipStream_t s)
+{
+    const size_t nnz = (size_t)L.nnz;
+    const dim3 g((unsigned)((nnz * dof * dof + 255) / 256));
+    if (dof == 4)
+        hipLaunchKernelGGL(k_bjscale<4>, g, dim3(256), 0, s, nnz, L.blockRow.as<int>(), K, dinv, B);
+    else if (dof == 3)
+        hipLaunchKernelGGL(k_bjscale<3>, g, dim3(256), 0, s, nnz, L.blockRow.as<int>(), K, dinv, B);
     else {
         set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;

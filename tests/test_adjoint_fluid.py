@@ -190,3 +190,40 @@ def test_viscosity_adjoint_vs_cpu_adjoint_and_fd(gpu, orc, mixed):
     assert err <= 1e-7
     elems = np.random.default_rng(13).choice(m.nEl, 20, replace=False)
     assert _fd_check(adj, z, du.detach(), c, g, elems) <= 1.0
+
+
+@pytest.mark.gpu
+def test_viscosity_adjoint_large_cylinder(gpu):
+    """The config-5 adjoint at full size: the ~3.2M-tet cylinder (h = 0.042),
+    mu_e = 0.04 exp(z_e) with z ~ N(0, 0.1^2) i.i.d., loss = the weighted
+    outlet flux of the linearised step, mixed-precision forward and transposed
+    solves.  Checked by central differences (relative step 1e-4 in mu) on the
+    five elements of largest |dL/dz| and five random ones, to 1e-5 relative
+    plus the absolute floor of _fd_check."""
+    import time
+
+    import torch
+
+    from adjoint_fn import FluidViscositySolve
+    from svfsi_amd.adjoint import FluidViscosityAdjoint
+
+    prob = fluid_problem(0.042)
+    prob.ls = dict(prob.ls, relTol=1e-12, mItr=200, sD=100)
+    m = prob.mesh
+    print("nEl", m.nEl, "nNo", m.nNo, flush=True)
+    adj = FluidViscosityAdjoint(prob, mixed=True)
+    z = torch.tensor(np.log(prob.prop_el / MU0), device=gpu)
+    c = torch.tensor(outlet_flux_weights(prob), device=gpu)
+    zr = z.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    du = FluidViscositySolve.apply(zr, adj, MU0)
+    L = torch.sum(c * du)
+    L.backward()
+    torch.cuda.synchronize()
+    print(f"forward + adjoint {time.perf_counter() - t0:.2f} s", flush=True)
+    g = zr.grad.detach()
+    assert torch.isfinite(g).all()
+    top = torch.argsort(g.abs(), descending=True)[:5].cpu().numpy()
+    rnd = np.random.default_rng(17).choice(m.nEl, 5, replace=False)
+    assert _fd_check(adj, z, du.detach(), c, g, np.concatenate([top, rnd])) <= 1.0
