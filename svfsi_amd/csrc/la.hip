@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
 // B = D^-1 K when dinv is given (block-Jacobi), else K.  coef is a device
 // array (c1, c2 per factor), so the launches do not depend on the values.
 // ---------------------------------------------------------------------------
-template <int MODE, bool NT>
+template <int MODE, bool NT, int H = 4>
 __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ rowPtr,
                                                const int* __restrict__ colPtr, const float* __restrict__ val,
                                                const float* __restrict__ dinv, const float* __restrict__ x,
@@ -220,7 +220,6 @@ __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ 
                                                const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    constexpr int H = 4;
     const double c1 = coef[2 * step], c2 = coef[2 * step + 1];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = t / (4 * H);
@@ -356,10 +355,28 @@ int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* di
 #define SV_POLY(KERN, M, N)                                                                                     \
     SV_LAUNCH(ps, (KERN<M, N>), dim3(g), dim3(256), 0, s, L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, \
               step, stop)
+    // lanes per output component of the dof-4 kernel (tuning knob
+    // SVFSI_POLY_H = 2, 4 or 8)
+    static const int ph = [] {
+        const char* e = getenv("SVFSI_POLY_H");
+        const int v = e ? atoi(e) : 4;
+        return (v == 2 || v == 8) ? v : 4;
+    }();
+#define SV_POLYH(M, N, HH)                                                                                       \
+    SV_LAUNCH(ps, (k_poly4<M, N, HH>), dim3((unsigned)(((size_t)L.nNo * 4 * HH + 255) / 256)), dim3(256), 0, s, \
+              L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, step, stop)
+#define SV_POLY4(M, N)                                                                                           \
+    do {                                                                                                         \
+        if (ph == 2) SV_POLYH(M, N, 2);                                                                          \
+        else if (ph == 8) SV_POLYH(M, N, 8);                                                                     \
+        else SV_POLYH(M, N, 4);                                                                                  \
+    } while (0)
     if (dof == 4) {
-        if (mode == 0) { if (nt) SV_POLY(k_poly4, 0, true); else SV_POLY(k_poly4, 0, false); }
-        else if (mode == 1) { if (nt) SV_POLY(k_poly4, 1, true); else SV_POLY(k_poly4, 1, false); }
-        else { if (nt) SV_POLY(k_poly4, 2, true); else SV_POLY(k_poly4, 2, false); }
+        if (mode == 0) { if (nt) SV_POLY4(0, true); else SV_POLY4(0, false); }
+        else if (mode == 1) { if (nt) SV_POLY4(1, true); else SV_POLY4(1, false); }
+        else { if (nt) SV_POLY4(2, true); else SV_POLY4(2, false); }
+#undef SV_POLY4
+#undef SV_POLYH
     } else {
         if (mode == 0) { if (nt) SV_POLY(k_poly3, 0, true); else SV_POLY(k_poly3, 0, false); }
         else if (mode == 1) { if (nt) SV_POLY(k_poly3, 1, true); else SV_POLY(k_poly3, 1, false); }
