@@ -18,9 +18,11 @@ SRC = HERE / "svfsi_oracle.c"
 
 
 def build() -> Path:
-    """Compile the oracle with gcc (no external libraries)."""
+    """Compile the oracle with gcc (OpenMP from gcc's own runtime; no other
+    libraries)."""
     tmp = SO.with_suffix(".so.tmp")
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-Wextra", "-o", str(tmp), str(SRC), "-lm"]
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-fopenmp", "-Wall", "-Wextra", "-o", str(tmp), str(SRC),
+           "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stderr}")
@@ -133,6 +135,59 @@ def assemble(prob, rowPtr: np.ndarray, colPtr: np.ndarray) -> tuple[np.ndarray, 
         L.orc_backflow_tri3(len(fi), P(fi), P(fe), P(ien), m.eNoN, P(x), P(Yg), Yg.shape[1], C.byref(eq),
                             P(rowPtr), P(colPtr), P(Val), P(R))
     return Val, R
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's solver kernels (CPU baseline timing only)."""
+    lib().orc_set_threads(int(n))
+
+
+def assemble_ranks_seconds(prob, rowPtr: np.ndarray, colPtr: np.ndarray, nranks: int) -> float:
+    """Wall time of the element loop split over nranks host threads, each
+    assembling a contiguous range of elements into its own zeroed Val and R
+    (the per-rank local systems of the reference's MPI partition; the
+    interface sum that follows there is a halo exchange and is not
+    emulated; the outputs are zeroed beforehand, as each rank's share of
+    LSALLOC's zeroing is 1/nranks of the matrix).  The ctypes calls release the GIL, so the ranks run
+    concurrently.  Face terms are left out (a few hundred elements)."""
+    import concurrent.futures as cf
+    import time
+
+    m = prob.mesh
+    dof = prob.dof
+    eq = eq_struct(prob.eq)
+    x, ien = f64(m.x), i32(m.ien)
+    Ag, Yg, Dg = f64(prob.Ag), f64(prob.Yg), f64(prob.Dg)
+    prop = None if prob.prop_el is None else f64(prob.prop_el)
+    rp, cp = i32(rowPtr), i32(colPtr)
+    bounds = [int(b) for b in np.linspace(0, m.nEl, nranks + 1)]
+    # per-rank inputs and zeroed outputs, prepared outside the timed region
+    bufs = [(np.zeros((len(colPtr), dof * dof)), np.zeros((m.nNo, dof))) for _ in range(nranks)]
+    subs = [np.ascontiguousarray(ien[bounds[k]:bounds[k + 1]]) for k in range(nranks)]
+    props = [None if prop is None else np.ascontiguousarray(prop[bounds[k]:bounds[k + 1]]) for k in range(nranks)]
+    L = lib()
+
+    def rank(k):
+        e0, e1 = bounds[k], bounds[k + 1]
+        Val, R = bufs[k]
+        sub = subs[k]
+        pr = None if props[k] is None else P(props[k])
+        if prob.phys == 2:
+            L.orc_construct_lelas(m.nNo, e1 - e0, P(sub), P(x), P(Ag), P(Dg), Ag.shape[1], C.byref(eq), pr,
+                                  P(rp), P(cp), P(Val), P(R))
+        elif prob.phys == 1:
+            L.orc_construct_fluid(m.nNo, e1 - e0, P(sub), P(x), P(Ag), P(Yg), Ag.shape[1], C.byref(eq), pr,
+                                  P(rp), P(cp), P(Val), P(R))
+        else:
+            L.orc_construct_struct(m.nNo, e1 - e0, m.eNoN, P(sub), P(x), P(Ag), P(Yg), P(Dg), Ag.shape[1],
+                                   C.byref(eq), prob.quad & 1, P(rp), P(cp), P(Val), P(R))
+
+    with cf.ThreadPoolExecutor(max_workers=nranks) as ex:
+        list(ex.map(lambda k: None, range(nranks)))  # start the workers
+        t0 = time.perf_counter()
+        list(ex.map(rank, range(nranks)))
+        t1 = time.perf_counter()
+    return t1 - t0
 
 
 # ---------------------------------------------------------------- solver

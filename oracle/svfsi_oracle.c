@@ -23,6 +23,18 @@
 /* NN.f                                                                   */
 /* ===================================================================== */
 
+/* Threads for the solver kernels below (SpMV, dots, vector updates).  The
+ * default 1 runs them serially, in the order of the reference's one-rank
+ * loops, and the tests use that.  More threads are used only by the CPU
+ * baseline timing, as a stand-in for the reference's MPI ranks on the
+ * host cores.  With one thread an OpenMP reduction adds its partial sum to
+ * 0.0, so the results are bitwise those of the plain loops. */
+static int orc_nthreads = 1;
+void orc_set_threads(int n) { orc_nthreads = n < 1 ? 1 : n; }
+#define ORC_PAR _Pragma("omp parallel for if(orc_nthreads > 1) num_threads(orc_nthreads) schedule(static)")
+#define ORC_PAR_SUM(v) _Pragma(ORC_STR(omp parallel for if(orc_nthreads > 1) num_threads(orc_nthreads) schedule(static) reduction(+:v)))
+#define ORC_STR(x) #x
+
 /* ISZERO (svFSI/MATFUN.f:763-779, UTIL.f:879-903 with ib absent):
  * true iff |a| / max(|a|, eps) < 10 eps. */
 int orc_iszero(double a)
@@ -766,6 +778,7 @@ void orc_spmv(int nNo, int dof, const int* rowPtr, const int* colPtr,
               const double* K, const double* U, double* KU)
 {
     int dd = dof * dof;
+    ORC_PAR
     for (int i = 0; i < nNo; ++i) {
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
@@ -806,6 +819,7 @@ void orc_spmv_t(int nNo, int dof, const int* rowPtr, const int* colPtr,
 double orc_dot(int n, const double* U, const double* V)
 {
     double s = 0.0;
+    ORC_PAR_SUM(s)
     for (int i = 0; i < n; ++i) s += U[i] * V[i];
     return s;
 }
@@ -831,6 +845,7 @@ void orc_precond_diag(int nNo, int dof, const int* rowPtr, const int* colPtr,
                       double* Val, double* R, double* W)
 {
     int dd = dof * dof;
+    ORC_PAR
     for (int a = 0; a < nNo; ++a)
         for (int i = 0; i < dof; ++i) {
             double d = Val[(size_t)diagPtr[a] * dd + i * dof + i];
@@ -839,12 +854,14 @@ void orc_precond_diag(int nNo, int dof, const int* rowPtr, const int* colPtr,
         }
     if (dirMask)
         for (size_t q = 0; q < (size_t)nNo * dof; ++q) W[q] *= dirMask[q];
+    ORC_PAR
     for (int a = 0; a < nNo; ++a)
         for (int j = rowPtr[a]; j < rowPtr[a + 1]; ++j)
             for (int i = 0; i < dof; ++i)
                 for (int m = 0; m < dof; ++m)
                     Val[(size_t)j * dd + i * dof + m] *= W[(size_t)a * dof + i];
     for (size_t q = 0; q < (size_t)nNo * dof; ++q) R[q] *= W[q];
+    ORC_PAR
     for (int a = 0; a < nNo; ++a)
         for (int j = rowPtr[a]; j < rowPtr[a + 1]; ++j) {
             int c = colPtr[j];
@@ -882,13 +899,17 @@ void orc_cgradv(int nNo, int dof, const int* rowPtr, const int* colPtr,
         errO = err;
         orc_spmv(nNo, dof, rowPtr, colPtr, K, P, KP);
         double alpha = errO / orc_dot((int)n, P, KP);
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) X[q] += alpha * P[q];
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) R[q] += -alpha * KP[q];
         err = orc_norm((int)n, R);
         err = err * err;
         double r = errO / err;
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) P[q] += r * R[q];
         r = err / errO;
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) P[q] *= r;
     }
     memcpy(R, X, sizeof(double) * n);
@@ -932,8 +953,10 @@ void orc_gmresv(int nNo, int dof, const int* rowPtr, const int* colPtr,
         ls->dB = ls->fNorm;
         ls->itr++;
         orc_spmv(nNo, dof, rowPtr, colPtr, K, X, u);
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) u[q] = R[q] - u[q];
         err[0] = orc_norm((int)n, u);
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) u[q] = u[q] / err[0];
         int i;
         for (i = 1; i <= sD; ++i) {
@@ -945,11 +968,13 @@ void orc_gmresv(int nNo, int dof, const int* rowPtr, const int* colPtr,
             for (int j = 1; j <= i; ++j) {
                 double hj = H(j - 1, i - 1);
                 const double* uj = u + n * (j - 1);
+                ORC_PAR
                 for (size_t q = 0; q < n; ++q) un[q] += -hj * uj[q];
                 H(i, i - 1) = H(i, i - 1) - hj * hj;
             }
             H(i, i - 1) = sqrt(fabs(H(i, i - 1)));
             double rr = 1.0 / H(i, i - 1);
+            ORC_PAR
             for (size_t q = 0; q < n; ++q) un[q] *= rr;
             for (int j = 1; j <= i - 1; ++j) {
                 double tmp = c[j - 1] * H(j - 1, i - 1) + s[j - 1] * H(j, i - 1);
@@ -973,6 +998,7 @@ void orc_gmresv(int nNo, int dof, const int* rowPtr, const int* colPtr,
         }
         for (int j = 1; j <= i; ++j) {
             const double* uj = u + n * (j - 1);
+            ORC_PAR
             for (size_t q = 0; q < n; ++q) X[q] += y[j - 1] * uj[q];
         }
         ls->fNorm = fabs(err[i]);

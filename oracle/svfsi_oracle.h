@@ -157,4 +157,7 @@ void orc_lelas_dkde_dot(int nEl, const int* IEN, const double* x,
 #ifdef __cplusplus
 }
 #endif
+/* threads of the solver kernels (CPU baseline timing only; default 1) */
+void orc_set_threads(int n);
+
 #endif
