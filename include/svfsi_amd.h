@@ -123,7 +123,8 @@ int fsils_gpu_bc_free(void* lhs, int faIn);
 /* Replaces fsils_solve_ (FSILS_api_c.h:64; FSILS_SOLVE, svFSILS/SOLVE.f:51-143)
  * with HOST arrays: Ri(dof,nNo) is the RHS on entry and the solution on
  * exit; Val(dof*dof,nnz) is left scaled in place (W K W), as the reference
- * does (SOLVE.f:57, PRECOND.f:124-130).  prec must be PRECOND_FSILS;
+ * does (SOLVE.f:57, PRECOND.f:124-130).  prec PRECOND_FSILS (diagonal
+ * scaling) or PRECOND_RCS (row/column equilibration, PRECOND.f:150-368);
  * LS_type GMRES or CG (dof >= 2).  incL(nFaces) may be NULL (all faces
  * included); res(nFaces) may be NULL. */
 int fsils_gpu_solve(void* lhs, FSILS_lsType* ls, int dof, double* Ri,
@@ -170,6 +171,13 @@ int svfsi_dot_dev(int n, const double* dU, const double* dV, double* out,
  * writes W(dof,nNo) to dW, honouring the Dirichlet faces of lhs. */
 int svfsi_precond_diag_dev(void* lhs, int dof, double* dVal, double* dR,
                            double* dW, const int* incL, void* stream);
+/* PRECONDRCS (svFSILS/PRECOND.f:150-368): scales dVal and dR in place
+ * (Dirichlet rows/columns removed, up to 10 row/column max-norm sweeps)
+ * and writes the row factors W1 and column factors W2 (dof,nNo); the
+ * solution of the scaled system is unscaled with W2 (SOLVE.f:134).  Needs
+ * a structurally symmetric pattern. */
+int svfsi_precond_rcs_dev(void* lhs, int dof, double* dVal, double* dR,
+                          double* dW1, double* dW2, const int* incL, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Sparsity pattern                                                        */

@@ -240,17 +240,68 @@ def gmresv(rowPtr, colPtr, dof, K, R, settings: dict):
     return x, s
 
 
-def fsils_solve(rowPtr, colPtr, dof, mask, settings: dict, R, Val):
-    """FSILS_SOLVE restatement.  Returns (x, scaled Val, SubLs)."""
+def precond_rcs(rowPtr, colPtr, dof, Val, R, mask):
+    """PRECONDRCS.  Returns (scaled Val, scaled R, W1, W2)."""
+    nNo = len(rowPtr) - 1
+    rp, cp = i32(rowPtr), i32(colPtr)
+    dg = np.zeros(nNo, np.int32)
+    assert lib().orc_diagptr(nNo, P(rp), P(cp), P(dg)) == 0
+    V, Rr = f64(Val).copy(), f64(R).copy()
+    W1 = np.zeros((nNo, dof))
+    W2 = np.zeros((nNo, dof))
+    lib().orc_precond_rcs(nNo, dof, P(rp), P(cp), P(dg), None if mask is None else P(f64(mask)), P(V), P(Rr),
+                          P(W1), P(W2))
+    return V, Rr, W1, W2
+
+
+def fsils_solve(rowPtr, colPtr, dof, mask, settings: dict, R, Val, prec: int = 701):
+    """FSILS_SOLVE restatement (prec 701 PRECOND_FSILS, 709 PRECOND_RCS).
+    Returns (x, scaled Val, SubLs)."""
     x = f64(R).copy()
     V = f64(Val).copy()
     s = _ls(settings)
-    rc = lib().orc_fsils_solve(len(rowPtr) - 1, dof, P(i32(rowPtr)), P(i32(colPtr)),
-                               None if mask is None else P(f64(mask)), settings["LS_type"], C.byref(s),
-                               P(x), P(V))
+    rc = lib().orc_fsils_solve_prec(len(rowPtr) - 1, dof, P(i32(rowPtr)), P(i32(colPtr)),
+                                    None if mask is None else P(f64(mask)), settings["LS_type"], int(prec),
+                                    C.byref(s), P(x), P(V))
     if rc != 0:
         raise RuntimeError(f"orc_fsils_solve rc={rc}")
     return x, V, s
+
+
+class Ls(C.Structure):
+    """FSILS_lsType: the outer record RI and the NS solver's inner GM, CG."""
+    _fields_ = [("foC", C.c_int), ("LS_type", C.c_int), ("Resm", C.c_int), ("Resc", C.c_int),
+                ("GM", SubLs), ("CG", SubLs), ("RI", SubLs)]
+
+
+def ns_ls(settings: dict) -> Ls:
+    """NS solver record: RI from settings (relTol, absTol, mItr), GM and CG
+    from settings["GM"] / settings["CG"] (relTol, absTol, mItr, sD), with the
+    reference's NS defaults (LS.f:69-95) for what is missing."""
+    ls = Ls()
+    ls.LS_type = 796
+    ls.RI.relTol = settings.get("relTol", 0.4)
+    ls.RI.absTol = settings.get("absTol", 1e-10)
+    ls.RI.mItr = settings.get("mItr", 10)
+    ls.RI.sD = settings.get("sD") or 100
+    gm = {**dict(relTol=1e-2, absTol=1e-10, mItr=2, sD=ls.RI.sD), **settings.get("GM", {})}
+    cg = {**dict(relTol=0.2, absTol=1e-10, mItr=500), **settings.get("CG", {})}
+    for rec, d in ((ls.GM, gm), (ls.CG, cg)):
+        rec.relTol, rec.absTol, rec.mItr = d["relTol"], d["absTol"], d["mItr"]
+        rec.sD = d.get("sD", 0) or 0
+    return ls
+
+
+def fsils_solve_ns(rowPtr, colPtr, dof, mask, settings: dict, R, Val, prec: int = 701):
+    """FSILS_SOLVE with the NS solver (NSSOLVER).  Returns (x, scaled Val, Ls)."""
+    x = f64(R).copy()
+    V = f64(Val).copy()
+    ls = ns_ls(settings)
+    rc = lib().orc_fsils_solve_ls(len(rowPtr) - 1, dof, P(i32(rowPtr)), P(i32(colPtr)),
+                                  None if mask is None else P(f64(mask)), 796, int(prec), C.byref(ls), P(x), P(V))
+    if rc != 0:
+        raise RuntimeError(f"orc_fsils_solve_ls rc={rc}")
+    return x, V, ls
 
 
 def fluid_dmu_dot(prob, lam, du) -> np.ndarray:

@@ -871,6 +871,87 @@ void orc_precond_diag(int nNo, int dof, const int* rowPtr, const int* colPtr,
         }
 }
 
+/* PRECONDRCS (svFSILS/PRECOND.f:150-368), one rank.  Dirichlet rows and
+ * columns are removed with the mask (PREMUL/POSMUL, 179-197) and their
+ * diagonal entries set to one (200-236).  Then up to maxiter = 10 sweeps of
+ * row/column max-norm scaling (239-352): Wr, Wc = the largest |entry| of
+ * each scalar row / column; the stop test max|1 - Wr| < tol and
+ * max|1 - Wc| < tol with tol = 2 is evaluated before the sweep's own
+ * scaling, which every sweep applies (the loop ends after it);
+ * Val <- diag(1/sqrt Wr) Val diag(1/sqrt Wc), W1 *= 1/sqrt Wr,
+ * W2 *= 1/sqrt Wc.  Finally R <- W1 R (355).  The solution of the scaled
+ * system is unscaled with W2 (SOLVE.f:134). */
+void orc_precond_rcs(int nNo, int dof, const int* rowPtr, const int* colPtr, const int* diagPtr,
+                     const double* dirMask, double* Val, double* R, double* W1, double* W2)
+{
+    const int dd = dof * dof;
+    const size_t n = (size_t)nNo * dof;
+    double* Wr = malloc(sizeof(double) * n);
+    double* Wc = malloc(sizeof(double) * n);
+    for (size_t q = 0; q < n; ++q) {
+        W1[q] = 1.0;
+        W2[q] = 1.0;
+        /* the face-value product renormalised to 0 / 1 (PRECOND.f:191-193) */
+        Wr[q] = dirMask ? (dirMask[q] > 0.5 ? 1.0 : 0.0) : 1.0;
+    }
+    /* kill the Dirichlet rows and columns (PREMUL, then POSMUL) */
+    for (int a = 0; a < nNo; ++a)
+        for (int k = rowPtr[a]; k < rowPtr[a + 1]; ++k)
+            for (int i = 0; i < dof; ++i)
+                for (int j = 0; j < dof; ++j) Val[(size_t)k * dd + i * dof + j] *= Wr[(size_t)a * dof + i];
+    for (size_t q = 0; q < n; ++q) R[q] = Wr[q] * R[q];
+    for (int a = 0; a < nNo; ++a)
+        for (int k = rowPtr[a]; k < rowPtr[a + 1]; ++k)
+            for (int i = 0; i < dof; ++i)
+                for (int j = 0; j < dof; ++j)
+                    Val[(size_t)k * dd + i * dof + j] *= Wr[(size_t)colPtr[k] * dof + j];
+    for (int a = 0; a < nNo; ++a)
+        for (int i = 0; i < dof; ++i) {
+            double* v = Val + (size_t)diagPtr[a] * dd + i * dof + i;
+            *v = Wr[(size_t)a * dof + i] * (*v - 1.0) + 1.0;
+        }
+    int iter = 0, flag = 1;
+    while (flag) {
+        for (size_t q = 0; q < n; ++q) Wr[q] = Wc[q] = 0.0;
+        ++iter;
+        if (iter >= 10) flag = 0;
+        for (int a = 0; a < nNo; ++a)
+            for (int k = rowPtr[a]; k < rowPtr[a + 1]; ++k)
+                for (int i = 0; i < dof; ++i)
+                    for (int j = 0; j < dof; ++j) {
+                        const double v = fabs(Val[(size_t)k * dd + i * dof + j]);
+                        if (v > Wr[(size_t)a * dof + i]) Wr[(size_t)a * dof + i] = v;
+                        if (v > Wc[(size_t)colPtr[k] * dof + j]) Wc[(size_t)colPtr[k] * dof + j] = v;
+                    }
+        double mr = 0.0, mc = 0.0;
+        for (size_t q = 0; q < n; ++q) {
+            mr = fmax(mr, fabs(1.0 - Wr[q]));
+            mc = fmax(mc, fabs(1.0 - Wc[q]));
+        }
+        if (mr < 2.0 && mc < 2.0) flag = 0;
+        for (size_t q = 0; q < n; ++q) {
+            Wr[q] = 1.0 / sqrt(Wr[q]);
+            Wc[q] = 1.0 / sqrt(Wc[q]);
+        }
+        for (int a = 0; a < nNo; ++a)
+            for (int k = rowPtr[a]; k < rowPtr[a + 1]; ++k)
+                for (int i = 0; i < dof; ++i)
+                    for (int j = 0; j < dof; ++j) Val[(size_t)k * dd + i * dof + j] *= Wr[(size_t)a * dof + i];
+        for (int a = 0; a < nNo; ++a)
+            for (int k = rowPtr[a]; k < rowPtr[a + 1]; ++k)
+                for (int i = 0; i < dof; ++i)
+                    for (int j = 0; j < dof; ++j)
+                        Val[(size_t)k * dd + i * dof + j] *= Wc[(size_t)colPtr[k] * dof + j];
+        for (size_t q = 0; q < n; ++q) {
+            W1[q] = W1[q] * Wr[q];
+            W2[q] = W2[q] * Wc[q];
+        }
+    }
+    for (size_t q = 0; q < n; ++q) R[q] = W1[q] * R[q];
+    free(Wr);
+    free(Wc);
+}
+
 static double wall_s(void)
 {
     struct timespec ts;
@@ -1012,23 +1093,475 @@ done:
     free(u); free(X); free(h); free(y); free(c); free(s); free(err);
 }
 
+/* ===================================================================== */
+/* NSSOLVER (svFSILS/NSSOLVER.f:52-383) and its parts                     */
+/* ===================================================================== */
+
+/* GMRES (svFSILS/GMRES.f:51-166): restarted GMRES(sD) for K X = R from
+ * X = 0.  Unlike GMRESV the first cycle starts from R itself (no product),
+ * R is left unchanged, and ls->itr accumulates over calls. */
+static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* K,
+                        const double* R, double* X, orc_subls* ls)
+{
+    size_t n = (size_t)nNo * dof;
+    int sD = ls->sD;
+    double* u = malloc(sizeof(double) * n * (sD + 1));
+    double* h = calloc((size_t)(sD + 1) * sD, sizeof(double));
+    double* y = malloc(sizeof(double) * sD);
+    double* c = malloc(sizeof(double) * sD);
+    double* s = malloc(sizeof(double) * sD);
+    double* err = malloc(sizeof(double) * (sD + 1));
+#define H(j, i) h[(size_t)(i) * (sD + 1) + (j)]
+    double time = wall_s();
+    ls->suc = 0;
+    double eps = 0.0;
+    memset(X, 0, sizeof(double) * n);
+    for (int l = 1; l <= ls->mItr; ++l) {
+        if (l == 1) {
+            memcpy(u, R, sizeof(double) * n);
+        } else {
+            ls->itr++;
+            orc_spmv(nNo, dof, rowPtr, colPtr, K, X, u);
+            for (size_t q = 0; q < n; ++q) u[q] = R[q] - u[q];
+        }
+        err[0] = orc_norm((int)n, u);
+        if (l == 1) {
+            eps = err[0];
+            if (eps <= ls->absTol) {
+                ls->callD = EPS;
+                ls->dB = 0.0;
+                goto done;
+            }
+            ls->iNorm = eps;
+            ls->fNorm = eps;
+            eps = fmax(ls->absTol, ls->relTol * eps);
+        }
+        ls->dB = ls->fNorm;
+        for (size_t q = 0; q < n; ++q) u[q] = u[q] / err[0];
+        int i;
+        for (i = 1; i <= sD; ++i) {
+            ls->itr++;
+            double* ui = u + n * (i - 1);
+            double* un = u + n * i;
+            orc_spmv(nNo, dof, rowPtr, colPtr, K, ui, un);
+            for (int j = 1; j <= i + 1; ++j) H(j - 1, i - 1) = orc_dot((int)n, u + n * (j - 1), un);
+            for (int j = 1; j <= i; ++j) {
+                double hj = H(j - 1, i - 1);
+                const double* uj = u + n * (j - 1);
+                for (size_t q = 0; q < n; ++q) un[q] += -hj * uj[q];
+                H(i, i - 1) = H(i, i - 1) - hj * hj;
+            }
+            H(i, i - 1) = sqrt(fabs(H(i, i - 1)));
+            double rr = 1.0 / H(i, i - 1);
+            for (size_t q = 0; q < n; ++q) un[q] *= rr;
+            for (int j = 1; j <= i - 1; ++j) {
+                double tmp = c[j - 1] * H(j - 1, i - 1) + s[j - 1] * H(j, i - 1);
+                H(j, i - 1) = -s[j - 1] * H(j - 1, i - 1) + c[j - 1] * H(j, i - 1);
+                H(j - 1, i - 1) = tmp;
+            }
+            double tmp = sqrt(H(i - 1, i - 1) * H(i - 1, i - 1) + H(i, i - 1) * H(i, i - 1));
+            c[i - 1] = H(i - 1, i - 1) / tmp;
+            s[i - 1] = H(i, i - 1) / tmp;
+            H(i - 1, i - 1) = tmp;
+            H(i, i - 1) = 0.0;
+            err[i] = -s[i - 1] * err[i - 1];
+            err[i - 1] = c[i - 1] * err[i - 1];
+            if (fabs(err[i]) < eps) { ls->suc = 1; break; }
+        }
+        if (i > sD) i = sD;
+        for (int j = 0; j < i; ++j) y[j] = err[j];
+        for (int j = i; j >= 1; --j) {
+            for (int k = j + 1; k <= i; ++k) y[j - 1] = y[j - 1] - H(j - 1, k - 1) * y[k - 1];
+            y[j - 1] = y[j - 1] / H(j - 1, j - 1);
+        }
+        for (int j = 1; j <= i; ++j) {
+            const double* uj = u + n * (j - 1);
+            for (size_t q = 0; q < n; ++q) X[q] = X[q] + y[j - 1] * uj[q];
+        }
+        ls->fNorm = fabs(err[i]);
+        if (ls->suc) break;
+    }
+    ls->callD = wall_s() - time + ls->callD;
+    ls->dB = 10.0 * log(ls->fNorm / ls->dB);
+done:
+#undef H
+    free(u); free(h); free(y); free(c); free(s); free(err);
+}
+
+/* FSILS_SPARMULVS / SV / SS (svFSILS/SPARMUL.f:135-297), one rank:
+ * vector-to-scalar (K(nsd,nnz), U(nsd,nNo) -> KU(nNo)), scalar-to-vector
+ * and scalar-to-scalar block products, in the reference's summation
+ * order. */
+static void orc_spmv_vs(int nNo, int nsd, const int* rowPtr, const int* colPtr, const double* K,
+                        const double* U, double* KU)
+{
+    for (int i = 0; i < nNo; ++i) {
+        double a = 0.0;
+        for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
+            const double* k = K + (size_t)j * nsd;
+            const double* u = U + (size_t)colPtr[j] * nsd;
+            double t = a + k[0] * u[0];
+            for (int m = 1; m < nsd; ++m) t = t + k[m] * u[m];
+            a = t;
+        }
+        KU[i] = a;
+    }
+}
+
+static void orc_spmv_sv(int nNo, int nsd, const int* rowPtr, const int* colPtr, const double* K,
+                        const double* U, double* KU)
+{
+    for (int i = 0; i < nNo; ++i) {
+        double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j)
+            for (int m = 0; m < nsd; ++m) a[m] = a[m] + K[(size_t)j * nsd + m] * U[colPtr[j]];
+        for (int m = 0; m < nsd; ++m) KU[(size_t)i * nsd + m] = a[m];
+    }
+}
+
+static void orc_spmv_ss(int nNo, const int* rowPtr, const int* colPtr, const double* K, const double* U,
+                        double* KU)
+{
+    for (int i = 0; i < nNo; ++i) {
+        double a = 0.0;
+        for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) a = a + K[j] * U[colPtr[j]];
+        KU[i] = a;
+    }
+}
+
+/* CGRAD_SCHUR (svFSILS/CGRAD.f:51-123): CG on the Schur complement
+ * S P = L P - D (G P) (D = Gt = -G^T blockwise, so S = L + G^T G); R in:
+ * right-hand side, out: solution; ls->itr accumulates. */
+static void orc_cgrad_schur(int nNo, int nsd, const int* rowPtr, const int* colPtr, const double* D,
+                            const double* G, const double* L, double* R, orc_subls* ls)
+{
+    double* X = calloc(nNo, sizeof(double));
+    double* P = malloc(sizeof(double) * nNo);
+    double* SP = malloc(sizeof(double) * nNo);
+    double* DGP = malloc(sizeof(double) * nNo);
+    double* GP = malloc(sizeof(double) * (size_t)nNo * nsd);
+    double time = wall_s();
+    ls->suc = 0;
+    ls->iNorm = orc_norm(nNo, R);
+    double eps = pow(fmax(ls->absTol, ls->relTol * ls->iNorm), 2.0);
+    double errO = ls->iNorm * ls->iNorm, err = errO;
+    memcpy(P, R, sizeof(double) * nNo);
+    int i;
+    for (i = 1; i <= ls->mItr; ++i) {
+        if (err < eps) { ls->suc = 1; break; }
+        errO = err;
+        orc_spmv_sv(nNo, nsd, rowPtr, colPtr, G, P, GP);
+        orc_spmv_vs(nNo, nsd, rowPtr, colPtr, D, GP, DGP);
+        orc_spmv_ss(nNo, rowPtr, colPtr, L, P, SP);
+        for (int q = 0; q < nNo; ++q) SP[q] = SP[q] + (-1.0) * DGP[q];
+        double alpha = errO / orc_dot(nNo, P, SP);
+        for (int q = 0; q < nNo; ++q) X[q] = X[q] + alpha * P[q];
+        for (int q = 0; q < nNo; ++q) R[q] = R[q] + (-alpha) * SP[q];
+        err = orc_norm(nNo, R);
+        err = err * err;
+        for (int q = 0; q < nNo; ++q) P[q] = P[q] + (errO / err) * R[q];
+        for (int q = 0; q < nNo; ++q) P[q] = P[q] * (err / errO);
+    }
+    memcpy(R, X, sizeof(double) * nNo);
+    ls->fNorm = sqrt(err);
+    ls->callD = wall_s() - time + ls->callD;
+    ls->itr = ls->itr + i - 1;
+    ls->dB = (errO < EPS) ? 0.0 : 5.0 * log(err / errO);
+    free(X); free(P); free(SP); free(DGP); free(GP);
+}
+
+/* GE (svFSILS/GE.f:51-151): A x = B for the leading N x N block of A
+ * (column-major, leading dimension nV) after symmetric diagonal scaling;
+ * partial pivoting; closed forms for N = 1, 2.  B is overwritten with x;
+ * returns 0 (and B = 0) for a singular system. */
+static int orc_ge(int nV, int N, const double* A, double* B)
+{
+#define AA(i, j) A[(size_t)(j) * nV + (i)]
+    if (N <= 0) return 0;
+    double* W = malloc(sizeof(double) * N);
+    for (int i = 0; i < N; ++i) {
+        if (fabs(AA(i, i)) < DBL_MIN) {
+            for (int q = 0; q < N; ++q) B[q] = 0.0;
+            free(W);
+            return 0;
+        }
+        W[i] = 1.0 / sqrt(fabs(AA(i, i)));
+    }
+    double* C = malloc(sizeof(double) * N * (N + 1)); /* C(i,j) at i*(N+1)+j */
+#define CC(i, j) C[(size_t)(i) * (N + 1) + (j)]
+    for (int i = 0; i < N; ++i) {
+        for (int j = 0; j < N; ++j) CC(i, j) = W[i] * W[j] * AA(i, j);
+        CC(i, N) = W[i] * B[i];
+    }
+    int ok = 1;
+    if (N == 1) {
+        B[0] = CC(0, 1) / CC(0, 0);
+        B[0] = B[0] * W[0];
+    } else if (N == 2) {
+        double pivot = CC(0, 0) * CC(1, 1) - CC(1, 0) * CC(0, 1);
+        if (fabs(pivot) < DBL_EPSILON) {
+            B[0] = B[1] = 0.0;
+            ok = 0;
+        } else {
+            B[0] = (CC(0, 2) * CC(1, 1) - CC(1, 2) * CC(0, 1)) / pivot;
+            B[1] = (CC(1, 2) * CC(0, 0) - CC(0, 2) * CC(1, 0)) / pivot;
+            B[0] = W[0] * B[0];
+            B[1] = W[1] * B[1];
+        }
+    } else {
+        for (int m = 0; m < N - 1 && ok; ++m) {
+            int ipv = m;
+            double pivot = fabs(CC(m, m));
+            for (int i = m + 1; i < N; ++i)
+                if (fabs(CC(i, m)) > pivot) { ipv = i; pivot = fabs(CC(i, m)); }
+            if (fabs(pivot) < DBL_EPSILON) {
+                for (int q = 0; q < N; ++q) B[q] = 0.0;
+                ok = 0;
+                break;
+            }
+            if (ipv != m)
+                for (int j = m; j <= N; ++j) {
+                    double t = CC(m, j);
+                    CC(m, j) = CC(ipv, j);
+                    CC(ipv, j) = t;
+                }
+            for (int i = m + 1; i < N; ++i) {
+                double f = CC(i, m) / CC(m, m);
+                CC(i, m) = 0.0;
+                for (int j = m + 1; j <= N; ++j) CC(i, j) = CC(i, j) - f * CC(m, j);
+            }
+        }
+        if (ok) {
+            for (int j = N - 1; j >= 0; --j) {
+                for (int i = j + 1; i < N; ++i) CC(j, N) = CC(j, N) - CC(j, i) * CC(i, N);
+                CC(j, N) = CC(j, N) / CC(j, j);
+            }
+            for (int i = 0; i < N; ++i) B[i] = W[i] * CC(i, N);
+        }
+    }
+#undef CC
+#undef AA
+    free(C);
+    free(W);
+    return ok;
+}
+
+/* GE exported for its test */
+int orc_ge_test(int nV, int N, const double* A, double* B) { return orc_ge(nV, N, A, B); }
+
+/* NSSOLVER (svFSILS/NSSOLVER.f:52-383), one rank, nsd = dof - 1 = 3,
+ * no coupled faces.  DEPART (236-303) splits each 4x4 block into the
+ * momentum block mK (3x3), the gradient column mG, the divergence row mD
+ * and mL, and Gt(:,trans(k)) = -mG(:,k).  Each outer iteration i adds two
+ * directions: [0; P_i] with P_i = S^-1 (Rc - D K^-1 Rm) (CGRAD_SCHUR) and
+ * [U_i; 0] with U_i = K^-1 (Rm - G P_i) (GMRES), their images MU/MP under
+ * the full matrix, the Gram system A xB = B over all directions so far
+ * (GE), and the residual.  ls holds RI, GM and CG in FSILS_lsType order. */
+int orc_nssolver(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* Val, double* Ri,
+                 orc_ls* ls)
+{
+    if (dof != 4) return -3;
+    const int nsd = 3;
+    const size_t n = (size_t)nNo, nnz = (size_t)rowPtr[nNo];
+    const int iBmax = ls->RI.mItr, nB = 2 * iBmax;
+    double* mK = malloc(sizeof(double) * nnz * 9);
+    double* mG = malloc(sizeof(double) * nnz * 3);
+    double* mD = malloc(sizeof(double) * nnz * 3);
+    double* mL = malloc(sizeof(double) * nnz);
+    double* Gt = malloc(sizeof(double) * nnz * 3);
+    for (size_t k = 0; k < nnz; ++k) {
+        const double* t = Val + k * 16;
+        const int kk[9] = {0, 1, 2, 4, 5, 6, 8, 9, 10};
+        for (int m = 0; m < 9; ++m) mK[k * 9 + m] = t[kk[m]];
+        mG[k * 3 + 0] = t[3];
+        mG[k * 3 + 1] = t[7];
+        mG[k * 3 + 2] = t[11];
+        mD[k * 3 + 0] = t[12];
+        mD[k * 3 + 1] = t[13];
+        mD[k * 3 + 2] = t[14];
+        mL[k] = t[15];
+    }
+    for (int i = 0; i < nNo; ++i)
+        for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
+            const int k = colPtr[j];
+            for (int l = rowPtr[k]; l < rowPtr[k + 1]; ++l)
+                if (colPtr[l] == i) {
+                    for (int m = 0; m < 3; ++m) Gt[(size_t)l * 3 + m] = -mG[(size_t)j * 3 + m];
+                    break;
+                }
+        }
+    double* Rm = malloc(sizeof(double) * n * nsd);
+    double* Rc = malloc(sizeof(double) * n);
+    double* Rmi = malloc(sizeof(double) * n * nsd);
+    double* Rci = malloc(sizeof(double) * n);
+    double* U = malloc(sizeof(double) * n * nsd * iBmax);
+    double* P = malloc(sizeof(double) * n * iBmax);
+    double* MU = malloc(sizeof(double) * n * nsd * nB);
+    double* MP = malloc(sizeof(double) * n * nB);
+    double* A = calloc((size_t)nB * nB, sizeof(double));
+    double* B = calloc(nB, sizeof(double));
+    double* xB = calloc(nB, sizeof(double));
+    double* oldxB = calloc(nB, sizeof(double));
+    for (size_t a = 0; a < n; ++a) {
+        for (int m = 0; m < nsd; ++m) Rmi[a * nsd + m] = Ri[a * dof + m];
+        Rci[a] = Ri[a * dof + nsd];
+    }
+    memcpy(Rm, Rmi, sizeof(double) * n * nsd);
+    memcpy(Rc, Rci, sizeof(double) * n);
+    double eps = sqrt(pow(orc_norm((int)(n * nsd), Rm), 2.0) + pow(orc_norm((int)n, Rc), 2.0));
+    ls->RI.iNorm = eps;
+    ls->RI.fNorm = eps * eps;
+    ls->CG.callD = 0.0;
+    ls->GM.callD = 0.0;
+    ls->CG.itr = 0;
+    ls->GM.itr = 0;
+    ls->RI.callD = wall_s();
+    ls->RI.suc = 0;
+    eps = fmax(ls->RI.absTol, ls->RI.relTol * eps);
+#define VU(i) (U + (size_t)((i) - 1) * n * nsd)
+#define VP(i) (P + (size_t)((i) - 1) * n)
+#define VMU(i) (MU + (size_t)((i) - 1) * n * nsd)
+#define VMP(i) (MP + (size_t)((i) - 1) * n)
+#define AM(j, k) A[(size_t)((k) - 1) * nB + ((j) - 1)]
+    int i, iB = 0, iBB = 0;
+    for (i = 1; i <= ls->RI.mItr; ++i) {
+        iB = 2 * i - 1;
+        iBB = 2 * i;
+        ls->RI.dB = ls->RI.fNorm;
+        orc_gmres_x(nNo, nsd, rowPtr, colPtr, mK, Rm, VU(i), &ls->GM);
+        orc_spmv_vs(nNo, nsd, rowPtr, colPtr, mD, VU(i), VP(i));
+        for (size_t q = 0; q < n; ++q) VP(i)[q] = Rc[q] - VP(i)[q];
+        orc_cgrad_schur(nNo, nsd, rowPtr, colPtr, Gt, mG, mL, VP(i), &ls->CG);
+        orc_spmv_sv(nNo, nsd, rowPtr, colPtr, mG, VP(i), VMU(iB));
+        for (size_t q = 0; q < n * nsd; ++q) VMU(iBB)[q] = Rm[q] - VMU(iB)[q];
+        orc_gmres_x(nNo, nsd, rowPtr, colPtr, mK, VMU(iBB), VU(i), &ls->GM);
+        orc_spmv(nNo, nsd, rowPtr, colPtr, mK, VU(i), VMU(iBB));
+        orc_spmv_ss(nNo, rowPtr, colPtr, mL, VP(i), VMP(iB));
+        orc_spmv_vs(nNo, nsd, rowPtr, colPtr, mD, VU(i), VMP(iBB));
+        for (int k = iB; k <= iBB; ++k) {
+            for (int j = 1; j <= k; ++j) {
+                const double t = orc_dot((int)(n * nsd), VMU(j), VMU(k)) + orc_dot((int)n, VMP(j), VMP(k));
+                AM(j, k) = t;
+                AM(k, j) = t;
+            }
+            B[k - 1] = orc_dot((int)(n * nsd), VMU(k), Rmi) + orc_dot((int)n, VMP(k), Rci);
+        }
+        memcpy(xB, B, sizeof(double) * nB);
+        if (orc_ge(nB, iBB, A, xB)) {
+            memcpy(oldxB, xB, sizeof(double) * nB);
+        } else {
+            memcpy(xB, oldxB, sizeof(double) * nB);
+            if (i > 1) {
+                iB -= 2;
+                iBB -= 2;
+            }
+            break;
+        }
+        double sxb = 0.0;
+        for (int k = 0; k < iBB; ++k) sxb += xB[k] * B[k];
+        ls->RI.fNorm = ls->RI.iNorm * ls->RI.iNorm - sxb;
+        if (ls->RI.fNorm < eps * eps) {
+            ls->RI.suc = 1;
+            break;
+        }
+        for (size_t q = 0; q < n * nsd; ++q) Rm[q] = Rmi[q] - xB[0] * VMU(1)[q];
+        for (size_t q = 0; q < n; ++q) Rc[q] = Rci[q] - xB[0] * VMP(1)[q];
+        for (int j = 2; j <= iBB; ++j) {
+            for (size_t q = 0; q < n * nsd; ++q) Rm[q] = Rm[q] - xB[j - 1] * VMU(j)[q];
+            for (size_t q = 0; q < n; ++q) Rc[q] = Rc[q] - xB[j - 1] * VMP(j)[q];
+        }
+    }
+    if (i > ls->RI.mItr) {
+        ls->RI.itr = ls->RI.mItr;
+    } else {
+        ls->RI.itr = i;
+        for (size_t q = 0; q < n; ++q) Rc[q] = Rci[q] - xB[0] * VMP(1)[q];
+        for (int j = 2; j <= iBB; ++j)
+            for (size_t q = 0; q < n; ++q) Rc[q] = Rc[q] - xB[j - 1] * VMP(j)[q];
+    }
+    ls->Resc = (int)lround(100.0 * pow(orc_norm((int)n, Rc), 2.0) / ls->RI.fNorm);
+    ls->Resm = 100 - ls->Resc;
+    for (size_t q = 0; q < n * nsd; ++q) Rmi[q] = xB[1] * VU(1)[q];
+    for (size_t q = 0; q < n; ++q) Rci[q] = xB[0] * VP(1)[q];
+    for (int k = 2; k <= ls->RI.itr; ++k) {
+        const int b1 = 2 * k - 1, b2 = 2 * k;
+        for (size_t q = 0; q < n * nsd; ++q) Rmi[q] = Rmi[q] + xB[b2 - 1] * VU(k)[q];
+        for (size_t q = 0; q < n; ++q) Rci[q] = Rci[q] + xB[b1 - 1] * VP(k)[q];
+    }
+    ls->RI.callD = wall_s() - ls->RI.callD;
+    ls->RI.dB = 5.0 * log(ls->RI.fNorm / ls->RI.dB);
+    if (ls->Resc < 0 || ls->Resm < 0) {
+        ls->Resc = 0;
+        ls->Resm = 0;
+        ls->RI.dB = 0.0;
+        ls->RI.fNorm = 0.0;
+    }
+    ls->RI.fNorm = sqrt(ls->RI.fNorm);
+    for (size_t a = 0; a < n; ++a) {
+        for (int m = 0; m < nsd; ++m) Ri[a * dof + m] = Rmi[a * nsd + m];
+        Ri[a * dof + nsd] = Rci[a];
+    }
+#undef VU
+#undef VP
+#undef VMU
+#undef VMP
+#undef AM
+    free(mK); free(mG); free(mD); free(mL); free(Gt);
+    free(Rm); free(Rc); free(Rmi); free(Rci); free(U); free(P); free(MU); free(MP);
+    free(A); free(B); free(xB); free(oldxB);
+    return 0;
+}
+
+/* FSILS_SOLVE with the full solver record (RI, GM, CG): LS_type 796 (NS),
+ * 797 (GMRES) or 798 (CG); prec 701 or 709. */
+int orc_fsils_solve_ls(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* dirMask,
+                       int LS_type, int prec, orc_ls* ls, double* Ri, double* Val)
+{
+    if (LS_type != 796)
+        return orc_fsils_solve_prec(nNo, dof, rowPtr, colPtr, dirMask, LS_type, prec, &ls->RI, Ri, Val);
+    int* diagPtr = malloc(sizeof(int) * nNo);
+    if (orc_diagptr(nNo, rowPtr, colPtr, diagPtr) != 0) { free(diagPtr); return -1; }
+    size_t n = (size_t)nNo * dof;
+    double* W = malloc(sizeof(double) * n);
+    double* W1 = malloc(sizeof(double) * n);
+    if (prec == 709)
+        orc_precond_rcs(nNo, dof, rowPtr, colPtr, diagPtr, dirMask, Val, Ri, W1, W);
+    else
+        orc_precond_diag(nNo, dof, rowPtr, colPtr, diagPtr, dirMask, Val, Ri, W);
+    int rc = orc_nssolver(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
+    for (size_t q = 0; q < n; ++q) Ri[q] = W[q] * Ri[q];
+    free(W); free(W1); free(diagPtr);
+    return rc;
+}
+
 /* FSILS_SOLVE (svFSILS/SOLVE.f:51-143), one rank, PRECOND_FSILS, no
  * coupled Neumann faces: scaling, CG or GMRES, un-scaling x = W y. */
-int orc_fsils_solve(int nNo, int dof, const int* rowPtr, const int* colPtr,
-                    const double* dirMask, int LS_type, orc_subls* ls,
-                    double* Ri, double* Val)
+int orc_fsils_solve_prec(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                         const double* dirMask, int LS_type, int prec, orc_subls* ls,
+                         double* Ri, double* Val)
 {
     int* diagPtr = malloc(sizeof(int) * nNo);
     if (orc_diagptr(nNo, rowPtr, colPtr, diagPtr) != 0) { free(diagPtr); return -1; }
     size_t n = (size_t)nNo * dof;
     double* W = malloc(sizeof(double) * n);
-    orc_precond_diag(nNo, dof, rowPtr, colPtr, diagPtr, dirMask, Val, Ri, W);
+    double* W1 = malloc(sizeof(double) * n);
+    if (prec == 709) /* PRECOND_RCS: unscale with W2 */
+        orc_precond_rcs(nNo, dof, rowPtr, colPtr, diagPtr, dirMask, Val, Ri, W1, W);
+    else
+        orc_precond_diag(nNo, dof, rowPtr, colPtr, diagPtr, dirMask, Val, Ri, W);
     if (LS_type == 797) orc_gmresv(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
     else if (LS_type == 798) orc_cgradv(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
-    else { free(W); free(diagPtr); return -2; }
+    else { free(W); free(W1); free(diagPtr); return -2; }
     for (size_t q = 0; q < n; ++q) Ri[q] = W[q] * Ri[q];
-    free(W); free(diagPtr);
+    free(W); free(W1); free(diagPtr);
     return 0;
+}
+
+int orc_fsils_solve(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                    const double* dirMask, int LS_type, orc_subls* ls,
+                    double* Ri, double* Val)
+{
+    return orc_fsils_solve_prec(nNo, dof, rowPtr, colPtr, dirMask, LS_type, 701, ls, Ri, Val);
 }
 
 /* ===================================================================== */

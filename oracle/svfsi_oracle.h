@@ -51,6 +51,13 @@ typedef struct {
     double absTol, relTol, iNorm, fNorm, dB, callD;
 } orc_subls;
 
+/* FSILS_lsType (FSILS_STRUCT.h:182-204): the NS solver's outer (RI) and
+ * inner (GM, CG) records */
+typedef struct {
+    int foC, LS_type, Resm, Resc;
+    orc_subls GM, CG, RI;
+} orc_ls;
+
 /* ---------------- NN.f: quadrature, shape functions, GNN ---------------- */
 void orc_rule_tet4(double w[4], double xi[4][3]);
 void orc_rule_tet10(double w[15], double xi[15][3]);
@@ -154,10 +161,28 @@ void orc_lelas_dkde_dot(int nEl, const int* IEN, const double* x,
                         const orc_eq* eq, const double* E_el,
                         const double* lam, const double* u, double* g);
 
+/* NSSOLVER (svFSILS/NSSOLVER.f:52-383) on the scaled system, dof 4; and
+ * FSILS_SOLVE with the full record (LS_type 796 NS, 797 GMRES, 798 CG) */
+int orc_nssolver(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* Val, double* Ri,
+                 orc_ls* ls);
+int orc_fsils_solve_ls(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* dirMask,
+                       int LS_type, int prec, orc_ls* ls, double* Ri, double* Val);
+/* GE (svFSILS/GE.f:51-151), A column-major with leading dimension nV */
+int orc_ge_test(int nV, int N, const double* A, double* B);
+
+/* threads of the solver kernels (CPU baseline timing only; default 1) */
+void orc_set_threads(int n);
+
+/* PRECONDRCS (svFSILS/PRECOND.f:150-368) and FSILS_SOLVE with either
+ * preconditioner (701 PRECOND_FSILS, 709 PRECOND_RCS) */
+void orc_precond_rcs(int nNo, int dof, const int* rowPtr, const int* colPtr, const int* diagPtr,
+                     const double* dirMask, double* Val, double* R, double* W1, double* W2);
+int orc_fsils_solve_prec(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                         const double* dirMask, int LS_type, int prec, orc_subls* ls,
+                         double* Ri, double* Val);
+
 #ifdef __cplusplus
 }
 #endif
-/* threads of the solver kernels (CPU baseline timing only; default 1) */
-void orc_set_threads(int n);
 
 #endif

@@ -93,6 +93,7 @@ SIGNATURES = {
     "svfsi_spmv_t_dev": [_P, _I, _P, _P, _P, _P],
     "svfsi_dot_dev": [_I, _P, _P, _DP, _P],
     "svfsi_precond_diag_dev": [_P, _I, _P, _P, _P, _IP, _P],
+    "svfsi_precond_rcs_dev": [_P, _I, _P, _P, _P, _P, _IP, _P],
     "svfsi_lhsa": [_I, _I, _I, _IP, _IP, _IP, _IP],
     "svfsi_mesh_create": [C.POINTER(_P), _P, _I, _I, _I, _IP, _DP],
     "svfsi_mesh_free": [_P],

@@ -228,8 +228,9 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
 {
     SV_ARG(ls, "ls is NULL");
     SV_ARG(dof >= 2 && dof <= 4, "dof %d not supported (2..4)", dof);
-    if (prec != PRECOND_FSILS) {
-        set_error("preconditioner %d not supported (only PRECOND_FSILS=%d)", prec, PRECOND_FSILS);
+    if (prec != PRECOND_FSILS && prec != PRECOND_RCS) {
+        set_error("preconditioner %d not supported (PRECOND_FSILS=%d, PRECOND_RCS=%d)", prec, PRECOND_FSILS,
+                  PRECOND_RCS);
         return SVFSI_ERR_NOTSUP;
     }
     for (size_t f = 0; f < L.faces.size(); ++f) {
@@ -252,7 +253,14 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
         SV_TRY(w.val32.ensure((size_t)L.nnz * dof * dof * sizeof(float)));
         v32 = w.val32.as<float>();
     }
-    SV_TRY(precond_diag(L, dof, Val, R, w.W.as<double>(), v32, incL, s));
+    // w.W ends up holding the factor that unscales the solution (SOLVE.f:134):
+    // W of PRECONDDIAG, or W2 (the column factor) of PRECONDRCS
+    if (prec == PRECOND_RCS) {
+        SV_TRY(w.W1.ensure(n * sizeof(double)));
+        SV_TRY(precond_rcs(L, dof, Val, R, w.W1.as<double>(), w.W.as<double>(), v32, incL, s));
+    } else {
+        SV_TRY(precond_diag(L, dof, Val, R, w.W.as<double>(), v32, incL, s));
+    }
     int rc;
     if (mode == 1) rc = mixed_solve(L, ls->RI, dof, Val, v32, R, s);
     else if (ls->LS_type == LS_TYPE_GMRES) rc = gmres_solve(L, ls->RI, dof, Val, R, s, false);
@@ -545,6 +553,18 @@ int svfsi_precond_diag_dev(void* lhs, int dof, double* dVal, double* dR, double*
     SV_TRY(require_device());
     hipStream_t s = static_cast<hipStream_t>(stream);
     SV_TRY(precond_diag(*static_cast<Lhs*>(lhs), dof, dVal, dR, dW, nullptr, incL, s));
+    SV_HIP(hipStreamSynchronize(s));
+    return SVFSI_OK;
+}
+
+int svfsi_precond_rcs_dev(void* lhs, int dof, double* dVal, double* dR, double* dW1, double* dW2,
+                          const int* incL, void* stream)
+{
+    SV_ARG(lhs && dVal && dR && dW1 && dW2, "NULL argument");
+    SV_ARG(dof >= 2 && dof <= 4, "dof %d not supported (2..4)", dof);
+    SV_TRY(require_device());
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    SV_TRY(precond_rcs(*static_cast<Lhs*>(lhs), dof, dVal, dR, dW1, dW2, nullptr, incL, s));
     SV_HIP(hipStreamSynchronize(s));
     return SVFSI_OK;
 }

@@ -116,6 +116,7 @@ struct Work {
     DBuf partial, state, hess;
     DBuf zdir, opartial, ostate, ohess; // flexible GMRES: FP32 directions, outer state
     DBuf mask;
+    DBuf rcs, W1;  // PRECOND_RCS: row/column factors, partial maxima; W1 (row scaling)
     DBuf valT; // K^T values for transposed solves
     DBuf dinv32; // inverse diagonal blocks (FP32) of the inner block-Jacobi
     DBuf polyCoef, polyVec; // polynomial preconditioner: factors, work vectors
@@ -237,6 +238,10 @@ int launch_transpose_blocks(const Lhs& L, int dof, const double* V, double* VT, 
 int launch_dot(int n, const double* U, const double* V, double* partial,
                double* result, hipStream_t s);
 int build_mask(Lhs& L, int dof, const int* incL, hipStream_t s);
+// PRECONDRCS: Val, R scaled in place; W1 row, W2 column factors (the
+// solution of the scaled system is unscaled with W2); optional FP32 copy.
+int precond_rcs(Lhs& L, int dof, double* Val, double* R, double* W1, double* W2, float* Val32, const int* incL,
+                hipStream_t s);
 int precond_diag(Lhs& L, int dof, double* Val, double* R, double* W,
                  float* Val32, const int* incL, hipStream_t s);
 int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s);

@@ -884,6 +884,204 @@ int precond_diag(Lhs& L, int dof, double* Val, double* R, double* W, float* Val3
     return SVFSI_OK;
 }
 
+// ---------------------------------------------------------------------------
+// PRECONDRCS (svFSILS/PRECOND.f:150-368): Dirichlet rows and columns
+// removed and their diagonal set to one, then up to 10 sweeps of row and
+// column max-norm scaling; the sweep loop runs on the host (one 2-double
+// read-back per sweep, as the reference's one allgather per sweep).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rcs_mask(double m) { return m > 0.5 ? 1.0 : 0.0; }
+
+// Val <- diag(m) Val diag(m) (PREMUL then POSMUL), m the renormalised mask.
+template <int DOF>
+__global__ __launch_bounds__(256) void k_rcs_kill(size_t nEnt, const int* __restrict__ blockRow,
+                                                  const int* __restrict__ colPtr, const double* __restrict__ mask,
+                                                  double* __restrict__ val)
+{
+    constexpr int DD = DOF * DOF;
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nEnt) return;
+    const size_t k = t / DD;
+    const int e = (int)(t - k * DD);
+    const int i = e / DOF, j = e - (e / DOF) * DOF;
+    const double mr = rcs_mask(mask[(size_t)blockRow[k] * DOF + i]);
+    const double mc = rcs_mask(mask[(size_t)colPtr[k] * DOF + j]);
+    val[t] = (val[t] * mr) * mc;
+}
+
+// diagonal of masked dofs set to one, R <- m R, W1 = W2 = 1.
+template <int DOF>
+__global__ __launch_bounds__(256) void k_rcs_diag(int nNo, const int* __restrict__ diagPtr,
+                                                  const double* __restrict__ mask, double* __restrict__ val,
+                                                  double* __restrict__ R, double* __restrict__ W1,
+                                                  double* __restrict__ W2)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nNo * DOF) return;
+    const int a = t / DOF, i = t - (t / DOF) * DOF;
+    const double m = rcs_mask(mask[t]);
+    double* v = val + (size_t)diagPtr[a] * DOF * DOF + i * DOF + i;
+    *v = m * (*v - 1.0) + 1.0;
+    R[t] = m * R[t];
+    W1[t] = 1.0;
+    W2[t] = 1.0;
+}
+
+// Wr = largest |entry| of each scalar row, Wc of each scalar column (the
+// blocks of column a are the transposes' of row a), and per block of
+// threads max |1 - Wr|, max |1 - Wc|.
+template <int DOF>
+__global__ __launch_bounds__(256) void k_rcs_max(int nNo, const int* __restrict__ rowPtr,
+                                                 const int* __restrict__ trans, const double* __restrict__ val,
+                                                 double* __restrict__ Wr, double* __restrict__ Wc,
+                                                 double* __restrict__ part)
+{
+    constexpr int DD = DOF * DOF;
+    __shared__ double sr[256], sc[256];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double dr = 0.0, dc = 0.0;
+    if (t < nNo * DOF) {
+        const int a = t / DOF, i = t - (t / DOF) * DOF;
+        double r = 0.0, c = 0.0;
+        for (int k = rowPtr[a]; k < rowPtr[a + 1]; ++k) {
+            const double* b = val + (size_t)k * DD + i * DOF;
+            const double* bt = val + (size_t)trans[k] * DD + i;
+#pragma unroll
+            for (int m = 0; m < DOF; ++m) {
+                r = fmax(r, fabs(b[m]));
+                c = fmax(c, fabs(bt[m * DOF]));
+            }
+        }
+        Wr[t] = r;
+        Wc[t] = c;
+        dr = fabs(1.0 - r);
+        dc = fabs(1.0 - c);
+    }
+    sr[threadIdx.x] = dr;
+    sc[threadIdx.x] = dc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + o]);
+            sc[threadIdx.x] = fmax(sc[threadIdx.x], sc[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sr[0];
+        part[2 * blockIdx.x + 1] = sc[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rcs_maxsum(int nb, const double* __restrict__ part, double* __restrict__ out)
+{
+    __shared__ double sr[256], sc[256];
+    double r = 0.0, c = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        r = fmax(r, part[2 * b]);
+        c = fmax(c, part[2 * b + 1]);
+    }
+    sr[threadIdx.x] = r;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + o]);
+            sc[threadIdx.x] = fmax(sc[threadIdx.x], sc[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sr[0];
+        out[1] = sc[0];
+    }
+}
+
+// Wr <- 1/sqrt(Wr), Wc <- 1/sqrt(Wc), W1 *= Wr, W2 *= Wc.
+__global__ __launch_bounds__(256) void k_rcs_inv(size_t n, double* __restrict__ Wr, double* __restrict__ Wc,
+                                                 double* __restrict__ W1, double* __restrict__ W2)
+{
+    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const double r = 1.0 / sqrt(Wr[q]), c = 1.0 / sqrt(Wc[q]);
+    Wr[q] = r;
+    Wc[q] = c;
+    W1[q] = W1[q] * r;
+    W2[q] = W2[q] * c;
+}
+
+// Val <- diag(Wr) Val diag(Wc) (PREMUL then POSMUL); optional FP32 copy.
+template <int DOF>
+__global__ __launch_bounds__(256) void k_rcs_scale(size_t nEnt, const int* __restrict__ blockRow,
+                                                   const int* __restrict__ colPtr, const double* __restrict__ Wr,
+                                                   const double* __restrict__ Wc, double* __restrict__ val,
+                                                   float* __restrict__ val32)
+{
+    constexpr int DD = DOF * DOF;
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nEnt) return;
+    const size_t k = t / DD;
+    const int e = (int)(t - k * DD);
+    const int i = e / DOF, j = e - (e / DOF) * DOF;
+    const double v = (val[t] * Wr[(size_t)blockRow[k] * DOF + i]) * Wc[(size_t)colPtr[k] * DOF + j];
+    val[t] = v;
+    if (val32) val32[t] = (float)v;
+}
+
+template <int DOF>
+static int precond_rcs_t(Lhs& L, double* Val, double* R, double* W1, double* W2, float* Val32, hipStream_t s)
+{
+    constexpr int DD = DOF * DOF;
+    const size_t n = (size_t)L.nNo * DOF;
+    const size_t nEnt = (size_t)L.nnz * DD;
+    const unsigned ge = (unsigned)((nEnt + 255) / 256);
+    const unsigned gn = (unsigned)((n + 255) / 256);
+    Work& w = L.ws;
+    SV_TRY(w.rcs.ensure((2 * n + 2 * (size_t)gn + 2) * sizeof(double)));
+    double* Wr = w.rcs.as<double>();
+    double* Wc = Wr + n;
+    double* part = Wc + n;
+    double* out = part + 2 * (size_t)gn;
+    const double* mask = L.ws.mask.as<double>();
+    hipLaunchKernelGGL(k_rcs_kill<DOF>, dim3(ge), dim3(256), 0, s, nEnt, L.blockRow.as<int>(), L.colPtr.as<int>(),
+                       mask, Val);
+    hipLaunchKernelGGL(k_rcs_diag<DOF>, dim3(gn), dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), mask, Val, R, W1, W2);
+    for (int iter = 1;; ++iter) {
+        bool last = iter >= 10;
+        hipLaunchKernelGGL(k_rcs_max<DOF>, dim3(gn), dim3(256), 0, s, L.nNo, L.rowPtr.as<int>(), L.trans.as<int>(),
+                           (const double*)Val, Wr, Wc, part);
+        hipLaunchKernelGGL(k_rcs_maxsum, dim3(1), dim3(256), 0, s, (int)gn, (const double*)part, out);
+        double h[2];
+        SV_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+        SV_HIP(hipStreamSynchronize(s));
+        if (h[0] < 2.0 && h[1] < 2.0) last = true;
+        hipLaunchKernelGGL(k_rcs_inv, dim3(gn), dim3(256), 0, s, n, Wr, Wc, W1, W2);
+        hipLaunchKernelGGL(k_rcs_scale<DOF>, dim3(ge), dim3(256), 0, s, nEnt, L.blockRow.as<int>(),
+                           L.colPtr.as<int>(), (const double*)Wr, (const double*)Wc, Val, last ? Val32 : nullptr);
+        if (last) break;
+    }
+    SV_TRY(launch_scale_vec(n, W1, R, s));
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int precond_rcs(Lhs& L, int dof, double* Val, double* R, double* W1, double* W2, float* Val32, const int* incL,
+                hipStream_t s)
+{
+    if (!L.symmetric) {
+        set_error("PRECOND_RCS needs a structurally symmetric pattern (column maxima via the transpose map)");
+        return SVFSI_ERR_NOTSUP;
+    }
+    SV_TRY(build_mask(L, dof, incL, s));
+    if (L.nNo == 0) return SVFSI_OK;
+    switch (dof) {
+    case 2: return precond_rcs_t<2>(L, Val, R, W1, W2, Val32, s);
+    case 3: return precond_rcs_t<3>(L, Val, R, W1, W2, Val32, s);
+    case 4: return precond_rcs_t<4>(L, Val, R, W1, W2, Val32, s);
+    default: set_error("PRECOND_RCS: dof %d not supported (2..4)", dof); return SVFSI_ERR_NOTSUP;
+    }
+}
+
 // Values of K^T on the (structurally symmetric) pattern of K:
 // VT[k](i,j) = V[trans[k]](j,i).
 template <int DOF>

@@ -60,7 +60,8 @@ class NewtonStep:
 
     def solve(self, mixed: bool = False) -> FSILS_lsType:
         ls = self.make_ls()
-        fsils.fsils_solve(self.lhs, ls, self.prob.dof, self.R, self.Val, mixed=mixed)
+        prec = self.prob.ls.get("prec", fsils.PRECOND_FSILS)
+        fsils.fsils_solve(self.lhs, ls, self.prob.dof, self.R, self.Val, prec=prec, mixed=mixed)
         return ls
 
     def step(self, mixed: bool = False) -> FSILS_lsType:

@@ -128,6 +128,57 @@ def _assembled(prob, orc):
     return st, Vg, Rg, Vo, Ro
 
 
+@pytest.mark.parametrize("which", ["fluid", "lelas"])
+def test_precond_rcs_parity(gpu, orc, which):
+    """PRECONDRCS on the device against the oracle: scaled matrix and
+    right-hand side and both factor vectors to 1e-14 (max and sqrt are exact
+    or correctly rounded, the products are taken in the same order)."""
+    import torch
+
+    from svfsi_amd._lib import call
+
+    prob = problems.fluid_cylinder(h=0.5) if which == "fluid" else problems.lelas_cube(5)
+    st = NewtonStep(prob)
+    st.assemble()
+    torch.cuda.synchronize()
+    V0, R0 = st.Val.cpu().numpy().copy(), st.R.cpu().numpy().copy()
+    W1 = torch.zeros_like(st.R)
+    W2 = torch.zeros_like(st.R)
+    call("svfsi_precond_rcs_dev", st.lhs.handle, prob.dof, fsils.dev_ptr(st.Val), fsils.dev_ptr(st.R),
+         fsils.dev_ptr(W1), fsils.dev_ptr(W2), None, fsils.cur_stream())
+    Vo, Ro, W1o, W2o = orc.precond_rcs(st.rowPtr - 1, st.colPtr - 1, prob.dof, V0, R0, prob.dir_mask())
+    say("rcs parity", which, relerr(st.Val.cpu().numpy(), Vo), relerr(W2.cpu().numpy(), W2o))
+    assert relerr(st.Val.cpu().numpy(), Vo) < 1e-14
+    assert relerr(st.R.cpu().numpy(), Ro) < 1e-14
+    assert relerr(W1.cpu().numpy(), W1o) < 1e-14
+    assert relerr(W2.cpu().numpy(), W2o) < 1e-14
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("which", ["fluid", "lelas"])
+def test_solve_with_rcs(gpu, orc, which, mixed):
+    """FSILS_SOLVE with PRECOND_RCS (GMRES for the fluid, CG for LELAS), FP64
+    and mixed paths, against the oracle's solve with the same
+    preconditioner."""
+    prob = problems.fluid_cylinder(h=0.5) if which == "fluid" else problems.lelas_cube(5)
+    prob.ls = dict(prob.ls, prec=709)
+    st = NewtonStep(prob)
+    st.assemble()
+    import torch
+
+    torch.cuda.synchronize()
+    V0, R0 = st.Val.cpu().numpy().copy(), st.R.cpu().numpy().copy()
+    ls = st.solve(mixed=mixed)
+    torch.cuda.synchronize()
+    xg = st.R.cpu().numpy()
+    xo, _, so = orc.fsils_solve(st.rowPtr - 1, st.colPtr - 1, prob.dof, prob.dir_mask(), prob.ls, R0, V0, prec=709)
+    say("rcs solve", which, "mixed", mixed, "itr", ls.RI.itr, "cpu", so.itr, "x relerr", relerr(xg, xo))
+    assert ls.RI.suc == 1 and so.suc == 1
+    if not mixed:
+        assert abs(ls.RI.itr - so.itr) <= max(3, so.itr // 50), (ls.RI.itr, so.itr)
+    assert relerr(xg, xo) < 1e-5
+
+
 def test_lelas_assembly_parity_and_determinism(gpu, orc):
     prob = problems.lelas_cube(6)
     rng = np.random.default_rng(3)

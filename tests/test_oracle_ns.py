@@ -1,0 +1,106 @@
+"""Pins of the oracle's NSSOLVER (svFSILS/NSSOLVER.f:52-383).
+
+The NS solver builds its solution from directions [0; P_i] and [U_i; 0]
+and the least-squares fit of their images over the whole scaled system,
+so whatever the inner accuracies:
+- it converges to the solution of the full system (checked against the
+  dense solve of the scaled system and against GMRESV at a tight
+  tolerance), and its reported final residual matches the recomputed one;
+- a one-iteration run is the exact least-squares combination of its two
+  directions (checked by recomputing them densely);
+- GE (GE.f) agrees with numpy on scaled symmetric positive systems, and
+  on N = 1, 2 (closed forms)."""
+import numpy as np
+import pytest
+
+from svfsi_amd import problems
+
+from test_adjoint import dense
+
+
+def fluid(h=0.6, seed=3):
+    prob = problems.fluid_cylinder(h=h)
+    rng = np.random.default_rng(seed)
+    prob.Yg[:, :3] += 0.1 * rng.normal(size=(prob.nNo, 3))
+    return prob
+
+
+def scaled_system(orc, prob):
+    rp, cp = orc.lhsa(prob.nNo, prob.mesh.ien)
+    V, R = orc.assemble(prob, rp, cp)
+    Vs, Rs, W = orc.precond_diag(rp, cp, 4, V, R, prob.dir_mask())
+    return rp, cp, V, R, Vs, Rs, W
+
+
+def test_ns_converges_to_the_full_solution(orc):
+    prob = fluid()
+    rp, cp, V, R, Vs, Rs, W = scaled_system(orc, prob)
+    st = dict(relTol=1e-10, absTol=1e-30, mItr=20, GM=dict(relTol=1e-4, mItr=4, sD=50), CG=dict(relTol=1e-4))
+    x, _, ls = orc.fsils_solve_ns(rp, cp, 4, prob.dir_mask(), st, R, V)
+    assert ls.RI.suc == 1
+    xg, _, sg = orc.fsils_solve(rp, cp, 4, prob.dir_mask(), dict(LS_type=797, relTol=1e-12, absTol=1e-30,
+                                                                  mItr=200, sD=50), R, V)
+    assert sg.suc == 1
+    assert np.abs(x - xg).max() <= 1e-6 * np.abs(xg).max()
+    # The reported residual is iNorm^2 - xB.B (normal equations), exact in
+    # exact arithmetic; at 1e-6 it is far above that formula's cancellation
+    # floor (eps iNorm^2) and matches the residual recomputed from x.  (At
+    # 1e-10 the formula can go negative, and the reference then reports 0.)
+    # Agreement to 1 %: the Gram matrix of the images is ill-conditioned.
+    st6 = dict(st, relTol=1e-6)
+    x6, _, ls6 = orc.fsils_solve_ns(rp, cp, 4, prob.dir_mask(), st6, R, V)
+    assert ls6.RI.suc == 1
+    y = np.where(W != 0, x6 / np.where(W != 0, W, 1), 0.0)
+    r = Rs - orc.spmv(rp, cp, 4, Vs, y)
+    assert abs(np.linalg.norm(r) - ls6.RI.fNorm) <= 1e-2 * ls6.RI.fNorm
+    assert ls6.RI.fNorm <= 1e-6 * ls6.RI.iNorm
+    assert ls6.Resm + ls6.Resc == 100 and 0 <= ls6.Resc <= 100
+
+
+def test_ns_one_iteration_is_the_least_squares_fit(orc):
+    prob = fluid(0.8)
+    rp, cp, V, R, Vs, Rs, W = scaled_system(orc, prob)
+    n = prob.nNo
+    K = dense(Vs, rp, cp, n, 4)
+    b = Rs.ravel()
+    st = dict(relTol=1e-14, absTol=1e-30, mItr=1, GM=dict(relTol=1e-12, mItr=50, sD=60), CG=dict(relTol=1e-12,
+                                                                                                  mItr=5000))
+    x, _, ls = orc.fsils_solve_ns(rp, cp, 4, prob.dir_mask(), st, R, V)
+    y = np.where(W != 0, x / np.where(W != 0, W, 1), 0.0).ravel()
+    # the two directions, from dense solves with the same (near-exact) inner accuracy
+    vi = np.arange(4 * n).reshape(n, 4)
+    iu, ip = vi[:, :3].ravel(), vi[:, 3]
+    Kuu, Kup, Kpu, Kpp = K[np.ix_(iu, iu)], K[np.ix_(iu, ip)], K[np.ix_(ip, iu)], K[np.ix_(ip, ip)]
+    free_u = np.abs(np.diag(Kuu)) > 0
+    Kuu_f = Kuu + np.diag(~free_u * 1.0)  # masked rows/cols are zero: their U entries are zero anyway
+    U1 = np.linalg.solve(Kuu_f, b[iu])
+    Pp = b[ip] - Kpu @ U1
+    S = Kpp + Kup.T @ Kup
+    Sf = S + np.diag((np.abs(np.diag(S)) == 0) * 1.0)
+    Pp = np.linalg.solve(Sf, Pp)
+    U2 = np.linalg.solve(Kuu_f, b[iu] - Kup @ Pp)
+    d1 = np.zeros(4 * n); d1[ip] = Pp
+    d2 = np.zeros(4 * n); d2[iu] = U2
+    Md = np.stack([K @ d1, K @ d2], axis=1)
+    coef, *_ = np.linalg.lstsq(Md, b, rcond=None)
+    y_ref = coef[0] * d1 + coef[1] * d2
+    assert np.abs(y - y_ref).max() <= 1e-6 * np.abs(y_ref).max()
+
+
+@pytest.mark.parametrize("N", [1, 2, 5, 9])
+def test_ge_matches_numpy(orc, N):
+    import ctypes as C
+
+    rng = np.random.default_rng(N)
+    M = rng.normal(size=(N, N))
+    A = M @ M.T + N * np.eye(N)
+    nV = N + 3
+    Ab = np.zeros((nV, nV))
+    Ab[:N, :N] = A
+    b = rng.normal(size=nV)
+    B = b.copy()
+    lib = orc.lib()
+    fn = lib.orc_ge_test
+    ok = fn(nV, N, Ab.T.copy().ctypes.data_as(C.c_void_p), B.ctypes.data_as(C.c_void_p))
+    assert ok == 1
+    assert np.allclose(B[:N], np.linalg.solve(A, b[:N]), rtol=1e-12, atol=1e-12)
