@@ -125,8 +125,10 @@ int fsils_gpu_bc_free(void* lhs, int faIn);
  * exit; Val(dof*dof,nnz) is left scaled in place (W K W), as the reference
  * does (SOLVE.f:57, PRECOND.f:124-130).  prec PRECOND_FSILS (diagonal
  * scaling) or PRECOND_RCS (row/column equilibration, PRECOND.f:150-368);
- * LS_type GMRES or CG (dof >= 2).  incL(nFaces) may be NULL (all faces
- * included); res(nFaces) may be NULL. */
+ * LS_type GMRES or CG (dof >= 2), or NS (dof 4: NSSOLVER,
+ * svFSILS/NSSOLVER.f:52-383, with the ls%GM and ls%CG records, filling
+ * ls%Resm/Resc).  incL(nFaces) may be NULL (all faces included);
+ * res(nFaces) may be NULL. */
 int fsils_gpu_solve(void* lhs, FSILS_lsType* ls, int dof, double* Ri,
                     double* Val, int prec, const int* incL, const double* res);
 

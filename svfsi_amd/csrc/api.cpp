@@ -241,8 +241,12 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
             return SVFSI_ERR_NOTSUP;
         }
     }
-    if (ls->LS_type != LS_TYPE_GMRES && ls->LS_type != LS_TYPE_CG) {
-        set_error("LS_type %d not supported (GMRES 797, CG 798)", ls->LS_type);
+    if (ls->LS_type != LS_TYPE_GMRES && ls->LS_type != LS_TYPE_CG && ls->LS_type != LS_TYPE_NS) {
+        set_error("LS_type %d not supported (NS 796, GMRES 797, CG 798)", ls->LS_type);
+        return SVFSI_ERR_NOTSUP;
+    }
+    if (ls->LS_type == LS_TYPE_NS && (mode != 0 || dof != 4)) {
+        set_error("the NS solver is FP64 only and needs dof 4 (velocity 3 + pressure)");
         return SVFSI_ERR_NOTSUP;
     }
     const size_t n = (size_t)L.nNo * dof;
@@ -263,6 +267,7 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
     }
     int rc;
     if (mode == 1) rc = mixed_solve(L, ls->RI, dof, Val, v32, R, s);
+    else if (ls->LS_type == LS_TYPE_NS) rc = ns_solve(L, *ls, dof, Val, R, s);
     else if (ls->LS_type == LS_TYPE_GMRES) rc = gmres_solve(L, ls->RI, dof, Val, R, s, false);
     else rc = cg_solve(L, ls->RI, dof, Val, R, s);
     if (rc != SVFSI_OK) return rc;

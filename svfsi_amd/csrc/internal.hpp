@@ -117,6 +117,7 @@ struct Work {
     DBuf zdir, opartial, ostate, ohess; // flexible GMRES: FP32 directions, outer state
     DBuf mask;
     DBuf rcs, W1;  // PRECOND_RCS: row/column factors, partial maxima; W1 (row scaling)
+    DBuf ns;       // NS solver: the split blocks, its vectors and directions
     DBuf valT; // K^T values for transposed solves
     DBuf dinv32; // inverse diagonal blocks (FP32) of the inner block-Jacobi
     DBuf polyCoef, polyVec; // polynomial preconditioner: factors, work vectors
@@ -254,5 +255,8 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
                 float* K32, double* R, hipStream_t s);
 
 double wall_seconds();
+// NSSOLVER (svFSILS/NSSOLVER.f:52-383) on the scaled system, dof 4: Ri in
+// (right-hand side), out (solution); ls.RI, ls.GM, ls.CG, Resm, Resc set.
+int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, hipStream_t s);
 
 } // namespace svfsi

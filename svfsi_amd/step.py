@@ -55,8 +55,16 @@ class NewtonStep:
 
     def make_ls(self) -> FSILS_lsType:
         c = self.prob.ls
+        inner = {}
+        if c["LS_type"] == fsils.LS_TYPE_NS:
+            # the NS solver's momentum GMRES and Schur-complement CG records
+            # (LS.f:69-95 defaults where the settings leave them out)
+            gm = {**dict(relTol=1e-2, absTol=1e-10, mItr=2), **c.get("GM", {})}
+            cg = {**dict(relTol=0.2, absTol=1e-10, mItr=500), **c.get("CG", {})}
+            inner = dict(relTolIn=[gm["relTol"], cg["relTol"]], absTolIn=[gm["absTol"], cg["absTol"]],
+                         maxItrIn=[gm["mItr"], cg["mItr"]])
         return fsils.fsils_ls_create(c["LS_type"], relTol=c["relTol"], absTol=c["absTol"],
-                                     maxItr=c["mItr"], dimKry=c["sD"] if c.get("sD") else None)
+                                     maxItr=c["mItr"], dimKry=c["sD"] if c.get("sD") else None, **inner)
 
     def solve(self, mixed: bool = False) -> FSILS_lsType:
         ls = self.make_ls()

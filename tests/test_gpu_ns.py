@@ -1,0 +1,118 @@
+"""GPU NSSOLVER (svFSILS/NSSOLVER.f:52-383) through the C ABI against the
+oracle's restatement on the same assembled fluid system: solution, outer
+iteration count, success flag, the Resm/Resc split; and the reported final
+residual against the residual recomputed from the GPU solution on the CPU
+(the least-squares formula iNorm^2 - xB.B agrees with it to about 1 % at
+relTol 1e-6: the Gram matrix of the images is ill-conditioned)."""
+import sys
+
+import numpy as np
+import pytest
+
+from svfsi_amd import fsils, problems
+from svfsi_amd._lib import SvfsiError
+from svfsi_amd.step import NewtonStep
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def fluid(h, seed=3):
+    prob = problems.fluid_cylinder(h=h)
+    rng = np.random.default_rng(seed)
+    prob.Yg[:, :3] += 0.1 * rng.normal(size=(prob.nNo, 3))
+    return prob
+
+
+# reference defaults (LS.f:69-95) and a tight variant
+SETTINGS = {
+    "default": dict(relTol=0.4, absTol=1e-10, mItr=10, sD=100),
+    "tight": dict(relTol=1e-6, absTol=1e-30, mItr=20, sD=50, GM=dict(relTol=1e-4, mItr=4),
+                  CG=dict(relTol=1e-4)),
+}
+
+
+def _solve_both(orc, prob, settings, prec):
+    import torch
+
+    prob.ls = dict(settings, LS_type=fsils.LS_TYPE_NS, prec=prec)
+    st = NewtonStep(prob)
+    st.assemble()
+    torch.cuda.synchronize()
+    V0, R0 = st.Val.cpu().numpy().copy(), st.R.cpu().numpy().copy()
+    ls = st.solve()
+    torch.cuda.synchronize()
+    xg = st.R.cpu().numpy()
+    xo, _, lo = orc.fsils_solve_ns(st.rowPtr - 1, st.colPtr - 1, 4, prob.dir_mask(), settings, R0, V0, prec=prec)
+    return st, V0, R0, xg, ls, xo, lo
+
+
+@pytest.mark.parametrize("prec", [fsils.PRECOND_FSILS, fsils.PRECOND_RCS])
+@pytest.mark.parametrize("which", ["default", "tight"])
+def test_ns_solver_parity(gpu, orc, which, prec):
+    prob = fluid(0.6)
+    say("ns solve", which, "prec", prec, "nNo", prob.nNo)
+    st, V0, R0, xg, ls, xo, lo = _solve_both(orc, prob, SETTINGS[which], prec)
+    say("  gpu RI.itr", ls.RI.itr, "GM.itr", ls.GM.itr, "CG.itr", ls.CG.itr, "suc", ls.RI.suc, "Resm/Resc",
+        ls.Resm, ls.Resc, "fNorm/iNorm", ls.RI.fNorm / ls.RI.iNorm)
+    say("  cpu RI.itr", lo.RI.itr, "GM.itr", lo.GM.itr, "CG.itr", lo.CG.itr, "suc", lo.RI.suc, "Resm/Resc",
+        lo.Resm, lo.Resc, "x relerr", relerr(xg, xo))
+    assert ls.RI.suc == lo.RI.suc == 1
+    assert ls.RI.itr == lo.RI.itr
+    assert abs(ls.GM.itr - lo.GM.itr) <= max(2, lo.GM.itr // 20)
+    assert abs(ls.CG.itr - lo.CG.itr) <= max(2, lo.CG.itr // 20)
+    # Resm + Resc = 100, or both 0 where the least-squares residual formula
+    # went negative by cancellation (NSSOLVER.f:217-221 then reports 0)
+    assert ls.Resm + ls.Resc in (0, 100) and abs(ls.Resc - lo.Resc) <= 1 and abs(ls.Resm - lo.Resm) <= 1
+    assert abs(ls.RI.iNorm - lo.RI.iNorm) <= 1e-12 * lo.RI.iNorm
+    assert relerr(xg, xo) < 1e-6
+    if which == "tight" and prec == fsils.PRECOND_FSILS:
+        # the reported residual against the one recomputed on the CPU from
+        # the GPU solution, on the diagonally scaled system
+        rp, cp = st.rowPtr - 1, st.colPtr - 1
+        Vs, Rs, W = orc.precond_diag(rp, cp, 4, V0, R0, prob.dir_mask())
+        y = np.where(W != 0, xg / np.where(W != 0, W, 1), 0.0)
+        r = Rs - orc.spmv(rp, cp, 4, Vs, y)
+        say("  residual recomputed", np.linalg.norm(r), "reported", ls.RI.fNorm)
+        assert abs(np.linalg.norm(r) - ls.RI.fNorm) <= 1e-2 * ls.RI.fNorm
+        assert ls.RI.fNorm <= 1e-6 * ls.RI.iNorm
+
+
+def test_ns_solver_larger_mesh(gpu, orc):
+    """A few thousand nodes, reference defaults: the same outer iterations
+    and solution as the oracle."""
+    prob = fluid(0.3)
+    say("ns solve larger", prob.nNo)
+    st, V0, R0, xg, ls, xo, lo = _solve_both(orc, prob, SETTINGS["default"], fsils.PRECOND_FSILS)
+    say("  itr", ls.RI.itr, lo.RI.itr, "GM", ls.GM.itr, lo.GM.itr, "CG", ls.CG.itr, lo.CG.itr, "x relerr",
+        relerr(xg, xo))
+    assert ls.RI.suc == lo.RI.suc == 1
+    assert ls.RI.itr == lo.RI.itr
+    assert relerr(xg, xo) < 1e-6
+
+
+def test_ns_solver_rejects_unsupported(gpu):
+    """NS needs dof 4 and runs FP64 only: dof 3 and the mixed path are
+    refused with an error (the system is left untouched by the refusal)."""
+    prob = problems.lelas_cube(3)
+    prob.ls = dict(prob.ls, LS_type=fsils.LS_TYPE_NS)
+    st = NewtonStep(prob)
+    st.assemble()
+    with pytest.raises(SvfsiError, match="dof 4"):
+        st.solve()
+    pf = fluid(0.8)
+    pf.ls = dict(SETTINGS["default"], LS_type=fsils.LS_TYPE_NS)
+    sf = NewtonStep(pf)
+    sf.assemble()
+    with pytest.raises(SvfsiError, match="FP64 only"):
+        sf.solve(mixed=True)
