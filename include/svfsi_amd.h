@@ -113,8 +113,9 @@ int fsils_gpu_lhs_free(void* lhs);
 
 /* Replaces fsils_bc_create_ (FSILS_api_c.h:61; FSILS_BC_CREATE,
  * svFSILS/BC.f:50-121).  faIn is 1-based; gNodes 1-based; Val(dof,nNo)
- * may be NULL (then 0, a homogeneous Dirichlet mask).  BC_TYPE_Neu faces
- * are stored but coupled Neumann (res != 0) solves are not supported. */
+ * may be NULL (then 0, a homogeneous Dirichlet mask).  For a BC_TYPE_Neu
+ * face Val is the face vector (the integral of N n, svFSI/BAFINI.f:514-533)
+ * that couples through res(faIn) in the solve. */
 int fsils_gpu_bc_create(void* lhs, int faIn, int nNo, int dof, int BC_type,
                         const int* gNodes, const double* Val);
 /* Replaces fsils_bc_free_ (FSILS_api_c.h:62). */
@@ -127,8 +128,11 @@ int fsils_gpu_bc_free(void* lhs, int faIn);
  * scaling) or PRECOND_RCS (row/column equilibration, PRECOND.f:150-368);
  * LS_type GMRES or CG (dof >= 2), or NS (dof 4: NSSOLVER,
  * svFSILS/NSSOLVER.f:52-383, with the ls%GM and ls%CG records, filling
- * ls%Resm/Resc).  incL(nFaces) may be NULL (all faces included);
- * res(nFaces) may be NULL. */
+ * ls%Resm/Resc).  incL(nFaces) may be NULL (all faces included).
+ * res(nFaces): a BC_TYPE_Neu face with res != 0 adds res v v^T (v its face
+ * vector scaled by W) to the GMRES and NS operators (ADDBCMUL,
+ * svFSILS/ADDBCMUL.f:53-114); CG ignores it, as CGRADV does.  res may be
+ * NULL only when no BC_TYPE_Neu face exists (SOLVE.f:74-77). */
 int fsils_gpu_solve(void* lhs, FSILS_lsType* ls, int dof, double* Ri,
                     double* Val, int prec, const int* incL, const double* res);
 

@@ -304,6 +304,44 @@ def fsils_solve_ns(rowPtr, colPtr, dof, mask, settings: dict, R, Val, prec: int 
     return x, V, ls
 
 
+def coupled_faces(faces, dof: int):
+    """Concatenated coupled Neumann faces for orc_fsils_solve_cpl: faces is a
+    list of (nodes 0-based, face dof, val (nNo_f, face dof), res)."""
+    off = [0]
+    for nodes, _, _, _ in faces:
+        off.append(off[-1] + len(nodes))
+    glob = np.concatenate([np.asarray(nd, dtype=np.int32) for nd, _, _, _ in faces])
+    fdof = np.array([fd for _, fd, _, _ in faces], dtype=np.int32)
+    val = np.zeros((off[-1], dof))
+    for f, (nodes, fd, v, _) in enumerate(faces):
+        k = min(fd, dof)
+        val[off[f]:off[f + 1], :k] = np.asarray(v, dtype=float).reshape(len(nodes), fd)[:, :k]
+    res = np.array([r for _, _, _, r in faces], dtype=float)
+    return i32(off), glob, fdof, val, res
+
+
+def fsils_solve_cpl(rowPtr, colPtr, dof, mask, settings: dict, R, Val, faces, prec: int = 701):
+    """FSILS_SOLVE with coupled Neumann faces (ADDBCMUL in GMRESV / NSSOLVER;
+    CGRADV ignores them).  settings["LS_type"] 796, 797 or 798.  Returns
+    (x, scaled Val, Ls): Ls.RI is the outer record for every solver."""
+    x = f64(R).copy()
+    V = f64(Val).copy()
+    lt = settings["LS_type"]
+    if lt == 796:
+        ls = ns_ls(settings)
+    else:
+        ls = Ls()
+        ls.LS_type = lt
+        ls.RI = _ls(settings)
+    off, glob, fdof, val, res = coupled_faces(faces, dof)
+    rc = lib().orc_fsils_solve_cpl(len(rowPtr) - 1, dof, P(i32(rowPtr)), P(i32(colPtr)),
+                                   None if mask is None else P(f64(mask)), lt, int(prec), C.byref(ls),
+                                   len(faces), P(off), P(glob), P(fdof), P(f64(val)), P(f64(res)), P(x), P(V))
+    if rc != 0:
+        raise RuntimeError(f"orc_fsils_solve_cpl rc={rc}")
+    return x, V, ls
+
+
 def fluid_dmu_dot(prob, lam, du) -> np.ndarray:
     """g[e] = lam_e . (dR_e/dmu_e - dK_e/dmu_e du_e) (complex step)."""
     m = prob.mesh

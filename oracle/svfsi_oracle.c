@@ -1004,8 +1004,53 @@ void orc_cgradv(int nNo, int dof, const int* rowPtr, const int* colPtr,
 /* GMRESV (svFSILS/GMRES.f:273-431) without coupled faces: restarted
  * GMRES(sD), classical Gram-Schmidt with all dots first, Pythagorean
  * sub-diagonal, Givens rotations, back substitution after each cycle. */
+/* Coupled Neumann faces of one solve (FSILS_SOLVE with res(faIn) != 0 on a
+ * BC_TYPE_Neu face, SOLVE.f:78-86): face f holds nodes glob[off[f] ..
+ * off[f+1]-1] and the preconditioned face vector vm (valM, PRECOND.f:133-140)
+ * with stride dof (fdof[f] components used), the coefficient res[f] and the
+ * squared norm nS[f] of BCPRE (GMRES.f:393-429, NSSOLVER.f:307-338). */
+typedef struct {
+    int nC, dof;
+    const int* off;
+    const int* glob;
+    const int* fdof;
+    const double* vm;
+    const double* res;
+    const double* nS;
+} orc_cpl;
+
+/* ADDBCMUL (svFSILS/ADDBCMUL.f:53-114), one rank: for each coupled face in
+ * order, S = coef (v . X) over its nodes and min(face dof, vd) components,
+ * then Y += v S.  op 0 = BCOP_TYPE_ADD (coef = res), 1 = BCOP_TYPE_PRE
+ * (coef = -res / (1 + res nS)); X and Y have vd components per node. */
+static void orc_addbcmul(const orc_cpl* c, int op, int vd, const double* X, double* Y)
+{
+    for (int f = 0; f < c->nC; ++f) {
+        const int nsd = c->fdof[f] < vd ? c->fdof[f] : vd;
+        const double coef = op == 0 ? c->res[f] : -c->res[f] / (1.0 + c->res[f] * c->nS[f]);
+        double S = 0.0;
+        for (int a = c->off[f]; a < c->off[f + 1]; ++a)
+            for (int i = 0; i < nsd; ++i) S = S + c->vm[(size_t)a * c->dof + i] * X[(size_t)c->glob[a] * vd + i];
+        S = coef * S;
+        for (int a = c->off[f]; a < c->off[f + 1]; ++a)
+            for (int i = 0; i < nsd; ++i)
+                Y[(size_t)c->glob[a] * vd + i] = Y[(size_t)c->glob[a] * vd + i] + c->vm[(size_t)a * c->dof + i] * S;
+    }
+}
+
+static void gmresv_impl(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* K, double* R,
+                        orc_subls* ls, const orc_cpl* cp);
+
 void orc_gmresv(int nNo, int dof, const int* rowPtr, const int* colPtr,
                 const double* K, double* R, orc_subls* ls)
+{
+    gmresv_impl(nNo, dof, rowPtr, colPtr, K, R, ls, NULL);
+}
+
+/* GMRESV with the coupled faces' ADDBCMUL(ADD) after every product
+ * (GMRES.f:317-331; its BCOP_TYPE_PRE branch is off, flag = .FALSE.) */
+static void gmresv_impl(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* K, double* R,
+                        orc_subls* ls, const orc_cpl* cp)
 {
     size_t n = (size_t)nNo * dof;
     int sD = ls->sD;
@@ -1034,6 +1079,7 @@ void orc_gmresv(int nNo, int dof, const int* rowPtr, const int* colPtr,
         ls->dB = ls->fNorm;
         ls->itr++;
         orc_spmv(nNo, dof, rowPtr, colPtr, K, X, u);
+        if (cp) orc_addbcmul(cp, 0, dof, X, u);
         ORC_PAR
         for (size_t q = 0; q < n; ++q) u[q] = R[q] - u[q];
         err[0] = orc_norm((int)n, u);
@@ -1045,6 +1091,7 @@ void orc_gmresv(int nNo, int dof, const int* rowPtr, const int* colPtr,
             double* ui = u + n * (i - 1);
             double* un = u + n * i;
             orc_spmv(nNo, dof, rowPtr, colPtr, K, ui, un);
+            if (cp) orc_addbcmul(cp, 0, dof, ui, un);
             for (int j = 1; j <= i + 1; ++j) H(j - 1, i - 1) = orc_dot((int)n, u + n * (j - 1), un);
             for (int j = 1; j <= i; ++j) {
                 double hj = H(j - 1, i - 1);
@@ -1101,7 +1148,7 @@ done:
  * X = 0.  Unlike GMRESV the first cycle starts from R itself (no product),
  * R is left unchanged, and ls->itr accumulates over calls. */
 static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* K,
-                        const double* R, double* X, orc_subls* ls)
+                        const double* R, double* X, orc_subls* ls, const orc_cpl* cp)
 {
     size_t n = (size_t)nNo * dof;
     int sD = ls->sD;
@@ -1111,6 +1158,7 @@ static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, 
     double* c = malloc(sizeof(double) * sD);
     double* s = malloc(sizeof(double) * sD);
     double* err = malloc(sizeof(double) * (sD + 1));
+    double* unc = cp ? malloc(sizeof(double) * n) : NULL;
 #define H(j, i) h[(size_t)(i) * (sD + 1) + (j)]
     double time = wall_s();
     ls->suc = 0;
@@ -1122,7 +1170,12 @@ static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, 
         } else {
             ls->itr++;
             orc_spmv(nNo, dof, rowPtr, colPtr, K, X, u);
+            if (cp) orc_addbcmul(cp, 0, dof, X, u);
             for (size_t q = 0; q < n; ++q) u[q] = R[q] - u[q];
+        }
+        if (cp) {
+            memcpy(unc, u, sizeof(double) * n);
+            orc_addbcmul(cp, 1, dof, unc, u);
         }
         err[0] = orc_norm((int)n, u);
         if (l == 1) {
@@ -1144,6 +1197,11 @@ static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, 
             double* ui = u + n * (i - 1);
             double* un = u + n * i;
             orc_spmv(nNo, dof, rowPtr, colPtr, K, ui, un);
+            if (cp) {
+                orc_addbcmul(cp, 0, dof, ui, un);
+                memcpy(unc, un, sizeof(double) * n);
+                orc_addbcmul(cp, 1, dof, unc, un);
+            }
             for (int j = 1; j <= i + 1; ++j) H(j - 1, i - 1) = orc_dot((int)n, u + n * (j - 1), un);
             for (int j = 1; j <= i; ++j) {
                 double hj = H(j - 1, i - 1);
@@ -1185,7 +1243,7 @@ static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, 
     ls->dB = 10.0 * log(ls->fNorm / ls->dB);
 done:
 #undef H
-    free(u); free(h); free(y); free(c); free(s); free(err);
+    free(u); free(h); free(y); free(c); free(s); free(err); free(unc);
 }
 
 /* FSILS_SPARMULVS / SV / SS (svFSILS/SPARMUL.f:135-297), one rank:
@@ -1233,8 +1291,9 @@ static void orc_spmv_ss(int nNo, const int* rowPtr, const int* colPtr, const dou
  * S P = L P - D (G P) (D = Gt = -G^T blockwise, so S = L + G^T G); R in:
  * right-hand side, out: solution; ls->itr accumulates. */
 static void orc_cgrad_schur(int nNo, int nsd, const int* rowPtr, const int* colPtr, const double* D,
-                            const double* G, const double* L, double* R, orc_subls* ls)
+                            const double* G, const double* L, double* R, orc_subls* ls, const orc_cpl* cp)
 {
+    double* unc = cp ? malloc(sizeof(double) * (size_t)nNo * nsd) : NULL;
     double* X = calloc(nNo, sizeof(double));
     double* P = malloc(sizeof(double) * nNo);
     double* SP = malloc(sizeof(double) * nNo);
@@ -1251,6 +1310,10 @@ static void orc_cgrad_schur(int nNo, int nsd, const int* rowPtr, const int* colP
         if (err < eps) { ls->suc = 1; break; }
         errO = err;
         orc_spmv_sv(nNo, nsd, rowPtr, colPtr, G, P, GP);
+        if (cp) {
+            memcpy(unc, GP, sizeof(double) * (size_t)nNo * nsd);
+            orc_addbcmul(cp, 1, nsd, unc, GP);
+        }
         orc_spmv_vs(nNo, nsd, rowPtr, colPtr, D, GP, DGP);
         orc_spmv_ss(nNo, rowPtr, colPtr, L, P, SP);
         for (int q = 0; q < nNo; ++q) SP[q] = SP[q] + (-1.0) * DGP[q];
@@ -1267,7 +1330,7 @@ static void orc_cgrad_schur(int nNo, int nsd, const int* rowPtr, const int* colP
     ls->callD = wall_s() - time + ls->callD;
     ls->itr = ls->itr + i - 1;
     ls->dB = (errO < EPS) ? 0.0 : 5.0 * log(err / errO);
-    free(X); free(P); free(SP); free(DGP); free(GP);
+    free(X); free(P); free(SP); free(DGP); free(GP); free(unc);
 }
 
 /* GE (svFSILS/GE.f:51-151): A x = B for the leading N x N block of A
@@ -1357,8 +1420,19 @@ int orc_ge_test(int nV, int N, const double* A, double* B) { return orc_ge(nV, N
  * [U_i; 0] with U_i = K^-1 (Rm - G P_i) (GMRES), their images MU/MP under
  * the full matrix, the Gram system A xB = B over all directions so far
  * (GE), and the residual.  ls holds RI, GM and CG in FSILS_lsType order. */
+static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* Val, double* Ri,
+                         orc_ls* ls, const orc_cpl* cp);
+
 int orc_nssolver(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* Val, double* Ri,
                  orc_ls* ls)
+{
+    return nssolver_impl(nNo, dof, rowPtr, colPtr, Val, Ri, ls, NULL);
+}
+
+/* NSSOLVER with the coupled faces: ADDBCMUL(ADD) on K U (NSSOLVER.f:128),
+ * ADD and PRE inside GMRES, PRE inside CGRAD_SCHUR */
+static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* Val, double* Ri,
+                         orc_ls* ls, const orc_cpl* cp)
 {
     if (dof != 4) return -3;
     const int nsd = 3;
@@ -1428,14 +1502,15 @@ int orc_nssolver(int nNo, int dof, const int* rowPtr, const int* colPtr, const d
         iB = 2 * i - 1;
         iBB = 2 * i;
         ls->RI.dB = ls->RI.fNorm;
-        orc_gmres_x(nNo, nsd, rowPtr, colPtr, mK, Rm, VU(i), &ls->GM);
+        orc_gmres_x(nNo, nsd, rowPtr, colPtr, mK, Rm, VU(i), &ls->GM, cp);
         orc_spmv_vs(nNo, nsd, rowPtr, colPtr, mD, VU(i), VP(i));
         for (size_t q = 0; q < n; ++q) VP(i)[q] = Rc[q] - VP(i)[q];
-        orc_cgrad_schur(nNo, nsd, rowPtr, colPtr, Gt, mG, mL, VP(i), &ls->CG);
+        orc_cgrad_schur(nNo, nsd, rowPtr, colPtr, Gt, mG, mL, VP(i), &ls->CG, cp);
         orc_spmv_sv(nNo, nsd, rowPtr, colPtr, mG, VP(i), VMU(iB));
         for (size_t q = 0; q < n * nsd; ++q) VMU(iBB)[q] = Rm[q] - VMU(iB)[q];
-        orc_gmres_x(nNo, nsd, rowPtr, colPtr, mK, VMU(iBB), VU(i), &ls->GM);
+        orc_gmres_x(nNo, nsd, rowPtr, colPtr, mK, VMU(iBB), VU(i), &ls->GM, cp);
         orc_spmv(nNo, nsd, rowPtr, colPtr, mK, VU(i), VMU(iBB));
+        if (cp) orc_addbcmul(cp, 0, nsd, VU(i), VMU(iBB));
         orc_spmv_ss(nNo, rowPtr, colPtr, mL, VP(i), VMP(iB));
         orc_spmv_vs(nNo, nsd, rowPtr, colPtr, mD, VU(i), VMP(iBB));
         for (int k = iB; k <= iBB; ++k) {
@@ -1531,6 +1606,41 @@ int orc_fsils_solve_ls(int nNo, int dof, const int* rowPtr, const int* colPtr, c
     int rc = orc_nssolver(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
     for (size_t q = 0; q < n; ++q) Ri[q] = W[q] * Ri[q];
     free(W); free(W1); free(diagPtr);
+    return rc;
+}
+
+int orc_fsils_solve_cpl(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* dirMask,
+                        int LS_type, int prec, orc_ls* ls, int nC, const int* cOff, const int* cGlob,
+                        const int* cDof, const double* cVal, const double* cRes, double* Ri, double* Val)
+{
+    if (nC <= 0) return orc_fsils_solve_ls(nNo, dof, rowPtr, colPtr, dirMask, LS_type, prec, ls, Ri, Val);
+    /* PRECONDRCS leaves valM unset (PRECOND.f:356-364 are commented out) */
+    if (prec != 701) return -4;
+    int* diagPtr = malloc(sizeof(int) * nNo);
+    if (orc_diagptr(nNo, rowPtr, colPtr, diagPtr) != 0) { free(diagPtr); return -1; }
+    const size_t n = (size_t)nNo * dof, T = (size_t)cOff[nC];
+    double* W = malloc(sizeof(double) * n);
+    double* vm = calloc(T * dof, sizeof(double));
+    double* nS = calloc(nC, sizeof(double));
+    orc_precond_diag(nNo, dof, rowPtr, colPtr, diagPtr, dirMask, Val, Ri, W);
+    /* valM = val W (PRECOND.f:133-140); nS over dof - 1 components (BCPRE) */
+    for (int f = 0; f < nC; ++f) {
+        const int nv = cDof[f] < dof ? cDof[f] : dof;
+        const int ns = cDof[f] < dof - 1 ? cDof[f] : dof - 1;
+        for (int a = cOff[f]; a < cOff[f + 1]; ++a) {
+            for (int i = 0; i < nv; ++i)
+                vm[(size_t)a * dof + i] = cVal[(size_t)a * dof + i] * W[(size_t)cGlob[a] * dof + i];
+            for (int i = 0; i < ns; ++i) nS[f] = nS[f] + vm[(size_t)a * dof + i] * vm[(size_t)a * dof + i];
+        }
+    }
+    orc_cpl c = {nC, dof, cOff, cGlob, cDof, vm, cRes, nS};
+    int rc = 0;
+    if (LS_type == 796) rc = nssolver_impl(nNo, dof, rowPtr, colPtr, Val, Ri, ls, &c);
+    else if (LS_type == 797) gmresv_impl(nNo, dof, rowPtr, colPtr, Val, Ri, &ls->RI, &c);
+    else if (LS_type == 798) orc_cgradv(nNo, dof, rowPtr, colPtr, Val, Ri, &ls->RI); /* CGRADV has no coupled terms */
+    else rc = -2;
+    for (size_t q = 0; q < n; ++q) Ri[q] = W[q] * Ri[q];
+    free(W); free(vm); free(nS); free(diagPtr);
     return rc;
 }
 

@@ -167,6 +167,14 @@ int orc_nssolver(int nNo, int dof, const int* rowPtr, const int* colPtr, const d
                  orc_ls* ls);
 int orc_fsils_solve_ls(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* dirMask,
                        int LS_type, int prec, orc_ls* ls, double* Ri, double* Val);
+/* FSILS_SOLVE with coupled Neumann faces (res(faIn) != 0, SOLVE.f:78-86)
+ * and PRECOND_FSILS: face f = nodes cGlob[cOff[f] .. cOff[f+1]-1] (0-based),
+ * face vector cVal (stride dof, cDof[f] components), coefficient cRes[f].
+ * ADDBCMUL enters GMRESV and NSSOLVER; CGRADV ignores the faces, as the
+ * reference does.  Returns -4 for PRECOND_RCS (valM unset there). */
+int orc_fsils_solve_cpl(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* dirMask,
+                        int LS_type, int prec, orc_ls* ls, int nC, const int* cOff, const int* cGlob,
+                        const int* cDof, const double* cVal, const double* cRes, double* Ri, double* Val);
 /* GE (svFSILS/GE.f:51-151), A column-major with leading dimension nV */
 int orc_ge_test(int nV, int N, const double* A, double* B);
 

@@ -233,20 +233,25 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
                   PRECOND_RCS);
         return SVFSI_ERR_NOTSUP;
     }
-    for (size_t f = 0; f < L.faces.size(); ++f) {
-        const Face& F = L.faces[f];
-        const bool inc = !incL || incL[f] != 0;
-        if (F.foC && inc && F.bGrp == BC_TYPE_Neu && res && res[f] != 0.0) {
-            set_error("coupled Neumann faces (res != 0) are not supported");
-            return SVFSI_ERR_NOTSUP;
-        }
-    }
     if (ls->LS_type != LS_TYPE_GMRES && ls->LS_type != LS_TYPE_CG && ls->LS_type != LS_TYPE_NS) {
         set_error("LS_type %d not supported (NS 796, GMRES 797, CG 798)", ls->LS_type);
         return SVFSI_ERR_NOTSUP;
     }
     if (ls->LS_type == LS_TYPE_NS && (mode != 0 || dof != 4)) {
         set_error("the NS solver is FP64 only and needs dof 4 (velocity 3 + pressure)");
+        return SVFSI_ERR_NOTSUP;
+    }
+    // coupled Neumann faces (SOLVE.f:66-86): the rank-one terms enter GMRES
+    // and NS products (ADDBCMUL); CGRADV has none, so CG solves without them
+    struct CplOff {
+        CplSet& c;
+        ~CplOff() { c.active = false; }
+    } cplOff{L.cpl};
+    SV_TRY(cpl_setup(L, dof, incL, res, s));
+    if (ls->LS_type == LS_TYPE_CG) L.cpl.active = false;
+    if (L.cpl.active && prec == PRECOND_RCS) {
+        set_error("coupled Neumann faces need PRECOND_FSILS: PRECONDRCS leaves the scaled face vectors unset "
+                  "(PRECOND.f:356-364)");
         return SVFSI_ERR_NOTSUP;
     }
     const size_t n = (size_t)L.nNo * dof;
@@ -264,11 +269,12 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
         SV_TRY(precond_rcs(L, dof, Val, R, w.W1.as<double>(), w.W.as<double>(), v32, incL, s));
     } else {
         SV_TRY(precond_diag(L, dof, Val, R, w.W.as<double>(), v32, incL, s));
+        SV_TRY(cpl_scale(L, w.W.as<double>(), s));
     }
     int rc;
     if (mode == 1) rc = mixed_solve(L, ls->RI, dof, Val, v32, R, s);
     else if (ls->LS_type == LS_TYPE_NS) rc = ns_solve(L, *ls, dof, Val, R, s);
-    else if (ls->LS_type == LS_TYPE_GMRES) rc = gmres_solve(L, ls->RI, dof, Val, R, s, false);
+    else if (ls->LS_type == LS_TYPE_GMRES) rc = gmres_solve(L, ls->RI, dof, Val, R, s, false, L.cpl.active ? 1 : 0);
     else rc = cg_solve(L, ls->RI, dof, Val, R, s);
     if (rc != SVFSI_OK) return rc;
     SV_TRY(launch_scale_vec(n, w.W.as<double>(), R, s));

@@ -126,6 +126,22 @@ struct Work {
     int maskVersion = -1;
 };
 
+// Coupled Neumann faces of the solve in progress (BC_TYPE_Neu faces with
+// res(faIn) != 0, svFSILS/SOLVE.f:78-86), concatenated: face f holds entries
+// off[f] .. off[f+1]-1 of glob (0-based node), val (the face vector, stride
+// dof, fdof[f] components) and vm (val scaled by the preconditioner,
+// PRECOND.f:133-140).  coef[f] = res, coef[nC+f] = -res/(1 + res nS) (BCPRE).
+struct CplSet {
+    int nC = 0, T = 0, dof = 0;
+    bool disjoint = true;  // no node on two faces: one launch, in place
+    bool active = false;   // set for the duration of a solve
+    unsigned gen = 0;      // bumped when the device layout changes
+    std::vector<int> ids;  // lhs face of each coupled face (layout key)
+    int keyDof = 0, keyVersion = -1;
+    DBuf off, glob, fdof, val, vm, coef, tmp;
+    std::vector<double> hres;
+};
+
 struct Lhs {
     int gnNo = 0, nNo = 0, nnz = 0, nFaces = 0;
     DBuf rowPtr, colPtr, diagPtr, blockRow, trans;
@@ -133,6 +149,7 @@ struct Lhs {
     std::vector<Face> faces;
     int facesVersion = 0;
     bool symmetric = false; // every block (i,j) has a (j,i) partner: trans[] valid
+    CplSet cpl;
     Work ws;
 };
 
@@ -249,12 +266,23 @@ int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s);
 
 int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
              double* R, hipStream_t s);
+// cpl: coupled faces in the operator, 0 none, 1 ADDBCMUL ADD (GMRESV),
+// 2 ADD then PRE (the NS solver's GMRES; the caller applies PRE to R)
 int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
-                double* R, hipStream_t s, bool transpose);
+                double* R, hipStream_t s, bool transpose, int cpl = 0);
 int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
                 float* K32, double* R, hipStream_t s);
 
 double wall_seconds();
+// Coupled Neumann faces: cpl_setup selects the faces of this solve (none:
+// L.cpl.active = false) and uploads their layout; cpl_scale forms vm = val W
+// and the BCPRE coefficients after the diagonal preconditioner;
+// cpl_apply is ADDBCMUL (svFSILS/ADDBCMUL.f:53-114): Y += coef v (v . X) per
+// face, op 0 ADD, 1 PRE, on vectors of vd components per node.  X may equal
+// Y (PRE in place).  stop: optional device flag that skips the launch.
+int cpl_setup(Lhs& L, int dof, const int* incL, const double* res, hipStream_t s);
+int cpl_scale(Lhs& L, const double* W, hipStream_t s);
+int cpl_apply(Lhs& L, int op, int vd, const double* X, double* Y, hipStream_t s, const int* stop = nullptr);
 // NSSOLVER (svFSILS/NSSOLVER.f:52-383) on the scaled system, dof 4: Ri in
 // (right-hand side), out (solution); ls.RI, ls.GM, ls.CG, Resm, Resc set.
 int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, hipStream_t s);

@@ -1051,9 +1051,13 @@ __global__ void k_sum_to(int m, const double* __restrict__ partial, double* __re
 template <typename BT> struct GmOp;
 
 template <> struct GmOp<double> {
-    const Lhs* L; int dof; const double* K; bool transpose;
+    Lhs* L; int dof; const double* K; bool transpose;
+    int cpl = 0;  // coupled faces: 0 none, 1 ADDBCMUL ADD, 2 ADD then PRE (GMRES.f:109-115)
     int apply(const double* x, double* y, hipStream_t s, const int* stop) const {
-        return transpose ? launch_spmv_t(*L, dof, K, x, y, s, stop) : launch_spmv(*L, dof, K, x, y, s, stop);
+        SV_TRY(transpose ? launch_spmv_t(*L, dof, K, x, y, s, stop) : launch_spmv(*L, dof, K, x, y, s, stop));
+        if (cpl >= 1) SV_TRY(cpl_apply(*L, 0, dof, x, y, s, stop));
+        if (cpl >= 2) SV_TRY(cpl_apply(*L, 1, dof, y, y, s, stop));
+        return SVFSI_OK;
     }
 };
 template <> struct GmOp<float> {
@@ -1066,6 +1070,16 @@ template <> struct GmOp<float> {
 
 static bool op_transposed(const GmOp<double>& op) { return op.transpose; }
 static bool op_transposed(const GmOp<float>&) { return false; }
+static int op_cpl(const GmOp<double>& op) { return op.cpl; }
+static int op_cpl(const GmOp<float>&) { return 0; }
+
+// the coupled-face launches baked into a captured graph
+static void cpl_sig(const Lhs& L, std::vector<uintptr_t>& sig)
+{
+    const CplSet& c = L.cpl;
+    sig.insert(sig.end(), {(uintptr_t)c.active, (uintptr_t)c.gen, (uintptr_t)c.vm.p, (uintptr_t)c.coef.p,
+                           (uintptr_t)c.glob.p, (uintptr_t)c.off.p, (uintptr_t)c.fdof.p, (uintptr_t)c.tmp.p});
+}
 
 // Restarted GMRES(sD) on n = nNo*dof unknowns.  R: RHS in, solution out
 // (GMRESV overwrites R with X, GMRES.f:383).  Fills suc/itr/iNorm/fNorm/dB.
@@ -1125,10 +1139,11 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
     const int chunk = 8;
     const bool useGraphs = graphs_enabled();
     if (useGraphs) {
-        const std::vector<uintptr_t> sig = {
+        std::vector<uintptr_t> sig = {
             (uintptr_t)n, (uintptr_t)dof, (uintptr_t)sD, (uintptr_t)op.K, (uintptr_t)sizeof(BT),
             (uintptr_t)op_transposed(op), (uintptr_t)basis, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.hess.p,
-            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)L.trans.p};
+            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)L.trans.p, (uintptr_t)op_cpl(op)};
+        if (op_cpl(op)) cpl_sig(L, sig);
         if (sig != w.gmGraphs.sig) {
             w.gmGraphs.clear();
             w.gmGraphs.sig = sig;
@@ -1199,9 +1214,9 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
 }
 
 int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s,
-                bool transpose)
+                bool transpose, int cpl)
 {
-    GmOp<double> op{&L, dof, K, transpose};
+    GmOp<double> op{&L, dof, K, transpose, cpl};
     return gmres_core<double>(L, op, dof, R, L.ws.basis, L.ws.X, ls, s);
 }
 
@@ -1534,12 +1549,13 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
         double tinv = tin;
         uintptr_t tbits;
         std::memcpy(&tbits, &tinv, sizeof(tbits));
-        const std::vector<uintptr_t> sig = {
+        std::vector<uintptr_t> sig = {
             (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)K32,
             (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
             (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
             (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv,
             (uintptr_t)cgsPasses};
+        cpl_sig(L, sig);
         if (sig != w.graphs.sig) {
             w.graphs.clear();
             w.graphs.sig = sig;
@@ -1550,6 +1566,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
         ls.dB = ls.fNorm;
         ls.itr += 1;
         SV_TRY(launch_spmv(L, dof, K, x, Q, s));
+        SV_TRY(cpl_apply(L, 0, dof, x, Q, s));
         hipLaunchKernelGGL(k_gm_resid<double>, dim3(gv), dim3(256), 0, s, n, (const double*)R, Q, part, st, A);
         hipLaunchKernelGGL(k_gm_div<double>, dim3(gv), dim3(256), 0, s, n, Q, A);
         outerMv += 1;
@@ -1597,6 +1614,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
                 }
                 hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, cs, n, (const float*)zi, t64);
                 SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, cs, &st->stop));
+                SV_TRY(cpl_apply(L, 0, dof, t64, Q + (size_t)i * n, cs, &st->stop));
                 // outer Arnoldi: classical Gram-Schmidt, two passes (CGS2) by
                 // default; SVFSI_OUTER_CGS=1 runs the reference's single pass
                 for (int pp = 1; pp <= cgsPasses; ++pp) {

@@ -247,9 +247,13 @@ int ns_gmres(NsCtx& c, const double* K, const double* R, double* X, FSILS_subLsT
 {
     const size_t len = c.n * kNsd;
     SV_HIP(hipMemcpyAsync(X, R, len * sizeof(double), hipMemcpyDeviceToDevice, c.s));
+    // coupled faces: GMRES on PRE (K + ADD) X = PRE R, PRE being linear
+    // (GMRES.f:79-115 applies PRE to each residual and product)
+    const bool cpl = c.L->cpl.active;
+    if (cpl) SV_TRY(cpl_apply(*c.L, 1, kNsd, X, X, c.s));
     FSILS_subLsType t = gm;
     t.itr = 0;
-    SV_TRY(gmres_solve(*c.L, t, kNsd, K, X, c.s, false));
+    SV_TRY(gmres_solve(*c.L, t, kNsd, K, X, c.s, false, cpl ? 2 : 0));
     if (t.iNorm <= gm.absTol) {
         SV_HIP(hipMemsetAsync(X, 0, len * sizeof(double), c.s));
         gm.callD += 2.220446049250313e-16;
@@ -289,6 +293,7 @@ int ns_cg_schur(NsCtx& c, const double* Gt, const double* G, const double* L, do
         }
         errO = err;
         c.sv(G, P, GP);
+        SV_TRY(cpl_apply(*c.L, 1, kNsd, GP, GP, c.s));  // CGRAD.f:89-92
         c.vs(Gt, GP, DGP);
         c.ss(L, P, SP);
         c.axpy(SP, 1.0, -1.0, DGP, n);
@@ -395,6 +400,7 @@ int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, h
                            (const double*)VMU(iB), VMU(iBB));
         SV_TRY(ns_gmres(c, mK, VMU(iBB), VU(i), ls.GM));
         SV_TRY(launch_spmv(L, kNsd, mK, VU(i), VMU(iBB), s));
+        SV_TRY(cpl_apply(L, 0, kNsd, VU(i), VMU(iBB), s));  // NSSOLVER.f:128
         c.ss(mL, VP(i), VMP(iB));
         c.vs(mD, VU(i), VMP(iBB));
         for (int k = iB; k <= iBB; ++k) {

@@ -41,6 +41,8 @@ class Problem:
     dir_faces: list            # [(nodes 0-based, face dof)] homogeneous Dirichlet masks
     trac_faces: list = field(default_factory=list)   # [(fIEN, fEl, h(3))]
     bflow_faces: list = field(default_factory=list)  # [(fIEN, fEl)]
+    # coupled Neumann (resistance) faces: [(nodes 0-based, face dof, val (nNo_f, face dof), res)]
+    res_faces: list = field(default_factory=list)
     ls: dict = field(default_factory=dict)
     prop_el: np.ndarray | None = None
     quad: int = 0
@@ -61,6 +63,28 @@ def _faces_where(m: meshmod.TetMesh, pred_nodes: np.ndarray):
     f, e = meshmod.boundary_faces(m.ien)
     sel = pred_nodes[f].all(axis=1)
     return f[sel], e[sel]
+
+
+def face_normal_integral(m: meshmod.TetMesh, fien: np.ndarray, fel: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Integral of N_a n over tri3 boundary faces, n the outward unit normal
+    (the face vector of a resistance face, svFSI/BAFINI.f:514-527): each
+    triangle adds area/3 * n to its three nodes.  Returns (nodes 0-based,
+    val (nNo_f, 3))."""
+    x = m.x
+    a, b, c = x[fien[:, 0]], x[fien[:, 1]], x[fien[:, 2]]
+    n = 0.5 * np.cross(b - a, c - a)  # area-weighted normal
+    # orient away from the element's node that is not on the face
+    el = m.ien[fel, :4]
+    inner = np.array([[v for v in el[k] if v not in fien[k]][0] for k in range(len(fel))])
+    flip = np.einsum("ij,ij->i", n, x[inner] - a) > 0
+    n[flip] *= -1.0
+    nodes = np.unique(fien)
+    pos = {int(v): k for k, v in enumerate(nodes)}
+    val = np.zeros((len(nodes), 3))
+    for k in range(len(fien)):
+        for v in fien[k]:
+            val[pos[int(v)]] += n[k] / 3.0
+    return nodes.astype(np.int32), val
 
 
 def lelas_cube(n: int = 20, E: float = 1e6, nu: float = 0.3, rho: float = 1.0, dt: float = 1.0,

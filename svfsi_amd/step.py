@@ -11,7 +11,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import fsils
-from ._lib import BC_TYPE_Dir, FSILS_lsType
+from ._lib import BC_TYPE_Dir, BC_TYPE_Neu, FSILS_lsType
 from .assembly import DeviceMesh
 from .problems import Problem
 
@@ -24,10 +24,19 @@ class NewtonStep:
         m = prob.mesh
         self.rowPtr, self.colPtr = fsils.lhsa(m.nNo, m.ien)
         self.nnz = int(self.colPtr.shape[0])
+        nDir = len(prob.dir_faces)
         self.lhs = fsils.fsils_lhs_create(m.nNo, self.nnz, self.rowPtr, self.colPtr,
-                                          nFaces=len(prob.dir_faces))
+                                          nFaces=nDir + len(prob.res_faces))
         for i, (nodes, fdof) in enumerate(prob.dir_faces):
             fsils.fsils_bc_create(self.lhs, i + 1, len(nodes), fdof, BC_TYPE_Dir, np.asarray(nodes) + 1)
+        # resistance faces (svFSI/BAFINI.f:512-533): Neumann faces whose face
+        # vector couples through res in FSILS_SOLVE
+        for i, (nodes, fdof, val, _) in enumerate(prob.res_faces):
+            fsils.fsils_bc_create(self.lhs, nDir + i + 1, len(nodes), fdof, BC_TYPE_Neu, np.asarray(nodes) + 1,
+                                  np.asarray(val, dtype=np.float64).reshape(len(nodes), fdof))
+        self.res = None
+        if prob.res_faces:
+            self.res = np.array([0.0] * nDir + [float(r) for _, _, _, r in prob.res_faces])
         self.dmesh = DeviceMesh(self.lhs, m)
         self.trac_ids = [(self.dmesh.add_face(f, e), h) for f, e, h in prob.trac_faces]
         self.bflow_ids = [self.dmesh.add_face(f, e) for f, e in prob.bflow_faces]
@@ -69,7 +78,7 @@ class NewtonStep:
     def solve(self, mixed: bool = False) -> FSILS_lsType:
         ls = self.make_ls()
         prec = self.prob.ls.get("prec", fsils.PRECOND_FSILS)
-        fsils.fsils_solve(self.lhs, ls, self.prob.dof, self.R, self.Val, prec=prec, mixed=mixed)
+        fsils.fsils_solve(self.lhs, ls, self.prob.dof, self.R, self.Val, prec=prec, res=self.res, mixed=mixed)
         return ls
 
     def step(self, mixed: bool = False) -> FSILS_lsType:
