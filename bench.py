@@ -79,6 +79,9 @@ def main() -> int:
                     help="only run a device-to-device copy of MIB MiB five times (calibrates PMC byte counters)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="skip the FP64 leg (for counter-collection runs, which serialise every dispatch)")
+    ap.add_argument("--ls", choices=["gmres", "ns"], default="gmres",
+                    help="ns: the fluid workloads with the NS solver (NSSOLVER, reference defaults) in place of "
+                         "GMRES; a separate measurement line (FP64 only, as the reference's NS solver)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -126,6 +129,11 @@ def main() -> int:
         wl = f"LELAS3D tet4 cube {n}^3 x 6, one assembly + CG to 1e-8"
     if args.mitr:
         prob.ls = dict(prob.ls, mItr=args.mitr)
+    if args.ls == "ns":
+        if prob.dof != 4:
+            print("bench.py: --ls ns needs a fluid workload (dof 4)", file=sys.stderr)
+            return 2
+        return bench_ns(args, prob, t0)
     st = NewtonStep(prob)
     t_setup = time.time() - t0
     m = prob.mesh
@@ -388,6 +396,130 @@ def main() -> int:
     }
     if cpu is not None:
         out["speedup_vs_cpu_mixed"] = round(cpu["value"] / mix_ms, 1)
+    print(json.dumps(out))
+    return 0
+
+
+def host_cores() -> int:
+    """Host cores of this process's CPU share (one oracle thread per core,
+    as the reference runs one MPI rank per core)."""
+    cores = len(os.sched_getaffinity(0))
+    return max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+
+
+def bench_ns(args, prob, t0) -> int:
+    """Assembly + FSILS_SOLVE with the NS solver (NSSOLVER.f, LS.f:69-95
+    defaults: RI 0.4 / 10, GM 1e-2 / 2 restarts of 100, CG 0.2 / 500) per
+    Newton step, FP64 on the GPU, beside the oracle's NSSOLVER on the host
+    cores.  Roofline: the Schur-complement product (k_cgs_sp), the kernel
+    that dominates the solve."""
+    import ctypes as C
+
+    import torch
+
+    from svfsi_amd._lib import LS_TYPE_NS, lib
+    from svfsi_amd.step import NewtonStep
+
+    prob.ls = dict(LS_type=LS_TYPE_NS, relTol=0.4, absTol=1e-10, mItr=10)
+    st = NewtonStep(prob)
+    m = prob.mesh
+    nnzb = st.nnz
+    log(f"setup {time.time() - t0:.1f}s nNo={m.nNo} nEl={m.nEl} nnzb={nnzb} colors={st.ncolors}")
+    for w in range(args.warmup):
+        ls = st.step()
+        log(f"warmup {w} ns: RI.itr {ls.RI.itr} GM.itr {ls.GM.itr} CG.itr {ls.CG.itr} suc {ls.RI.suc}")
+    torch.cuda.synchronize()
+    L = lib()
+    L.svfsi_prof_enable(0)
+    a = time.perf_counter()
+    lss = [st.step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    ns_ms = (time.perf_counter() - a) * 1e3 / args.steps
+    ls = lss[-1]
+    a = time.perf_counter()
+    for _ in range(args.steps):
+        st.assemble()
+    torch.cuda.synchronize()
+    asm_ms = (time.perf_counter() - a) * 1e3 / args.steps
+    log(f"ns step {ns_ms:.3f} ms (assembly {asm_ms:.3f} ms): RI.itr {ls.RI.itr} GM.itr {ls.GM.itr} "
+        f"CG.itr {ls.CG.itr}")
+    # profiled step: dispatch-attached events on the Schur product
+    L.svfsi_prof_reset()
+    L.svfsi_prof_enable(1)
+    lsp = st.step()
+    torch.cuda.synchronize()
+    L.svfsi_prof_enable(0)
+    cnt, tot, byt = C.c_longlong(), C.c_double(), C.c_double()
+    L.svfsi_prof_get(4, C.byref(cnt), C.byref(tot))
+    L.svfsi_prof_get_bytes(4, C.byref(byt))
+    full = max(1, min(cnt.value, lsp.CG.itr))  # launches queued after convergence exit at once
+    bpl = byt.value / max(cnt.value, 1)
+    gbs = bpl * full / (tot.value * 1e-3) / 1e9 if cnt.value else None
+    kname = "k_cgs_sp"
+    roof = None if gbs is None else {
+        "kernel": f"{kname} (Schur-complement product SP = L P - Gt GP with its partial dot)", "bound": "hbm",
+        "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(nnzb, kname), "bytes_per_launch": round(bpl),
+        "avg_launch_us": round(tot.value / full * 1e3, 2), "launches": cnt.value, "full_work_launches": full,
+        "timing": "events attached to each dispatch (hipExtLaunchKernelGGL)"}
+    cpu = None
+    acc = {}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        st.assemble()
+        torch.cuda.synchronize()
+        V0 = st.Val.cpu().numpy().copy()
+        R0 = st.R.cpu().numpy().copy()
+        st.solve()
+        torch.cuda.synchronize()
+        xg = st.R.cpu().numpy()
+        cores = host_cores()
+        rp, cp = st.rowPtr - 1, st.colPtr - 1
+        log("cpu leg: oracle assembly and NSSOLVER")
+        c0 = time.perf_counter()
+        Vo, Ro = orc.assemble(prob, rp, cp)
+        asm1 = time.perf_counter() - c0
+        asm_par = orc.assemble_ranks_seconds(prob, rp, cp, cores)
+        t_solve = {}
+        for nt in (1, cores):
+            orc.set_threads(nt)
+            c0 = time.perf_counter()
+            xo, _, lo = orc.fsils_solve_ns(rp, cp, 4, prob.dir_mask(), prob.ls, R0, V0)
+            t_solve[nt] = time.perf_counter() - c0
+        orc.set_threads(1)
+        acc["assembly_rel_err_vs_cpu"] = float(np.abs(V0 - Vo).max() / np.abs(Vo).max())
+        acc["solution_rel_diff_vs_cpu"] = float(np.abs(xg - xo).max() / np.abs(xo).max())
+        acc["cpu_itr"] = [lo.RI.itr, lo.GM.itr, lo.CG.itr]
+        Vs, Rs, W = orc.precond_diag(rp, cp, 4, V0, R0, prob.dir_mask())
+        for name, x in (("gpu", xg), ("cpu", xo)):
+            y = np.where(W != 0, x / np.where(W != 0, W, 1.0), 0.0)
+            r = Rs - orc.spmv(rp, cp, 4, Vs, y)
+            acc[f"scaled_rel_residual_{name}"] = float(np.linalg.norm(r) / np.linalg.norm(Rs))
+        cpu_ms = (asm_par + t_solve[cores]) * 1e3
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms/step", "cores": cores, "kind": "port",
+               "sample": (f"C oracle on {cores} host cores (OpenMP threads standing in for MPI ranks): full FP64 "
+                          f"assembly split over {cores} ranks ({round(asm_par * 1e3)} ms) + the whole NSSOLVER "
+                          f"solve with PRECONDDIAG ({round(t_solve[cores] * 1e3)} ms)"),
+               "single_core_value": round((asm1 + t_solve[1]) * 1e3, 1),
+               "single_core_sample": (f"the same on one core: assembly {round(asm1 * 1e3)} ms, solve "
+                                      f"{round(t_solve[1] * 1e3)} ms")}
+    wl = ("FLUID3D VMS tet4 cylinder r=1 L=10 (h=%g), one Newton step, NSSOLVER with the reference defaults "
+          "(RI 0.4/10, GM 1e-2/2x100, CG 0.2/500) and diagonal scaling" % (0.042 if args.workload == "fluid-large"
+                                                                            else args.h))
+    out = {
+        "metric": "Assembly + NS-solver time per Newton step (FP64)",
+        "value": round(ns_ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ns_ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl, "nNo": m.nNo, "nEl": m.nEl, "nnzb": nnzb, "dof": 4, "colors": st.ncolors},
+        "assembly_ms": round(asm_ms, 3),
+        "ns_itr": {"RI": ls.RI.itr, "GM": ls.GM.itr, "CG": ls.CG.itr, "suc": int(ls.RI.suc),
+                   "fNorm_over_iNorm": ls.RI.fNorm / ls.RI.iNorm, "Resm": ls.Resm, "Resc": ls.Resc},
+        "accuracy": acc, "roofline": roof, "cpu_baseline": cpu,
+    }
+    if cpu is not None:
+        out["speedup_vs_cpu"] = round(cpu["value"] / ns_ms, 1)
     print(json.dumps(out))
     return 0
 

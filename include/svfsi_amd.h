@@ -323,7 +323,8 @@ int svfsi_prof_enable(int on);
 /* kernel: 0 = FP64 SpMV, 1 = FP32 SpMV (including the inner solver's SpMV
  * fused with its multi-dot, and the polynomial preconditioner's steps),
  * 2 = STRUCT element kernels (one per colour), 3 = STRUCT material-point
- * kernel.  Returns launches and total ms. */
+ * kernel, 4 = the NS solver's Schur-complement product SP = L P - Gt GP
+ * (with its partial dot).  Returns launches and total ms. */
 int svfsi_prof_get(int kernel, long long* count, double* total_ms);
 /* Algorithmic bytes of the profiled launches of `kernel` (values, indices,
  * row pointers, x and y once; for the fused inner step also the basis

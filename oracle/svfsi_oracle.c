@@ -1171,6 +1171,7 @@ static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, 
             ls->itr++;
             orc_spmv(nNo, dof, rowPtr, colPtr, K, X, u);
             if (cp) orc_addbcmul(cp, 0, dof, X, u);
+            ORC_PAR
             for (size_t q = 0; q < n; ++q) u[q] = R[q] - u[q];
         }
         if (cp) {
@@ -1190,6 +1191,7 @@ static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, 
             eps = fmax(ls->absTol, ls->relTol * eps);
         }
         ls->dB = ls->fNorm;
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) u[q] = u[q] / err[0];
         int i;
         for (i = 1; i <= sD; ++i) {
@@ -1206,11 +1208,13 @@ static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, 
             for (int j = 1; j <= i; ++j) {
                 double hj = H(j - 1, i - 1);
                 const double* uj = u + n * (j - 1);
+                ORC_PAR
                 for (size_t q = 0; q < n; ++q) un[q] += -hj * uj[q];
                 H(i, i - 1) = H(i, i - 1) - hj * hj;
             }
             H(i, i - 1) = sqrt(fabs(H(i, i - 1)));
             double rr = 1.0 / H(i, i - 1);
+            ORC_PAR
             for (size_t q = 0; q < n; ++q) un[q] *= rr;
             for (int j = 1; j <= i - 1; ++j) {
                 double tmp = c[j - 1] * H(j - 1, i - 1) + s[j - 1] * H(j, i - 1);
@@ -1234,6 +1238,7 @@ static void orc_gmres_x(int nNo, int dof, const int* rowPtr, const int* colPtr, 
         }
         for (int j = 1; j <= i; ++j) {
             const double* uj = u + n * (j - 1);
+            ORC_PAR
             for (size_t q = 0; q < n; ++q) X[q] = X[q] + y[j - 1] * uj[q];
         }
         ls->fNorm = fabs(err[i]);
@@ -1253,6 +1258,7 @@ done:
 static void orc_spmv_vs(int nNo, int nsd, const int* rowPtr, const int* colPtr, const double* K,
                         const double* U, double* KU)
 {
+    ORC_PAR
     for (int i = 0; i < nNo; ++i) {
         double a = 0.0;
         for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
@@ -1269,6 +1275,7 @@ static void orc_spmv_vs(int nNo, int nsd, const int* rowPtr, const int* colPtr, 
 static void orc_spmv_sv(int nNo, int nsd, const int* rowPtr, const int* colPtr, const double* K,
                         const double* U, double* KU)
 {
+    ORC_PAR
     for (int i = 0; i < nNo; ++i) {
         double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j)
@@ -1280,6 +1287,7 @@ static void orc_spmv_sv(int nNo, int nsd, const int* rowPtr, const int* colPtr, 
 static void orc_spmv_ss(int nNo, const int* rowPtr, const int* colPtr, const double* K, const double* U,
                         double* KU)
 {
+    ORC_PAR
     for (int i = 0; i < nNo; ++i) {
         double a = 0.0;
         for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) a = a + K[j] * U[colPtr[j]];
@@ -1316,13 +1324,18 @@ static void orc_cgrad_schur(int nNo, int nsd, const int* rowPtr, const int* colP
         }
         orc_spmv_vs(nNo, nsd, rowPtr, colPtr, D, GP, DGP);
         orc_spmv_ss(nNo, rowPtr, colPtr, L, P, SP);
+        ORC_PAR
         for (int q = 0; q < nNo; ++q) SP[q] = SP[q] + (-1.0) * DGP[q];
         double alpha = errO / orc_dot(nNo, P, SP);
+        ORC_PAR
         for (int q = 0; q < nNo; ++q) X[q] = X[q] + alpha * P[q];
+        ORC_PAR
         for (int q = 0; q < nNo; ++q) R[q] = R[q] + (-alpha) * SP[q];
         err = orc_norm(nNo, R);
         err = err * err;
+        ORC_PAR
         for (int q = 0; q < nNo; ++q) P[q] = P[q] + (errO / err) * R[q];
+        ORC_PAR
         for (int q = 0; q < nNo; ++q) P[q] = P[q] * (err / errO);
     }
     memcpy(R, X, sizeof(double) * nNo);
@@ -1443,6 +1456,7 @@ static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr,
     double* mD = malloc(sizeof(double) * nnz * 3);
     double* mL = malloc(sizeof(double) * nnz);
     double* Gt = malloc(sizeof(double) * nnz * 3);
+    ORC_PAR
     for (size_t k = 0; k < nnz; ++k) {
         const double* t = Val + k * 16;
         const int kk[9] = {0, 1, 2, 4, 5, 6, 8, 9, 10};
@@ -1455,6 +1469,7 @@ static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr,
         mD[k * 3 + 2] = t[14];
         mL[k] = t[15];
     }
+    ORC_PAR
     for (int i = 0; i < nNo; ++i)
         for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
             const int k = colPtr[j];
@@ -1476,6 +1491,7 @@ static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr,
     double* B = calloc(nB, sizeof(double));
     double* xB = calloc(nB, sizeof(double));
     double* oldxB = calloc(nB, sizeof(double));
+    ORC_PAR
     for (size_t a = 0; a < n; ++a) {
         for (int m = 0; m < nsd; ++m) Rmi[a * nsd + m] = Ri[a * dof + m];
         Rci[a] = Ri[a * dof + nsd];
@@ -1504,9 +1520,11 @@ static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr,
         ls->RI.dB = ls->RI.fNorm;
         orc_gmres_x(nNo, nsd, rowPtr, colPtr, mK, Rm, VU(i), &ls->GM, cp);
         orc_spmv_vs(nNo, nsd, rowPtr, colPtr, mD, VU(i), VP(i));
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) VP(i)[q] = Rc[q] - VP(i)[q];
         orc_cgrad_schur(nNo, nsd, rowPtr, colPtr, Gt, mG, mL, VP(i), &ls->CG, cp);
         orc_spmv_sv(nNo, nsd, rowPtr, colPtr, mG, VP(i), VMU(iB));
+        ORC_PAR
         for (size_t q = 0; q < n * nsd; ++q) VMU(iBB)[q] = Rm[q] - VMU(iB)[q];
         orc_gmres_x(nNo, nsd, rowPtr, colPtr, mK, VMU(iBB), VU(i), &ls->GM, cp);
         orc_spmv(nNo, nsd, rowPtr, colPtr, mK, VU(i), VMU(iBB));
@@ -1539,10 +1557,13 @@ static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr,
             ls->RI.suc = 1;
             break;
         }
+        ORC_PAR
         for (size_t q = 0; q < n * nsd; ++q) Rm[q] = Rmi[q] - xB[0] * VMU(1)[q];
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) Rc[q] = Rci[q] - xB[0] * VMP(1)[q];
         for (int j = 2; j <= iBB; ++j) {
             for (size_t q = 0; q < n * nsd; ++q) Rm[q] = Rm[q] - xB[j - 1] * VMU(j)[q];
+            ORC_PAR
             for (size_t q = 0; q < n; ++q) Rc[q] = Rc[q] - xB[j - 1] * VMP(j)[q];
         }
     }
@@ -1550,17 +1571,22 @@ static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr,
         ls->RI.itr = ls->RI.mItr;
     } else {
         ls->RI.itr = i;
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) Rc[q] = Rci[q] - xB[0] * VMP(1)[q];
         for (int j = 2; j <= iBB; ++j)
             for (size_t q = 0; q < n; ++q) Rc[q] = Rc[q] - xB[j - 1] * VMP(j)[q];
     }
     ls->Resc = (int)lround(100.0 * pow(orc_norm((int)n, Rc), 2.0) / ls->RI.fNorm);
     ls->Resm = 100 - ls->Resc;
+    ORC_PAR
     for (size_t q = 0; q < n * nsd; ++q) Rmi[q] = xB[1] * VU(1)[q];
+    ORC_PAR
     for (size_t q = 0; q < n; ++q) Rci[q] = xB[0] * VP(1)[q];
     for (int k = 2; k <= ls->RI.itr; ++k) {
         const int b1 = 2 * k - 1, b2 = 2 * k;
+        ORC_PAR
         for (size_t q = 0; q < n * nsd; ++q) Rmi[q] = Rmi[q] + xB[b2 - 1] * VU(k)[q];
+        ORC_PAR
         for (size_t q = 0; q < n; ++q) Rci[q] = Rci[q] + xB[b1 - 1] * VP(k)[q];
     }
     ls->RI.callD = wall_s() - ls->RI.callD;
@@ -1572,6 +1598,7 @@ static int nssolver_impl(int nNo, int dof, const int* rowPtr, const int* colPtr,
         ls->RI.fNorm = 0.0;
     }
     ls->RI.fNorm = sqrt(ls->RI.fNorm);
+    ORC_PAR
     for (size_t a = 0; a < n; ++a) {
         for (int m = 0; m < nsd; ++m) Ri[a * dof + m] = Rmi[a * nsd + m];
         Ri[a * dof + nsd] = Rci[a];

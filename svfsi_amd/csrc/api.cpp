@@ -874,7 +874,7 @@ int svfsi_prof_enable(int on)
 
 int svfsi_prof_get(int kernel, long long* count, double* total_ms)
 {
-    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..3");
+    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..4");
     prof_resolve();
     if (count) *count = prof().count[kernel];
     if (total_ms) *total_ms = prof().ms[kernel];
@@ -883,7 +883,7 @@ int svfsi_prof_get(int kernel, long long* count, double* total_ms)
 
 int svfsi_prof_get_bytes(int kernel, double* bytes)
 {
-    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..3");
+    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..4");
     prof_resolve();
     if (bytes) *bytes = prof().bytes[kernel];
     return SVFSI_OK;

@@ -111,6 +111,7 @@ struct Work {
     GraphCache graphs;   // mixed solve: one graph per outer step
     GraphCache gmGraphs; // FP64 GMRES: one graph per chunk of Arnoldi steps
     GraphCache cgGraphs; // CG: one graph for a chunk of iterations
+    GraphCache csGraphs; // NS solver's Schur-complement CG: likewise
     PinnedPoll poll;
     DBuf W, X, R0, tmp, basis, basis32, Z32, val32, x32, y32;
     DBuf partial, state, hess;
@@ -195,11 +196,11 @@ struct ProfPending {
 struct Prof {
     bool on = false;
     // slots: 0 FP64 SpMV, 1 FP32 SpMV (inner solver), 2 STRUCT element
-    // kernels, 3 STRUCT material-point kernel
-    static constexpr int kSlots = 4;
-    long long count[kSlots] = {0, 0, 0, 0};
-    double ms[kSlots] = {0.0, 0.0, 0.0, 0.0};
-    double bytes[kSlots] = {0.0, 0.0, 0.0, 0.0}; // algorithmic bytes (slot 2: MFMA flops) of the timed launches
+    // kernels, 3 STRUCT material-point kernel, 4 NS Schur-complement product
+    static constexpr int kSlots = 5;
+    long long count[kSlots] = {};
+    double ms[kSlots] = {};
+    double bytes[kSlots] = {}; // algorithmic bytes (slot 2: MFMA flops) of the timed launches
     std::vector<ProfPending> pending;
 };
 Prof& prof();
@@ -266,6 +267,12 @@ int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s);
 
 int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
              double* R, hipStream_t s);
+// CGRAD_SCHUR (svFSILS/CGRAD.f:51-123) on the device: (L - Gt G) x = R with
+// G (3, nnz), Gt (3, nnz), Lm (nnz); R in: right-hand side, out: solution.
+// X, P, SP (nNo) and GP (3, nNo) are work vectors; ls.itr and ls.callD
+// accumulate, as the reference's do.
+int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* G, const double* Lm, double* R,
+                   double* X, double* P, double* SP, double* GP, hipStream_t s);
 // cpl: coupled faces in the operator, 0 none, 1 ADDBCMUL ADD (GMRESV),
 // 2 ADD then PRE (the NS solver's GMRES; the caller applies PRE to R)
 int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,

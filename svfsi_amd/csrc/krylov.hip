@@ -309,6 +309,14 @@ static int graph_run(GraphCache& gc, int key, hipStream_t s, F&& enqueue)
     return SVFSI_OK;
 }
 
+// the coupled-face launches baked into a captured graph
+static void cpl_sig(const Lhs& L, std::vector<uintptr_t>& sig)
+{
+    const CplSet& c = L.cpl;
+    sig.insert(sig.end(), {(uintptr_t)c.active, (uintptr_t)c.gen, (uintptr_t)c.vm.p, (uintptr_t)c.coef.p,
+                           (uintptr_t)c.glob.p, (uintptr_t)c.off.p, (uintptr_t)c.fdof.p, (uintptr_t)c.tmp.p});
+}
+
 // Graph replay is on unless SVFSI_GRAPHS=0; it is off while SpMV profiling
 // is on (profiled launches carry events that a capture cannot hold).
 static bool graphs_enabled()
@@ -431,6 +439,216 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
     ls.cM = hs.itr;
     ls.fNorm = std::sqrt(hs.err);
     ls.callD = wall_seconds() - ls.callD;
+    ls.dB = (hs.errO < 2.220446049250313e-16) ? 0.0 : 5.0 * std::log(hs.err / hs.errO);
+    return SVFSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CGRAD_SCHUR (svFSILS/CGRAD.f:51-123): CG on S P = L P - Gt (G P), the
+// pressure Schur complement of the NS solver (Gt = -G^T blockwise, so
+// S = L + G^T G).  The same device-resident iteration as cg_solve (the
+// update kernels are CGRADV's: CGRAD_SCHUR makes the same scalar and vector
+// steps), with the operator in two row passes: GP = G P, then
+// SP = L P - Gt GP and the partial sums of P . SP.
+// ---------------------------------------------------------------------------
+// LANES lanes per row (a tet4 node has about 15 to 18 neighbours): lane l
+// takes the row's blocks l, l+LANES, ..., so a
+// wavefront's loads of G, Gt, L and colPtr are contiguous; the partial sums
+// meet by shuffles.
+template <int LANES>
+__global__ __launch_bounds__(256) void k_cgs_gp(int nNo, const int* __restrict__ rowPtr,
+                                                const int* __restrict__ colPtr, const double* __restrict__ G,
+                                                const double* __restrict__ P, double* __restrict__ GP,
+                                                const CgState* __restrict__ st)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x % LANES;
+    const int rowsPerGrid = gridDim.x * (256 / LANES);
+    for (int i = blockIdx.x * (256 / LANES) + threadIdx.x / LANES; i < nNo; i += rowsPerGrid) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int j = rowPtr[i] + lane; j < rowPtr[i + 1]; j += LANES) {
+            const double u = P[colPtr[j]];
+            a0 += G[(size_t)j * 3] * u;
+            a1 += G[(size_t)j * 3 + 1] * u;
+            a2 += G[(size_t)j * 3 + 2] * u;
+        }
+        for (int o = LANES / 2; o > 0; o >>= 1) {
+            a0 += __shfl_xor(a0, o, LANES);
+            a1 += __shfl_xor(a1, o, LANES);
+            a2 += __shfl_xor(a2, o, LANES);
+        }
+        if (lane == 0) {
+            GP[(size_t)i * 3] = a0;
+            GP[(size_t)i * 3 + 1] = a1;
+            GP[(size_t)i * 3 + 2] = a2;
+        }
+    }
+}
+
+template <int LANES>
+__global__ __launch_bounds__(256) void k_cgs_sp(int nNo, const int* __restrict__ rowPtr,
+                                                const int* __restrict__ colPtr, const double* __restrict__ Lm,
+                                                const double* __restrict__ Gt, const double* __restrict__ P,
+                                                const double* __restrict__ GP, double* __restrict__ SP,
+                                                double* __restrict__ part, const CgState* __restrict__ st)
+{
+    if (st->done) return;
+    __shared__ double lds[4];
+    const int lane = threadIdx.x % LANES;
+    const int rowsPerGrid = gridDim.x * (256 / LANES);
+    double contrib = 0.0;
+    for (int i = blockIdx.x * (256 / LANES) + threadIdx.x / LANES; i < nNo; i += rowsPerGrid) {
+        double a = 0.0, d = 0.0;
+        for (int j = rowPtr[i] + lane; j < rowPtr[i + 1]; j += LANES) {
+            const int c = colPtr[j];
+            a += Lm[j] * P[c];
+            const double* k = Gt + (size_t)j * 3;
+            const double* u = GP + (size_t)c * 3;
+            d += k[0] * u[0] + k[1] * u[1] + k[2] * u[2];
+        }
+        for (int o = LANES / 2; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, LANES);
+            d += __shfl_xor(d, o, LANES);
+        }
+        if (lane == 0) {
+            const double sp = a + (-1.0) * d;
+            SP[i] = sp;
+            contrib += P[i] * sp;
+        }
+    }
+    const double sb = block_sum<256>(contrib, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = sb;
+}
+
+int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* G, const double* Lm, double* R,
+                   double* X, double* P, double* SP, double* GP, hipStream_t s)
+{
+    const int nNo = L.nNo;
+    const size_t n = (size_t)nNo;
+    Work& w = L.ws;
+    // lanes per row (tuning knob SVFSI_CGS_LANES = 8, 16 or 32; 8 measured
+    // fastest on the 200k-tet cylinder, 32 slowest)
+    static const int lanes = [] {
+        const char* e = getenv("SVFSI_CGS_LANES");
+        const int v = e ? atoi(e) : 8;
+        return (v == 16 || v == 32) ? v : 8;
+    }();
+    constexpr int kMaxGs = 2048;
+    int gs = (nNo + 256 / lanes - 1) / (256 / lanes);
+    if (gs > kMaxGs) gs = kMaxGs;
+    if (gs < 1) gs = 1;
+    unsigned gv = vec_grid(n);
+    if (gv > 512) gv = 512;
+    const size_t nPart = kMaxGs + 512 + 16;
+    SV_TRY(w.partial.ensure(nPart * sizeof(double)));
+    SV_TRY(w.state.ensure(sizeof(CgState)));
+    double* part1 = w.partial.as<double>();
+    double* part2 = part1 + kMaxGs;
+    double* scal = part1 + nPart - 8;
+    CgState* st = w.state.as<CgState>();
+
+    const double time = wall_seconds();
+    ls.suc = 0;
+    SV_TRY(launch_dot((int)n, R, R, part1, scal, s));
+    double rr = 0.0;
+    SV_HIP(hipMemcpyAsync(&rr, scal, sizeof(double), hipMemcpyDeviceToHost, s));
+    SV_HIP(hipStreamSynchronize(s));
+    ls.iNorm = std::sqrt(rr);
+    double eps = std::fmax(ls.absTol, ls.relTol * ls.iNorm);
+    eps = eps * eps;
+    CgState h0{};
+    h0.errO = ls.iNorm * ls.iNorm;
+    h0.err = h0.errO;
+    h0.e[0] = h0.err;
+    h0.eps = eps;
+    h0.mItr = ls.mItr;
+    if (h0.err < eps) { h0.done = 1; h0.suc = 1; }
+    if (ls.mItr <= 0) h0.done = 1;
+    SV_HIP(hipMemcpyAsync(st, &h0, sizeof(CgState), hipMemcpyHostToDevice, s));
+    SV_HIP(hipMemsetAsync(X, 0, n * sizeof(double), s));
+    SV_HIP(hipMemcpyAsync(P, R, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+
+    PinnedPoll& poll = w.poll;
+    SV_TRY(poll.ensure(256));
+    constexpr int chunk = 16; // even: iteration parity = position in the chunk
+    const bool cpl = L.cpl.active;
+    auto iters = [&](hipStream_t cs) -> int {
+        for (int it = 0; it < chunk; ++it) {
+            const int par = it & 1;
+            const int* rp = L.rowPtr.as<int>();
+            const int* cp = L.colPtr.as<int>();
+            if (lanes == 8)
+                hipLaunchKernelGGL(k_cgs_gp<8>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, (const double*)P, GP,
+                                   (const CgState*)st);
+            else if (lanes == 32)
+                hipLaunchKernelGGL(k_cgs_gp<32>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, (const double*)P, GP,
+                                   (const CgState*)st);
+            else
+                hipLaunchKernelGGL(k_cgs_gp<16>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, (const double*)P, GP,
+                                   (const CgState*)st);
+            // coupled faces: GP <- PRE GP (CGRAD.f:89-92)
+            if (cpl) SV_TRY(cpl_apply(L, 1, 3, GP, GP, cs, &st->done));
+            {
+                // algorithmic bytes: L, Gt, colPtr, rowPtr; P and GP once; SP
+                ProfScope ps(4, 36.0 * L.nnz + 4.0 * (nNo + 1) + 40.0 * nNo);
+                if (lanes == 8)
+                    SV_LAUNCH(ps, k_cgs_sp<8>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)P,
+                              (const double*)GP, SP, part1, (const CgState*)st);
+                else if (lanes == 32)
+                    SV_LAUNCH(ps, k_cgs_sp<32>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)P,
+                              (const double*)GP, SP, part1, (const CgState*)st);
+                else
+                    SV_LAUNCH(ps, k_cgs_sp<16>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)P,
+                              (const double*)GP, SP, part1, (const CgState*)st);
+            }
+            hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, cs, n, X, R, (const double*)P, (const double*)SP,
+                               (const double*)part1, gs, part2, st, par);
+            hipLaunchKernelGGL(k_cg_pupdate, dim3(gv), dim3(256), 0, cs, n, P, (const double*)R,
+                               (const double*)part2, (int)gv, st, par);
+        }
+        SV_HIP(hipGetLastError());
+        return SVFSI_OK;
+    };
+    const bool useGraphs = graphs_enabled();
+    if (useGraphs) {
+        std::vector<uintptr_t> sig = {(uintptr_t)n, (uintptr_t)Gt, (uintptr_t)G, (uintptr_t)Lm, (uintptr_t)R,
+                                      (uintptr_t)X, (uintptr_t)P, (uintptr_t)SP, (uintptr_t)GP, (uintptr_t)part1,
+                                      (uintptr_t)st, (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)cpl,
+                                      (uintptr_t)lanes};
+        if (cpl) cpl_sig(L, sig);
+        if (sig != w.csGraphs.sig) {
+            w.csGraphs.clear();
+            w.csGraphs.sig = sig;
+        }
+    }
+    int posted = 0;
+    CgState hs = h0;
+    bool finished = h0.done != 0;
+    int launched = 0;
+    while (!finished) {
+        const int q = posted & 1;
+        if (useGraphs)
+            SV_TRY(graph_run(w.csGraphs, 0, s, iters));
+        else
+            SV_TRY(iters(s));
+        launched += chunk;
+        SV_TRY(poll.post(q, st, sizeof(CgState), s));
+        ++posted;
+        if (posted >= 2) {
+            SV_TRY(poll.wait(q ^ 1));
+            std::memcpy(&hs, poll.h[q ^ 1], sizeof(CgState));
+            if (hs.done) finished = true;
+        }
+        if (launched > ls.mItr + 2 * chunk) break; // safety: the flag is set by now
+    }
+    SV_HIP(hipStreamSynchronize(s));
+    SV_HIP(hipMemcpy(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost));
+    SV_HIP(hipMemcpyAsync(R, X, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SV_HIP(hipStreamSynchronize(s));
+    ls.suc = hs.suc;
+    ls.fNorm = std::sqrt(hs.err);
+    ls.callD = wall_seconds() - time + ls.callD;
+    ls.itr = ls.itr + hs.itr;
     ls.dB = (hs.errO < 2.220446049250313e-16) ? 0.0 : 5.0 * std::log(hs.err / hs.errO);
     return SVFSI_OK;
 }
@@ -1073,13 +1291,6 @@ static bool op_transposed(const GmOp<float>&) { return false; }
 static int op_cpl(const GmOp<double>& op) { return op.cpl; }
 static int op_cpl(const GmOp<float>&) { return 0; }
 
-// the coupled-face launches baked into a captured graph
-static void cpl_sig(const Lhs& L, std::vector<uintptr_t>& sig)
-{
-    const CplSet& c = L.cpl;
-    sig.insert(sig.end(), {(uintptr_t)c.active, (uintptr_t)c.gen, (uintptr_t)c.vm.p, (uintptr_t)c.coef.p,
-                           (uintptr_t)c.glob.p, (uintptr_t)c.off.p, (uintptr_t)c.fdof.p, (uintptr_t)c.tmp.p});
-}
 
 // Restarted GMRES(sD) on n = nNo*dof unknowns.  R: RHS in, solution out
 // (GMRESV overwrites R with X, GMRES.f:383).  Fills suc/itr/iNorm/fNorm/dB.

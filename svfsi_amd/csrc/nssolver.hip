@@ -46,50 +46,69 @@ __global__ __launch_bounds__(256) void k_depart(size_t nnz, const int* __restric
     mL[k] = t[15];
 }
 
-// FSILS_SPARMULVS / SV / SS (SPARMUL.f:135-297), a thread per row, in the
-// reference's summation order.
+// FSILS_SPARMULVS / SV / SS (SPARMUL.f:135-297): 16 lanes per row (a tet4
+// node has about 15 neighbours), lane l taking the row's blocks l, l+16, ...
+// so that the loads of K and colPtr are contiguous; shuffles add the lanes.
+constexpr int kLanes = 16;
+
+__device__ __forceinline__ double lanes_sum(double v)
+{
+    for (int o = kLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kLanes);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_sp_vs(int nNo, const int* __restrict__ rowPtr,
                                                const int* __restrict__ colPtr, const double* __restrict__ K,
                                                const double* __restrict__ U, double* __restrict__ KU)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.x * (256 / kLanes) + threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
     if (i >= nNo) return;
     double a = 0.0;
-    for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
+    for (int j = rowPtr[i] + lane; j < rowPtr[i + 1]; j += kLanes) {
         const double* k = K + (size_t)j * kNsd;
         const double* u = U + (size_t)colPtr[j] * kNsd;
-        a = ((a + k[0] * u[0]) + k[1] * u[1]) + k[2] * u[2];
+        a += k[0] * u[0] + k[1] * u[1] + k[2] * u[2];
     }
-    KU[i] = a;
+    a = lanes_sum(a);
+    if (lane == 0) KU[i] = a;
 }
 
 __global__ __launch_bounds__(256) void k_sp_sv(int nNo, const int* __restrict__ rowPtr,
                                                const int* __restrict__ colPtr, const double* __restrict__ K,
                                                const double* __restrict__ U, double* __restrict__ KU)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.x * (256 / kLanes) + threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
     if (i >= nNo) return;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) {
+    for (int j = rowPtr[i] + lane; j < rowPtr[i + 1]; j += kLanes) {
         const double u = U[colPtr[j]];
-        a0 = a0 + K[(size_t)j * 3] * u;
-        a1 = a1 + K[(size_t)j * 3 + 1] * u;
-        a2 = a2 + K[(size_t)j * 3 + 2] * u;
+        a0 += K[(size_t)j * 3] * u;
+        a1 += K[(size_t)j * 3 + 1] * u;
+        a2 += K[(size_t)j * 3 + 2] * u;
     }
-    KU[(size_t)i * 3] = a0;
-    KU[(size_t)i * 3 + 1] = a1;
-    KU[(size_t)i * 3 + 2] = a2;
+    a0 = lanes_sum(a0);
+    a1 = lanes_sum(a1);
+    a2 = lanes_sum(a2);
+    if (lane == 0) {
+        KU[(size_t)i * 3] = a0;
+        KU[(size_t)i * 3 + 1] = a1;
+        KU[(size_t)i * 3 + 2] = a2;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_sp_ss(int nNo, const int* __restrict__ rowPtr,
                                                const int* __restrict__ colPtr, const double* __restrict__ K,
                                                const double* __restrict__ U, double* __restrict__ KU)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.x * (256 / kLanes) + threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
     if (i >= nNo) return;
     double a = 0.0;
-    for (int j = rowPtr[i]; j < rowPtr[i + 1]; ++j) a = a + K[j] * U[colPtr[j]];
-    KU[i] = a;
+    for (int j = rowPtr[i] + lane; j < rowPtr[i + 1]; j += kLanes) a += K[j] * U[colPtr[j]];
+    a = lanes_sum(a);
+    if (lane == 0) KU[i] = a;
 }
 
 // y <- a*y + c*x (a = 1: the reference's OMPSUMS/V, y = y + c x; a = 0:
@@ -107,12 +126,6 @@ __global__ __launch_bounds__(256) void k_ns_sub(size_t n, const double* __restri
 {
     const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (q < n) z[q] = x[q] - y[q];
-}
-
-__global__ __launch_bounds__(256) void k_ns_scale(size_t n, double* __restrict__ y, double c)
-{
-    const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (q < n) y[q] = y[q] * c;
 }
 
 // Ri (4, nNo) <-> Rm (3, nNo), Rc (nNo)
@@ -135,6 +148,48 @@ __global__ __launch_bounds__(256) void k_ns_merge(int nNo, const double* __restr
 }
 
 inline dim3 grid_of(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+inline dim3 row_grid(size_t nRows) { return dim3((unsigned)((nRows + 256 / kLanes - 1) / (256 / kLanes))); }
+
+// The Gram entries of one outer iteration (NSSOLVER.f:135-146) in one pass:
+// pair p is (k, j) with k = kLo for p <= kLo, else k = kLo + 1, and j = 0..k
+// (j = 0: the right-hand side Rmi, Rci).  Each pair gets block partials of
+// MU_j . MU_k over 3 nNo values (slot 2p) and of MP_j . MP_k over nNo
+// values (slot 2p+1); k_ns_gram_sum adds each slot's partials in a fixed
+// order.
+constexpr int kGramBlocks = 64;
+
+__global__ __launch_bounds__(256) void k_ns_gram(int nNo, int kLo, const double* __restrict__ MU,
+                                                 const double* __restrict__ MP, const double* __restrict__ Rmi,
+                                                 const double* __restrict__ Rci, double* __restrict__ part)
+{
+    __shared__ double lds[4];
+    const int p = blockIdx.y;
+    const int k = p <= kLo ? kLo : kLo + 1;
+    const int j = p <= kLo ? p : p - kLo - 1;
+    const size_t n3 = (size_t)nNo * kNsd;
+    const double* a = j == 0 ? Rmi : MU + (size_t)(j - 1) * n3;
+    const double* b = MU + (size_t)(k - 1) * n3;
+    const double* ap = j == 0 ? Rci : MP + (size_t)(j - 1) * nNo;
+    const double* bp = MP + (size_t)(k - 1) * nNo;
+    double su = 0.0, sp = 0.0;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n3; q += (size_t)gridDim.x * 256) su += a[q] * b[q];
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < (size_t)nNo; q += (size_t)gridDim.x * 256)
+        sp += ap[q] * bp[q];
+    su = block_sum<256>(su, lds);
+    sp = block_sum<256>(sp, lds);
+    if (threadIdx.x == 0) {
+        part[(size_t)(2 * p) * gridDim.x + blockIdx.x] = su;
+        part[(size_t)(2 * p + 1) * gridDim.x + blockIdx.x] = sp;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_ns_gram_sum(int gx, const double* __restrict__ part, double* __restrict__ out)
+{
+    double v = 0.0;
+    for (int q = threadIdx.x; q < gx; q += 64) v += part[(size_t)blockIdx.x * gx + q];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
 
 // GE (svFSILS/GE.f:51-151), host: A column-major with leading dimension nV;
 // B overwritten with the solution; false (and B = 0) if singular.
@@ -223,17 +278,17 @@ struct NsCtx {
     }
     void vs(const double* K, const double* U, double* KU)
     {
-        hipLaunchKernelGGL(k_sp_vs, grid_of(n), dim3(256), 0, s, (int)n, L->rowPtr.as<int>(), L->colPtr.as<int>(),
+        hipLaunchKernelGGL(k_sp_vs, row_grid(n), dim3(256), 0, s, (int)n, L->rowPtr.as<int>(), L->colPtr.as<int>(),
                            K, U, KU);
     }
     void sv(const double* K, const double* U, double* KU)
     {
-        hipLaunchKernelGGL(k_sp_sv, grid_of(n), dim3(256), 0, s, (int)n, L->rowPtr.as<int>(), L->colPtr.as<int>(),
+        hipLaunchKernelGGL(k_sp_sv, row_grid(n), dim3(256), 0, s, (int)n, L->rowPtr.as<int>(), L->colPtr.as<int>(),
                            K, U, KU);
     }
     void ss(const double* K, const double* U, double* KU)
     {
-        hipLaunchKernelGGL(k_sp_ss, grid_of(n), dim3(256), 0, s, (int)n, L->rowPtr.as<int>(), L->colPtr.as<int>(),
+        hipLaunchKernelGGL(k_sp_ss, row_grid(n), dim3(256), 0, s, (int)n, L->rowPtr.as<int>(), L->colPtr.as<int>(),
                            K, U, KU);
     }
 };
@@ -269,53 +324,6 @@ int ns_gmres(NsCtx& c, const double* K, const double* R, double* X, FSILS_subLsT
     return SVFSI_OK;
 }
 
-// CGRAD_SCHUR (svFSILS/CGRAD.f:51-123): R in: right-hand side, out: the
-// solution of (L - Gt G) x = R; itr and callD accumulate.
-int ns_cg_schur(NsCtx& c, const double* Gt, const double* G, const double* L, double* R, double* X, double* P,
-                double* SP, double* DGP, double* GP, FSILS_subLsType& cg)
-{
-    int rc = SVFSI_OK;
-    const size_t n = c.n;
-    const double time = wall_seconds();
-    cg.suc = 0;
-    cg.iNorm = std::sqrt(c.dot(R, R, n, &rc));
-    SV_TRY(rc);
-    const double eps0 = std::fmax(cg.absTol, cg.relTol * cg.iNorm);
-    const double eps = eps0 * eps0;
-    double errO = cg.iNorm * cg.iNorm, err = errO;
-    SV_HIP(hipMemsetAsync(X, 0, n * sizeof(double), c.s));
-    SV_HIP(hipMemcpyAsync(P, R, n * sizeof(double), hipMemcpyDeviceToDevice, c.s));
-    int i;
-    for (i = 1; i <= cg.mItr; ++i) {
-        if (err < eps) {
-            cg.suc = 1;
-            break;
-        }
-        errO = err;
-        c.sv(G, P, GP);
-        SV_TRY(cpl_apply(*c.L, 1, kNsd, GP, GP, c.s));  // CGRAD.f:89-92
-        c.vs(Gt, GP, DGP);
-        c.ss(L, P, SP);
-        c.axpy(SP, 1.0, -1.0, DGP, n);
-        const double alpha = errO / c.dot(P, SP, n, &rc);
-        SV_TRY(rc);
-        c.axpy(X, 1.0, alpha, P, n);
-        c.axpy(R, 1.0, -alpha, SP, n);
-        err = std::sqrt(c.dot(R, R, n, &rc));
-        SV_TRY(rc);
-        err = err * err;
-        c.axpy(P, 1.0, errO / err, R, n);
-        hipLaunchKernelGGL(k_ns_scale, grid_of(n), dim3(256), 0, c.s, n, P, err / errO);
-    }
-    SV_HIP(hipMemcpyAsync(R, X, n * sizeof(double), hipMemcpyDeviceToDevice, c.s));
-    cg.fNorm = std::sqrt(err);
-    cg.callD = wall_seconds() - time + cg.callD;
-    cg.itr = cg.itr + i - 1;
-    cg.dB = (errO < 2.220446049250313e-16) ? 0.0 : 5.0 * std::log(err / errO);
-    SV_HIP(hipGetLastError());
-    return SVFSI_OK;
-}
-
 } // namespace
 
 int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, hipStream_t s)
@@ -336,7 +344,7 @@ int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, h
     }
     Work& w = L.ws;
     const size_t need = 19 * nnz + n * (3 + 1 + 3 + 1) + n * (3 + 1) * (size_t)iBmax + n * (3 + 1) * (size_t)nB +
-                        n * (1 + 1 + 1 + 1 + 3) + 520;
+                        n * (1 + 1 + 1 + 3) + 520 + (size_t)2 * (2 * nB + 1) * (kGramBlocks + 1);
     SV_TRY(w.ns.ensure(need * sizeof(double)));
     double* p = w.ns.as<double>();
     auto take = [&](size_t k) { double* q = p; p += k; return q; };
@@ -356,9 +364,12 @@ int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, h
     double* cX = take(n);
     double* cP = take(n);
     double* cSP = take(n);
-    double* cDGP = take(n);
     double* cGP = take(3 * n);
     NsCtx c{&L, s, n, take(520)};
+    const int npMax = 2 * nB + 1;  // pairs of the last outer iteration
+    double* gpart = take((size_t)2 * npMax * kGramBlocks);
+    double* gout = take((size_t)2 * npMax);
+    std::vector<double> gram(2 * npMax);
     auto VU = [&](int i) { return U + (size_t)(i - 1) * 3 * n; };
     auto VP = [&](int i) { return P + (size_t)(i - 1) * n; };
     auto VMU = [&](int i) { return MU + (size_t)(i - 1) * 3 * n; };
@@ -394,7 +405,7 @@ int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, h
         SV_TRY(ns_gmres(c, mK, Rm, VU(i), ls.GM));
         c.vs(mD, VU(i), VP(i));
         hipLaunchKernelGGL(k_ns_sub, grid_of(n), dim3(256), 0, s, n, (const double*)Rc, (const double*)VP(i), VP(i));
-        SV_TRY(ns_cg_schur(c, Gt, mG, mL, VP(i), cX, cP, cSP, cDGP, cGP, ls.CG));
+        SV_TRY(cg_schur_solve(L, ls.CG, Gt, mG, mL, VP(i), cX, cP, cSP, cGP, s));
         c.sv(mG, VP(i), VMU(iB));
         hipLaunchKernelGGL(k_ns_sub, grid_of(3 * n), dim3(256), 0, s, 3 * n, (const double*)Rm,
                            (const double*)VMU(iB), VMU(iBB));
@@ -403,15 +414,26 @@ int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, h
         SV_TRY(cpl_apply(L, 0, kNsd, VU(i), VMU(iBB), s));  // NSSOLVER.f:128
         c.ss(mL, VP(i), VMP(iB));
         c.vs(mD, VU(i), VMP(iBB));
-        for (int k = iB; k <= iBB; ++k) {
-            for (int j = 1; j <= k; ++j) {
-                const double t = c.dot(VMU(j), VMU(k), 3 * n, &rc) + c.dot(VMP(j), VMP(k), n, &rc);
+        // A(j,k) = MU_j . MU_k + MP_j . MP_k, B(k) = MU_k . Rmi + MP_k . Rci
+        // for k = iB, iBB: all pairs in one pass, one read-back
+        const int np = 2 * iB + 3;
+        hipLaunchKernelGGL(k_ns_gram, dim3(kGramBlocks, np), dim3(256), 0, s, (int)n, iB, (const double*)MU,
+                           (const double*)MP, (const double*)Rmi, (const double*)Rci, gpart);
+        hipLaunchKernelGGL(k_ns_gram_sum, dim3(2 * np), dim3(64), 0, s, kGramBlocks, (const double*)gpart, gout);
+        SV_HIP(hipGetLastError());
+        SV_HIP(hipMemcpyAsync(gram.data(), gout, 2 * np * sizeof(double), hipMemcpyDeviceToHost, s));
+        SV_HIP(hipStreamSynchronize(s));
+        for (int p = 0; p < np; ++p) {
+            const int k = p <= iB ? iB : iBB;
+            const int j = p <= iB ? p : p - iB - 1;
+            const double t = gram[2 * p] + gram[2 * p + 1];
+            if (j == 0) {
+                B[k - 1] = t;
+            } else {
                 AM(j, k) = t;
                 AM(k, j) = t;
             }
-            B[k - 1] = c.dot(VMU(k), Rmi, 3 * n, &rc) + c.dot(VMP(k), Rci, n, &rc);
         }
-        SV_TRY(rc);
         xB = B;
         if (ge_host(nB, iBB, A, xB)) {
             oldxB = xB;

@@ -129,7 +129,9 @@ def test_mixed_resistance(gpu, orc):
 
 @pytest.mark.parametrize("split", [False, True])
 def test_ns_resistance_parity(gpu, orc, split):
-    settings = dict(LS_type=fsils.LS_TYPE_NS, relTol=1e-6, absTol=1e-30, mItr=20, sD=50,
+    # relTol 1e-5: the NS stopping test resolves iNorm^2 - xB.B only to
+    # about 1e-12 iNorm^2
+    settings = dict(LS_type=fsils.LS_TYPE_NS, relTol=1e-5, absTol=1e-30, mItr=20, sD=50,
                     GM=dict(relTol=1e-4, mItr=4), CG=dict(relTol=1e-4))
     prob, st, V0, R0 = setup(orc, settings, split)
     res = res_values(orc, st, prob, V0, R0, 10.0)

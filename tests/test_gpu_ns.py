@@ -3,7 +3,11 @@ oracle's restatement on the same assembled fluid system: solution, outer
 iteration count, success flag, the Resm/Resc split; and the reported final
 residual against the residual recomputed from the GPU solution on the CPU
 (the least-squares formula iNorm^2 - xB.B agrees with it to about 1 % at
-relTol 1e-6: the Gram matrix of the images is ill-conditioned)."""
+relTol 1e-5: the Gram matrix of the images is ill-conditioned).  The
+tight case stops at relTol 1e-5, not lower: the stopping test compares
+iNorm^2 - xB.B with (relTol iNorm)^2, a difference that rounding resolves
+only to about 1e-12 iNorm^2, so below 1e-6 the outcome of the test is
+decided by rounding on either side."""
 import sys
 
 import numpy as np
@@ -37,7 +41,7 @@ def fluid(h, seed=3):
 # reference defaults (LS.f:69-95) and a tight variant
 SETTINGS = {
     "default": dict(relTol=0.4, absTol=1e-10, mItr=10, sD=100),
-    "tight": dict(relTol=1e-6, absTol=1e-30, mItr=20, sD=50, GM=dict(relTol=1e-4, mItr=4),
+    "tight": dict(relTol=1e-5, absTol=1e-30, mItr=20, sD=50, GM=dict(relTol=1e-4, mItr=4),
                   CG=dict(relTol=1e-4)),
 }
 
@@ -72,8 +76,12 @@ def test_ns_solver_parity(gpu, orc, which, prec):
     assert abs(ls.GM.itr - lo.GM.itr) <= max(2, lo.GM.itr // 20)
     assert abs(ls.CG.itr - lo.CG.itr) <= max(2, lo.CG.itr // 20)
     # Resm + Resc = 100, or both 0 where the least-squares residual formula
-    # went negative by cancellation (NSSOLVER.f:217-221 then reports 0)
-    assert ls.Resm + ls.Resc in (0, 100) and abs(ls.Resc - lo.Resc) <= 1 and abs(ls.Resm - lo.Resm) <= 1
+    # went negative by cancellation (NSSOLVER.f:217-221 then reports 0).
+    # The split divides by fNorm^2 = iNorm^2 - xB.B: compared only above
+    # that formula's cancellation floor (rounding of iNorm^2 times ~1e4).
+    assert ls.Resm + ls.Resc in (0, 100) and lo.Resm + lo.Resc in (0, 100)
+    if min(ls.RI.fNorm, lo.RI.fNorm) ** 2 > 1e-12 * lo.RI.iNorm ** 2:
+        assert abs(ls.Resc - lo.Resc) <= 1 and abs(ls.Resm - lo.Resm) <= 1
     assert abs(ls.RI.iNorm - lo.RI.iNorm) <= 1e-12 * lo.RI.iNorm
     assert relerr(xg, xo) < 1e-6
     if which == "tight" and prec == fsils.PRECOND_FSILS:
@@ -85,7 +93,7 @@ def test_ns_solver_parity(gpu, orc, which, prec):
         r = Rs - orc.spmv(rp, cp, 4, Vs, y)
         say("  residual recomputed", np.linalg.norm(r), "reported", ls.RI.fNorm)
         assert abs(np.linalg.norm(r) - ls.RI.fNorm) <= 1e-2 * ls.RI.fNorm
-        assert ls.RI.fNorm <= 1e-6 * ls.RI.iNorm
+        assert ls.RI.fNorm <= 1e-5 * ls.RI.iNorm
 
 
 def test_ns_solver_larger_mesh(gpu, orc):
