@@ -212,7 +212,19 @@ typedef struct {
     double C10, Kpen;             /* STRUCT nHK (MATMODELS.f:136-150) */
     int volType;                  /* 1 Quad, 2 ST91 (default), 3 M94 */
     int pad;
+    /* FLUID viscosity model (GETVISCOSITY, FLUID.f:1690-1739; READVISCMODEL,
+     * READFILES.f:2878-2914): viscType SVFSI_VISC_CONST (mu), SVFSI_VISC_CY
+     * (Carreau-Yasuda: mu_i = mu, mu_o, lam, visc_a, visc_n) or
+     * SVFSI_VISC_CASS (Casson: mu_i = mu, mu_o, lam = low shear-rate
+     * threshold).  0 is read as SVFSI_VISC_CONST. */
+    int viscType;
+    int pad2;
+    double mu_o, lam, visc_a, visc_n;
 } svfsi_eq_params;
+
+#define SVFSI_VISC_CONST 698  /* viscType_Const (CONSTS.f:201-202) */
+#define SVFSI_VISC_CY    697  /* viscType_CY */
+#define SVFSI_VISC_CASS  696  /* viscType_Cass */
 
 #define SVFSI_PHYS_FLUID   1  /* CONSTRUCT_FLUID, FLUID.f:40-190 (tet4 VMS) */
 #define SVFSI_PHYS_LELAS   2  /* CONSTRUCT_LELAS, LELAS.f:38-128 (tet4) */

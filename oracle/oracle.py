@@ -36,6 +36,8 @@ class Eq(C.Structure):
         ("dt", C.c_double), ("rho", C.c_double), ("E", C.c_double), ("nu", C.c_double),
         ("fb", C.c_double * 3), ("mu", C.c_double), ("bstab", C.c_double),
         ("C10", C.c_double), ("Kpen", C.c_double), ("volType", C.c_int), ("pad", C.c_int),
+        ("viscType", C.c_int), ("pad2", C.c_int), ("mu_o", C.c_double), ("lam", C.c_double),
+        ("visc_a", C.c_double), ("visc_n", C.c_double),
     ]
 
 
@@ -76,6 +78,9 @@ def eq_struct(d: dict) -> Eq:
     for i in range(3):
         e.fb[i] = float(fb[i])
     e.volType = int(d.get("volType", 2))
+    e.viscType = int(d.get("viscType", 698))
+    for k in ("mu_o", "lam", "visc_a", "visc_n"):
+        setattr(e, k, float(d.get(k, 0.0)))
     return e
 
 
@@ -135,6 +140,18 @@ def assemble(prob, rowPtr: np.ndarray, colPtr: np.ndarray) -> tuple[np.ndarray, 
         L.orc_backflow_tri3(len(fi), P(fi), P(fe), P(ien), m.eNoN, P(x), P(Yg), Yg.shape[1], C.byref(eq),
                             P(rowPtr), P(colPtr), P(Val), P(R))
     return Val, R
+
+
+def getviscosity(eq: dict, gamma: float, mu_i: float | None = None):
+    """GETVISCOSITY at shear rate gamma: (mu, mu_s, d mu / d gamma, gamma
+    after the call: Casson raises it to its threshold)."""
+    e = eq_struct(eq)
+    g, mu, mus, mux = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    fn = lib().orc_getviscosity
+    fn.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    fn(C.byref(e), float(eq.get("mu", 0.0) if mu_i is None else mu_i), float(gamma), C.byref(g), C.byref(mu),
+       C.byref(mus), C.byref(mux))
+    return mu.value, mus.value, mux.value, g.value
 
 
 def set_threads(n: int) -> None:

@@ -358,6 +358,50 @@ void orc_construct_lelas(int nNo, int nEl, const int* IEN, const double* x,
 /* FLUID.f                                                                */
 /* ===================================================================== */
 
+/* GETVISCOSITY (svFSI/FLUID.f:1690-1739): viscosity mu and d mu / d gamma
+ * (mu_x) at shear rate *gamma for the model eq->viscType, mu_i = mu_in.
+ * Casson raises *gamma to the threshold lam below it (gamma is INOUT in
+ * the reference, and the caller divides by the raised value). */
+static void orc_getviscosity_d(const orc_eq* eq, double mu_in, double* gamma, double* mu, double* mu_s,
+                               double* mu_x)
+{
+    const int vt = eq->viscType == 0 ? 698 : eq->viscType;
+    if (vt == 697) {
+        const double mu_i = mu_in, mu_o = eq->mu_o, lam = eq->lam, a = eq->visc_a, n = eq->visc_n;
+        double T1 = 1.0 + pow(lam * *gamma, a);
+        double T2 = pow(T1, (n - 1.0) / a);
+        *mu = mu_i + (mu_o - mu_i) * T2;
+        *mu_s = mu_i;
+        T1 = T2 / T1;
+        T2 = pow(lam, a) * pow(*gamma, a - 1.0) * T1;
+        *mu_x = (mu_o - mu_i) * (n - 1.0) * T2;
+    } else if (vt == 696) {
+        const double mu_i = mu_in, lam = eq->lam;
+        double mu_o = eq->mu_o;
+        if (*gamma < lam) {
+            mu_o = mu_o / sqrt(lam);
+            *gamma = lam;
+        } else {
+            mu_o = mu_o / sqrt(*gamma);
+        }
+        *mu = (mu_i + mu_o) * (mu_i + mu_o);
+        *mu_s = mu_i * mu_i;
+        *mu_x = 2.0 * mu_o * (mu_o + mu_i) / *gamma;
+    } else {
+        *mu = mu_in;
+        *mu_s = mu_in;
+        *mu_x = 0.0;
+    }
+}
+
+void orc_getviscosity(const orc_eq* eq, double mu_i, double gamma_in, double* gamma_out, double* mu,
+                      double* mu_s, double* mu_x)
+{
+    double g = gamma_in;
+    orc_getviscosity_d(eq, mu_i, &g, mu, mu_s, mu_x);
+    if (gamma_out) *gamma_out = g;
+}
+
 #define FREAL double
 #define FSQRT sqrt
 #define FRE(x) (x)

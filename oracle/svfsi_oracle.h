@@ -43,6 +43,11 @@ typedef struct {
     double C10, Kpen;               /* STRUCT nHK: C10 = mu/2, penalty (bulk modulus) */
     int volType;                    /* STRUCT: 1 Quad, 2 ST91, 3 M94 (GETSVOLP) */
     int pad;
+    /* FLUID viscosity model (GETVISCOSITY, FLUID.f:1690-1739): 698 (or 0)
+     * constant mu; 697 Carreau-Yasuda (mu_i = mu, mu_o, lam, a, n); 696
+     * Casson (mu_i = mu, mu_o, lam) */
+    int viscType, pad2;
+    double mu_o, lam, visc_a, visc_n;
 } orc_eq;
 
 /* FSILS_subLsType (FSILS_STRUCT.h:151-180) */
@@ -95,6 +100,11 @@ void orc_getpk2cc_nhk(const orc_eq* eq, const double F[3][3],
 void orc_struct3d(int eNoN, double w, const double* N, const double (*Nx)[3],
                   const double* al, const double* yl, const double* dl,
                   int tDof, const orc_eq* eq, double* lR, double* lK);
+
+/* GETVISCOSITY (svFSI/FLUID.f:1690-1739): mu, mu_s and d mu / d gamma at
+ * shear rate gamma_in (Casson returns the raised rate in *gamma_out) */
+void orc_getviscosity(const orc_eq* eq, double mu_i, double gamma_in, double* gamma_out, double* mu,
+                      double* mu_s, double* mu_x);
 
 /* ---------------- construct loops (element loop + DOASSEM) ---------------- */
 /* rule: 0 = reference rule for the element (tet4: 4-pt, tet10: 15-pt),

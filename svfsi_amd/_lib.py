@@ -66,8 +66,14 @@ class EqParams(C.Structure):
         ("dt", C.c_double), ("rho", C.c_double), ("E", C.c_double), ("nu", C.c_double),
         ("fb", C.c_double * 3), ("mu", C.c_double), ("bstab", C.c_double),
         ("C10", C.c_double), ("Kpen", C.c_double), ("volType", C.c_int), ("pad", C.c_int),
+        ("viscType", C.c_int), ("pad2", C.c_int), ("mu_o", C.c_double), ("lam", C.c_double),
+        ("visc_a", C.c_double), ("visc_n", C.c_double),
     ]
 
+
+SVFSI_VISC_CONST = 698
+SVFSI_VISC_CY = 697
+SVFSI_VISC_CASS = 696
 
 _P = C.c_void_p
 _I = C.c_int

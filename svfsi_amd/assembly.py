@@ -25,6 +25,10 @@ def eq_params(d: dict) -> EqParams:
     for i in range(3):
         p.fb[i] = float(fb[i])
     p.volType = int(d.get("volType", 2))
+    # viscosity model (READVISCMODEL): constant unless viscType says otherwise
+    p.viscType = int(d.get("viscType", 698))
+    for k in ("mu_o", "lam", "visc_a", "visc_n"):
+        setattr(p, k, float(d.get(k, 0.0)))
     return p
 
 

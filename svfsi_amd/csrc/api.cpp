@@ -162,6 +162,8 @@ AsmParams to_asm(const svfsi_eq_params& p)
     a.fb[0] = p.fb[0]; a.fb[1] = p.fb[1]; a.fb[2] = p.fb[2];
     a.mu = p.mu; a.bstab = p.bstab; a.C10 = p.C10; a.Kpen = p.Kpen;
     a.volType = p.volType; a.pad = 0;
+    a.viscType = p.viscType == 0 ? SVFSI_VISC_CONST : p.viscType; a.pad2 = 0;
+    a.mu_o = p.mu_o; a.lam = p.lam; a.va = p.visc_a; a.vn = p.visc_n;
     return a;
 }
 
@@ -734,6 +736,10 @@ int svfsi_assemble_dev(void* mesh, int phys, const svfsi_eq_params* p, int tDof,
     } else if (phys == SVFSI_PHYS_FLUID) {
         SV_ARG(M.eNoN == 4, "FLUID assembly supports tet4 only");
         SV_ARG(tDof >= 4 && dAg && dYg, "FLUID needs Ag, Yg with tDof >= 4");
+        const int vt = p->viscType;
+        SV_ARG(vt == 0 || vt == SVFSI_VISC_CONST || vt == SVFSI_VISC_CY || vt == SVFSI_VISC_CASS,
+               "viscType %d: SVFSI_VISC_CONST (698), SVFSI_VISC_CY (697) or SVFSI_VISC_CASS (696)", vt);
+        SV_ARG(vt != SVFSI_VISC_CY || p->visc_a > 0.0, "Carreau-Yasuda needs a > 0");
     } else if (phys == SVFSI_PHYS_STRUCT) {
         SV_ARG(M.eNoN == 10, "STRUCT assembly supports tet10 only");
         SV_ARG(tDof >= 3 && dAg && dDg, "STRUCT needs Ag, Dg with tDof >= 3");
@@ -796,6 +802,8 @@ int svfsi_fluid_dmu_dot_dev(void* mesh, const svfsi_eq_params* p, int tDof, cons
     Mesh& M = *static_cast<Mesh*>(mesh);
     SV_ARG(M.eNoN == 4, "tet4 only");
     SV_ARG(tDof >= 4, "FLUID needs tDof >= 4");
+    SV_ARG(p->viscType == 0 || p->viscType == SVFSI_VISC_CONST,
+           "the viscosity sensitivity is defined for the constant (Newtonian) model only");
     return launch_fluid_dmu(M, to_asm(*p), make_tet4_rule(), tDof, dAg, dYg, dmu_el, dlam, ddu, dg,
                             static_cast<hipStream_t>(stream));
 }

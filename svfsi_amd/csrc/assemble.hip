@@ -353,6 +353,49 @@ __device__ __forceinline__ void fluid_gp(const FluidElem& E, T mu, const AsmPara
     G.tauB = tauB;
 }
 
+// Viscosity of a tet4 element (GETVISCOSITY, svFSI/FLUID.f:1690-1739, at
+// the shear rate gamma = sqrt(es:es / 2) of FLUID.f:359-373): the velocity
+// gradient of a linear tet is constant, so mu and mu_g = (d mu / d gamma) /
+// gamma are element constants.  mu_i is the element's (or the equation's)
+// viscosity; the constant model returns it with mu_g = 0.  Casson raises
+// gamma to its threshold lam below it, as the reference does.
+__device__ __forceinline__ void fluid_visc(const FluidElem& E, const AsmParams& P, double mu_i, double& mu,
+                                           double& mu_g)
+{
+    mu = mu_i;
+    mu_g = 0.0;
+    if (P.viscType != 697 && P.viscType != 696) return;
+    double gam = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double es = E.ux[i][j] + E.ux[j][i];
+            gam += es * es;
+        }
+    gam = sqrt(0.5 * gam);
+    double mux;
+    if (P.viscType == 697) {
+        double T1 = 1.0 + pow(P.lam * gam, P.va);
+        double T2 = pow(T1, (P.vn - 1.0) / P.va);
+        mu = mu_i + (P.mu_o - mu_i) * T2;
+        T1 = T2 / T1;
+        T2 = pow(P.lam, P.va) * pow(gam, P.va - 1.0) * T1;
+        mux = (P.mu_o - mu_i) * (P.vn - 1.0) * T2;
+    } else {
+        double mu_o = P.mu_o;
+        if (gam < P.lam) {
+            mu_o = mu_o / sqrt(P.lam);
+            gam = P.lam;
+        } else {
+            mu_o = mu_o / sqrt(gam);
+        }
+        mu = (mu_i + mu_o) * (mu_i + mu_o);
+        mux = 2.0 * mu_o * (mu_o + mu_i) / gam;
+    }
+    mu_g = iszero_d(gam) ? 0.0 : mux / gam;
+}
+
 // FLUID3D_M + FLUID3D_C (svFSI/FLUID.f:192-560, 813-1084), the Gauss point
 // g contribution to the (a,b) block of one tet4 element, Newtonian
 // viscosity mu, second derivatives zero: k (4x4 block, row-major) and, when
@@ -362,6 +405,7 @@ __device__ __forceinline__ void fluid_gp(const FluidElem& E, T mu, const AsmPara
 struct FluidAB {
     double Nxa[3], Nxb[3];
     double Na[4], Nb[4]; // N_a, N_b at the four Gauss points
+    double esNxa[3], esNxb[3]; // esNx(c,a) = sum_j es(j,c) Nx(j,a) (FLUID.f:321-328)
 };
 
 __device__ __forceinline__ void fluid_ab(const FluidElem& E, const Tet4Rule& rule, int a, int b, FluidAB& S)
@@ -389,12 +433,18 @@ __device__ __forceinline__ void fluid_ab(const FluidElem& E, const Tet4Rule& rul
             if (q == b) S.Nb[g] = rule.N[g][q];
         }
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double e0 = E.ux[0][c] + E.ux[c][0], e1 = E.ux[1][c] + E.ux[c][1], e2 = E.ux[2][c] + E.ux[c][2];
+        S.esNxa[c] = e0 * S.Nxa[0] + e1 * S.Nxa[1] + e2 * S.Nxa[2];
+        S.esNxb[c] = e0 * S.Nxb[0] + e1 * S.Nxb[1] + e2 * S.Nxb[2];
+    }
 }
 
 template <typename T>
-__device__ __forceinline__ void fluid_pair_gp(const FluidElem& E, T mu, const AsmParams& P, const Tet4Rule& rule,
-                                              int g, const FluidGP<T>& G, const FluidAB& S, bool want_r, T k[16],
-                                              T r[4])
+__device__ __forceinline__ void fluid_pair_gp(const FluidElem& E, T mu, double mug, const AsmParams& P,
+                                              const Tet4Rule& rule, int g, const FluidGP<T>& G, const FluidAB& S,
+                                              bool want_r, T k[16], T r[4])
 {
     const double* Nxa = S.Nxa;
     const double* Nxb = S.Nxb;
@@ -427,7 +477,7 @@ __device__ __forceinline__ void fluid_pair_gp(const FluidElem& E, T mu, const As
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const double updu = (rr == c) ? T1b : 0.0;
-            const T T2 = mu * Nxa[c] * Nxb[rr] + tauC * Nxa[rr] * Nxb[c]
+            const T T2 = mu * Nxa[c] * Nxb[rr] + tauC * Nxa[rr] * Nxb[c] + S.esNxa[rr] * mug * S.esNxb[c]
                        - rho * tauM * uaNxa * updu;
             k[rr * 4 + c] = k[rr * 4 + c] + wl * ((rr == c) ? T2 + T1 : T2);
         }
@@ -487,7 +537,7 @@ __device__ __forceinline__ void fluid_pair(const FluidElem& E, T mu, const AsmPa
     for (int g = 0; g < 4; ++g) {
         FluidGP<T> G;
         fluid_gp(E, mu, P, rule, g, G);
-        fluid_pair_gp(E, mu, P, rule, g, G, S, want_r, k, r);
+        fluid_pair_gp(E, mu, 0.0, P, rule, g, G, S, want_r, k, r);
     }
 }
 
@@ -510,7 +560,8 @@ __global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const i
     const int a = ab >> 2, b = ab & 3;
     FluidElem E;
     fluid_elem(p, ien, x, tDof, Ag, Yg, E);
-    const double mu = muEl ? muEl[elemId[p]] : P.mu;
+    double mu, mug;
+    fluid_visc(E, P, muEl ? muEl[elemId[p]] : P.mu, mu, mug);
     double k[16], r[4];
     // the 16 lanes of an element (consecutive lanes of one wave; count * 16
     // threads, so a group is active or inactive as a whole) evaluate the
@@ -546,7 +597,7 @@ __global__ __launch_bounds__(256) void k_asm_fluid(int begin, int count, const i
         G.tauM = __shfl(mine.tauM, src, kWave);
         G.tauC = __shfl(mine.tauC, src, kWave);
         G.tauB = __shfl(mine.tauB, src, kWave);
-        fluid_pair_gp<double>(E, mu, P, rule, g, G, S, b == 0, k, r);
+        fluid_pair_gp<double>(E, mu, mug, P, rule, g, G, S, b == 0, k, r);
     }
     if (ke) {
         double* kb = ke + (size_t)gslot[(size_t)p * 16 + ab] * 16;

@@ -14,6 +14,9 @@ struct AsmParams {
     double C10, Kpen;
     int volType;
     int pad;
+    int viscType;  // 698 constant, 697 Carreau-Yasuda, 696 Casson
+    int pad2;
+    double mu_o, lam, va, vn;
 };
 
 // TET4 4-point rule and shape values (GETGIP/GETGNN, svFSI/NN.f:266-275,

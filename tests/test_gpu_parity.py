@@ -214,6 +214,43 @@ def test_fluid_assembly_parity(gpu, orc):
     assert relerr(Rg, Ro) < 1e-12
 
 
+VISC = {
+    "cy": dict(viscType=697, mu=0.0345, mu_o=0.56, lam=3.313, visc_a=2.0, visc_n=0.3568),
+    "cass": dict(viscType=696, mu=0.187, mu_o=0.2, lam=0.5),
+}
+
+
+@pytest.mark.parametrize("model", ["cy", "cass"])
+def test_fluid_assembly_nonnewtonian(gpu, orc, model):
+    """Carreau-Yasuda and Casson viscosity (GETVISCOSITY) in the fluid
+    element kernel against the oracle to 1e-12, with shear rates on both
+    sides of the models' transitions."""
+    prob = problems.fluid_cylinder(h=0.4)
+    rng = np.random.default_rng(6)
+    prob.Yg[:, :3] = (2.0 if model == "cy" else 0.1) * rng.normal(size=(prob.nNo, 3))
+    prob.Ag[:, :3] = rng.normal(size=(prob.nNo, 3))
+    prob.eq = dict(prob.eq, **VISC[model])
+    say("fluid assembly", model, prob.nNo, prob.mesh.nEl)
+    st, Vg, Rg, Vo, Ro = _assembled(prob, orc)
+    newt = problems.fluid_cylinder(h=0.4)
+    newt.Yg, newt.Ag = prob.Yg, prob.Ag
+    V0, _ = orc.assemble(newt, st.rowPtr - 1, st.colPtr - 1)
+    say("relerr", relerr(Vg, Vo), relerr(Rg, Ro), "vs Newtonian", relerr(Vo, V0))
+    assert relerr(Vo, V0) > 1e-3  # the model changed the matrix
+    assert relerr(Vg, Vo) < 1e-12
+    assert relerr(Rg, Ro) < 1e-12
+
+
+def test_viscosity_model_errors(gpu):
+    from svfsi_amd._lib import SvfsiError
+
+    prob = problems.fluid_cylinder(h=0.8)
+    prob.eq = dict(prob.eq, viscType=695)
+    st = NewtonStep(prob)
+    with pytest.raises(SvfsiError, match="viscType"):
+        st.assemble()
+
+
 @pytest.mark.parametrize("kern", [SVFSI_STRUCT_VALU, SVFSI_STRUCT_MFMA64])
 @pytest.mark.parametrize("quad", [0, 1])
 def test_struct_assembly_parity(gpu, orc, quad, kern):
