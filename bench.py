@@ -315,7 +315,10 @@ def main() -> int:
             del Vk
         prob.quad = base_quad
         log(f"cpu leg: solver samples in {c3 - c2:.1f} s; accuracy checks")
-        # FP64 scaled residual of each GPU solution, recomputed on the CPU
+        # FP64 residual of each GPU solution, recomputed on the CPU: in the
+        # reference's scaled norm |W (R - K x)| / |W R|, and unscaled,
+        # |f - K u| / |f| over the dofs that are not Dirichlet-masked
+        free = mask != 0
         for name, mixed in (("fp64", False), ("mixed", True)):
             st.assemble()
             st.solve(mixed=mixed)
@@ -324,6 +327,8 @@ def main() -> int:
             y = np.where(W != 0, xg / np.where(W != 0, W, 1.0), 0.0)
             r = Rs - orc.spmv(st.rowPtr - 1, st.colPtr - 1, dof, Vs, y)
             acc[f"scaled_rel_residual_{name}"] = float(np.linalg.norm(r) / np.linalg.norm(Rs))
+            ru = (Ro - orc.spmv(st.rowPtr - 1, st.colPtr - 1, dof, Vo, xg))[free]
+            acc[f"unscaled_rel_residual_{name}"] = float(np.linalg.norm(ru) / np.linalg.norm(Ro[free]))
         cpu = {
             "value": round(cpu_ms, 1), "unit": "ms/step", "cores": cores, "kind": "port",
             "sample": (f"C oracle on {cores} host cores (OpenMP threads standing in for MPI ranks): full FP64 "
