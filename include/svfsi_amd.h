@@ -345,6 +345,31 @@ int svfsi_prof_get(int kernel, long long* count, double* total_ms);
 int svfsi_prof_get_bytes(int kernel, double* bytes);
 int svfsi_prof_reset(void);
 
+/* ---------------------------------------------------------------------- */
+/* Mesh files (host): the input side of the path                           */
+/* ---------------------------------------------------------------------- */
+/* Replace READCCNE (svFSI/LOADMSH.f:125-239): the connectivity file has one
+ * element per line, "index node_1 .. node_eNoN" (1-based node ids), the
+ * coordinates file one node per line, "index x_1 .. x_nsd"; the widths come
+ * from the first lines (CheckNoNumbers).  Blank lines are skipped.
+ * svfsi_read_ccne fills ien(eNoN, nEl) (1-based, as read) and x(nsd, nNo). */
+int svfsi_read_ccne_sizes(const char* connPath, const char* coordPath, int* nEl, int* eNoN, int* nNo,
+                          int* nsd);
+int svfsi_read_ccne(const char* connPath, const char* coordPath, int nEl, int eNoN, int nNo, int nsd, int* ien,
+                    double* x);
+/* Replace the face-file branch of READCCNE (LOADMSH.f:196-216): one face
+ * element per line, "element index node_1 .. node_feNoN"; fills gE(nFe)
+ * and fien(feNoN, nFe) as read. */
+int svfsi_read_ebc_sizes(const char* path, int* nFe, int* feNoN);
+int svfsi_read_ebc(const char* path, int nFe, int feNoN, int* gE, int* fien);
+/* Replaces CHECKIEN (svFSI/READMSH.f:1010-1185) for tet4 and tet10 in 3-D,
+ * x(3, nNo): bounds and isolated-node checks, then every element whose
+ * (x4 - x3) . ((x2 - x1) x (x3 - x2)) is positive (READMSH.f:1131-1141)
+ * has nodes 1 and 2 swapped in place (and 6/7, 8/9 for tet10), which makes
+ * the isoparametric Jacobian of GNN positive; *nReordered (may be NULL)
+ * counts the swapped elements.  A zero sign is an error ("distorted"). */
+int svfsi_checkien(int eNoN, int nEl, int nNo, int* ien, const double* x, int* nReordered);
+
 #ifdef __cplusplus
 }
 #endif

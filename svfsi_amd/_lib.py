@@ -112,6 +112,11 @@ SIGNATURES = {
     "svfsi_lelas_dkde_dot_dev": [_P, C.POINTER(EqParams), _P, _P, _P, _P],
     "svfsi_fluid_dmu_dot_dev": [_P, C.POINTER(EqParams), _I, _P, _P, _P, _P, _P, _P, _P],
     "svfsi_poly_factors": [_I, _I, _DP, _DP, _IP, _I],
+    "svfsi_read_ccne_sizes": [C.c_char_p, C.c_char_p, _IP, _IP, _IP, _IP],
+    "svfsi_read_ccne": [C.c_char_p, C.c_char_p, _I, _I, _I, _I, _IP, _DP],
+    "svfsi_read_ebc_sizes": [C.c_char_p, _IP, _IP],
+    "svfsi_read_ebc": [C.c_char_p, _I, _I, _IP, _IP],
+    "svfsi_checkien": [_I, _I, _I, _IP, _DP, _IP],
     "svfsi_prof_enable": [_I],
     "svfsi_prof_get": [_I, C.POINTER(C.c_longlong), _DP],
     "svfsi_prof_get_bytes": [_I, C.POINTER(C.c_double)],
