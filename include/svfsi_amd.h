@@ -126,7 +126,8 @@ int fsils_gpu_bc_free(void* lhs, int faIn);
  * exit; Val(dof*dof,nnz) is left scaled in place (W K W), as the reference
  * does (SOLVE.f:57, PRECOND.f:124-130).  prec PRECOND_FSILS (diagonal
  * scaling) or PRECOND_RCS (row/column equilibration, PRECOND.f:150-368);
- * LS_type GMRES or CG (dof >= 2), or NS (dof 4: NSSOLVER,
+ * LS_type GMRES, CG or BICGS (BICGSV, svFSILS/BICGS.f:115-180) (dof >= 2),
+ * or NS (dof 4: NSSOLVER,
  * svFSILS/NSSOLVER.f:52-383, with the ls%GM and ls%CG records, filling
  * ls%Resm/Resc).  incL(nFaces) may be NULL (all faces included).
  * res(nFaces): a BC_TYPE_Neu face with res != 0 adds res v v^T (v its face

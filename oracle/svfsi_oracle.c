@@ -1082,6 +1082,61 @@ static void orc_addbcmul(const orc_cpl* c, int op, int vd, const double* X, doub
     }
 }
 
+/* BICGSV (svFSILS/BICGS.f:115-180): unpreconditioned BiCGStab on the
+ * scaled system; R in: right-hand side, out: solution. */
+void orc_bicgsv(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* K, double* R,
+                orc_subls* ls)
+{
+    const size_t n = (size_t)nNo * dof;
+    double* P = malloc(sizeof(double) * n);
+    double* Rh = malloc(sizeof(double) * n);
+    double* X = calloc(n, sizeof(double));
+    double* V = malloc(sizeof(double) * n);
+    double* S = malloc(sizeof(double) * n);
+    double* T = malloc(sizeof(double) * n);
+    ls->callD = wall_s();
+    ls->suc = 0;
+    double err = orc_norm((int)n, R);
+    double errO = err;
+    ls->iNorm = err;
+    const double eps = fmax(ls->absTol, ls->relTol * err);
+    double rho = err * err;
+    memcpy(P, R, sizeof(double) * n);
+    memcpy(Rh, R, sizeof(double) * n);
+    int i;
+    for (i = 1; i <= ls->mItr; ++i) {
+        if (err < eps) {
+            ls->suc = 1;
+            break;
+        }
+        orc_spmv(nNo, dof, rowPtr, colPtr, K, P, V);
+        const double alpha = rho / orc_dot((int)n, Rh, V);
+        ORC_PAR
+        for (size_t q = 0; q < n; ++q) S[q] = R[q] - alpha * V[q];
+        orc_spmv(nNo, dof, rowPtr, colPtr, K, S, T);
+        double omega = orc_norm((int)n, T);
+        omega = orc_dot((int)n, T, S) / (omega * omega);
+        ORC_PAR
+        for (size_t q = 0; q < n; ++q) {
+            X[q] = X[q] + alpha * P[q] + omega * S[q];
+            R[q] = S[q] - omega * T[q];
+        }
+        errO = err;
+        err = orc_norm((int)n, R);
+        const double rhoO = rho;
+        rho = orc_dot((int)n, R, Rh);
+        const double beta = rho * alpha / (rhoO * omega);
+        ORC_PAR
+        for (size_t q = 0; q < n; ++q) P[q] = R[q] + beta * (P[q] - omega * V[q]);
+    }
+    memcpy(R, X, sizeof(double) * n);
+    ls->itr = i - 1;
+    ls->fNorm = err;
+    ls->callD = wall_s() - ls->callD;
+    ls->dB = (errO < EPS) ? 0.0 : 10.0 * log(err / errO);
+    free(P); free(Rh); free(X); free(V); free(S); free(T);
+}
+
 static void gmresv_impl(int nNo, int dof, const int* rowPtr, const int* colPtr, const double* K, double* R,
                         orc_subls* ls, const orc_cpl* cp);
 
@@ -1709,6 +1764,7 @@ int orc_fsils_solve_cpl(int nNo, int dof, const int* rowPtr, const int* colPtr, 
     if (LS_type == 796) rc = nssolver_impl(nNo, dof, rowPtr, colPtr, Val, Ri, ls, &c);
     else if (LS_type == 797) gmresv_impl(nNo, dof, rowPtr, colPtr, Val, Ri, &ls->RI, &c);
     else if (LS_type == 798) orc_cgradv(nNo, dof, rowPtr, colPtr, Val, Ri, &ls->RI); /* CGRADV has no coupled terms */
+    else if (LS_type == 795) orc_bicgsv(nNo, dof, rowPtr, colPtr, Val, Ri, &ls->RI); /* nor BICGSV */
     else rc = -2;
     for (size_t q = 0; q < n; ++q) Ri[q] = W[q] * Ri[q];
     free(W); free(vm); free(nS); free(diagPtr);
@@ -1732,6 +1788,7 @@ int orc_fsils_solve_prec(int nNo, int dof, const int* rowPtr, const int* colPtr,
         orc_precond_diag(nNo, dof, rowPtr, colPtr, diagPtr, dirMask, Val, Ri, W);
     if (LS_type == 797) orc_gmresv(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
     else if (LS_type == 798) orc_cgradv(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
+    else if (LS_type == 795) orc_bicgsv(nNo, dof, rowPtr, colPtr, Val, Ri, ls);
     else { free(W); free(W1); free(diagPtr); return -2; }
     for (size_t q = 0; q < n; ++q) Ri[q] = W[q] * Ri[q];
     free(W); free(W1); free(diagPtr);

@@ -158,7 +158,10 @@ void   orc_cgradv(int nNo, int dof, const int* rowPtr, const int* colPtr,
                   const double* K, double* R, orc_subls* ls);
 void   orc_gmresv(int nNo, int dof, const int* rowPtr, const int* colPtr,
                   const double* K, double* R, orc_subls* ls);
-/* FSILS_SOLVE for LS_TYPE_GMRES (797) or LS_TYPE_CG (798) with PRECOND_FSILS.
+void   orc_bicgsv(int nNo, int dof, const int* rowPtr, const int* colPtr,
+                  const double* K, double* R, orc_subls* ls);
+/* FSILS_SOLVE for LS_TYPE_GMRES (797), LS_TYPE_CG (798) or, through
+ * orc_fsils_solve_prec, LS_TYPE_BICGS (795), with PRECOND_FSILS.
  * dirMask[nNo][dof]: product of the Dirichlet face values (1 elsewhere). */
 int    orc_fsils_solve(int nNo, int dof, const int* rowPtr, const int* colPtr,
                        const double* dirMask, int LS_type, orc_subls* ls,

@@ -235,8 +235,9 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
                   PRECOND_RCS);
         return SVFSI_ERR_NOTSUP;
     }
-    if (ls->LS_type != LS_TYPE_GMRES && ls->LS_type != LS_TYPE_CG && ls->LS_type != LS_TYPE_NS) {
-        set_error("LS_type %d not supported (NS 796, GMRES 797, CG 798)", ls->LS_type);
+    if (ls->LS_type != LS_TYPE_GMRES && ls->LS_type != LS_TYPE_CG && ls->LS_type != LS_TYPE_NS &&
+        ls->LS_type != LS_TYPE_BICGS) {
+        set_error("LS_type %d not supported (BICGS 795, NS 796, GMRES 797, CG 798)", ls->LS_type);
         return SVFSI_ERR_NOTSUP;
     }
     if (ls->LS_type == LS_TYPE_NS && (mode != 0 || dof != 4)) {
@@ -250,7 +251,8 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
         ~CplOff() { c.active = false; }
     } cplOff{L.cpl};
     SV_TRY(cpl_setup(L, dof, incL, res, s));
-    if (ls->LS_type == LS_TYPE_CG) L.cpl.active = false;
+    // BICGSV has no coupled terms either
+    if (ls->LS_type == LS_TYPE_CG || ls->LS_type == LS_TYPE_BICGS) L.cpl.active = false;
     if (L.cpl.active && prec == PRECOND_RCS) {
         set_error("coupled Neumann faces need PRECOND_FSILS: PRECONDRCS leaves the scaled face vectors unset "
                   "(PRECOND.f:356-364)");
@@ -277,6 +279,7 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
     if (mode == 1) rc = mixed_solve(L, ls->RI, dof, Val, v32, R, s);
     else if (ls->LS_type == LS_TYPE_NS) rc = ns_solve(L, *ls, dof, Val, R, s);
     else if (ls->LS_type == LS_TYPE_GMRES) rc = gmres_solve(L, ls->RI, dof, Val, R, s, false, L.cpl.active ? 1 : 0);
+    else if (ls->LS_type == LS_TYPE_BICGS) rc = bicgs_solve(L, ls->RI, dof, Val, R, s);
     else rc = cg_solve(L, ls->RI, dof, Val, R, s);
     if (rc != SVFSI_OK) return rc;
     SV_TRY(launch_scale_vec(n, w.W.as<double>(), R, s));

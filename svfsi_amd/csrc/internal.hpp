@@ -119,6 +119,7 @@ struct Work {
     DBuf mask;
     DBuf rcs, W1;  // PRECOND_RCS: row/column factors, partial maxima; W1 (row scaling)
     DBuf ns;       // NS solver: the split blocks, its vectors and directions
+    DBuf bicgs;    // BiCGStab: P, Rh, X, V, S, T and the dot partials
     DBuf valT; // K^T values for transposed solves
     DBuf dinv32; // inverse diagonal blocks (FP32) of the inner block-Jacobi
     DBuf polyCoef, polyVec; // polynomial preconditioner: factors, work vectors
@@ -267,6 +268,8 @@ int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s);
 
 int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
              double* R, hipStream_t s);
+// BICGSV (svFSILS/BICGS.f:115-180): R in (scaled RHS), out (solution).
+int bicgs_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s);
 // CGRAD_SCHUR (svFSILS/CGRAD.f:51-123) on the device: (L - Gt G) x = R with
 // G (3, nnz), Gt (3, nnz), Lm (nnz); R in: right-hand side, out: solution.
 // X, P, SP (nNo) and GP (3, nNo) are work vectors; ls.itr and ls.callD

@@ -16,12 +16,14 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BC_TYPE_Dir, BC_TYPE_Neu, FSILS_lsType, LS_TYPE_CG, LS_TYPE_GMRES, LS_TYPE_NS, PRECOND_FSILS,
+from ._lib import (BC_TYPE_Dir, BC_TYPE_Neu, FSILS_lsType, LS_TYPE_BICGS, LS_TYPE_CG, LS_TYPE_GMRES, LS_TYPE_NS,
+                   PRECOND_FSILS,
                    PRECOND_RCS, SvfsiError, call, lib)
 
 __all__ = [
     "Lhs", "fsils_lhs_create", "fsils_bc_create", "fsils_ls_create", "fsils_solve", "lhsa",
-    "BC_TYPE_Dir", "BC_TYPE_Neu", "LS_TYPE_CG", "LS_TYPE_GMRES", "LS_TYPE_NS", "PRECOND_FSILS", "PRECOND_RCS",
+    "BC_TYPE_Dir", "BC_TYPE_Neu", "LS_TYPE_BICGS", "LS_TYPE_CG", "LS_TYPE_GMRES", "LS_TYPE_NS", "PRECOND_FSILS",
+    "PRECOND_RCS",
 ]
 
 
