@@ -363,6 +363,18 @@ int svfsi_read_ccne(const char* connPath, const char* coordPath, int nEl, int eN
  * and fien(feNoN, nFe) as read. */
 int svfsi_read_ebc_sizes(const char* path, int* nFe, int* feNoN);
 int svfsi_read_ebc(const char* path, int nFe, int feNoN, int* gE, int* fien);
+/* VTK XML meshes and faces, the .vtu/.vtp files svFSI reads
+ * (svFSI/VTKXML.f, vtkXMLParser.f90): UnstructuredGrid cells or PolyData
+ * polys of one size, 3-component Points, optional PointData
+ * "GlobalNodeID" and CellData "GlobalElementID"; DataArrays in ascii,
+ * binary (base64) or appended (raw or base64) form, UInt32 or UInt64
+ * headers, optionally vtkZLibDataCompressor blocks, little-endian.
+ * svfsi_read_vtk fills x(3, nPoints), conn(nodesPerCell, nCells) (0-based,
+ * as stored) and, when not NULL, the two id arrays. */
+int svfsi_read_vtk_sizes(const char* path, int* nPoints, int* nCells, int* nodesPerCell, int* hasNodeIds,
+                         int* hasElementIds);
+int svfsi_read_vtk(const char* path, int nPoints, int nCells, int nodesPerCell, double* x, int* conn,
+                   int* nodeIds, int* elementIds);
 /* Replaces CHECKIEN (svFSI/READMSH.f:1010-1185) for tet4 and tet10 in 3-D,
  * x(3, nNo): bounds and isolated-node checks, then every element whose
  * (x4 - x3) . ((x2 - x1) x (x3 - x2)) is positive (READMSH.f:1131-1141)

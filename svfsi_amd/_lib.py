@@ -117,6 +117,8 @@ SIGNATURES = {
     "svfsi_read_ebc_sizes": [C.c_char_p, _IP, _IP],
     "svfsi_read_ebc": [C.c_char_p, _I, _I, _IP, _IP],
     "svfsi_checkien": [_I, _I, _I, _IP, _DP, _IP],
+    "svfsi_read_vtk_sizes": [C.c_char_p, _IP, _IP, _IP, _IP, _IP],
+    "svfsi_read_vtk": [C.c_char_p, _I, _I, _I, _DP, _IP, _IP, _IP],
     "svfsi_prof_enable": [_I],
     "svfsi_prof_get": [_I, C.POINTER(C.c_longlong), _DP],
     "svfsi_prof_get_bytes": [_I, C.POINTER(C.c_double)],

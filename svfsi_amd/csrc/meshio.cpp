@@ -5,11 +5,15 @@
 // Host code: the files are read once, before anything reaches the device.
 #include "internal.hpp"
 
+#include <zlib.h>
+
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
-#include <utility>
+#include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace svfsi {
@@ -83,6 +87,311 @@ int to_int(double v, int* out)
 
 int sgn(double v) { return v > 0.0 ? 1 : (v < 0.0 ? -1 : 0); }
 
+// ---- VTK XML (.vtu UnstructuredGrid, .vtp PolyData), the files svFSI's
+// vtkXMLParser reads for meshes and faces (svFSI/VTKXML.f) ----
+
+struct Tag {
+    size_t begin = std::string::npos, end = 0; // tag text [begin, end)
+    size_t body = 0, bodyEnd = 0;              // element content
+    std::string attrs;
+};
+
+std::string attr(const std::string& attrs, const char* name)
+{
+    const std::string key = std::string(" ") + name + "=\"";
+    size_t p = (" " + attrs).find(key);
+    if (p == std::string::npos) return "";
+    p += key.size() - 1; // index in attrs (the added space shifts by one)
+    const size_t q = attrs.find('"', p);
+    return q == std::string::npos ? "" : attrs.substr(p, q - p);
+}
+
+// next <name ...> at or after `from`, with its content up to </name>
+Tag find_tag(const std::string& s, const char* name, size_t from, size_t limit)
+{
+    Tag t;
+    const std::string open = std::string("<") + name;
+    for (size_t p = s.find(open, from); p != std::string::npos && p < limit; p = s.find(open, p + 1)) {
+        const char c = s[p + open.size()];
+        if (c != ' ' && c != '>' && c != '/' && c != '\n' && c != '\t' && c != '\r') continue;
+        const size_t gt = s.find('>', p);
+        if (gt == std::string::npos) break;
+        t.begin = p;
+        t.attrs = s.substr(p + open.size(), gt - p - open.size());
+        for (char& ch : t.attrs)
+            if (ch == '\n' || ch == '\t' || ch == '\r') ch = ' ';
+        t.body = gt + 1;
+        if (s[gt - 1] == '/') {
+            t.bodyEnd = t.body;
+            t.end = gt + 1;
+        } else {
+            const size_t close = s.find(std::string("</") + name, gt);
+            t.bodyEnd = close == std::string::npos ? s.size() : close;
+            t.end = t.bodyEnd;
+        }
+        return t;
+    }
+    return t;
+}
+
+int b64val(char c)
+{
+    if (c >= 'A' && c <= 'Z') return c - 'A';
+    if (c >= 'a' && c <= 'z') return c - 'a' + 26;
+    if (c >= '0' && c <= '9') return c - '0' + 52;
+    if (c == '+') return 62;
+    if (c == '/') return 63;
+    return -1;
+}
+
+// base64 in groups of four characters, each group decoded on its own, so
+// separately encoded streams that were concatenated (VTK's compressed
+// header and blocks) decode too; whitespace is skipped.
+std::vector<uint8_t> b64decode(const char* p, const char* e)
+{
+    std::vector<uint8_t> out;
+    int q[4], k = 0;
+    for (; p < e; ++p) {
+        if (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t') continue;
+        q[k++] = *p == '=' ? -2 : b64val(*p);
+        if (k < 4) continue;
+        k = 0;
+        if (q[0] < 0 || q[1] < 0) break;
+        out.push_back((uint8_t)((q[0] << 2) | (q[1] >> 4)));
+        if (q[2] >= 0) out.push_back((uint8_t)(((q[1] & 15) << 4) | (q[2] >> 2)));
+        if (q[2] >= 0 && q[3] >= 0) out.push_back((uint8_t)(((q[2] & 3) << 6) | q[3]));
+    }
+    return out;
+}
+
+size_t type_size(const std::string& t)
+{
+    if (t == "Int8" || t == "UInt8") return 1;
+    if (t == "Int16" || t == "UInt16") return 2;
+    if (t == "Int32" || t == "UInt32" || t == "Float32") return 4;
+    if (t == "Int64" || t == "UInt64" || t == "Float64") return 8;
+    return 0;
+}
+
+double typed_at(const uint8_t* b, const std::string& t)
+{
+    if (t == "Float64") { double v; std::memcpy(&v, b, 8); return v; }
+    if (t == "Float32") { float v; std::memcpy(&v, b, 4); return v; }
+    if (t == "Int64") { int64_t v; std::memcpy(&v, b, 8); return (double)v; }
+    if (t == "UInt64") { uint64_t v; std::memcpy(&v, b, 8); return (double)v; }
+    if (t == "Int32") { int32_t v; std::memcpy(&v, b, 4); return v; }
+    if (t == "UInt32") { uint32_t v; std::memcpy(&v, b, 4); return v; }
+    if (t == "Int16") { int16_t v; std::memcpy(&v, b, 2); return v; }
+    if (t == "UInt16") { uint16_t v; std::memcpy(&v, b, 2); return v; }
+    if (t == "Int8") return (double)(int8_t)b[0];
+    return (double)b[0];
+}
+
+struct VtkFile {
+    std::string s, path;
+    size_t hdr = 4;        // header integer size (UInt32 or UInt64)
+    bool zlib = false;
+    size_t app = std::string::npos; // first byte after the '_' of AppendedData
+    bool appB64 = false;
+};
+
+uint64_t read_uint(const uint8_t* b, size_t n)
+{
+    if (n == 8) { uint64_t v; std::memcpy(&v, b, 8); return v; }
+    uint32_t v; std::memcpy(&v, b, 4); return v;
+}
+
+// payload bytes of a binary block (header + data, optionally zlib blocks)
+int unpack_block(const VtkFile& f, const uint8_t* b, size_t avail, std::vector<uint8_t>& out, size_t* used)
+{
+    const size_t h = f.hdr;
+    if (!f.zlib) {
+        if (avail < h) return SVFSI_ERR_ARG;
+        const uint64_t n = read_uint(b, h);
+        if (avail < h + n) return SVFSI_ERR_ARG;
+        out.assign(b + h, b + h + n);
+        if (used) *used = h + n;
+        return SVFSI_OK;
+    }
+    if (avail < 3 * h) return SVFSI_ERR_ARG;
+    const uint64_t nb = read_uint(b, h), bs = read_uint(b + h, h), last = read_uint(b + 2 * h, h);
+    if (avail < (3 + nb) * h) return SVFSI_ERR_ARG;
+    size_t off = (3 + nb) * h;
+    out.clear();
+    for (uint64_t k = 0; k < nb; ++k) {
+        const uint64_t cs = read_uint(b + (3 + k) * h, h);
+        const uint64_t us = (k + 1 == nb && last) ? last : bs;
+        if (avail < off + cs) return SVFSI_ERR_ARG;
+        const size_t o = out.size();
+        out.resize(o + us);
+        uLongf dl = (uLongf)us;
+        if (uncompress(out.data() + o, &dl, b + off, (uLong)cs) != Z_OK || dl != us) return SVFSI_ERR_ARG;
+        off += cs;
+    }
+    if (used) *used = off;
+    return SVFSI_OK;
+}
+
+// the values of one DataArray as doubles
+int array_values(const VtkFile& f, const Tag& t, std::vector<double>& v)
+{
+    const std::string type = attr(t.attrs, "type"), fmt = attr(t.attrs, "format");
+    const size_t ts = type_size(type);
+    if (!ts) {
+        set_error("%s: DataArray type '%s' not supported", f.path.c_str(), type.c_str());
+        return SVFSI_ERR_ARG;
+    }
+    v.clear();
+    std::vector<uint8_t> raw;
+    if (fmt == "ascii") {
+        std::vector<double> tmp;
+        if (!parse_line(f.s.substr(t.body, t.bodyEnd - t.body), tmp)) {
+            set_error("%s: bad ascii DataArray", f.path.c_str());
+            return SVFSI_ERR_ARG;
+        }
+        v = tmp;
+        return SVFSI_OK;
+    } else if (fmt == "binary") {
+        // one base64 stream, or (compressed) the header's and the blocks'
+        // streams one after the other: b64decode takes both
+        const std::vector<uint8_t> enc = b64decode(f.s.data() + t.body, f.s.data() + t.bodyEnd);
+        if (unpack_block(f, enc.data(), enc.size(), raw, nullptr) != SVFSI_OK) {
+            set_error("%s: bad binary DataArray", f.path.c_str());
+            return SVFSI_ERR_ARG;
+        }
+    } else if (fmt == "appended") {
+        if (f.app == std::string::npos) {
+            set_error("%s: appended DataArray without AppendedData", f.path.c_str());
+            return SVFSI_ERR_ARG;
+        }
+        const size_t off = (size_t)std::strtoull(attr(t.attrs, "offset").c_str(), nullptr, 10);
+        if (f.appB64) {
+            const size_t e = f.s.find('<', f.app + off);
+            const std::vector<uint8_t> enc = b64decode(f.s.data() + f.app + off, f.s.data() + e);
+            if (unpack_block(f, enc.data(), enc.size(), raw, nullptr) != SVFSI_OK) {
+                set_error("%s: bad appended DataArray", f.path.c_str());
+                return SVFSI_ERR_ARG;
+            }
+        } else {
+            const uint8_t* b = reinterpret_cast<const uint8_t*>(f.s.data()) + f.app + off;
+            if (f.app + off > f.s.size() ||
+                unpack_block(f, b, f.s.size() - f.app - off, raw, nullptr) != SVFSI_OK) {
+                set_error("%s: bad appended DataArray", f.path.c_str());
+                return SVFSI_ERR_ARG;
+            }
+        }
+    } else {
+        set_error("%s: DataArray format '%s' not supported", f.path.c_str(), fmt.c_str());
+        return SVFSI_ERR_ARG;
+    }
+    v.resize(raw.size() / ts);
+    for (size_t k = 0; k < v.size(); ++k) v[k] = typed_at(raw.data() + k * ts, type);
+    return SVFSI_OK;
+}
+
+int vtk_open(const char* path, VtkFile& f)
+{
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) {
+        set_error("cannot open %s", path);
+        return SVFSI_ERR_ARG;
+    }
+    std::fseek(fp, 0, SEEK_END);
+    const long n = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    f.s.resize(n > 0 ? (size_t)n : 0);
+    const size_t got = n > 0 ? std::fread(&f.s[0], 1, (size_t)n, fp) : 0;
+    std::fclose(fp);
+    f.path = path;
+    if (got != f.s.size()) {
+        set_error("cannot read %s", path);
+        return SVFSI_ERR_ARG;
+    }
+    const Tag vf = find_tag(f.s, "VTKFile", 0, f.s.size());
+    if (vf.begin == std::string::npos) {
+        set_error("%s: not a VTK XML file", path);
+        return SVFSI_ERR_ARG;
+    }
+    const std::string bo = attr(vf.attrs, "byte_order");
+    if (!bo.empty() && bo != "LittleEndian") {
+        set_error("%s: big-endian data not supported", path);
+        return SVFSI_ERR_ARG;
+    }
+    f.hdr = attr(vf.attrs, "header_type") == "UInt64" ? 8 : 4;
+    const std::string comp = attr(vf.attrs, "compressor");
+    if (!comp.empty() && comp != "vtkZLibDataCompressor") {
+        set_error("%s: compressor %s not supported", path, comp.c_str());
+        return SVFSI_ERR_ARG;
+    }
+    f.zlib = !comp.empty();
+    const Tag ad = find_tag(f.s, "AppendedData", 0, f.s.size());
+    if (ad.begin != std::string::npos) {
+        f.appB64 = attr(ad.attrs, "encoding") == "base64";
+        const size_t u = f.s.find('_', ad.body);
+        if (u != std::string::npos) f.app = u + 1;
+    }
+    return SVFSI_OK;
+}
+
+// DataArray named `name` inside the first <section> (empty name: the first)
+int section_array(const VtkFile& f, const char* section, const char* name, std::vector<double>& v, bool* found)
+{
+    if (found) *found = false;
+    const Tag sec = find_tag(f.s, section, 0, f.app == std::string::npos ? f.s.size() : f.app);
+    if (sec.begin == std::string::npos) return SVFSI_OK;
+    for (size_t p = sec.body;;) {
+        const Tag da = find_tag(f.s, "DataArray", p, sec.bodyEnd);
+        if (da.begin == std::string::npos) break;
+        if (!*name || attr(da.attrs, "Name") == name) {
+            if (found) *found = true;
+            return array_values(f, da, v);
+        }
+        p = da.end;
+    }
+    return SVFSI_OK;
+}
+
+struct VtkMesh {
+    std::vector<double> x, conn, offs, gnid, geid;
+    bool hasN = false, hasE = false;
+    int npc = 0, nCells = 0;
+};
+
+int vtk_mesh(const char* path, VtkMesh& m)
+{
+    VtkFile f;
+    SV_TRY(vtk_open(path, f));
+    bool ok = false;
+    SV_TRY(section_array(f, "Points", "", m.x, &ok));
+    if (!ok || m.x.size() % 3) {
+        set_error("%s: no 3-component Points array", path);
+        return SVFSI_ERR_ARG;
+    }
+    const bool poly = find_tag(f.s, "PolyData", 0, f.s.size()).begin != std::string::npos;
+    const char* cells = poly ? "Polys" : "Cells";
+    SV_TRY(section_array(f, cells, "connectivity", m.conn, &ok));
+    bool ok2 = false;
+    SV_TRY(section_array(f, cells, "offsets", m.offs, &ok2));
+    if (!ok || !ok2 || m.offs.empty()) {
+        set_error("%s: no %s connectivity/offsets", path, cells);
+        return SVFSI_ERR_ARG;
+    }
+    m.nCells = (int)m.offs.size();
+    m.npc = (int)m.offs[0];
+    for (int c = 0; c < m.nCells; ++c)
+        if ((int)m.offs[c] != (c + 1) * m.npc) {
+            set_error("%s: cells of different sizes", path);
+            return SVFSI_ERR_ARG;
+        }
+    if ((int)m.conn.size() != m.nCells * m.npc) {
+        set_error("%s: connectivity size does not match the offsets", path);
+        return SVFSI_ERR_ARG;
+    }
+    SV_TRY(section_array(f, "PointData", "GlobalNodeID", m.gnid, &m.hasN));
+    SV_TRY(section_array(f, "CellData", "GlobalElementID", m.geid, &m.hasE));
+    return SVFSI_OK;
+}
+
 } // namespace
 
 } // namespace svfsi
@@ -150,6 +459,44 @@ int svfsi_read_ebc(const char* path, int nFe, int feNoN, int* gE, int* fien)
             set_error("face element %d: not integers", e + 1);
             return SVFSI_ERR_ARG;
         }
+    }
+    return SVFSI_OK;
+}
+
+int svfsi_read_vtk_sizes(const char* path, int* nPoints, int* nCells, int* nodesPerCell, int* hasNodeIds,
+                         int* hasElementIds)
+{
+    SV_ARG(path && nPoints && nCells && nodesPerCell, "NULL argument");
+    VtkMesh m;
+    SV_TRY(vtk_mesh(path, m));
+    *nPoints = (int)(m.x.size() / 3);
+    *nCells = m.nCells;
+    *nodesPerCell = m.npc;
+    if (hasNodeIds) *hasNodeIds = m.hasN;
+    if (hasElementIds) *hasElementIds = m.hasE;
+    return SVFSI_OK;
+}
+
+int svfsi_read_vtk(const char* path, int nPoints, int nCells, int nodesPerCell, double* x, int* conn,
+                   int* nodeIds, int* elementIds)
+{
+    SV_ARG(path && x && conn, "NULL argument");
+    VtkMesh m;
+    SV_TRY(vtk_mesh(path, m));
+    SV_ARG((int)(m.x.size() / 3) == nPoints && m.nCells == nCells && m.npc == nodesPerCell,
+           "sizes do not match the file");
+    for (size_t k = 0; k < m.x.size(); ++k) x[k] = m.x[k];
+    for (size_t k = 0; k < m.conn.size(); ++k) {
+        SV_ARG(m.conn[k] >= 0 && m.conn[k] < nPoints, "cell node out of range");
+        conn[k] = (int)m.conn[k];
+    }
+    if (nodeIds) {
+        SV_ARG(m.hasN && (int)m.gnid.size() == nPoints, "no GlobalNodeID point array");
+        for (int k = 0; k < nPoints; ++k) nodeIds[k] = (int)m.gnid[k];
+    }
+    if (elementIds) {
+        SV_ARG(m.hasE && (int)m.geid.size() == nCells, "no GlobalElementID cell array");
+        for (int k = 0; k < nCells; ++k) elementIds[k] = (int)m.geid[k];
     }
     return SVFSI_OK;
 }

@@ -62,3 +62,18 @@ def load_mesh(conn_path: str, coord_path: str) -> TetMesh:
     """A TetMesh from READCCNE files, CHECKIEN applied."""
     ien, x, _ = read_ccne(conn_path, coord_path)
     return TetMesh(x=x, ien=ien)
+
+
+def read_vtk(path: str):
+    """(x (nPoints, 3), cells (nCells, nodesPerCell) 0-based, GlobalNodeID or
+    None, GlobalElementID or None) from a .vtu or .vtp file."""
+    n = [C.c_int() for _ in range(5)]
+    call("svfsi_read_vtk_sizes", path.encode(), *[C.byref(v) for v in n])
+    nP, nC, npc, hasN, hasE = (v.value for v in n)
+    x = np.zeros((nP, 3))
+    conn = np.zeros((nC, npc), dtype=np.int32)
+    gn = np.zeros(nP, dtype=np.int32) if hasN else None
+    ge = np.zeros(nC, dtype=np.int32) if hasE else None
+    call("svfsi_read_vtk", path.encode(), nP, nC, npc, _dp(x), _ip(conn), None if gn is None else _ip(gn),
+         None if ge is None else _ip(ge))
+    return x, conn, gn, ge
