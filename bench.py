@@ -13,8 +13,10 @@ scaled norm.  Inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line.  `value` is the mixed-precision path's time per
 Newton step (ms, lower is better); the FP64 path, the SpMV roofline and the
-CPU baseline (the FP64 C oracle on this host, one core) are reported beside
-it.
+CPU baseline (the FP64 C oracle on this host's cores, one thread per core
+standing in for the reference's MPI ranks; one core beside it) are reported
+beside it.  `--ls ns` measures the same Newton step with the NS solver
+(FP64, the reference's NSSOLVER defaults) as a separate line.
 """
 from __future__ import annotations
 
