@@ -7,9 +7,10 @@
 // U_i = K^-1 (Rm - G P_i) (GMRES on the momentum block), forms their images
 // under the full matrix, and solves the Gram system of all images so far
 // for the least-squares combination (GE).  The block products, the vector
-// updates and the dots run on the device; the outer loop, the Gram matrix
-// and GE (at most 2 mItr unknowns) on the host, one read-back per dot as
-// the reference has one allreduce per dot.
+// updates, the dots and the Schur-complement CG run on the device; the
+// outer loop and GE (at most 2 mItr unknowns) on the host, with all Gram
+// entries of an outer iteration brought back in one read-back (the
+// reference makes one allreduce per dot).
 #include "internal.hpp"
 #include "devutil.hpp"
 
