@@ -272,10 +272,10 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
 int bicgs_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s);
 // CGRAD_SCHUR (svFSILS/CGRAD.f:51-123) on the device: (L - Gt G) x = R with
 // G (3, nnz), Gt (3, nnz), Lm (nnz); R in: right-hand side, out: solution.
-// X, P, SP (nNo) and GP (3, nNo) are work vectors; ls.itr and ls.callD
+// X, P, P2, SP (nNo) and GP (3, nNo) are work vectors; ls.itr and ls.callD
 // accumulate, as the reference's do.
 int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* G, const double* Lm, double* R,
-                   double* X, double* P, double* SP, double* GP, hipStream_t s);
+                   double* X, double* P, double* P2, double* SP, double* GP, hipStream_t s);
 // cpl: coupled faces in the operator, 0 none, 1 ADDBCMUL ADD (GMRESV),
 // 2 ADD then PRE (the NS solver's GMRES; the caller applies PRE to R)
 int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,

@@ -448,43 +448,14 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
 // pressure Schur complement of the NS solver (Gt = -G^T blockwise, so
 // S = L + G^T G).  The same device-resident iteration as cg_solve (the
 // update kernels are CGRADV's: CGRAD_SCHUR makes the same scalar and vector
-// steps), with the operator in two row passes: GP = G P, then
-// SP = L P - Gt GP and the partial sums of P . SP.
+// steps), with the operator in two row passes: GP = G P (k_cgs_pgp, which
+// also makes the previous iteration's P update), then SP = L P - Gt GP and
+// the partial sums of P . SP (k_cgs_sp).  Three launches per iteration.
 // ---------------------------------------------------------------------------
 // LANES lanes per row (a tet4 node has about 15 to 18 neighbours): lane l
 // takes the row's blocks l, l+LANES, ..., so a
 // wavefront's loads of G, Gt, L and colPtr are contiguous; the partial sums
 // meet by shuffles.
-template <int LANES>
-__global__ __launch_bounds__(256) void k_cgs_gp(int nNo, const int* __restrict__ rowPtr,
-                                                const int* __restrict__ colPtr, const double* __restrict__ G,
-                                                const double* __restrict__ P, double* __restrict__ GP,
-                                                const CgState* __restrict__ st)
-{
-    if (st->done) return;
-    const int lane = threadIdx.x % LANES;
-    const int rowsPerGrid = gridDim.x * (256 / LANES);
-    for (int i = blockIdx.x * (256 / LANES) + threadIdx.x / LANES; i < nNo; i += rowsPerGrid) {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int j = rowPtr[i] + lane; j < rowPtr[i + 1]; j += LANES) {
-            const double u = P[colPtr[j]];
-            a0 += G[(size_t)j * 3] * u;
-            a1 += G[(size_t)j * 3 + 1] * u;
-            a2 += G[(size_t)j * 3 + 2] * u;
-        }
-        for (int o = LANES / 2; o > 0; o >>= 1) {
-            a0 += __shfl_xor(a0, o, LANES);
-            a1 += __shfl_xor(a1, o, LANES);
-            a2 += __shfl_xor(a2, o, LANES);
-        }
-        if (lane == 0) {
-            GP[(size_t)i * 3] = a0;
-            GP[(size_t)i * 3 + 1] = a1;
-            GP[(size_t)i * 3 + 2] = a2;
-        }
-    }
-}
-
 template <int LANES>
 __global__ __launch_bounds__(256) void k_cgs_sp(int nNo, const int* __restrict__ rowPtr,
                                                 const int* __restrict__ colPtr, const double* __restrict__ Lm,
@@ -520,8 +491,73 @@ __global__ __launch_bounds__(256) void k_cgs_sp(int nNo, const int* __restrict__
     if (threadIdx.x == 0) part[blockIdx.x] = sb;
 }
 
+// The P update of the iteration just finished (k_cg_pupdate's step) fused
+// with the next iteration's GP = G P: every block sums the |R|^2 partials
+// itself, so each row can form the new P of its neighbours on the fly,
+// (Pin + (errO/err) R) * (err/errO) with the same operations as the row
+// that owns it; the owner writes it to Pout (a second buffer: other blocks
+// still read Pin).  Block 0 takes over k_cg_pupdate's bookkeeping.  With
+// first set (the solve's first iteration) P is Pin unchanged.
+template <int LANES>
+__global__ __launch_bounds__(256) void k_cgs_pgp(int nNo, const int* __restrict__ rowPtr,
+                                                 const int* __restrict__ colPtr, const double* __restrict__ G,
+                                                 const double* __restrict__ Pin, const double* __restrict__ R,
+                                                 const double* __restrict__ part2, int m2, double* __restrict__ Pout,
+                                                 double* __restrict__ GP, CgState* __restrict__ st, int par, int first)
+{
+    __shared__ double lds[4];
+    __shared__ int done_s;
+    if (threadIdx.x == 0) done_s = *(volatile int*)&st->done;
+    __syncthreads();
+    if (done_s) return;
+    double r1 = 0.0, r2 = 1.0;
+    if (!first) {
+        const double t = sum_partials(part2, m2, lds);
+        const double nrm = sqrt(t);
+        const double err = nrm * nrm;
+        const double errO = st->e[par ^ 1];
+        r1 = errO / err;
+        r2 = err / errO;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->errO = errO;
+            st->err = err;
+            st->e[par] = err;
+            st->itr += 1;
+            if (st->itr >= st->mItr) {
+                st->done = 1;
+            } else if (err < st->eps) {
+                st->done = 1;
+                st->suc = 1;
+            }
+        }
+    }
+    const int lane = threadIdx.x % LANES;
+    const int rowsPerGrid = gridDim.x * (256 / LANES);
+    for (int i = blockIdx.x * (256 / LANES) + threadIdx.x / LANES; i < nNo; i += rowsPerGrid) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int j = rowPtr[i] + lane; j < rowPtr[i + 1]; j += LANES) {
+            const int c = colPtr[j];
+            const double u = (Pin[c] + r1 * R[c]) * r2;
+            a0 += G[(size_t)j * 3] * u;
+            a1 += G[(size_t)j * 3 + 1] * u;
+            a2 += G[(size_t)j * 3 + 2] * u;
+        }
+        for (int o = LANES / 2; o > 0; o >>= 1) {
+            a0 += __shfl_xor(a0, o, LANES);
+            a1 += __shfl_xor(a1, o, LANES);
+            a2 += __shfl_xor(a2, o, LANES);
+        }
+        if (lane == 0) {
+            Pout[i] = (Pin[i] + r1 * R[i]) * r2;
+            GP[(size_t)i * 3] = a0;
+            GP[(size_t)i * 3 + 1] = a1;
+            GP[(size_t)i * 3 + 2] = a2;
+        }
+    }
+}
+
 int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* G, const double* Lm, double* R,
-                   double* X, double* P, double* SP, double* GP, hipStream_t s)
+                   double* X, double* P, double* P2, double* SP, double* GP, hipStream_t s)
 {
     const int nNo = L.nNo;
     const size_t n = (size_t)nNo;
@@ -566,45 +602,51 @@ int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* 
     if (ls.mItr <= 0) h0.done = 1;
     SV_HIP(hipMemcpyAsync(st, &h0, sizeof(CgState), hipMemcpyHostToDevice, s));
     SV_HIP(hipMemsetAsync(X, 0, n * sizeof(double), s));
-    SV_HIP(hipMemcpyAsync(P, R, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // P double-buffered: the fused update reads one buffer and writes the
+    // other; chunk position it reads buf[(it+1)&1] and writes buf[it&1]
+    double* buf[2] = {P, P2};
+    SV_HIP(hipMemcpyAsync(buf[1], R, n * sizeof(double), hipMemcpyDeviceToDevice, s));
 
     PinnedPoll& poll = w.poll;
     SV_TRY(poll.ensure(256));
     constexpr int chunk = 16; // even: iteration parity = position in the chunk
     const bool cpl = L.cpl.active;
-    auto iters = [&](hipStream_t cs) -> int {
+    const int* rp = L.rowPtr.as<int>();
+    const int* cp = L.colPtr.as<int>();
+    // a chunk of iterations (the first chunk of a solve differs only in its
+    // first position, which has no finished iteration to update P for)
+    auto iters = [&](hipStream_t cs, bool firstChunk) -> int {
         for (int it = 0; it < chunk; ++it) {
             const int par = it & 1;
-            const int* rp = L.rowPtr.as<int>();
-            const int* cp = L.colPtr.as<int>();
+            const double* Pin = buf[(it + 1) & 1];
+            double* Pc = buf[it & 1];
+            const int first = (firstChunk && it == 0) ? 1 : 0;
             if (lanes == 8)
-                hipLaunchKernelGGL(k_cgs_gp<8>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, (const double*)P, GP,
-                                   (const CgState*)st);
+                hipLaunchKernelGGL(k_cgs_pgp<8>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, Pin, (const double*)R,
+                                   (const double*)part2, (int)gv, Pc, GP, st, par, first);
             else if (lanes == 32)
-                hipLaunchKernelGGL(k_cgs_gp<32>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, (const double*)P, GP,
-                                   (const CgState*)st);
+                hipLaunchKernelGGL(k_cgs_pgp<32>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, Pin, (const double*)R,
+                                   (const double*)part2, (int)gv, Pc, GP, st, par, first);
             else
-                hipLaunchKernelGGL(k_cgs_gp<16>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, (const double*)P, GP,
-                                   (const CgState*)st);
+                hipLaunchKernelGGL(k_cgs_pgp<16>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, Pin, (const double*)R,
+                                   (const double*)part2, (int)gv, Pc, GP, st, par, first);
             // coupled faces: GP <- PRE GP (CGRAD.f:89-92)
             if (cpl) SV_TRY(cpl_apply(L, 1, 3, GP, GP, cs, &st->done));
             {
                 // algorithmic bytes: L, Gt, colPtr, rowPtr; P and GP once; SP
                 ProfScope ps(4, 36.0 * L.nnz + 4.0 * (nNo + 1) + 40.0 * nNo);
                 if (lanes == 8)
-                    SV_LAUNCH(ps, k_cgs_sp<8>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)P,
+                    SV_LAUNCH(ps, k_cgs_sp<8>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)Pc,
                               (const double*)GP, SP, part1, (const CgState*)st);
                 else if (lanes == 32)
-                    SV_LAUNCH(ps, k_cgs_sp<32>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)P,
+                    SV_LAUNCH(ps, k_cgs_sp<32>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)Pc,
                               (const double*)GP, SP, part1, (const CgState*)st);
                 else
-                    SV_LAUNCH(ps, k_cgs_sp<16>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)P,
+                    SV_LAUNCH(ps, k_cgs_sp<16>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)Pc,
                               (const double*)GP, SP, part1, (const CgState*)st);
             }
-            hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, cs, n, X, R, (const double*)P, (const double*)SP,
+            hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, cs, n, X, R, (const double*)Pc, (const double*)SP,
                                (const double*)part1, gs, part2, st, par);
-            hipLaunchKernelGGL(k_cg_pupdate, dim3(gv), dim3(256), 0, cs, n, P, (const double*)R,
-                               (const double*)part2, (int)gv, st, par);
         }
         SV_HIP(hipGetLastError());
         return SVFSI_OK;
@@ -612,8 +654,8 @@ int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* 
     const bool useGraphs = graphs_enabled();
     if (useGraphs) {
         std::vector<uintptr_t> sig = {(uintptr_t)n, (uintptr_t)Gt, (uintptr_t)G, (uintptr_t)Lm, (uintptr_t)R,
-                                      (uintptr_t)X, (uintptr_t)P, (uintptr_t)SP, (uintptr_t)GP, (uintptr_t)part1,
-                                      (uintptr_t)st, (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)cpl,
+                                      (uintptr_t)X, (uintptr_t)P, (uintptr_t)P2, (uintptr_t)SP, (uintptr_t)GP,
+                                      (uintptr_t)part1, (uintptr_t)st, (uintptr_t)rp, (uintptr_t)cp, (uintptr_t)cpl,
                                       (uintptr_t)lanes};
         if (cpl) cpl_sig(L, sig);
         if (sig != w.csGraphs.sig) {
@@ -627,10 +669,11 @@ int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* 
     int launched = 0;
     while (!finished) {
         const int q = posted & 1;
+        const bool firstChunk = launched == 0;
         if (useGraphs)
-            SV_TRY(graph_run(w.csGraphs, 0, s, iters));
+            SV_TRY(graph_run(w.csGraphs, firstChunk ? 0 : 1, s, [&](hipStream_t cs) { return iters(cs, firstChunk); }));
         else
-            SV_TRY(iters(s));
+            SV_TRY(iters(s, firstChunk));
         launched += chunk;
         SV_TRY(poll.post(q, st, sizeof(CgState), s));
         ++posted;
@@ -639,7 +682,9 @@ int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* 
             std::memcpy(&hs, poll.h[q ^ 1], sizeof(CgState));
             if (hs.done) finished = true;
         }
-        if (launched > ls.mItr + 2 * chunk) break; // safety: the flag is set by now
+        // the last iteration's bookkeeping runs at the next chunk's start:
+        // one chunk beyond mItr at most
+        if (launched > ls.mItr + 3 * chunk) break; // safety: the flag is set by now
     }
     SV_HIP(hipStreamSynchronize(s));
     SV_HIP(hipMemcpy(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost));
