@@ -79,6 +79,34 @@ module fsils_gpu
        real(c_double), intent(inout) :: Ri(*), Val(*)
        type(c_ptr), value :: incL, res
      end function fsils_gpu_solve
+
+     ! CHECKIEN (svFSI/READMSH.f:1010-1185) for tet4 / tet10: ien(eNoN, nEl)
+     ! 1-based, reordered in place; x(3, nNo)
+     integer(c_int) function svfsi_checkien(eNoN, nEl, nNo, ien, x, nReordered) bind(C, name="svfsi_checkien")
+       import :: c_int, c_double
+       integer(c_int), value :: eNoN, nEl, nNo
+       integer(c_int), intent(inout) :: ien(*)
+       real(c_double), intent(in) :: x(*)
+       integer(c_int), intent(out) :: nReordered
+     end function svfsi_checkien
+
+     ! .vtu / .vtp meshes and faces (svFSI/VTKXML.f): sizes, then the data
+     integer(c_int) function svfsi_read_vtk_sizes(path, nPoints, nCells, nodesPerCell, hasNodeIds, hasElementIds) &
+          bind(C, name="svfsi_read_vtk_sizes")
+       import :: c_int, c_char
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), intent(out) :: nPoints, nCells, nodesPerCell, hasNodeIds, hasElementIds
+     end function svfsi_read_vtk_sizes
+
+     integer(c_int) function svfsi_read_vtk(path, nPoints, nCells, nodesPerCell, x, conn, nodeIds, elementIds) &
+          bind(C, name="svfsi_read_vtk")
+       import :: c_int, c_char, c_double, c_ptr
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: nPoints, nCells, nodesPerCell
+       real(c_double), intent(out) :: x(*)
+       integer(c_int), intent(out) :: conn(*)
+       type(c_ptr), value :: nodeIds, elementIds
+     end function svfsi_read_vtk
   end interface
 
 contains
@@ -93,6 +121,19 @@ contains
     integer(c_int), intent(out) :: stat
     stat = fsils_gpu_solve(lhs, ls, dof, Ri, Val, prec, c_null_ptr, c_null_ptr)
   end subroutine FSILS_GPU_SOLVE_F
+
+  ! the same with the optional incL(nFaces) and res(nFaces) of FSILS_SOLVE
+  ! (coupled Neumann faces, SOLVE.f:66-86)
+  subroutine FSILS_GPU_SOLVE_RES_F(lhs, ls, dof, Ri, Val, prec, incL, res, stat)
+    type(c_ptr), intent(in) :: lhs
+    type(fsils_ls_c), intent(inout) :: ls
+    integer(c_int), intent(in) :: dof, prec
+    real(c_double), intent(inout) :: Ri(:,:), Val(:,:)
+    integer(c_int), intent(in), target :: incL(:)
+    real(c_double), intent(in), target :: res(:)
+    integer(c_int), intent(out) :: stat
+    stat = fsils_gpu_solve(lhs, ls, dof, Ri, Val, prec, c_loc(incL), c_loc(res))
+  end subroutine FSILS_GPU_SOLVE_RES_F
 
   function last_error() result(msg)
     character(len=256) :: msg
