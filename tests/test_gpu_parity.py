@@ -242,6 +242,10 @@ def test_fluid_assembly_nonnewtonian(gpu, orc, model):
 
 
 def test_viscosity_model_errors(gpu):
+    """An unknown viscType is refused, and so is the viscosity sensitivity
+    (defined for the constant model) with a non-Newtonian model."""
+    import torch
+
     from svfsi_amd._lib import SvfsiError
 
     prob = problems.fluid_cylinder(h=0.8)
@@ -249,6 +253,12 @@ def test_viscosity_model_errors(gpu):
     st = NewtonStep(prob)
     with pytest.raises(SvfsiError, match="viscType"):
         st.assemble()
+    eq = dict(prob.eq, **VISC["cy"])
+    n = prob.nNo
+    z = torch.zeros((n, 4), dtype=torch.float64, device=gpu)
+    g = torch.zeros(prob.mesh.nEl, dtype=torch.float64, device=gpu)
+    with pytest.raises(SvfsiError, match="constant"):
+        st.dmesh.fluid_dmu_dot(eq, st.Ag, st.Yg, None, z, z, g)
 
 
 @pytest.mark.parametrize("kern", [SVFSI_STRUCT_VALU, SVFSI_STRUCT_MFMA64])
