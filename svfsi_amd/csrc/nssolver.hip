@@ -312,7 +312,7 @@ int ns_gmres(NsCtx& c, const double* K, const double* R, double* X, FSILS_subLsT
     SV_TRY(gmres_solve(*c.L, t, kNsd, K, X, c.s, false, cpl ? 2 : 0));
     if (t.iNorm <= gm.absTol) {
         SV_HIP(hipMemsetAsync(X, 0, len * sizeof(double), c.s));
-        gm.callD += 2.220446049250313e-16;
+        gm.callD = 2.220446049250313e-16;  // GMRES.f:97: assigned, as the reference does
         gm.dB = 0.0;
         return SVFSI_OK;
     }
