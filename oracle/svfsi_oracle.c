@@ -817,7 +817,11 @@ void orc_construct_struct(int nNo, int nEl, int eNoN, const int* IEN,
 /* svFSILS                                                                */
 /* ===================================================================== */
 
-/* FSILS_SPARMULVV (svFSILS/SPARMUL.f:51-133), one rank. */
+/* FSILS_SPARMULVV (svFSILS/SPARMUL.f:51-133), one rank.  The sums
+ * associate as the reference's: for dof 1-4 (SPARMUL.f:66-113) each
+ * product is added to the running row sum in turn,
+ * KU = ((KU + K1 U1) + K2 U2) + ..., and for other dofs
+ * (SPARMUL.f:114-126) a block row's SUM is formed first and then added. */
 void orc_spmv(int nNo, int dof, const int* rowPtr, const int* colPtr,
               const double* K, const double* U, double* KU)
 {
@@ -829,9 +833,13 @@ void orc_spmv(int nNo, int dof, const int* rowPtr, const int* colPtr,
             const double* k = K + (size_t)j * dd;
             const double* u = U + (size_t)colPtr[j] * dof;
             for (int l = 0; l < dof; ++l) {
-                double s = 0.0;
-                for (int m = 0; m < dof; ++m) s += k[l * dof + m] * u[m];
-                acc[l] += s;
+                if (dof <= 4) {
+                    for (int m = 0; m < dof; ++m) acc[l] += k[l * dof + m] * u[m];
+                } else {
+                    double s = 0.0;
+                    for (int m = 0; m < dof; ++m) s += k[l * dof + m] * u[m];
+                    acc[l] += s;
+                }
             }
         }
         for (int l = 0; l < dof; ++l) KU[(size_t)i * dof + l] = acc[l];
