@@ -5,6 +5,7 @@
 #include <hip/hip_ext.h>
 
 #include <cstdint>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -37,6 +38,22 @@ void set_error(const char* fmt, ...);
             return SVFSI_ERR_ARG;                                              \
         }                                                                      \
     } while (0)
+
+// Runs the body of an extern "C" entry point and turns any C++ exception
+// (std::bad_alloc, std::length_error of a container) into an error code, so
+// that none crosses the C ABI.
+template <class F>
+int guarded(F&& body) noexcept
+{
+    try {
+        return body();
+    } catch (const std::exception& e) {
+        set_error("internal error: %s", e.what());
+    } catch (...) {
+        set_error("internal error: unknown exception");
+    }
+    return SVFSI_ERR_ARG;
+}
 
 int require_device();
 

@@ -201,27 +201,37 @@ uint64_t read_uint(const uint8_t* b, size_t n)
     uint32_t v; std::memcpy(&v, b, 4); return v;
 }
 
-// payload bytes of a binary block (header + data, optionally zlib blocks)
+// Largest decoded array accepted (bytes).  The sizes in a binary block's
+// header come from the file; bounding them keeps a corrupt or crafted header
+// from asking for an allocation the process cannot make.
+constexpr uint64_t kMaxArrayBytes = uint64_t(1) << 36;
+
+// payload bytes of a binary block (header + data, optionally zlib blocks).
+// Every size read from the block is checked against the bytes available
+// before it is used in an offset, in a form that cannot overflow.
 int unpack_block(const VtkFile& f, const uint8_t* b, size_t avail, std::vector<uint8_t>& out, size_t* used)
 {
     const size_t h = f.hdr;
     if (!f.zlib) {
         if (avail < h) return SVFSI_ERR_ARG;
         const uint64_t n = read_uint(b, h);
-        if (avail < h + n) return SVFSI_ERR_ARG;
+        if (n > avail - h) return SVFSI_ERR_ARG;
         out.assign(b + h, b + h + n);
         if (used) *used = h + n;
         return SVFSI_OK;
     }
     if (avail < 3 * h) return SVFSI_ERR_ARG;
     const uint64_t nb = read_uint(b, h), bs = read_uint(b + h, h), last = read_uint(b + 2 * h, h);
-    if (avail < (3 + nb) * h) return SVFSI_ERR_ARG;
+    // the nb compressed sizes follow the three header integers
+    if (nb > avail / h - 3) return SVFSI_ERR_ARG;
+    if (bs > kMaxArrayBytes || last > bs || (nb && nb - 1 > kMaxArrayBytes / (bs ? bs : 1))) return SVFSI_ERR_ARG;
     size_t off = (3 + nb) * h;
     out.clear();
     for (uint64_t k = 0; k < nb; ++k) {
         const uint64_t cs = read_uint(b + (3 + k) * h, h);
         const uint64_t us = (k + 1 == nb && last) ? last : bs;
-        if (avail < off + cs) return SVFSI_ERR_ARG;
+        if (cs > avail - off) return SVFSI_ERR_ARG;
+        if (us > kMaxArrayBytes - out.size()) return SVFSI_ERR_ARG;
         const size_t o = out.size();
         out.resize(o + us);
         uLongf dl = (uLongf)us;
@@ -265,8 +275,13 @@ int array_values(const VtkFile& f, const Tag& t, std::vector<double>& v)
             return SVFSI_ERR_ARG;
         }
         const size_t off = (size_t)std::strtoull(attr(t.attrs, "offset").c_str(), nullptr, 10);
+        if (off > f.s.size() - f.app) {
+            set_error("%s: appended DataArray offset beyond the end of the file", f.path.c_str());
+            return SVFSI_ERR_ARG;
+        }
         if (f.appB64) {
-            const size_t e = f.s.find('<', f.app + off);
+            size_t e = f.s.find('<', f.app + off);
+            if (e == std::string::npos) e = f.s.size();
             const std::vector<uint8_t> enc = b64decode(f.s.data() + f.app + off, f.s.data() + e);
             if (unpack_block(f, enc.data(), enc.size(), raw, nullptr) != SVFSI_OK) {
                 set_error("%s: bad appended DataArray", f.path.c_str());
@@ -274,8 +289,7 @@ int array_values(const VtkFile& f, const Tag& t, std::vector<double>& v)
             }
         } else {
             const uint8_t* b = reinterpret_cast<const uint8_t*>(f.s.data()) + f.app + off;
-            if (f.app + off > f.s.size() ||
-                unpack_block(f, b, f.s.size() - f.app - off, raw, nullptr) != SVFSI_OK) {
+            if (unpack_block(f, b, f.s.size() - f.app - off, raw, nullptr) != SVFSI_OK) {
                 set_error("%s: bad appended DataArray", f.path.c_str());
                 return SVFSI_ERR_ARG;
             }
@@ -376,6 +390,16 @@ int vtk_mesh(const char* path, VtkMesh& m)
         set_error("%s: no %s connectivity/offsets", path, cells);
         return SVFSI_ERR_ARG;
     }
+    // offsets come from the file: check them before any conversion to int
+    if (m.offs.size() > (size_t)INT32_MAX || m.x.size() / 3 > (size_t)INT32_MAX) {
+        set_error("%s: too many cells or points", path);
+        return SVFSI_ERR_ARG;
+    }
+    for (const double o : m.offs)
+        if (!(o >= 1.0 && o <= (double)m.conn.size())) {
+            set_error("%s: cell offset out of range", path);
+            return SVFSI_ERR_ARG;
+        }
     m.nCells = (int)m.offs.size();
     m.npc = (int)m.offs[0];
     for (int c = 0; c < m.nCells; ++c)
@@ -383,7 +407,7 @@ int vtk_mesh(const char* path, VtkMesh& m)
             set_error("%s: cells of different sizes", path);
             return SVFSI_ERR_ARG;
         }
-    if ((int)m.conn.size() != m.nCells * m.npc) {
+    if (m.conn.size() != (size_t)m.nCells * (size_t)m.npc) {
         set_error("%s: connectivity size does not match the offsets", path);
         return SVFSI_ERR_ARG;
     }
@@ -398,9 +422,10 @@ int vtk_mesh(const char* path, VtkMesh& m)
 
 using namespace svfsi;
 
-extern "C" {
+namespace {
 
-int svfsi_read_ccne_sizes(const char* connPath, const char* coordPath, int* nEl, int* eNoN, int* nNo, int* nsd)
+
+int svfsi_read_ccne_sizes_body(const char* connPath, const char* coordPath, int* nEl, int* eNoN, int* nNo, int* nsd)
 {
     SV_ARG(connPath && coordPath && nEl && eNoN && nNo && nsd, "NULL argument");
     std::vector<std::vector<double>> c, x;
@@ -415,7 +440,7 @@ int svfsi_read_ccne_sizes(const char* connPath, const char* coordPath, int* nEl,
     return SVFSI_OK;
 }
 
-int svfsi_read_ccne(const char* connPath, const char* coordPath, int nEl, int eNoN, int nNo, int nsd, int* ien,
+int svfsi_read_ccne_body(const char* connPath, const char* coordPath, int nEl, int eNoN, int nNo, int nsd, int* ien,
                     double* x)
 {
     SV_ARG(connPath && coordPath && ien && x, "NULL argument");
@@ -435,7 +460,7 @@ int svfsi_read_ccne(const char* connPath, const char* coordPath, int nEl, int eN
     return SVFSI_OK;
 }
 
-int svfsi_read_ebc_sizes(const char* path, int* nFe, int* feNoN)
+int svfsi_read_ebc_sizes_body(const char* path, int* nFe, int* feNoN)
 {
     SV_ARG(path && nFe && feNoN, "NULL argument");
     std::vector<std::vector<double>> r;
@@ -446,7 +471,7 @@ int svfsi_read_ebc_sizes(const char* path, int* nFe, int* feNoN)
     return SVFSI_OK;
 }
 
-int svfsi_read_ebc(const char* path, int nFe, int feNoN, int* gE, int* fien)
+int svfsi_read_ebc_body(const char* path, int nFe, int feNoN, int* gE, int* fien)
 {
     SV_ARG(path && gE && fien, "NULL argument");
     std::vector<std::vector<double>> r;
@@ -463,7 +488,7 @@ int svfsi_read_ebc(const char* path, int nFe, int feNoN, int* gE, int* fien)
     return SVFSI_OK;
 }
 
-int svfsi_read_vtk_sizes(const char* path, int* nPoints, int* nCells, int* nodesPerCell, int* hasNodeIds,
+int svfsi_read_vtk_sizes_body(const char* path, int* nPoints, int* nCells, int* nodesPerCell, int* hasNodeIds,
                          int* hasElementIds)
 {
     SV_ARG(path && nPoints && nCells && nodesPerCell, "NULL argument");
@@ -477,7 +502,7 @@ int svfsi_read_vtk_sizes(const char* path, int* nPoints, int* nCells, int* nodes
     return SVFSI_OK;
 }
 
-int svfsi_read_vtk(const char* path, int nPoints, int nCells, int nodesPerCell, double* x, int* conn,
+int svfsi_read_vtk_body(const char* path, int nPoints, int nCells, int nodesPerCell, double* x, int* conn,
                    int* nodeIds, int* elementIds)
 {
     SV_ARG(path && x && conn, "NULL argument");
@@ -492,16 +517,18 @@ int svfsi_read_vtk(const char* path, int nPoints, int nCells, int nodesPerCell, 
     }
     if (nodeIds) {
         SV_ARG(m.hasN && (int)m.gnid.size() == nPoints, "no GlobalNodeID point array");
-        for (int k = 0; k < nPoints; ++k) nodeIds[k] = (int)m.gnid[k];
+        for (int k = 0; k < nPoints; ++k)
+            SV_ARG(to_int(m.gnid[k], &nodeIds[k]) == SVFSI_OK, "GlobalNodeID %d is not an integer", k + 1);
     }
     if (elementIds) {
         SV_ARG(m.hasE && (int)m.geid.size() == nCells, "no GlobalElementID cell array");
-        for (int k = 0; k < nCells; ++k) elementIds[k] = (int)m.geid[k];
+        for (int k = 0; k < nCells; ++k)
+            SV_ARG(to_int(m.geid[k], &elementIds[k]) == SVFSI_OK, "GlobalElementID %d is not an integer", k + 1);
     }
     return SVFSI_OK;
 }
 
-int svfsi_checkien(int eNoN, int nEl, int nNo, int* ien, const double* x, int* nReordered)
+int svfsi_checkien_body(int eNoN, int nEl, int nNo, int* ien, const double* x, int* nReordered)
 {
     SV_ARG(ien && x, "NULL argument");
     SV_ARG(eNoN == 4 || eNoN == 10, "CHECKIEN here covers tet4 and tet10 (eNoN 4 or 10)");
@@ -549,6 +576,49 @@ int svfsi_checkien(int eNoN, int nEl, int nNo, int* ien, const double* x, int* n
     }
     if (nReordered) *nReordered = swapped;
     return SVFSI_OK;
+}
+
+
+} // namespace
+
+extern "C" {
+
+int svfsi_read_ccne_sizes(const char* connPath, const char* coordPath, int* nEl, int* eNoN, int* nNo, int* nsd)
+{
+    return guarded([&] { return svfsi_read_ccne_sizes_body(connPath, coordPath, nEl, eNoN, nNo, nsd); });
+}
+
+int svfsi_read_ccne(const char* connPath, const char* coordPath, int nEl, int eNoN, int nNo, int nsd, int* ien,
+                    double* x)
+{
+    return guarded([&] { return svfsi_read_ccne_body(connPath, coordPath, nEl, eNoN, nNo, nsd, ien, x); });
+}
+
+int svfsi_read_ebc_sizes(const char* path, int* nFe, int* feNoN)
+{
+    return guarded([&] { return svfsi_read_ebc_sizes_body(path, nFe, feNoN); });
+}
+
+int svfsi_read_ebc(const char* path, int nFe, int feNoN, int* gE, int* fien)
+{
+    return guarded([&] { return svfsi_read_ebc_body(path, nFe, feNoN, gE, fien); });
+}
+
+int svfsi_read_vtk_sizes(const char* path, int* nPoints, int* nCells, int* nodesPerCell, int* hasNodeIds,
+                         int* hasElementIds)
+{
+    return guarded([&] { return svfsi_read_vtk_sizes_body(path, nPoints, nCells, nodesPerCell, hasNodeIds, hasElementIds); });
+}
+
+int svfsi_read_vtk(const char* path, int nPoints, int nCells, int nodesPerCell, double* x, int* conn,
+                   int* nodeIds, int* elementIds)
+{
+    return guarded([&] { return svfsi_read_vtk_body(path, nPoints, nCells, nodesPerCell, x, conn, nodeIds, elementIds); });
+}
+
+int svfsi_checkien(int eNoN, int nEl, int nNo, int* ien, const double* x, int* nReordered)
+{
+    return guarded([&] { return svfsi_checkien_body(eNoN, nEl, nNo, ien, x, nReordered); });
 }
 
 } // extern "C"

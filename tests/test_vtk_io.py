@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from svfsi_amd import mesh, meshio
+from svfsi_amd._lib import SvfsiError
 
 VT = {np.dtype("float64"): "Float64", np.dtype("float32"): "Float32", np.dtype("int32"): "Int32",
       np.dtype("int64"): "Int64", np.dtype("uint8"): "UInt8"}
@@ -107,3 +108,64 @@ def test_vtu_float32_int64_tet10_and_vtp_face(tmp_path):
     xf, cf, gnf, gef = meshio.read_vtk(q)
     assert np.array_equal(gnf[cf], f + 1) and np.array_equal(gef, e + 1)
     assert np.array_equal(xf, m.x[nodes])
+
+
+BAD_HEADERS = {
+    # uncompressed: the byte count of the array
+    False: [lambda h, n: struct.pack(h, n), ],
+    # zlib: [nblocks, blocksize, lastblocksize, compressed sizes...]
+    True: [lambda h, n: struct.pack("<" + h[1] * 4, n, 1000, 0, 10),     # block count
+           lambda h, n: struct.pack("<" + h[1] * 4, 1, n, 0, 10),        # block size
+           lambda h, n: struct.pack("<" + h[1] * 4, 1, 1000, 8, n),      # compressed size
+           lambda h, n: struct.pack("<" + h[1] * 4, 2, 1000, 0, 10)],    # sizes beyond the data
+}
+
+
+@pytest.mark.parametrize("fmt", ["binary", "appended", "appended64"])
+@pytest.mark.parametrize("hdr", ["UInt32", "UInt64"])
+@pytest.mark.parametrize("comp", [False, True])
+def test_corrupt_block_headers_are_errors(tmp_path, monkeypatch, fmt, hdr, comp):
+    """Sizes read from a block header that exceed the file (including values
+    near 2^32 and 2^64, where an unchecked offset sum wraps) give an error
+    return, never an abort of the calling process."""
+    m = mesh.cube_tet4(1)
+    h = "<I" if hdr == "UInt32" else "<Q"
+    huge = (1 << 32) - 1 if hdr == "UInt32" else (1 << 64) - 1
+    for make in BAD_HEADERS[comp]:
+        for n in (huge, huge - 7, 1 << 31):
+            def bad(a, hdr_, comp_, block=1000, make=make, n=n):
+                return make(h, n), (b"" if comp_ else None)
+            monkeypatch.setattr(__import__(__name__), "payload", bad, raising=True)
+            p = str(tmp_path / f"bad_{fmt}_{hdr}_{comp}.vtu")
+            write_vtk(p, m.x, m.ien, 10, fmt, hdr, comp)
+            with pytest.raises(SvfsiError):
+                meshio.read_vtk(p)
+
+
+@pytest.mark.parametrize("fmt", ["binary", "appended", "appended64"])
+@pytest.mark.parametrize("comp", [False, True])
+def test_truncated_files_are_errors(tmp_path, fmt, comp):
+    """Every prefix cut of a valid file is either read or refused with an
+    error; none crashes the reader."""
+    m = mesh.cube_tet4(1)
+    p = tmp_path / "m.vtu"
+    write_vtk(str(p), m.x, m.ien, 10, fmt, "UInt64", comp)
+    data = p.read_bytes()
+    q = tmp_path / "cut.vtu"
+    for cut in range(0, len(data), max(1, len(data) // 97)):
+        q.write_bytes(data[:cut])
+        try:
+            meshio.read_vtk(str(q))
+        except SvfsiError:
+            pass
+
+
+def test_appended_offset_beyond_the_file_is_an_error(tmp_path):
+    m = mesh.cube_tet4(1)
+    for fmt in ("appended", "appended64"):
+        p = tmp_path / f"m_{fmt}.vtu"
+        write_vtk(str(p), m.x, m.ien, 10, fmt, "UInt32", False)
+        s = p.read_bytes().replace(b'offset="0"', b'offset="99999999999"', 1)
+        p.write_bytes(s)
+        with pytest.raises(SvfsiError):
+            meshio.read_vtk(str(p))
