@@ -34,6 +34,9 @@
 #ifndef SVFSI_AMD_H
 #define SVFSI_AMD_H
 
+#include <stddef.h>
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -56,6 +59,32 @@ extern "C" {
 #define SVFSI_ERR_STATE    -3  /* object not created / already created */
 #define SVFSI_ERR_NOTSUP   -4  /* combination the reference supports but this build does not */
 #define SVFSI_ERR_NODEVICE -5  /* no HIP device */
+
+/* MPI_Aint of the reference's C structs and prototypes: taken from mpi.h
+ * when the caller includes it first, otherwise the pointer-sized signed
+ * integer MPI_Aint is in MPICH and Open MPI. */
+#if defined(FSILS_C_H) || defined(MPI_VERSION)
+typedef MPI_Aint fsils_aint_t;
+#else
+typedef intptr_t fsils_aint_t;
+#endif
+
+/* When the reference's own FSILS_c.h has been included first (FSILS_C_H
+ * defined), its struct types are used, and the fsils_*_ prototypes further
+ * down redeclare its functions: a C compiler then checks them against the
+ * reference's. */
+#ifndef FSILS_C_H
+
+/* FSILS_commuType, same layout as svFSILS/FSILS_struct_c.h:60-68 */
+typedef struct {
+    int foC;            /* created (USE) */
+    int masF;           /* this rank is the master (USE) */
+    int master;         /* master rank (USE) */
+    int task;           /* rank (USE) */
+    int tF;             /* rank + 1 (USE) */
+    int nTasks;         /* number of ranks (USE) */
+    fsils_aint_t comm;  /* MPI communicator (IN) */
+} FSILS_commuType;
 
 /* FSILS_subLsType, same layout as svFSILS/FSILS_struct_c.h:115-130 */
 typedef struct {
@@ -82,6 +111,8 @@ typedef struct {
     FSILS_subLsType CG;
     FSILS_subLsType RI;
 } FSILS_lsType;
+
+#endif /* FSILS_C_H */
 
 const char* svfsi_last_error(void);
 /* Number of visible HIP devices (0 when none). */
@@ -144,18 +175,77 @@ int fsils_gpu_solve_dev(void* lhs, FSILS_lsType* ls, int dof, double* dRi,
 
 /* Mixed-precision form of fsils_gpu_solve_dev (no reference counterpart;
  * the reference is FP64-only, svFSILS/FSILS_TYPEDEF.h:55-57).  Same
- * arguments, scaling, convergence test and outputs.  Flexible GMRES(30) in
- * FP64 (outer residual, two-pass classical Gram-Schmidt Arnoldi,
- * Hessenberg/Givens, solution) whose right preconditioner is one cycle of
- * an inner GMRES(30) on an FP32 copy of the scaled matrix (FP32 SpMV, basis
- * and directions; FP64 dot accumulation and least squares), stopped at a
- * relative reduction of 0.1.  The reported residual is the FP64 one in the
+ * arguments, scaling, convergence test and outputs.  The solver is always
+ * flexible GMRES in FP64 (outer residual f - Ku, two-pass classical
+ * Gram-Schmidt Arnoldi, Hessenberg/Givens least squares, solution), with
+ * restart length ls->RI.sD (30 when 0), whose right preconditioner works on
+ * an FP32 copy of the scaled matrix: a fixed GMRES polynomial of the
+ * (block-Jacobi scaled) operator, or one cycle of an inner FP32 GMRES(30)
+ * stopped at a relative reduction of 0.1.  LS_type GMRES, CG and BICGS all
+ * select this one solver (it replaces each of them; their defaults of
+ * fsils_gpu_ls_create still apply); LS_TYPE_NS is refused
+ * (SVFSI_ERR_NOTSUP).  The reported residual is the FP64 one in the
  * reference's scaled norm; ls->RI.mItr bounds the outer restarts.
  * ls->RI.itr counts outer (FP64) Arnoldi steps and restarts;
  * ls->RI.cM counts all SpMVs (inner FP32 plus outer FP64). */
 int fsils_gpu_solve_mixed_dev(void* lhs, FSILS_lsType* ls, int dof,
                               double* dRi, double* dVal, int prec,
                               const int* incL, const double* res, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* The reference's C API, svFSILS/FSILS_api_c.h:52-64, same prototypes     */
+/* ---------------------------------------------------------------------- */
+/* A driver written against FSILS_c.h links this library in place of
+ * svFSILS.  lhs is an opaque handle (the reference's is a pointer to its
+ * SAVEd Fortran lhs, LHS.f:330-348; here each fsils_lhs_create_c_ call makes
+ * a new one).  Optional Fortran arguments are NULL pointers.  As in the
+ * reference these return nothing and stop the program on misuse, printing
+ * the reason and "FSILS: FATAL ERROR"; see svfsi_fsils_fatal below. */
+
+/* FSILS_COMMU_CREATE (svFSILS/COMMU.f:50-86): reads the Fortran MPI handle
+ * (the low 4 bytes of *commi); size and rank come from the process's MPI
+ * library when one is loaded and initialised, else 1 and 0. */
+void fsils_commu_create_(FSILS_commuType* commu, fsils_aint_t* commi);
+/* FSILS_COMMU_FREE (COMMU.f:88-96). */
+void fsils_commu_free_(FSILS_commuType* commu);
+/* FSILS_LHS_CREATE_C (svFSILS/LHS.f:330-348) over fsils_gpu_lhs_create; a
+ * communicator of more than one rank stops with "not supported". */
+void fsils_lhs_create_c_(void** lhs, FSILS_commuType* commu, int* gnNo, int* nNo, int* nnz, int* gNodes,
+                         int* rowPtr, int* colPtr, int* nFaces);
+/* FSILS_LHS_FREE (LHS.f:295-328). */
+void fsils_lhs_free_(void* lhs);
+/* FSILS_LS_CREATE (svFSILS/LS.f:50-119) over fsils_gpu_ls_create. */
+void fsils_ls_create_(FSILS_lsType* ls, int* LS_type, double* relTol, double* absTol, int* maxItr, int* dimKry,
+                      double* relTolIn, double* absTolIn, int* maxItrIn);
+/* FSILS_LS_FREE (LS.f:121-133). */
+void fsils_ls_free_(FSILS_lsType* ls);
+/* FSILS_BC_CREATE (svFSILS/BC.f:50-121) over fsils_gpu_bc_create. */
+void fsils_bc_create_(void* lhs, int* faIn, int* nNo, int* dof, int* BC_type, int* gNodes, double* Val);
+/* FSILS_BC_FREE (BC.f:195-214). */
+void fsils_bc_free_(void* lhs, int* faIn);
+/* FSILS_SOLVE (svFSILS/SOLVE.f:51-143) over fsils_gpu_solve: host arrays,
+ * Ri(dof,nNo) RHS in and solution out, Val left scaled in place. */
+void fsils_solve_(void* lhs, FSILS_lsType* ls, int* dof, double* Ri, double* Val, int* prec, int* incL,
+                  double* res);
+
+/* No counterpart in the reference: on = 0 makes the fsils_*_ functions
+ * above return after a failure instead of stopping the program; the status
+ * of the last call (0 or an SVFSI_ERR_* code) is then svfsi_fsils_status()
+ * and the reason svfsi_last_error().  Returns the previous setting. */
+int svfsi_fsils_fatal(int on);
+int svfsi_fsils_status(void);
+
+/* ---------------------------------------------------------------------- */
+/* Device memory for callers without a HIP binding (the Fortran driver)    */
+/* ---------------------------------------------------------------------- */
+/* No counterpart in the reference (its arrays live in host memory).  They
+ * wrap hipMalloc / hipFree / synchronous hipMemcpy on the current device, so
+ * a driver can keep Ag/Yg/Dg, R and Val resident in HBM across Newton
+ * iterations and call the _dev entry points. */
+int svfsi_dev_malloc(void** dptr, size_t bytes);
+int svfsi_dev_free(void* dptr);
+int svfsi_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int svfsi_memcpy_d2h(void* dst, const void* src, size_t bytes);
 
 /* ---------------------------------------------------------------------- */
 /* Kernel-level entry points of the solve (device arrays)                  */

@@ -58,6 +58,15 @@ class FSILS_lsType(C.Structure):
     ]
 
 
+class FSILS_commuType(C.Structure):
+    """FSILS_commuType (svFSILS/FSILS_struct_c.h:60-68); comm is an MPI_Aint."""
+
+    _fields_ = [
+        ("foC", C.c_int), ("masF", C.c_int), ("master", C.c_int), ("task", C.c_int),
+        ("tF", C.c_int), ("nTasks", C.c_int), ("comm", C.c_ssize_t),
+    ]
+
+
 class EqParams(C.Structure):
     """svfsi_eq_params: eq(cEq) and domain properties of COMMOD."""
 
@@ -123,6 +132,27 @@ SIGNATURES = {
     "svfsi_prof_get": [_I, C.POINTER(C.c_longlong), _DP],
     "svfsi_prof_get_bytes": [_I, C.POINTER(C.c_double)],
     "svfsi_prof_reset": [],
+    "svfsi_fsils_fatal": [_I],
+    "svfsi_fsils_status": [],
+    "svfsi_dev_malloc": [C.POINTER(_P), C.c_size_t],
+    "svfsi_dev_free": [_P],
+    "svfsi_memcpy_h2d": [_P, _P, C.c_size_t],
+    "svfsi_memcpy_d2h": [_P, _P, C.c_size_t],
+}
+
+# The reference's own C API (svFSILS/FSILS_api_c.h:52-64): every argument by
+# pointer, no return value; misuse stops the program unless
+# svfsi_fsils_fatal(0) was called (then read svfsi_fsils_status()).
+VOID_SIGNATURES = {
+    "fsils_commu_create_": [C.POINTER(FSILS_commuType), C.POINTER(C.c_ssize_t)],
+    "fsils_commu_free_": [C.POINTER(FSILS_commuType)],
+    "fsils_lhs_create_c_": [C.POINTER(_P), C.POINTER(FSILS_commuType), _IP, _IP, _IP, _IP, _IP, _IP, _IP],
+    "fsils_lhs_free_": [_P],
+    "fsils_ls_create_": [C.POINTER(FSILS_lsType), _IP, _DP, _DP, _IP, _IP, _DP, _DP, _IP],
+    "fsils_ls_free_": [C.POINTER(FSILS_lsType)],
+    "fsils_bc_create_": [_P, _IP, _IP, _IP, _IP, _IP, _DP],
+    "fsils_bc_free_": [_P, _IP],
+    "fsils_solve_": [_P, C.POINTER(FSILS_lsType), _IP, _DP, _DP, _IP, _IP, _DP],
 }
 
 
@@ -152,6 +182,10 @@ def lib() -> C.CDLL:
             f = getattr(L, name)
             f.argtypes = args
             f.restype = C.c_int
+        for name, args in VOID_SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
         L.svfsi_last_error.argtypes = []
         L.svfsi_last_error.restype = C.c_char_p
         _lib = L

@@ -305,6 +305,35 @@ int svfsi_device_count(void)
     return n;
 }
 
+int svfsi_dev_malloc(void** dptr, size_t bytes)
+{
+    SV_ARG(dptr, "dptr is NULL");
+    SV_TRY(require_device());
+    *dptr = nullptr;
+    SV_HIP(hipMalloc(dptr, bytes ? bytes : 8));
+    return SVFSI_OK;
+}
+
+int svfsi_dev_free(void* dptr)
+{
+    if (dptr) SV_HIP(hipFree(dptr));
+    return SVFSI_OK;
+}
+
+int svfsi_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    SV_ARG(dst && src, "NULL argument");
+    SV_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SVFSI_OK;
+}
+
+int svfsi_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    SV_ARG(dst && src, "NULL argument");
+    SV_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SVFSI_OK;
+}
+
 int fsils_gpu_ls_create(FSILS_lsType* ls, int LS_type, const double* relTol, const double* absTol,
                         const int* maxItr, const int* dimKry, const double* relTolIn,
                         const double* absTolIn, const int* maxItrIn)
@@ -455,6 +484,10 @@ int fsils_gpu_bc_free(void* lhs, int faIn)
     SV_ARG(lhs, "lhs is NULL");
     Lhs& L = *static_cast<Lhs*>(lhs);
     SV_ARG(faIn >= 1 && faIn <= L.nFaces, "faIn out of range");
+    if (!L.faces[faIn - 1].foC) {
+        set_error("FSILS: Cannot free a face that is not created yet (BC.f:200-203)");
+        return SVFSI_ERR_STATE;
+    }
     L.faces[faIn - 1] = Face();
     L.facesVersion++;
     return SVFSI_OK;
