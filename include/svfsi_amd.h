@@ -192,6 +192,22 @@ int fsils_gpu_solve_mixed_dev(void* lhs, FSILS_lsType* ls, int dof,
                               double* dRi, double* dVal, int prec,
                               const int* incL, const double* res, void* stream);
 
+/* No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  fsils_gpu_solve_dev restricted to LS_TYPE_CG with
+ * PRECOND_FSILS (CGRADV with diagonal scaling; coupled faces play no part in
+ * CG), with the same arguments, results and outputs, that calls no hipMalloc,
+ * hipFree or other allocating routine.  Its work memory (W, the Dirichlet
+ * mask, the CG vectors, partial sums and state) is one fixed device array of
+ * svfsi_noalloc_scratch_bytes() bytes (64 MiB: about 1.6 million unknowns,
+ * nNo * dof), which each GPU the library runs on holds until the process
+ * ends.  Larger systems are refused with SVFSI_ERR_ARG before any GPU work.
+ * All calls on one GPU share that array: calls that overlap in time on one
+ * GPU (from two host threads or two streams) race and must not be made. */
+int fsils_gpu_solve_cg_noalloc_dev(void* lhs, FSILS_lsType* ls, int dof, double* dRi, double* dVal,
+                                   const int* incL, void* stream);
+size_t svfsi_noalloc_scratch_bytes(void);
+
 /* ---------------------------------------------------------------------- */
 /* The reference's C API, svFSILS/FSILS_api_c.h:52-64, same prototypes     */
 /* ---------------------------------------------------------------------- */

@@ -138,6 +138,7 @@ SIGNATURES = {
     "svfsi_dev_free": [_P],
     "svfsi_memcpy_h2d": [_P, _P, C.c_size_t],
     "svfsi_memcpy_d2h": [_P, _P, C.c_size_t],
+    "fsils_gpu_solve_cg_noalloc_dev": [_P, C.POINTER(FSILS_lsType), _I, _P, _P, _IP, _P],
 }
 
 # The reference's own C API (svFSILS/FSILS_api_c.h:52-64): every argument by
@@ -186,6 +187,8 @@ def lib() -> C.CDLL:
             f = getattr(L, name)
             f.argtypes = args
             f.restype = None
+        L.svfsi_noalloc_scratch_bytes.argtypes = []
+        L.svfsi_noalloc_scratch_bytes.restype = C.c_size_t
         L.svfsi_last_error.argtypes = []
         L.svfsi_last_error.restype = C.c_char_p
         _lib = L

@@ -281,10 +281,32 @@ int precond_rcs(Lhs& L, int dof, double* Val, double* R, double* W1, double* W2,
                 hipStream_t s);
 int precond_diag(Lhs& L, int dof, double* Val, double* R, double* W,
                  float* Val32, const int* incL, hipStream_t s);
+// PRECONDDIAG with a Dirichlet mask the caller provides (device, (dof,nNo)).
+int precond_diag_masked(const Lhs& L, int dof, double* Val, double* R, double* W, float* Val32,
+                        const double* mask, hipStream_t s);
+// The Dirichlet mask of build_mask, built on the device into `mask` with no
+// allocation; `staging` holds one face's nodes and values at a time.
+int build_mask_into(const Lhs& L, int dof, const int* incL, double* mask, void* staging, size_t stagingBytes,
+                    hipStream_t s);
+int launch_fill(size_t n, double v, double* x, hipStream_t s);
 int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s);
 
 int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
              double* R, hipStream_t s);
+// CGRADV's device buffers: X, P, KP (n each), the partial sums
+// (kCgPartials doubles) and the iteration state (kCgStateBytes).
+constexpr size_t kCgPartials = 1024 + 512 + 16;
+constexpr size_t kCgStateBytes = 128;
+struct CgBufs {
+    double *X, *P, *KP, *part;
+    void* state;
+};
+// CGRADV on caller buffers.  graphs: replay chunks of iterations as one
+// captured graph (cached in L.ws); pinned: poll the state through L.ws's
+// pinned mirror one chunk behind.  With neither, nothing is allocated: the
+// host reads the state with a synchronous copy after each chunk.
+int cg_core(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s, const CgBufs& b,
+            bool graphs, bool pinned);
 // BICGSV (svFSILS/BICGS.f:115-180): R in (scaled RHS), out (solution).
 int bicgs_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s);
 // CGRAD_SCHUR (svFSILS/CGRAD.f:51-123) on the device: (L - Gt G) x = R with

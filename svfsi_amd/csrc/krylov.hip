@@ -333,6 +333,25 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
     }
     const size_t n = (size_t)L.nNo * dof;
     Work& w = L.ws;
+    SV_TRY(w.X.ensure(n * sizeof(double)));
+    SV_TRY(w.tmp.ensure(2 * n * sizeof(double)));
+    SV_TRY(w.partial.ensure(kCgPartials * sizeof(double)));
+    SV_TRY(w.state.ensure(kCgStateBytes));
+    SV_TRY(w.poll.ensure(256));
+    const CgBufs b{w.X.as<double>(), w.tmp.as<double>(), w.tmp.as<double>() + n, w.partial.as<double>(), w.state.p};
+    return cg_core(L, ls, dof, K, R, s, b, graphs_enabled(), true);
+}
+
+int cg_core(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s, const CgBufs& b,
+            bool useGraphs, bool pinned)
+{
+    static_assert(sizeof(CgState) <= kCgStateBytes, "CG state slot");
+    if (dof < 2 || dof > 4) {
+        set_error("CG: dof %d not supported", dof);
+        return SVFSI_ERR_NOTSUP;
+    }
+    const size_t n = (size_t)L.nNo * dof;
+    Work& w = L.ws;
     const bool vs = dof <= 3;  // value-streaming SpMV for dof 2 / 3
     const int rowsPerBlock = vs ? 256 / 16 : 4 * (kWave / (dof * dof));
     int gs = (L.nNo + rowsPerBlock - 1) / rowsPerBlock;
@@ -340,20 +359,15 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
     if (gs < 1) gs = 1;
     unsigned gv = vec_grid(n);
     if (gv > 512) gv = 512;
-    SV_TRY(w.X.ensure(n * sizeof(double)));
-    SV_TRY(w.tmp.ensure(2 * n * sizeof(double)));
     // partials: SpMV grid (<= 1024), vector grid (<= 512), launch_dot (512),
     // then the scalar result slot
-    const size_t nPart = 1024 + 512 + 16;
-    SV_TRY(w.partial.ensure(nPart * sizeof(double)));
-    SV_TRY(w.state.ensure(sizeof(CgState)));
-    double* X = w.X.as<double>();
-    double* P = w.tmp.as<double>();
-    double* KP = P + n;
-    double* part1 = w.partial.as<double>();
+    double* X = b.X;
+    double* P = b.P;
+    double* KP = b.KP;
+    double* part1 = b.part;
     double* part2 = part1 + 1024;
-    double* scal = part1 + nPart - 8;
-    CgState* st = w.state.as<CgState>();
+    double* scal = part1 + kCgPartials - 8;
+    CgState* st = static_cast<CgState*>(b.state);
 
     ls.callD = wall_seconds();
     ls.suc = 0;
@@ -376,11 +390,9 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
     if (h0.err < eps) { h0.done = 1; h0.suc = 1; }
     if (ls.mItr <= 0) h0.done = 1;
     SV_HIP(hipMemcpyAsync(st, &h0, sizeof(CgState), hipMemcpyHostToDevice, s));
-    SV_HIP(hipMemsetAsync(X, 0, n * sizeof(double), s));
+    SV_TRY(launch_fill(n, 0.0, X, s));
     SV_HIP(hipMemcpyAsync(P, R, n * sizeof(double), hipMemcpyDeviceToDevice, s));
 
-    PinnedPoll& poll = w.poll;
-    SV_TRY(poll.ensure(256));
     constexpr int chunk = 16; // even: iteration parity = position in the chunk
     // a chunk of iterations: the same launches every time, one graph
     auto iters = [&](hipStream_t cs) -> int {
@@ -399,7 +411,6 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
         SV_HIP(hipGetLastError());
         return SVFSI_OK;
     };
-    const bool useGraphs = graphs_enabled();
     if (useGraphs) {
         const std::vector<uintptr_t> sig = {(uintptr_t)n, (uintptr_t)dof, (uintptr_t)K, (uintptr_t)R, (uintptr_t)X,
                                             (uintptr_t)P, (uintptr_t)part1, (uintptr_t)st, (uintptr_t)L.rowPtr.p,
@@ -420,18 +431,24 @@ int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, h
         else
             SV_TRY(iters(s));
         launched += chunk;
-        SV_TRY(poll.post(q, st, sizeof(CgState), s));
-        ++posted;
-        // check the previous chunk while this one runs
-        if (posted >= 2) {
-            SV_TRY(poll.wait(q ^ 1));
-            std::memcpy(&hs, poll.h[q ^ 1], sizeof(CgState));
+        if (pinned) {
+            SV_TRY(w.poll.post(q, st, sizeof(CgState), s));
+            ++posted;
+            // check the previous chunk while this one runs
+            if (posted >= 2) {
+                SV_TRY(w.poll.wait(q ^ 1));
+                std::memcpy(&hs, w.poll.h[q ^ 1], sizeof(CgState));
+                if (hs.done) finished = true;
+            }
+        } else {
+            SV_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+            SV_HIP(hipStreamSynchronize(s));
             if (hs.done) finished = true;
         }
         if (launched > ls.mItr + 2 * chunk) break; // safety: flag must be set by now
     }
     SV_HIP(hipStreamSynchronize(s));
-    SV_HIP(hipMemcpy(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost));
+    SV_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     SV_HIP(hipMemcpyAsync(R, X, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     SV_HIP(hipStreamSynchronize(s));
     ls.suc = hs.suc;

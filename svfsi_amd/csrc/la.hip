@@ -863,14 +863,71 @@ int build_mask(Lhs& L, int dof, const int* incL, hipStream_t s)
     return SVFSI_OK;
 }
 
+// The Dirichlet mask built on the device into caller memory, with no
+// allocation: mask = 1, then each included Dirichlet face multiplies its
+// entries by its values, face by face in the order of build_mask, so the
+// products are the same.  Each face's node list and values are copied into
+// `staging` (at least nNo * (4 + 8 dof) bytes) before its kernel runs.
+__global__ __launch_bounds__(256) void k_fill(size_t n, double v, double* __restrict__ x)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) x[t] = v;
+}
+
+__global__ __launch_bounds__(256) void k_mask_face(int nNo, int fdof, int dof, int nd, const int* __restrict__ glob,
+                                                   const double* __restrict__ val, double* __restrict__ mask)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nNo * nd) return;
+    const int a = t / nd, i = t - a * nd;
+    mask[(size_t)glob[a] * dof + i] *= val[(size_t)a * fdof + i];
+}
+
+int launch_fill(size_t n, double v, double* x, hipStream_t s)
+{
+    if (n == 0) return SVFSI_OK;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, v, x);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int build_mask_into(const Lhs& L, int dof, const int* incL, double* mask, void* staging, size_t stagingBytes,
+                    hipStream_t s)
+{
+    SV_TRY(launch_fill((size_t)L.nNo * dof, 1.0, mask, s));
+    for (size_t f = 0; f < L.faces.size(); ++f) {
+        const Face& F = L.faces[f];
+        if (!F.foC || (incL && !incL[f]) || F.bGrp != BC_TYPE_Dir || F.nNo == 0) continue;
+        const int nd = F.dof < dof ? F.dof : dof;
+        const size_t gb = (size_t)F.nNo * sizeof(int), vb = (size_t)F.nNo * F.dof * sizeof(double);
+        const size_t voff = (gb + 255) & ~size_t(255);
+        SV_ARG(voff + vb <= stagingBytes, "Dirichlet face %zu does not fit the staging buffer", f + 1);
+        int* dglob = static_cast<int*>(staging);
+        double* dval = reinterpret_cast<double*>(static_cast<char*>(staging) + voff);
+        SV_HIP(hipMemcpyAsync(dglob, F.glob.data(), gb, hipMemcpyHostToDevice, s));
+        SV_HIP(hipMemcpyAsync(dval, F.val.data(), vb, hipMemcpyHostToDevice, s));
+        const int n = F.nNo * nd;
+        hipLaunchKernelGGL(k_mask_face, dim3((n + 255) / 256), dim3(256), 0, s, F.nNo, F.dof, dof, nd,
+                           (const int*)dglob, (const double*)dval, mask);
+        SV_HIP(hipGetLastError());
+    }
+    return SVFSI_OK;
+}
+
 int precond_diag(Lhs& L, int dof, double* Val, double* R, double* W, float* Val32,
                  const int* incL, hipStream_t s)
 {
     SV_TRY(build_mask(L, dof, incL, s));
+    return precond_diag_masked(L, dof, Val, R, W, Val32, L.ws.mask.as<double>(), s);
+}
+
+int precond_diag_masked(const Lhs& L, int dof, double* Val, double* R, double* W, float* Val32,
+                        const double* mask, hipStream_t s)
+{
     const int n = L.nNo * dof;
     if (n == 0) return SVFSI_OK;
     hipLaunchKernelGGL(k_diag_w, dim3((n + 255) / 256), dim3(256), 0, s, L.nNo, dof,
-                       L.diagPtr.as<int>(), Val, L.ws.mask.as<double>(), W);
+                       L.diagPtr.as<int>(), Val, mask, W);
     const size_t nEnt = (size_t)L.nnz * dof * dof;
     const unsigned g = (unsigned)((nEnt + 255) / 256);
     switch (dof) {

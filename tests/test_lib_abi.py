@@ -29,7 +29,8 @@ def test_library_exports_all_declared():
     missing = [n for n in declared() if not hasattr(L, n)]
     assert not missing, missing
     # and the ctypes signature table covers them
-    assert set(declared()) <= set(_lib.SIGNATURES) | set(_lib.VOID_SIGNATURES) | {"svfsi_last_error"}
+    assert set(declared()) <= set(_lib.SIGNATURES) | set(_lib.VOID_SIGNATURES) | {
+        "svfsi_last_error", "svfsi_noalloc_scratch_bytes"}
 
 
 def test_ls_create_defaults_match_reference():
