@@ -125,6 +125,12 @@ def assemble(prob, rowPtr: np.ndarray, colPtr: np.ndarray) -> tuple[np.ndarray, 
     if prob.phys == 2:
         L.orc_construct_lelas(m.nNo, m.nEl, P(ien), P(x), P(Ag), P(Dg), Ag.shape[1], C.byref(eq), prop,
                               P(rowPtr), P(colPtr), P(Val), P(R))
+    elif prob.phys == 1 and m.eNoN == 10:
+        # Taylor-Hood (nFs = 2): element loop, then THOOD_ValRC after the
+        # face terms, where svFSI's MAIN calls it (MAIN.f:167-173)
+        if L.orc_construct_fluid_th(m.nNo, m.nEl, P(ien), P(x), P(Ag), P(Yg), Ag.shape[1], C.byref(eq), prop,
+                                    P(rowPtr), P(colPtr), P(Val), P(R)):
+            raise RuntimeError("GNNxx: singular element")
     elif prob.phys == 1:
         L.orc_construct_fluid(m.nNo, m.nEl, P(ien), P(x), P(Ag), P(Yg), Ag.shape[1], C.byref(eq), prop,
                               P(rowPtr), P(colPtr), P(Val), P(R))
@@ -139,7 +145,28 @@ def assemble(prob, rowPtr: np.ndarray, colPtr: np.ndarray) -> tuple[np.ndarray, 
         fi, fe = i32(fien), i32(fel)
         L.orc_backflow_tri3(len(fi), P(fi), P(fe), P(ien), m.eNoN, P(x), P(Yg), Yg.shape[1], C.byref(eq),
                             P(rowPtr), P(colPtr), P(Val), P(R))
+    if prob.phys == 1 and m.eNoN == 10:
+        L.orc_thood_valrc(m.nNo, m.nEl, P(ien), P(rowPtr), P(colPtr), P(Val), P(R))
     return Val, R
+
+
+def gnnxx(Nxi: np.ndarray, Nxi2: np.ndarray, xl: np.ndarray, Nx: np.ndarray) -> np.ndarray:
+    """GNNxx (svFSI/NN.f:1568-1678): physical second derivatives (eNoN, 6)
+    in the order xx, yy, zz, xy, yz, xz."""
+    Nxx = np.zeros((len(Nxi), 6))
+    if lib().orc_gnnxx(len(Nxi), P(f64(Nxi)), P(f64(Nxi2)), P(f64(xl)), P(f64(Nx)), P(Nxx)):
+        raise RuntimeError("GNNxx: singular system")
+    return Nxx
+
+
+def shape_tet10(xi) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(N (10), Nxi (10, 3), Nxi2 (10, 6)) of the tet10 basis at xi."""
+    N = np.zeros(10)
+    Nxi = np.zeros((10, 3))
+    Nxi2 = np.zeros((10, 6))
+    lib().orc_shape_tet10(P(f64(xi)), P(N), P(Nxi))
+    lib().orc_shape_tet10_xx(P(Nxi2))
+    return N, Nxi, Nxi2
 
 
 def getviscosity(eq: dict, gamma: float, mu_i: float | None = None):

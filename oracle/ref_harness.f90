@@ -163,6 +163,9 @@ contains
          select case (hdr(1))
          case (phys_fluid)
             call CONSTRUCT_FLUID(msh(1), Ag, Yg)
+            ! P2P1 (nFs = 2): the pressure rows of the mid-edge nodes,
+            ! as svFSI's MAIN does after the assembly (MAIN.f:167-173)
+            if (msh(1)%nFs == 2) call THOOD_ValRC()
          case (phys_lElas)
             call CONSTRUCT_LELAS(msh(1), Ag, Dg)
          case (phys_struct)

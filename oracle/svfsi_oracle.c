@@ -448,15 +448,191 @@ void orc_construct_fluid(int nNo, int nEl, const int* IEN, const double* x,
         double mu = mu_el ? mu_el[e] : eq->mu;
         orc_gnn(eNoN, (const double (*)[3])Nxi, (const double (*)[3])xl, Nx, &Jac, ks);
         for (int g = 0; g < 4; ++g)
-            orc_fluid3d_m(eNoN, wg[g] * Jac, (const double (*)[3])ks, Ng[g],
-                          (const double (*)[3])Nx, (const double (*)[6])Nxx,
+            orc_fluid3d_m(1, eNoN, eNoN, wg[g] * Jac, (const double (*)[3])ks, Ng[g], Ng[g],
+                          (const double (*)[3])Nx, (const double (*)[3])Nx, (const double (*)[6])Nxx,
                           al, yl, 4, eq, mu, lR, lK);
         for (int g = 0; g < 4; ++g)
-            orc_fluid3d_c(eNoN, wg[g] * Jac, (const double (*)[3])ks, Ng[g],
-                          (const double (*)[3])Nx, (const double (*)[6])Nxx,
+            orc_fluid3d_c(1, eNoN, eNoN, wg[g] * Jac, (const double (*)[3])ks, Ng[g], Ng[g],
+                          (const double (*)[3])Nx, (const double (*)[3])Nx, (const double (*)[6])Nxx,
                           al, yl, 4, eq, mu, lR, lK);
         orc_doassem(eNoN, ptr, dof, lK, lR, rowPtr, colPtr, Val, R);
     }
+}
+
+/* GETGNNxx, TET10 (svFSI/NN.f:1285-1295): the constant second derivatives
+ * d2N/dxi_i dxi_j in the order (11, 22, 33, 12, 23, 13). */
+void orc_shape_tet10_xx(double Nxi2[10][6])
+{
+    static const double t[10][6] = {
+        {4, 0, 0, 0, 0, 0}, {0, 4, 0, 0, 0, 0}, {0, 0, 4, 0, 0, 0}, {4, 4, 4, 4, 4, 4},
+        {0, 0, 0, 4, 0, 0}, {0, 0, 0, 0, 4, 0}, {0, 0, 0, 0, 0, 4},
+        {-8, 0, 0, -4, 0, -4}, {0, -8, 0, -4, -4, 0}, {0, 0, -8, 0, -4, -4}};
+    memcpy(Nxi2, t, sizeof(t));
+}
+
+/* Solves K X = B for X (6 x 6 K, nrhs right-hand sides stored B[r][i]) by
+ * LU factorisation with partial (row) pivoting, the algorithm of LAPACK's
+ * DGESV that GNNxx calls (svFSI/NN.f:1672).  Returns 0, or 1 if K is
+ * singular. */
+static int solve6(double K[6][6], double (*B)[6], int nrhs)
+{
+    int piv[6];
+    for (int k = 0; k < 6; ++k) {
+        int p = k;
+        for (int i = k + 1; i < 6; ++i)
+            if (fabs(K[i][k]) > fabs(K[p][k])) p = i;
+        piv[k] = p;
+        if (K[p][k] == 0.0) return 1;
+        if (p != k)
+            for (int j = 0; j < 6; ++j) { double t = K[k][j]; K[k][j] = K[p][j]; K[p][j] = t; }
+        for (int i = k + 1; i < 6; ++i) {
+            K[i][k] /= K[k][k];
+            for (int j = k + 1; j < 6; ++j) K[i][j] -= K[i][k] * K[k][j];
+        }
+    }
+    for (int r = 0; r < nrhs; ++r) {
+        double* b = B[r];
+        for (int k = 0; k < 6; ++k)
+            if (piv[k] != k) { double t = b[k]; b[k] = b[piv[k]]; b[piv[k]] = t; }
+        for (int i = 1; i < 6; ++i)
+            for (int j = 0; j < i; ++j) b[i] -= K[i][j] * b[j];
+        for (int i = 5; i >= 0; --i) {
+            for (int j = i + 1; j < 6; ++j) b[i] -= K[i][j] * b[j];
+            b[i] /= K[i][i];
+        }
+    }
+    return 0;
+}
+
+/* GNNxx (svFSI/NN.f:1568-1678), 3-D: the physical second derivatives
+ * Nxx(a) (order xx, yy, zz, xy, yz, xz) from the chain rule
+ * d2N/dxi_i dxi_j = sum_kl x_k,i x_l,j N,kl + sum_k x_k,ij N,k, i.e. the
+ * 6 x 6 system K Nxx(a) = Nxi2(a) - Nx(a) . xXi2 with K built from the
+ * Jacobian xXi (NN.f:1632-1662).  Returns nonzero if K is singular. */
+int orc_gnnxx(int eNoN, const double (*Nxi)[3], const double (*Nxi2)[6], const double (*xl)[3],
+              const double (*Nx)[3], double (*Nxx)[6])
+{
+    double xXi[3][3] = {{0}}, xXi2[3][6] = {{0}};
+    for (int a = 0; a < eNoN; ++a)
+        for (int k = 0; k < 3; ++k) {
+            for (int i = 0; i < 3; ++i) xXi[k][i] += xl[a][k] * Nxi[a][i];
+            for (int i = 0; i < 6; ++i) xXi2[k][i] += xl[a][k] * Nxi2[a][i];
+        }
+    double K[6][6];
+    for (int i = 0; i < 3; ++i) {
+        K[i][0] = xXi[0][i] * xXi[0][i];
+        K[i][1] = xXi[1][i] * xXi[1][i];
+        K[i][2] = xXi[2][i] * xXi[2][i];
+        K[i][3] = 2.0 * xXi[0][i] * xXi[1][i];
+        K[i][4] = 2.0 * xXi[1][i] * xXi[2][i];
+        K[i][5] = 2.0 * xXi[0][i] * xXi[2][i];
+    }
+    const int pr[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+    for (int r = 0; r < 3; ++r) {
+        const int i = pr[r][0], j = pr[r][1];
+        double* k = K[3 + r];
+        k[0] = xXi[0][i] * xXi[0][j];
+        k[1] = xXi[1][i] * xXi[1][j];
+        k[2] = xXi[2][i] * xXi[2][j];
+        k[3] = xXi[0][i] * xXi[1][j] + xXi[0][j] * xXi[1][i];
+        k[4] = xXi[1][i] * xXi[2][j] + xXi[1][j] * xXi[2][i];
+        k[5] = xXi[0][i] * xXi[2][j] + xXi[0][j] * xXi[2][i];
+    }
+    for (int a = 0; a < eNoN; ++a)
+        for (int i = 0; i < 6; ++i)
+            Nxx[a][i] = Nxi2[a][i] - Nx[a][0] * xXi2[0][i] - Nx[a][1] * xXi2[1][i] - Nx[a][2] * xXi2[2][i];
+    return solve6(K, Nxx, eNoN);
+}
+
+/* CONSTRUCT_FLUID (svFSI/FLUID.f:40-190) with a Taylor-Hood basis (nFs =
+ * 2, P2P1): velocity on the tet10 mesh basis, pressure on the tet4 basis of
+ * the corner nodes (SETTHOODFS, FS.f:201-255), vmsFlag = .FALSE.
+ *   Integration 1 (GETTHOODFS iOpt 1, FS.f:288-308): the tet10 15-point
+ *   rule; at every point GNN of the tet10 map gives Nwx, Jac and ksix and
+ *   GNNxx gives Nwxx; the tet4 map's GNN (Nqx) at the first point only.
+ *   Then FLUID3D_M.
+ *   Integration 2 (iOpt 2, FS.f:310-330): the tet4 4-point rule; at every
+ *   point GNN of the tet10 map (Nwx, Jac, ksix), then at the first point
+ *   GNN of the tet4 map, whose Jac and ksix replace the tet10 ones there;
+ *   Nwxx keeps its value from the last point of integration 1.  Then
+ *   FLUID3D_C.
+ * IEN[nEl][10], state (nNo, tDof).  Returns nonzero on a singular GNNxx. */
+int orc_construct_fluid_th(int nNo, int nEl, const int* IEN, const double* x,
+                           const double* Ag, const double* Yg, int tDof,
+                           const orc_eq* eq, const double* mu_el,
+                           const int* rowPtr, const int* colPtr,
+                           double* Val, double* R)
+{
+    (void)nNo;
+    enum { NW = 10, NQ = 4, dof = 4 };
+    double w1[15], xi1[15][3], w2[4], xi2[4][3];
+    orc_rule_tet10(w1, xi1);
+    orc_rule_tet4(w2, xi2);
+    double Nw1[15][NW], Nwxi1[15][NW][3], Nq1[15][NQ], Nqxi[NQ][3];
+    double Nw2[4][NW], Nwxi2[4][NW][3], Nq2[4][NQ], Nxi2[NW][6];
+    for (int g = 0; g < 15; ++g) {
+        orc_shape_tet10(xi1[g], Nw1[g], Nwxi1[g]);
+        orc_shape_tet4(xi1[g], Nq1[g], Nqxi);
+    }
+    for (int g = 0; g < 4; ++g) {
+        orc_shape_tet10(xi2[g], Nw2[g], Nwxi2[g]);
+        orc_shape_tet4(xi2[g], Nq2[g], Nqxi);
+    }
+    orc_shape_tet10_xx(Nxi2);
+    double xl[NW][3], al[NW * 4], yl[NW * 4], Nwx[NW][3], Nqx[NQ][3], Nwxx[NW][6], ks[3][3], Jac = 0.0;
+    double lR[NW * 4], lK[NW * NW * 16];
+    int ptr[NW];
+    for (int e = 0; e < nEl; ++e) {
+        for (int a = 0; a < NW; ++a) {
+            int Ac = IEN[(size_t)e * NW + a];
+            ptr[a] = Ac;
+            for (int i = 0; i < 3; ++i) xl[a][i] = x[(size_t)Ac * 3 + i];
+            for (int i = 0; i < 4; ++i) {
+                al[a * 4 + i] = Ag[(size_t)Ac * tDof + i];
+                yl[a * 4 + i] = Yg[(size_t)Ac * tDof + i];
+            }
+        }
+        memset(lR, 0, sizeof(lR));
+        memset(lK, 0, sizeof(lK));
+        const double mu = mu_el ? mu_el[e] : eq->mu;
+        for (int g = 0; g < 15; ++g) {
+            if (g == 0) orc_gnn(NQ, (const double (*)[3])Nqxi, (const double (*)[3])xl, Nqx, &Jac, ks);
+            orc_gnn(NW, (const double (*)[3])Nwxi1[g], (const double (*)[3])xl, Nwx, &Jac, ks);
+            if (orc_gnnxx(NW, (const double (*)[3])Nwxi1[g], (const double (*)[6])Nxi2, (const double (*)[3])xl,
+                          (const double (*)[3])Nwx, Nwxx))
+                return 1;
+            orc_fluid3d_m(0, NW, NQ, w1[g] * Jac, (const double (*)[3])ks, Nw1[g], Nq1[g],
+                          (const double (*)[3])Nwx, (const double (*)[3])Nqx, (const double (*)[6])Nwxx,
+                          al, yl, 4, eq, mu, lR, lK);
+        }
+        for (int g = 0; g < 4; ++g) {
+            orc_gnn(NW, (const double (*)[3])Nwxi2[g], (const double (*)[3])xl, Nwx, &Jac, ks);
+            if (g == 0) orc_gnn(NQ, (const double (*)[3])Nqxi, (const double (*)[3])xl, Nqx, &Jac, ks);
+            orc_fluid3d_c(0, NW, NQ, w2[g] * Jac, (const double (*)[3])ks, Nw2[g], Nq2[g],
+                          (const double (*)[3])Nwx, (const double (*)[3])Nqx, (const double (*)[6])Nwxx,
+                          al, yl, 4, eq, mu, lR, lK);
+        }
+        orc_doassem(NW, ptr, dof, lK, lR, rowPtr, colPtr, Val, R);
+    }
+    return 0;
+}
+
+/* THOOD_ValRC (svFSI/FS.f:352-407): every node that is a mid-edge node
+ * (local 5..10) of a tet10 element has its continuity residual set to 0
+ * and, in its block row, the pressure-pressure entry (4,4) set to 1 on the
+ * diagonal block and to 0 elsewhere. */
+void orc_thood_valrc(int nNo, int nEl, const int* IEN, const int* rowPtr, const int* colPtr, double* Val,
+                     double* R)
+{
+    char* mid = (char*)calloc((size_t)nNo, 1);
+    for (int e = 0; e < nEl; ++e)
+        for (int a = 4; a < 10; ++a) mid[IEN[(size_t)e * 10 + a]] = 1;
+    for (int a = 0; a < nNo; ++a) {
+        if (!mid[a]) continue;
+        R[(size_t)a * 4 + 3] = 0.0;
+        for (int k = rowPtr[a]; k < rowPtr[a + 1]; ++k) Val[(size_t)k * 16 + 15] = colPtr[k] == a ? 1.0 : 0.0;
+    }
+    free(mid);
 }
 
 /* Complex-step instantiation of FLUID3D_M/C: with mu = mu_e + i h the
@@ -508,11 +684,13 @@ void orc_fluid_dmu_dot(int nEl, const int* IEN, const double* x, const double* A
         const double complex mu = mu0 + h * I;
         orc_gnn(eNoN, (const double (*)[3])Nxi, (const double (*)[3])xl, Nx, &Jac, ks);
         for (int q = 0; q < 4; ++q)
-            orc_fluid3d_m_cs(eNoN, wg[q] * Jac, (const double (*)[3])ks, Ng[q], (const double (*)[3])Nx,
-                             (const double (*)[6])Nxx, al, yl, 4, eq, mu, lR, lK);
+            orc_fluid3d_m_cs(1, eNoN, eNoN, wg[q] * Jac, (const double (*)[3])ks, Ng[q], Ng[q],
+                             (const double (*)[3])Nx, (const double (*)[3])Nx, (const double (*)[6])Nxx, al, yl,
+                             4, eq, mu, lR, lK);
         for (int q = 0; q < 4; ++q)
-            orc_fluid3d_c_cs(eNoN, wg[q] * Jac, (const double (*)[3])ks, Ng[q], (const double (*)[3])Nx,
-                             (const double (*)[6])Nxx, al, yl, 4, eq, mu, lR, lK);
+            orc_fluid3d_c_cs(1, eNoN, eNoN, wg[q] * Jac, (const double (*)[3])ks, Ng[q], Ng[q],
+                             (const double (*)[3])Nx, (const double (*)[3])Nx, (const double (*)[6])Nxx, al, yl,
+                             4, eq, mu, lR, lK);
         double s = 0.0;
         for (int a = 0; a < eNoN; ++a)
             for (int r = 0; r < dof; ++r) {

@@ -87,14 +87,14 @@ void orc_doassem(int d, const int* eqN, int dof, const double* lK,
 void orc_lelas3d(int eNoN, double w, const double* N, const double (*Nx)[3],
                  const double* al, const double* dl, int tDof,
                  const orc_eq* eq, double E, double* lR, double* lK);
-void orc_fluid3d_m(int eNoN, double w, const double ks[3][3], const double* N,
-                   const double (*Nx)[3], const double (*Nxx)[6],
-                   const double* al, const double* yl, int tDof,
-                   const orc_eq* eq, double mu, double* lR, double* lK);
-void orc_fluid3d_c(int eNoN, double w, const double ks[3][3], const double* N,
-                   const double (*Nx)[3], const double (*Nxx)[6],
-                   const double* al, const double* yl, int tDof,
-                   const orc_eq* eq, double mu, double* lR, double* lK);
+void orc_fluid3d_m(int vms, int eNoN, int eNoNq, double w, const double ks[3][3], const double* N,
+                   const double* Nq, const double (*Nx)[3], const double (*Nqx)[3], const double (*Nxx)[6],
+                   const double* al, const double* yl, int tDof, const orc_eq* eq, double mu, double* lR,
+                   double* lK);
+void orc_fluid3d_c(int vms, int eNoN, int eNoNq, double w, const double ks[3][3], const double* N,
+                   const double* Nq, const double (*Nx)[3], const double (*Nqx)[3], const double (*Nxx)[6],
+                   const double* al, const double* yl, int tDof, const orc_eq* eq, double mu, double* lR,
+                   double* lK);
 void orc_getpk2cc_nhk(const orc_eq* eq, const double F[3][3],
                       double S[3][3], double Dm[6][6]);
 void orc_struct3d(int eNoN, double w, const double* N, const double (*Nx)[3],
@@ -120,6 +120,21 @@ void orc_construct_fluid(int nNo, int nEl, const int* IEN, const double* x,
                          const orc_eq* eq, const double* mu_el,
                          const int* rowPtr, const int* colPtr,
                          double* Val, double* R);
+/* CONSTRUCT_FLUID with the Taylor-Hood basis (nFs = 2): tet10 velocity,
+ * tet4 pressure, no VMS terms; IEN[nEl][10].  Returns nonzero if GNNxx
+ * meets a singular element.  THOOD_ValRC (FS.f:352-407) is separate, as
+ * svFSI calls it after the assembly. */
+int orc_construct_fluid_th(int nNo, int nEl, const int* IEN, const double* x,
+                           const double* Ag, const double* Yg, int tDof,
+                           const orc_eq* eq, const double* mu_el,
+                           const int* rowPtr, const int* colPtr,
+                           double* Val, double* R);
+void orc_thood_valrc(int nNo, int nEl, const int* IEN, const int* rowPtr, const int* colPtr, double* Val,
+                     double* R);
+/* GNNxx (svFSI/NN.f:1568-1678) and the TET10 second-derivative table. */
+int orc_gnnxx(int eNoN, const double (*Nxi)[3], const double (*Nxi2)[6], const double (*xl)[3],
+              const double (*Nx)[3], double (*Nxx)[6]);
+void orc_shape_tet10_xx(double Nxi2[10][6]);
 /* Viscosity sensitivity (no reference counterpart): g[e] = lam_e .
  * (dR_e/dmu_e - dK_e/dmu_e du_e), derivative by complex step. */
 void orc_fluid_dmu_dot(int nEl, const int* IEN, const double* x, const double* Ag,

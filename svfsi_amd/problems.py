@@ -46,6 +46,10 @@ class Problem:
     ls: dict = field(default_factory=dict)
     prop_el: np.ndarray | None = None
     quad: int = 0
+    # function spaces of a FLUID mesh (svFSI msh%nFs): 1 = the mesh basis for
+    # velocity and pressure (VMS), 2 = Taylor-Hood (tet10 velocity, tet4
+    # pressure on the corner nodes)
+    nFs: int = 1
 
     @property
     def nNo(self) -> int:

@@ -124,6 +124,30 @@ def fluid():
         save(f"ref_fluid_{tag}", p, rp, cp, V, R, ext)
 
 
+def fluid_th_problem(n, seed, dt, umax, mu, viscType=698, **visc):
+    """Taylor-Hood (P2P1) fluid on a jittered tet10 cube: random velocity,
+    acceleration and corner pressure."""
+    m = jitter(mesh.cube_tet10(n), 0.2, seed)
+    rng = np.random.default_rng(seed + 200)
+    eq = dict(problems.gen_alpha(0.2, False), dt=dt, rho=1.06, mu=mu, bstab=0.2, fb=(0.3, -0.2, 0.1),
+              viscType=viscType, **visc)
+    Y = np.zeros((m.nNo, 4))
+    Y[:, :3] = rng.uniform(-umax, umax, (m.nNo, 3))
+    corners = np.unique(m.ien[:, :4])
+    Y[corners, 3] = rng.uniform(-1.0, 1.0, len(corners))
+    A = rng.normal(size=(m.nNo, 4)) * umax / dt * 1e-2
+    return problems.Problem(name="fluid_th", phys=problems.SVFSI_PHYS_FLUID, dof=4, mesh=m, eq=eq, Ag=A, Yg=Y,
+                            Dg=np.zeros_like(Y), dir_faces=[], nFs=2)
+
+
+def fluid_th():
+    for tag, vt, visc, mu in (("const", 698, {}, 0.04),
+                              ("cy", 697, dict(mu_o=0.56, lam=3.313, visc_a=2.0, visc_n=0.3568), 0.0345)):
+        p = fluid_th_problem(2, 6, dt=0.01, umax=5.0, mu=mu, viscType=vt, **visc)
+        rp, cp, V, R, _ = rh.assemble(p, nFs=2)
+        save(f"ref_fluid_th10_{tag}", p, rp, cp, V, R, dict(nFs=2))
+
+
 def lelas():
     m = jitter(mesh.cube_tet4(4), 0.2, 3)
     rng = np.random.default_rng(103)
@@ -153,6 +177,6 @@ def struct():
 if __name__ == "__main__":
     if not rh.available():
         rh.build()
-    fluid()
-    lelas()
-    struct()
+    which = sys.argv[1:] or ["fluid", "fluid_th", "lelas", "struct"]
+    for name in which:
+        globals()[name]()
