@@ -276,6 +276,13 @@ int svfsi_spmv_f32_dev(void* lhs, int dof, const float* dK, const float* dU,
 /* Transpose product KU = K^T U (adjoint solve; no reference counterpart). */
 int svfsi_spmv_t_dev(void* lhs, int dof, const double* dK, const double* dU,
                      double* dKU, void* stream);
+/* Residual dR = dB - K dU (transpose != 0: dB - K^T dU) with every row's sum
+ * carried in double-double (FMA-split products, TwoSum accumulation) and
+ * rounded once: the residual of iterative refinement on ill-conditioned
+ * systems (adjoint gradients; no reference counterpart).  dR may alias dB
+ * but not dU. */
+int svfsi_resid_dd_dev(void* lhs, int dof, const double* dK, const double* dU,
+                       const double* dB, double* dR, int transpose, void* stream);
 /* FSILS_DOTV / FSILS_NORMV one rank (svFSILS/DOT.f:56-113, NORM.f:55-113):
  * deterministic two-stage reduction; the result is written to *out (host). */
 int svfsi_dot_dev(int n, const double* dU, const double* dV, double* out,
@@ -343,6 +350,13 @@ typedef struct {
  *   MFMA64 -- B^T Dm B on FP64 matrix cores (the default);
  *   MFMA32 -- B^T Dm B on FP32 matrix cores with FP32 accumulation (the
  *             rest, and the accumulation into Val, stays FP64). */
+/* FLUID on a tet10 mesh: the Taylor-Hood (P2P1) basis of svFSI's nFs = 2
+ * (SETTHOODFS/GETTHOODFS, svFSI/FS.f:201-346): quadratic velocity on the
+ * tet10 nodes, linear pressure on the corner nodes, no VMS terms; the
+ * assembly ends with THOOD_ValRC (FS.f:352-407), which sets the pressure
+ * rows of the mid-edge nodes to the identity.  Passed as quad. */
+#define SVFSI_FLUID_P2P1    0x400
+
 #define SVFSI_STRUCT_AUTO   0x000
 #define SVFSI_STRUCT_VALU   0x100
 #define SVFSI_STRUCT_MFMA64 0x200
@@ -371,7 +385,8 @@ int svfsi_zero_dev(void* mesh, int dof, double* dVal, double* dR, void* stream);
  * device (tDof,nNo).  prop_el: optional per-element property (LELAS E_e,
  * FLUID mu_e, NULL = uniform).  quad: 0 = the reference rule (tet4 4-pt,
  * tet10 15-pt), 1 = tet10 4-point rule; for STRUCT, OR one of the
- * SVFSI_STRUCT_* kernel selectors below into quad (default: MFMA64). */
+ * SVFSI_STRUCT_* kernel selectors above into quad (default: MFMA64); FLUID
+ * on a tet10 mesh takes quad = SVFSI_FLUID_P2P1 (the reference's rules). */
 int svfsi_assemble_dev(void* mesh, int phys, const svfsi_eq_params* p,
                        int tDof, const double* dAg, const double* dYg,
                        const double* dDg, const double* dprop_el, int quad,

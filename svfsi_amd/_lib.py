@@ -24,6 +24,9 @@ SVFSI_PHYS_FLUID = 1
 SVFSI_PHYS_LELAS = 2
 SVFSI_PHYS_STRUCT = 3
 
+# FLUID on tet10: the Taylor-Hood (P2P1, nFs = 2) basis, passed as quad
+SVFSI_FLUID_P2P1 = 0x400
+
 # STRUCT element kernel, ORed into the assembly's quad argument
 SVFSI_STRUCT_AUTO = 0x000
 SVFSI_STRUCT_VALU = 0x100
@@ -106,6 +109,7 @@ SIGNATURES = {
     "svfsi_spmv_dev": [_P, _I, _P, _P, _P, _P],
     "svfsi_spmv_f32_dev": [_P, _I, _P, _P, _P, _P],
     "svfsi_spmv_t_dev": [_P, _I, _P, _P, _P, _P],
+    "svfsi_resid_dd_dev": [_P, _I, _P, _P, _P, _P, _I, _P],
     "svfsi_dot_dev": [_I, _P, _P, _DP, _P],
     "svfsi_precond_diag_dev": [_P, _I, _P, _P, _P, _IP, _P],
     "svfsi_precond_rcs_dev": [_P, _I, _P, _P, _P, _P, _IP, _P],

@@ -78,7 +78,7 @@ Tet4Rule make_tet4_rule()
 // GETGIP + GETGNN for TET10 (svFSI/NN.f:277-307, 674-715): the 15-point
 // rule (quad 0) or the 4-point TET4 rule (quad 1), tabulated as
 // w, N(10), Nxi(10,3) per Gauss point.
-int make_tet10_rule(int quad, std::vector<double>& tab)
+int make_tet10_rule(int quad, std::vector<double>& tab, std::vector<double>* xiOut)
 {
     double w[15], xi[15][3];
     int nG;
@@ -137,7 +137,28 @@ int make_tet10_rule(int quad, std::vector<double>& tab)
         for (int a = 0; a < 10; ++a)
             for (int j = 0; j < 3; ++j) r[11 + a * 3 + j] = d[a][j];
     }
+    if (xiOut) xiOut->assign(&xi[0][0], &xi[0][0] + (size_t)nG * 3);
     return nG;
+}
+
+int make_th_rule(std::vector<double>& tab)
+{
+    tab.assign((size_t)19 * kThRec, 0.0);
+    for (int q = 0; q < 2; ++q) {
+        std::vector<double> t10, xi;
+        const int nG = make_tet10_rule(q, t10, &xi);
+        for (int g = 0; g < nG; ++g) {
+            double* r = tab.data() + (size_t)(q * 15 + g) * kThRec;
+            for (int k = 0; k < kTet10Rec; ++k) r[k] = t10[(size_t)g * kTet10Rec + k];
+            // the tet4 basis at the same point (GETGNN, svFSI/NN.f:655-671)
+            const double* x = &xi[(size_t)g * 3];
+            r[41] = x[0];
+            r[42] = x[1];
+            r[43] = x[2];
+            r[44] = 1.0 - x[0] - x[1] - x[2];
+        }
+    }
+    return 19;
 }
 
 Tri3Rule make_tri3_rule()
@@ -261,6 +282,12 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
     const size_t n = (size_t)L.nNo * dof;
     Work& w = L.ws;
     SV_TRY(w.W.ensure(n * sizeof(double)));
+    // SVFSI_INNER_FP64=1: the mixed entry points run the same nested solver
+    // with its inner preconditioner in FP64 (a comparison of precisions)
+    if (mode == 1) {
+        const char* e = getenv("SVFSI_INNER_FP64");
+        if (e && e[0] == '1') mode = 2;
+    }
     float* v32 = nullptr;
     if (mode == 1) {
         SV_TRY(w.val32.ensure((size_t)L.nnz * dof * dof * sizeof(float)));
@@ -277,6 +304,7 @@ static int solve_impl(Lhs& L, FSILS_lsType* ls, int dof, double* R, double* Val,
     }
     int rc;
     if (mode == 1) rc = mixed_solve(L, ls->RI, dof, Val, v32, R, s);
+    else if (mode == 2) rc = nested_solve_fp64(L, ls->RI, dof, Val, R, s);
     else if (ls->LS_type == LS_TYPE_NS) rc = ns_solve(L, *ls, dof, Val, R, s);
     else if (ls->LS_type == LS_TYPE_GMRES) rc = gmres_solve(L, ls->RI, dof, Val, R, s, false, L.cpl.active ? 1 : 0);
     else if (ls->LS_type == LS_TYPE_BICGS) rc = bicgs_solve(L, ls->RI, dof, Val, R, s);
@@ -581,6 +609,16 @@ int svfsi_spmv_t_dev(void* lhs, int dof, const double* dK, const double* dU, dou
     return launch_spmv_t(*static_cast<Lhs*>(lhs), dof, dK, dU, dKU, static_cast<hipStream_t>(stream));
 }
 
+int svfsi_resid_dd_dev(void* lhs, int dof, const double* dK, const double* dU, const double* dB, double* dR,
+                       int transpose, void* stream)
+{
+    SV_ARG(lhs && dK && dU && dB && dR, "NULL argument");
+    SV_ARG(dR != dU, "dR must not alias dU");
+    SV_TRY(require_device());
+    return launch_resid_dd(*static_cast<Lhs*>(lhs), dof, dK, dU, dB, dR, transpose != 0,
+                           static_cast<hipStream_t>(stream));
+}
+
 int svfsi_dot_dev(int n, const double* dU, const double* dV, double* out, void* stream)
 {
     SV_ARG(dU && dV && out && n >= 0, "bad argument");
@@ -673,10 +711,22 @@ int svfsi_mesh_create(void** mesh, void* lhs, int nNo, int nEl, int eNoN, const 
         }
     }
     std::vector<double> xv(x, x + (size_t)3 * nNo);
+    // mid-edge nodes (local 5..10 of a tet10), whose pressure rows
+    // THOOD_ValRC resets in a Taylor-Hood FLUID assembly (FS.f:383-391)
+    std::vector<int> mids;
+    if (eNoN == 10) {
+        std::vector<char> isMid(nNo, 0);
+        for (int e = 0; e < nEl; ++e)
+            for (int a = 4; a < 10; ++a) isMid[ien[(size_t)e * 10 + a]] = 1;
+        for (int n = 0; n < nNo; ++n)
+            if (isMid[n]) mids.push_back(n);
+    }
+    M->nMid = (int)mids.size();
     int rc = SVFSI_OK;
     if ((rc = upload(M->ien, sien)) || (rc = upload(M->elemId, order)) || (rc = upload(M->emap, emap)) ||
         (rc = upload(M->x, xv)) || (rc = upload(M->gPtr, gptr)) || (rc = upload(M->gSlot, gslot)) ||
-        (rc = upload(M->rPtr, rptr)) || (rc = upload(M->rSlot, rslot))) {
+        (rc = upload(M->rPtr, rptr)) || (rc = upload(M->rSlot, rslot)) ||
+        (!mids.empty() && (rc = upload(M->midNodes, mids)))) {
         delete M;
         return rc;
     }
@@ -770,12 +820,20 @@ int svfsi_assemble_dev(void* mesh, int phys, const svfsi_eq_params* p, int tDof,
         SV_ARG(M.eNoN == 4, "LELAS assembly supports tet4 only");
         SV_ARG(tDof >= 3 && dAg && dDg, "LELAS needs Ag, Dg with tDof >= 3");
     } else if (phys == SVFSI_PHYS_FLUID) {
-        SV_ARG(M.eNoN == 4, "FLUID assembly supports tet4 only");
         SV_ARG(tDof >= 4 && dAg && dYg, "FLUID needs Ag, Yg with tDof >= 4");
         const int vt = p->viscType;
         SV_ARG(vt == 0 || vt == SVFSI_VISC_CONST || vt == SVFSI_VISC_CY || vt == SVFSI_VISC_CASS,
                "viscType %d: SVFSI_VISC_CONST (698), SVFSI_VISC_CY (697) or SVFSI_VISC_CASS (696)", vt);
         SV_ARG(vt != SVFSI_VISC_CY || p->visc_a > 0.0, "Carreau-Yasuda needs a > 0");
+        if (M.eNoN == 10) {
+            if (quad != SVFSI_FLUID_P2P1) {
+                set_error("FLUID on tet10: pass quad = SVFSI_FLUID_P2P1 (Taylor-Hood, nFs = 2); the VMS tet10 "
+                          "element (nFs = 1) is not built");
+                return SVFSI_ERR_NOTSUP;
+            }
+            return launch_asm_fluid_th(M, to_asm(*p), tDof, dAg, dYg, dprop_el, dVal, dR, s);
+        }
+        SV_ARG(quad == 0, "FLUID on tet4: quad must be 0 (SVFSI_FLUID_P2P1 needs a tet10 mesh)");
     } else if (phys == SVFSI_PHYS_STRUCT) {
         SV_ARG(M.eNoN == 10, "STRUCT assembly supports tet10 only");
         SV_ARG(tDof >= 3 && dAg && dDg, "STRUCT needs Ag, Dg with tDof >= 3");

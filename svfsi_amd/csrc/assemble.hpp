@@ -57,7 +57,18 @@ namespace svfsi {
 // Quadrature table of a TET10 rule (GETGIP/GETGNN, svFSI/NN.f:277-307,
 // 674-715): per Gauss point w, N(10), Nxi(10,3) -> 41 doubles.
 constexpr int kTet10Rec = 41;
-int make_tet10_rule(int quad, std::vector<double>& tab); // returns nG
+// returns nG; xiOut (optional, nG x 3) receives the points
+int make_tet10_rule(int quad, std::vector<double>& tab, std::vector<double>* xiOut = nullptr);
+// Taylor-Hood (P2P1) FLUID quadrature: the 15 points of the tet10 rule (the
+// momentum loop) then the 4 points of the tet4 rule (the continuity loop),
+// each as w, Nw(10), Nwxi(10,3) of the tet10 basis and Nq(4) of the tet4
+// basis -> 45 doubles per point.
+constexpr int kThRec = 45;
+int make_th_rule(std::vector<double>& tab);
+// CONSTRUCT_FLUID with nFs = 2 on a tet10 mesh, colour by colour, then
+// THOOD_ValRC on the mid-edge nodes.
+int launch_asm_fluid_th(Mesh& M, const AsmParams& P, int tDof, const double* Ag, const double* Yg,
+                        const double* muEl, double* val, double* R, hipStream_t s);
 // kern: one of SVFSI_STRUCT_*.  The MFMA kernels keep material-point
 // records in M.srec.
 int launch_asm_struct(Mesh& M, const AsmParams& P, int nG, const double* rule, int tDof,

@@ -92,4 +92,63 @@ __device__ __forceinline__ double dot4_nt(const float* __restrict__ v, const flo
          + (double)a.w * (double)xa.w;
 }
 
+
+// One dof-3 block row of K x, L lanes per row, each lane whole 3x3 blocks
+// (blocks k0 + l, k0 + l + L, ...; two in flight): the lane reads a block's
+// nine contiguous values (two 16-byte and one 4-byte load in FP32, four
+// 16-byte and one 8-byte in FP64) and x(:, col) once, with one column index
+// per block, and accumulates the three row sums in AT.  Consecutive lanes
+// read consecutive blocks, so a row's values stream contiguously.  The L
+// partial sums are then combined by lane shuffles: every lane of the group
+// returns the row's three sums.
+template <bool NT, typename VT>
+__device__ __forceinline__ VT ld_nt(const VT* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <typename VT, typename XT, typename AT, int L, bool NT>
+__device__ __forceinline__ void row3(const int* __restrict__ rowPtr, const int* __restrict__ colPtr,
+                                     const VT* __restrict__ val, const XT* __restrict__ x, int row, int l,
+                                     bool active, AT a[3])
+{
+    AT s0 = 0, s1 = 0, s2 = 0, t0 = 0, t1 = 0, t2 = 0;
+    if (active) {
+        const int e = rowPtr[row + 1];
+        int k = rowPtr[row] + l;
+        for (; k + L < e; k += 2 * L) {
+            const VT* u = val + (size_t)k * 9;
+            const VT* w = u + 9 * L;
+            const XT* xu = x + (size_t)colPtr[k] * 3;
+            const XT* xw = x + (size_t)colPtr[k + L] * 3;
+            const AT u0 = (AT)xu[0], u1 = (AT)xu[1], u2 = (AT)xu[2];
+            const AT w0 = (AT)xw[0], w1 = (AT)xw[1], w2 = (AT)xw[2];
+            s0 += (AT)ld_nt<NT>(u + 0) * u0 + (AT)ld_nt<NT>(u + 1) * u1 + (AT)ld_nt<NT>(u + 2) * u2;
+            s1 += (AT)ld_nt<NT>(u + 3) * u0 + (AT)ld_nt<NT>(u + 4) * u1 + (AT)ld_nt<NT>(u + 5) * u2;
+            s2 += (AT)ld_nt<NT>(u + 6) * u0 + (AT)ld_nt<NT>(u + 7) * u1 + (AT)ld_nt<NT>(u + 8) * u2;
+            t0 += (AT)ld_nt<NT>(w + 0) * w0 + (AT)ld_nt<NT>(w + 1) * w1 + (AT)ld_nt<NT>(w + 2) * w2;
+            t1 += (AT)ld_nt<NT>(w + 3) * w0 + (AT)ld_nt<NT>(w + 4) * w1 + (AT)ld_nt<NT>(w + 5) * w2;
+            t2 += (AT)ld_nt<NT>(w + 6) * w0 + (AT)ld_nt<NT>(w + 7) * w1 + (AT)ld_nt<NT>(w + 8) * w2;
+        }
+        if (k < e) {
+            const VT* u = val + (size_t)k * 9;
+            const XT* xu = x + (size_t)colPtr[k] * 3;
+            const AT u0 = (AT)xu[0], u1 = (AT)xu[1], u2 = (AT)xu[2];
+            s0 += (AT)ld_nt<NT>(u + 0) * u0 + (AT)ld_nt<NT>(u + 1) * u1 + (AT)ld_nt<NT>(u + 2) * u2;
+            s1 += (AT)ld_nt<NT>(u + 3) * u0 + (AT)ld_nt<NT>(u + 4) * u1 + (AT)ld_nt<NT>(u + 5) * u2;
+            s2 += (AT)ld_nt<NT>(u + 6) * u0 + (AT)ld_nt<NT>(u + 7) * u1 + (AT)ld_nt<NT>(u + 8) * u2;
+        }
+    }
+    a[0] = s0 + t0;
+    a[1] = s1 + t1;
+    a[2] = s2 + t2;
+#pragma unroll
+    for (int o = L >> 1; o > 0; o >>= 1) {
+        a[0] += __shfl_xor(a[0], o, kWave);
+        a[1] += __shfl_xor(a[1], o, kWave);
+        a[2] += __shfl_xor(a[2], o, kWave);
+    }
+}
+
 } // namespace svfsi

@@ -138,7 +138,8 @@ struct Work {
     DBuf ns;       // NS solver: the split blocks, its vectors and directions
     DBuf bicgs;    // BiCGStab: P, Rh, X, V, S, T and the dot partials
     DBuf valT; // K^T values for transposed solves
-    DBuf dinv32; // inverse diagonal blocks (FP32) of the inner block-Jacobi
+    DBuf dinv32; // inverse diagonal blocks (inner precision) of the inner block-Jacobi
+    DBuf valIn;  // FP64 nested solve: the block-Jacobi scaled inner matrix
     DBuf polyCoef, polyVec; // polynomial preconditioner: factors, work vectors
     int maskDof = -1;
     std::vector<int> maskInc;
@@ -191,6 +192,9 @@ struct Mesh {
     };
     std::vector<FaceSet*> faceSets;
     DBuf rule10[2];  // TET10 quadrature tables (15-point, 4-point), built on first use
+    DBuf ruleTH;     // Taylor-Hood FLUID quadrature (19 points), built on first use
+    DBuf midNodes;   // tet10: the nodes that are a mid-edge node of some element
+    int nMid = 0;
     int rule10nG[2] = {0, 0};
     DBuf srec;       // TET10 material-point records (nEl x nG x 97 doubles), grown on demand
     // Gather assembly: element block (slot p, a, b) is stored at position
@@ -258,20 +262,28 @@ int launch_spmv(const Lhs& L, int dof, const double* K, const double* U,
 int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
                     float* KU, hipStream_t s, const int* stop = nullptr,
                     const float* dinv = nullptr);
-int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s);
-int launch_bjscale(const Lhs& L, int dof, const double* K, const float* dinv, float* B, hipStream_t s);
-int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* dinv, const float* x,
-                const float* prodOld, float* w, float* y, float* out, const double* coef, int step, hipStream_t s,
-                const int* stop);
-int launch_poly_last(size_t n, const float* prod, float* y, const double* coef, int step, hipStream_t s,
-                     const int* stop);
-int launch_rand_masked(size_t n, unsigned seed, const double* W, float* v, hipStream_t s);
+// The inner (preconditioner) kernels of the nested solve, instantiated for
+// IT = float (the mixed solve) and IT = double (the same algorithm in FP64).
+template <typename OT>
+int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, OT* dinv, hipStream_t s);
+template <typename OT>
+int launch_bjscale(const Lhs& L, int dof, const double* K, const OT* dinv, OT* B, hipStream_t s);
+template <typename IT>
+int launch_poly(const Lhs& L, int dof, int mode, const IT* K, const IT* dinv, const IT* x, const IT* prodOld, IT* w,
+                IT* y, IT* out, const double* coef, int step, hipStream_t s, const int* stop);
+template <typename IT>
+int launch_poly_last(size_t n, const IT* prod, IT* y, const double* coef, int step, hipStream_t s, const int* stop);
+template <typename IT>
+int launch_rand_masked(size_t n, unsigned seed, const double* W, IT* v, hipStream_t s);
 int poly_factors(int d, int ld, const double* Hu, std::vector<double>& coef, std::vector<int>& kind);
-int launch_blockdiag_apply(const Lhs& L, int dof, const float* dinv, float* x, hipStream_t s,
-                           const int* stop);
+template <typename IT>
+int launch_blockdiag_apply(const Lhs& L, int dof, const IT* dinv, IT* x, hipStream_t s, const int* stop);
 int launch_spmv_t(const Lhs& L, int dof, const double* K, const double* U,
                   double* KU, hipStream_t s, const int* stop = nullptr);
 int launch_transpose_blocks(const Lhs& L, int dof, const double* V, double* VT, hipStream_t s);
+// r = b - K x (or b - K^T x) with double-double row sums, rounded once
+int launch_resid_dd(const Lhs& L, int dof, const double* K, const double* x, const double* b, double* r,
+                    bool transpose, hipStream_t s);
 int launch_dot(int n, const double* U, const double* V, double* partial,
                double* result, hipStream_t s);
 int build_mask(Lhs& L, int dof, const int* incL, hipStream_t s);
@@ -289,6 +301,8 @@ int precond_diag_masked(const Lhs& L, int dof, double* Val, double* R, double* W
 int build_mask_into(const Lhs& L, int dof, const int* incL, double* mask, void* staging, size_t stagingBytes,
                     hipStream_t s);
 int launch_fill(size_t n, double v, double* x, hipStream_t s);
+// lanes per block row of the dof-3 SpMV kernels for this pattern
+int spmv3_lanes(const Lhs& L);
 int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s);
 
 int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
@@ -321,6 +335,8 @@ int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
                 double* R, hipStream_t s, bool transpose, int cpl = 0);
 int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
                 float* K32, double* R, hipStream_t s);
+// The same nested solver with the inner preconditioner in FP64.
+int nested_solve_fp64(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s);
 
 double wall_seconds();
 // Coupled Neumann faces: cpl_setup selects the faces of this solve (none:

@@ -1626,18 +1626,17 @@ static int gmres_inner_dev(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& bas
     return SVFSI_OK;
 }
 
-__global__ void k_cvt_f64_f32(size_t n, const double* __restrict__ a, float* __restrict__ b)
+// b = a converted between the outer (FP64) and inner (IT) precisions
+template <typename TI, typename TO>
+__global__ void k_cvt(size_t n, const TI* __restrict__ a, TO* __restrict__ b)
 {
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) b[q] = (float)a[q];
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) b[q] = (TO)a[q];
 }
 
-__global__ void k_cvt_f32_f64(size_t n, const float* __restrict__ a, double* __restrict__ b)
-{
-    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) b[q] = (double)a[q];
-}
-
-// x += sum_j y_j z_j with FP32 directions z_j (flexible GMRES update)
-__global__ __launch_bounds__(256) void k_fg_xupdate(size_t n, double* __restrict__ X, const float* __restrict__ Z,
+// x += sum_j y_j z_j with the inner-precision directions z_j (flexible
+// GMRES update)
+template <typename IT>
+__global__ __launch_bounds__(256) void k_fg_xupdate(size_t n, double* __restrict__ X, const IT* __restrict__ Z,
                                                     const GmState* __restrict__ st, GmArrays A)
 {
     const int i = st->iconv;
@@ -1670,8 +1669,15 @@ static double env_dbl(const char* k, double d)
 // a loose relative tolerance; the outer Krylov space keeps what restarted
 // inner cycles would lose.  Convergence is GMRESV's FP64 test
 // |W(R - K x)| < max(absTol, relTol |W R|) on the outer estimate.
-int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K32, double* R,
-                hipStream_t s)
+//
+// IT is the inner precision: float for the mixed solve, double for the same
+// nested algorithm run entirely in FP64 (the comparison that separates what
+// the algorithm gains from what FP32 storage gains).  Kin is the inner
+// matrix: the FP32 copy of the scaled matrix written by the diagonal
+// preconditioner (IT = float), or nullptr (IT = double: the scaled FP64
+// matrix itself, or a copy in w.valIn when block Jacobi rescales it).
+template <typename IT>
+static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, IT* Kin, double* R, hipStream_t s)
 {
     const size_t n = (size_t)L.nNo * dof;
     Work& w = L.ws;
@@ -1690,7 +1696,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
     SV_TRY(w.X.ensure(n * sizeof(double)));
     SV_TRY(w.tmp.ensure(n * sizeof(double)));
     SV_TRY(w.basis.ensure((size_t)(mo + 1) * n * sizeof(double)));
-    SV_TRY(w.zdir.ensure((size_t)mo * n * sizeof(float)));
+    SV_TRY(w.zdir.ensure((size_t)mo * n * sizeof(IT)));
     const size_t partN = ((size_t)(mo + 2) * nbr > gv ? (size_t)(mo + 2) * nbr : gv) + 16;
     SV_TRY(w.opartial.ensure(partN * sizeof(double)));
     SV_TRY(w.ostate.ensure(256));
@@ -1698,7 +1704,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
     double* x = w.X.as<double>();
     double* t64 = w.tmp.as<double>();
     double* Q = w.basis.as<double>();
-    float* Z = w.zdir.as<float>();
+    IT* Z = w.zdir.as<IT>();
     double* part = w.opartial.as<double>();
     double* scal = part + partN - 8;
     GmState* st = w.ostate.as<GmState>();
@@ -1713,18 +1719,34 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
     // overrides.
     const bool polyWanted = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", 1) != 0;
     const bool bj = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_BJ", dof == 4 ? 1 : 0) != 0;
-    // With block Jacobi the FP32 inner matrix is replaced by B = D^-1 K (K32
-    // is overwritten; formed once per solve from the FP64 matrix), so the
-    // inner products carry no epilogue, and each inner right-hand side is
-    // multiplied by D^-1.
-    const float* dinv = nullptr;
-    if (bj) {
-        SV_TRY(w.dinv32.ensure((size_t)L.nNo * dof * dof * sizeof(float)));
-        SV_TRY(launch_blockdiag_inv(L, dof, K, w.dinv32.as<float>(), s));
-        SV_TRY(launch_bjscale(L, dof, K, w.dinv32.as<float>(), K32, s));
-        dinv = w.dinv32.as<float>();
+    // With block Jacobi the inner matrix is replaced by B = D^-1 K (formed
+    // once per solve from the FP64 matrix into Kin, or into w.valIn for an
+    // FP64 inner operator), so the inner products carry no epilogue, and
+    // each inner right-hand side is multiplied by D^-1.
+    const IT* dinv = nullptr;
+    if (!Kin) {
+        if constexpr (std::is_same<IT, double>::value) {
+            if (bj) {
+                SV_TRY(w.valIn.ensure((size_t)L.nnz * dof * dof * sizeof(double)));
+                Kin = w.valIn.as<double>();
+            } else {
+                Kin = const_cast<double*>(K); // read only: no block Jacobi
+            }
+        } else {
+            set_error("nested solve: FP32 inner matrix missing");
+            return SVFSI_ERR_ARG;
+        }
     }
-    GmOp<float> op32{&L, dof, K32, nullptr};
+    if (bj) {
+        SV_TRY(w.dinv32.ensure((size_t)L.nNo * dof * dof * sizeof(IT)));
+        SV_TRY(launch_blockdiag_inv<IT>(L, dof, K, w.dinv32.as<IT>(), s));
+        SV_TRY(launch_bjscale<IT>(L, dof, K, w.dinv32.as<IT>(), Kin, s));
+        dinv = w.dinv32.as<IT>();
+    }
+    GmOp<IT> opIn{}; // no transpose, no coupled faces, no D^-1 epilogue
+    opIn.L = &L;
+    opIn.dof = dof;
+    opIn.K = Kin;
 
     ls.callD = wall_seconds();
     ls.suc = 0;
@@ -1748,8 +1770,8 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
     // inner-cycle buffers (allocated here so that no allocation happens
     // while a graph is being captured)
     {
-        const size_t nbi = (n + (size_t)256 * VecE<float>::E - 1) / ((size_t)256 * VecE<float>::E);
-        SV_TRY(w.basis32.ensure((size_t)(mi + 1) * n * sizeof(float)));
+        const size_t nbi = (n + (size_t)256 * VecE<IT>::E - 1) / ((size_t)256 * VecE<IT>::E);
+        SV_TRY(w.basis32.ensure((size_t)(mi + 1) * n * sizeof(IT)));
         SV_TRY(w.partial.ensure(inner_partials(L.nNo, n, mi, nbi, gv) * sizeof(double)));
         SV_TRY(w.hess.ensure(gm_arrays_doubles(mi) * sizeof(double)));
     }
@@ -1768,7 +1790,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
     std::vector<int> kinds;
     int polySpmv = 0;
     long long polySetup = 0;
-    float *P0 = nullptr, *P1 = nullptr, *Wb = nullptr;
+    IT *P0 = nullptr, *P1 = nullptr, *Wb = nullptr;
     const double* coefD = nullptr;
     bool usePoly = polyWanted;
     if (usePoly) {
@@ -1777,18 +1799,18 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
             set_error("SVFSI_POLY_DEG must be in 2..%d", kMaxSD);
             return SVFSI_ERR_ARG;
         }
-        SV_TRY(w.polyVec.ensure(3 * n * sizeof(float)));
-        P0 = w.polyVec.as<float>();
+        SV_TRY(w.polyVec.ensure(3 * n * sizeof(IT)));
+        P0 = w.polyVec.as<IT>();
         P1 = P0 + n;
         Wb = P1 + n;
-        SV_TRY(w.basis32.ensure((size_t)(std::max(mi, d) + 1) * n * sizeof(float)));
+        SV_TRY(w.basis32.ensure((size_t)(std::max(mi, d) + 1) * n * sizeof(IT)));
         {
-            const size_t nbi = (n + (size_t)256 * VecE<float>::E - 1) / ((size_t)256 * VecE<float>::E);
+            const size_t nbi = (n + (size_t)256 * VecE<IT>::E - 1) / ((size_t)256 * VecE<IT>::E);
             SV_TRY(w.partial.ensure(inner_partials(L.nNo, n, std::max(mi, d), nbi, gv) * sizeof(double)));
             SV_TRY(w.hess.ensure(gm_arrays_doubles(std::max(mi, d)) * sizeof(double)));
         }
-        SV_TRY(launch_rand_masked(n, 0x5f3759dfu, w.W.as<double>(), P0, s));
-        SV_TRY(gmres_inner_dev<float>(L, op32, dof, P0, w.basis32, d, 0.0, nullptr, s));
+        SV_TRY(launch_rand_masked<IT>(n, 0x5f3759dfu, w.W.as<double>(), P0, s));
+        SV_TRY(gmres_inner_dev<IT>(L, opIn, dof, P0, w.basis32, d, 0.0, nullptr, s));
         GmArrays Ad = gm_carve(w.hess.as<double>(), d);
         std::vector<double> Hu((size_t)(d + 1) * d);
         SV_HIP(hipMemcpyAsync(Hu.data(), Ad.Hu, Hu.size() * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1823,11 +1845,11 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
         uintptr_t tbits;
         std::memcpy(&tbits, &tinv, sizeof(tbits));
         std::vector<uintptr_t> sig = {
-            (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)K32,
+            (uintptr_t)n, (uintptr_t)dof, (uintptr_t)mo, (uintptr_t)mi, tbits, (uintptr_t)K, (uintptr_t)Kin,
             (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
             (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
             (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv,
-            (uintptr_t)cgsPasses};
+            (uintptr_t)cgsPasses, (uintptr_t)sizeof(IT)};
         cpl_sig(L, sig);
         if (sig != w.graphs.sig) {
             w.graphs.clear();
@@ -1845,47 +1867,47 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
         outerMv += 1;
         int posted = 0;
         for (int i = 1; i <= mo; ++i) {
-            // one outer step: inner FP32 cycle on z_i = M^-1 q_i, then the
+            // one outer step: inner cycle on z_i = M^-1 q_i, then the
             // FP64 product and the CGS2 Arnoldi step; the same launches
             // every cycle, so they replay as one graph per i
             auto step = [&, i](hipStream_t cs) -> int {
-                float* zi = Z + (size_t)(i - 1) * n;
+                IT* zi = Z + (size_t)(i - 1) * n;
                 if (usePoly) {
                     // z_i = p(B) D^-1 q_i, factor by factor
-                    hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, cs, n,
+                    hipLaunchKernelGGL((k_cvt<double, IT>), dim3(gv), dim3(256), 0, cs, n,
                                        (const double*)(Q + (size_t)(i - 1) * n), P0);
-                    if (dinv) SV_TRY(launch_blockdiag_apply(L, dof, dinv, P0, cs, &st->stop));
-                    hipLaunchKernelGGL(k_set_zero<float>, dim3(gv), dim3(256), 0, cs, n, zi);
-                    float* cur = P0;
-                    float* nxt = P1;
+                    if (dinv) SV_TRY(launch_blockdiag_apply<IT>(L, dof, dinv, P0, cs, &st->stop));
+                    hipLaunchKernelGGL(k_set_zero<IT>, dim3(gv), dim3(256), 0, cs, n, zi);
+                    IT* cur = P0;
+                    IT* nxt = P1;
                     const int nf = (int)kinds.size();
                     for (int f = 0; f < nf; ++f) {
                         const bool last = f == nf - 1;
                         if (kinds[f] == 1) {
                             if (last) {
-                                SV_TRY(launch_poly_last(n, cur, zi, coefD, f, cs, &st->stop));
+                                SV_TRY(launch_poly_last<IT>(n, cur, zi, coefD, f, cs, &st->stop));
                             } else {
-                                SV_TRY(launch_poly(L, dof, 0, K32, nullptr, cur, nullptr, nullptr, zi, nxt, coefD, f, cs,
+                                SV_TRY(launch_poly<IT>(L, dof, 0, (const IT*)Kin, nullptr, cur, nullptr, nullptr, zi, nxt, coefD, f, cs,
                                                     &st->stop));
                                 std::swap(cur, nxt);
                             }
                         } else {
-                            SV_TRY(launch_poly(L, dof, 1, K32, nullptr, cur, nullptr, Wb, zi, nullptr, coefD, f, cs,
+                            SV_TRY(launch_poly<IT>(L, dof, 1, (const IT*)Kin, nullptr, cur, nullptr, Wb, zi, nullptr, coefD, f, cs,
                                                 &st->stop));
                             if (!last) {
-                                SV_TRY(launch_poly(L, dof, 2, K32, nullptr, Wb, cur, nullptr, nullptr, nxt, coefD, f, cs,
+                                SV_TRY(launch_poly<IT>(L, dof, 2, (const IT*)Kin, nullptr, Wb, cur, nullptr, nullptr, nxt, coefD, f, cs,
                                                     &st->stop));
                                 std::swap(cur, nxt);
                             }
                         }
                     }
                 } else {
-                    hipLaunchKernelGGL(k_cvt_f64_f32, dim3(gv), dim3(256), 0, cs, n,
+                    hipLaunchKernelGGL((k_cvt<double, IT>), dim3(gv), dim3(256), 0, cs, n,
                                        (const double*)(Q + (size_t)(i - 1) * n), zi);
-                    if (dinv) SV_TRY(launch_blockdiag_apply(L, dof, dinv, zi, cs, &st->stop));
-                    SV_TRY(gmres_inner_dev<float>(L, op32, dof, zi, w.basis32, mi, tin, &st->stop, cs));
+                    if (dinv) SV_TRY(launch_blockdiag_apply<IT>(L, dof, dinv, zi, cs, &st->stop));
+                    SV_TRY(gmres_inner_dev<IT>(L, opIn, dof, zi, w.basis32, mi, tin, &st->stop, cs));
                 }
-                hipLaunchKernelGGL(k_cvt_f32_f64, dim3(gv), dim3(256), 0, cs, n, (const float*)zi, t64);
+                hipLaunchKernelGGL((k_cvt<IT, double>), dim3(gv), dim3(256), 0, cs, n, (const IT*)zi, t64);
                 SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, cs, &st->stop));
                 SV_TRY(cpl_apply(L, 0, dof, t64, Q + (size_t)i * n, cs, &st->stop));
                 // outer Arnoldi: classical Gram-Schmidt, two passes (CGS2) by
@@ -1923,7 +1945,7 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
             const size_t ldsB = (tri <= (size_t)kBackLds ? tri : 0) * sizeof(double);
             hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), ldsB, s, mo, st, A);
         }
-        hipLaunchKernelGGL(k_fg_xupdate, dim3(gv), dim3(256), 0, s, n, x, (const float*)Z, (const GmState*)st, A);
+        hipLaunchKernelGGL(k_fg_xupdate<IT>, dim3(gv), dim3(256), 0, s, n, x, (const IT*)Z, (const GmState*)st, A);
         SV_HIP(hipGetLastError());
         SV_HIP(hipMemcpyAsync(&hs, st, sizeof(GmState), hipMemcpyDeviceToHost, s));
         SV_HIP(hipStreamSynchronize(s));
@@ -1948,6 +1970,16 @@ int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K3
     ls.callD = wall_seconds() - ls.callD;
     ls.dB = 10.0 * std::log(ls.fNorm / ls.dB);
     return SVFSI_OK;
+}
+
+int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, float* K32, double* R, hipStream_t s)
+{
+    return nested_solve<float>(L, ls, dof, K, K32, R, s);
+}
+
+int nested_solve_fp64(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s)
+{
+    return nested_solve<double>(L, ls, dof, K, nullptr, R, s);
 }
 
 } // namespace svfsi

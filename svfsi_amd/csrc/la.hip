@@ -144,6 +144,49 @@ __global__ __launch_bounds__(256) void k_bspmv_vs(int nNo, const int* __restrict
     }
 }
 
+// dof 3 on the block-per-lane layout: L lanes per block row, each lane
+// whole 3x3 blocks (row3), sums accumulated in AT (FP32 for the FP32
+// matrix, FP64 for the FP64 one).  dinv (optional): per-node inverse
+// diagonal blocks, the output is then D^-1 (K x).
+template <typename VT, typename XT, typename AT, int L, bool NT>
+__global__ __launch_bounds__(256) void k_bspmv3(int nNo, const int* __restrict__ rowPtr,
+                                                const int* __restrict__ colPtr, const VT* __restrict__ val,
+                                                const XT* __restrict__ x, XT* __restrict__ y,
+                                                const int* __restrict__ stop, const VT* __restrict__ dinv)
+{
+    if (stop && *stop) return;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int row = t / L;
+    const int l = t - row * L;
+    const bool active = row < nNo;
+    AT a[3];
+    row3<VT, XT, AT, L, NT>(rowPtr, colPtr, val, x, row, l, active, a);
+    if (!active || l >= 3) return;
+    AT out = l == 0 ? a[0] : (l == 1 ? a[1] : a[2]);
+    if (dinv) {
+        const VT* d = dinv + (size_t)row * 9 + l * 3;
+        out = (AT)d[0] * a[0] + (AT)d[1] * a[1] + (AT)d[2] * a[2];
+    }
+    y[(size_t)row * 3 + l] = (XT)out;
+}
+
+// Lanes per block row of the block-per-lane dof-3 kernels; SVFSI_SPMV3_L =
+// 4, 8, 16 or 32 overrides.
+int spmv3_lanes(const Lhs& L)
+{
+    static const int forced = [] {
+        const char* e = getenv("SVFSI_SPMV3_L");
+        const int v = e ? atoi(e) : 0;
+        return (v == 4 || v == 8 || v == 16 || v == 32) ? v : 0;
+    }();
+    if (forced) return forced;
+    (void)L;
+    // 16 lanes per block row measured fastest in FP32 on both the tet4
+    // (15 blocks per row: 83 us against 108 at 8 and 135 at 4 lanes) and the
+    // tet10 cube matrices (28 blocks per row: 190 us against 210 and 252)
+    return 16;
+}
+
 // dof = 4 specialisation: 4 lanes per block row, lane i owns output
 // component i and streams row i of every block of the row (4 contiguous
 // values: 32 B FP64 / 16 B FP32) plus the 4-vector U(:,col) with vector
@@ -210,12 +253,12 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
 // B = D^-1 K when dinv is given (block-Jacobi), else K.  coef is a device
 // array (c1, c2 per factor), so the launches do not depend on the values.
 // ---------------------------------------------------------------------------
-template <int MODE, bool NT, int H = 4>
+template <typename IT, int MODE, bool NT, int H = 4>
 __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ rowPtr,
-                                               const int* __restrict__ colPtr, const float* __restrict__ val,
-                                               const float* __restrict__ dinv, const float* __restrict__ x,
-                                               const float* __restrict__ prodOld, float* __restrict__ w,
-                                               float* __restrict__ y, float* __restrict__ out,
+                                               const int* __restrict__ colPtr, const IT* __restrict__ val,
+                                               const IT* __restrict__ dinv, const IT* __restrict__ x,
+                                               const IT* __restrict__ prodOld, IT* __restrict__ w,
+                                               IT* __restrict__ y, IT* __restrict__ out,
                                                const double* __restrict__ coef, int step,
                                                const int* __restrict__ stop)
 {
@@ -247,7 +290,7 @@ __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ 
         for (int m = 0; m < 4; ++m) kx[m] = __shfl(acc, base + m, kWave);
         double v = 0.0;
         if (row < nNo) {
-            const float* d = dinv + (size_t)row * 16 + i * 4;
+            const IT* d = dinv + (size_t)row * 16 + i * 4;
 #pragma unroll
             for (int m = 0; m < 4; ++m) v += (double)d[m] * kx[m];
         }
@@ -257,86 +300,72 @@ __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ 
     const size_t q = (size_t)row * 4 + i;
     if constexpr (MODE == 0) {
         const double pv = (double)x[q];
-        y[q] = (float)((double)y[q] + c1 * pv);
-        out[q] = (float)(pv - c1 * acc);
+        y[q] = (IT)((double)y[q] + c1 * pv);
+        out[q] = (IT)(pv - c1 * acc);
     } else if constexpr (MODE == 1) {
         const double pv = (double)x[q];
-        w[q] = (float)acc;
-        y[q] = (float)((double)y[q] + c1 * pv - c2 * acc);
+        w[q] = (IT)acc;
+        y[q] = (IT)((double)y[q] + c1 * pv - c2 * acc);
     } else {
-        out[q] = (float)((double)prodOld[q] - c1 * (double)x[q] + c2 * acc);
+        out[q] = (IT)((double)prodOld[q] - c1 * (double)x[q] + c2 * acc);
     }
 }
 
-// dof 3: the same factor steps on the value-streaming SpMV layout (LPR lanes
-// per block row stream the row's values; D^-1 applied after the reduction).
-template <int MODE, bool NT>
+// dof 3: the same factor steps on the block-per-lane layout (row3, L lanes
+// per block row, sums in the inner precision IT; D^-1 applied after the
+// reduction).
+template <typename IT, int MODE, bool NT, int L>
 __global__ __launch_bounds__(256) void k_poly3(int nNo, const int* __restrict__ rowPtr,
-                                               const int* __restrict__ colPtr, const float* __restrict__ val,
-                                               const float* __restrict__ dinv, const float* __restrict__ x,
-                                               const float* __restrict__ prodOld, float* __restrict__ w,
-                                               float* __restrict__ y, float* __restrict__ out,
+                                               const int* __restrict__ colPtr, const IT* __restrict__ val,
+                                               const IT* __restrict__ dinv, const IT* __restrict__ x,
+                                               const IT* __restrict__ prodOld, IT* __restrict__ w,
+                                               IT* __restrict__ y, IT* __restrict__ out,
                                                const double* __restrict__ coef, int step,
                                                const int* __restrict__ stop)
 {
     if (stop && *stop) return;
-    constexpr int DOF = 3, DD = 9, LPR = 16;
     const double c1 = coef[2 * step], c2 = coef[2 * step + 1];
     const int t = blockIdx.x * 256 + threadIdx.x;
-    const int row = t / LPR;
-    const int l = t - row * LPR;
+    const int row = t / L;
+    const int l = t - row * L;
     const bool skipSpmv = (MODE == 2) && c2 == 0.0;
-    double acc[DOF] = {0.0, 0.0, 0.0};
-    if (row < nNo && !skipSpmv) {
-        const int k0 = rowPtr[row];
-        const int nv = (rowPtr[row + 1] - k0) * DD;
-        const float* vr = val + (size_t)k0 * DD;
-        const int* cr = colPtr + k0;
-        for (int v = l; v < nv; v += LPR) {
-            const double a = NT ? (double)__builtin_nontemporal_load(vr + v) : (double)vr[v];
-            const int ka = v / DD, ea = v - ka * DD;
-            const double xa = (double)x[(size_t)cr[ka] * DOF + (ea % DOF)];
-#pragma unroll
-            for (int i = 0; i < DOF; ++i) acc[i] += (ea / DOF == i) ? a * xa : 0.0;
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < DOF; ++i)
-#pragma unroll
-        for (int o = LPR >> 1; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, kWave);
-    if (row >= nNo || l >= DOF) return;
-    double a = l == 0 ? acc[0] : (l == 1 ? acc[1] : acc[2]);
+    IT acc[3];
+    row3<IT, IT, IT, L, NT>(rowPtr, colPtr, val, x, row, l, row < nNo && !skipSpmv, acc);
+    if (row >= nNo || l >= 3) return;
+    IT a = l == 0 ? acc[0] : (l == 1 ? acc[1] : acc[2]);
     if (dinv && !skipSpmv) {
-        const float* d = dinv + (size_t)row * DD + l * DOF;
-        a = (double)d[0] * acc[0] + (double)d[1] * acc[1] + (double)d[2] * acc[2];
+        const IT* d = dinv + (size_t)row * 9 + l * 3;
+        a = d[0] * acc[0] + d[1] * acc[1] + d[2] * acc[2];
     }
-    const size_t q = (size_t)row * DOF + l;
+    const size_t q = (size_t)row * 3 + l;
     if constexpr (MODE == 0) {
         const double pv = (double)x[q];
-        y[q] = (float)((double)y[q] + c1 * pv);
-        out[q] = (float)(pv - c1 * a);
+        y[q] = (IT)((double)y[q] + c1 * pv);
+        out[q] = (IT)(pv - c1 * a);
     } else if constexpr (MODE == 1) {
         const double pv = (double)x[q];
-        w[q] = (float)a;
-        y[q] = (float)((double)y[q] + c1 * pv - c2 * a);
+        w[q] = a;
+        y[q] = (IT)((double)y[q] + c1 * pv - c2 * a);
     } else {
-        out[q] = (float)((double)prodOld[q] - c1 * (double)x[q] + c2 * a);
+        out[q] = (IT)((double)prodOld[q] - c1 * (double)x[q] + c2 * a);
     }
 }
 
 // y += c1 prod (the last, linear factor: no product needed)
-__global__ __launch_bounds__(256) void k_poly_last(size_t n, const float* __restrict__ prod, float* __restrict__ y,
+template <typename IT>
+__global__ __launch_bounds__(256) void k_poly_last(size_t n, const IT* __restrict__ prod, IT* __restrict__ y,
                                                    const double* __restrict__ coef, int step,
                                                    const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const double c1 = coef[2 * step];
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
-        y[q] = (float)((double)y[q] + c1 * (double)prod[q]);
+        y[q] = (IT)((double)y[q] + c1 * (double)prod[q]);
 }
 
-int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* dinv, const float* x,
-                const float* prodOld, float* w, float* y, float* out, const double* coef, int step, hipStream_t s,
+template <typename IT>
+int launch_poly(const Lhs& L, int dof, int mode, const IT* K, const IT* dinv, const IT* x,
+                const IT* prodOld, IT* w, IT* y, IT* out, const double* coef, int step, hipStream_t s,
                 const int* stop)
 {
     if (dof != 3 && dof != 4) {
@@ -344,17 +373,26 @@ int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* di
         return SVFSI_ERR_NOTSUP;
     }
     const int dd = dof * dof;
-    const bool nt = (size_t)L.nnz * dd * sizeof(float) > ((size_t)128 << 20);
+    const bool nt = (size_t)L.nnz * dd * sizeof(IT) > ((size_t)128 << 20);
     const int* rp = L.rowPtr.as<int>();
     const int* cp = L.colPtr.as<int>();
     // algorithmic bytes: matrix, indices, x, the vectors the mode touches
-    const double base = (double)L.nnz * dd * 4.0 + 4.0 * L.nnz + 4.0 * (L.nNo + 1) + (dinv ? L.nNo * dd * 4.0 : 0.0);
-    const double vec = (double)L.nNo * dof * 4.0;
-    ProfScope ps(1, base + vec * (mode == 0 ? 4 : mode == 1 ? 4 : 3));
-    const unsigned g = (unsigned)(((size_t)L.nNo * 16 + 255) / 256); // 16 lanes per block row, both layouts
-#define SV_POLY(KERN, M, N)                                                                                     \
-    SV_LAUNCH(ps, (KERN<M, N>), dim3(g), dim3(256), 0, s, L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, \
-              step, stop)
+    const double sz = (double)sizeof(IT);
+    const double base = (double)L.nnz * dd * sz + 4.0 * L.nnz + 4.0 * (L.nNo + 1) + (dinv ? L.nNo * dd * sz : 0.0);
+    const double vec = (double)L.nNo * dof * sz;
+    ProfScope ps(std::is_same<IT, float>::value ? 1 : 0, base + vec * (mode == 0 ? 4 : mode == 1 ? 4 : 3));
+    const int l3 = spmv3_lanes(L);
+    const unsigned g3 = (unsigned)(((size_t)L.nNo * l3 + 255) / 256);
+#define SV_POLY3L(M, N, LL)                                                                                    \
+    SV_LAUNCH(ps, (k_poly3<IT, M, N, LL>), dim3(g3), dim3(256), 0, s, L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, \
+              coef, step, stop)
+#define SV_POLY3(M, N)                                                                                         \
+    do {                                                                                                       \
+        if (l3 == 4) SV_POLY3L(M, N, 4);                                                                       \
+        else if (l3 == 16) SV_POLY3L(M, N, 16);                                                                \
+        else if (l3 == 32) SV_POLY3L(M, N, 32);                                                                \
+        else SV_POLY3L(M, N, 8);                                                                               \
+    } while (0)
     // lanes per output component of the dof-4 kernel (tuning knob
     // SVFSI_POLY_H = 2, 4 or 8)
     static const int ph = [] {
@@ -363,7 +401,7 @@ int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* di
         return (v == 2 || v == 8) ? v : 4;
     }();
 #define SV_POLYH(M, N, HH)                                                                                       \
-    SV_LAUNCH(ps, (k_poly4<M, N, HH>), dim3((unsigned)(((size_t)L.nNo * 4 * HH + 255) / 256)), dim3(256), 0, s, \
+    SV_LAUNCH(ps, (k_poly4<IT, M, N, HH>), dim3((unsigned)(((size_t)L.nNo * 4 * HH + 255) / 256)), dim3(256), 0, s, \
               L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, step, stop)
 #define SV_POLY4(M, N)                                                                                           \
     do {                                                                                                         \
@@ -378,31 +416,39 @@ int launch_poly(const Lhs& L, int dof, int mode, const float* K, const float* di
 #undef SV_POLY4
 #undef SV_POLYH
     } else {
-        if (mode == 0) { if (nt) SV_POLY(k_poly3, 0, true); else SV_POLY(k_poly3, 0, false); }
-        else if (mode == 1) { if (nt) SV_POLY(k_poly3, 1, true); else SV_POLY(k_poly3, 1, false); }
-        else { if (nt) SV_POLY(k_poly3, 2, true); else SV_POLY(k_poly3, 2, false); }
+        if (mode == 0) { if (nt) SV_POLY3(0, true); else SV_POLY3(0, false); }
+        else if (mode == 1) { if (nt) SV_POLY3(1, true); else SV_POLY3(1, false); }
+        else { if (nt) SV_POLY3(2, true); else SV_POLY3(2, false); }
     }
-#undef SV_POLY
+#undef SV_POLY3
+#undef SV_POLY3L
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }
 
-int launch_poly_last(size_t n, const float* prod, float* y, const double* coef, int step, hipStream_t s,
+template <typename IT>
+int launch_poly_last(size_t n, const IT* prod, IT* y, const double* coef, int step, hipStream_t s,
                      const int* stop)
 {
     size_t g = (n + 255) / 256;
     if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(k_poly_last, dim3((unsigned)g), dim3(256), 0, s, n, prod, y, coef, step, stop);
+    hipLaunchKernelGGL(k_poly_last<IT>, dim3((unsigned)g), dim3(256), 0, s, n, prod, y, coef, step, stop);
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }
+template int launch_poly<float>(const Lhs&, int, int, const float*, const float*, const float*, const float*, float*,
+                                float*, float*, const double*, int, hipStream_t, const int*);
+template int launch_poly<double>(const Lhs&, int, int, const double*, const double*, const double*, const double*,
+                                 double*, double*, double*, const double*, int, hipStream_t, const int*);
+template int launch_poly_last<float>(size_t, const float*, float*, const double*, int, hipStream_t, const int*);
+template int launch_poly_last<double>(size_t, const double*, double*, const double*, int, hipStream_t, const int*);
 
 // Inverse of each node's diagonal block (dof 4, Gauss-Jordan with partial
 // pivoting in FP64), stored FP32 row-major.  Rows and columns zeroed by the
 // Dirichlet mask get a unit diagonal first.
-template <int DOF>
+template <int DOF, typename OT>
 __global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __restrict__ diagPtr,
-                                                       const double* __restrict__ val, float* __restrict__ dinv)
+                                                       const double* __restrict__ val, OT* __restrict__ dinv)
 {
     constexpr int DD = DOF * DOF;
     const int r = blockIdx.x * 256 + threadIdx.x;
@@ -450,20 +496,20 @@ __global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __res
                 for (int q = 0; q < 2 * DOF; ++q) a[p][q] -= f * a[c][q];
             }
     }
-    float* o = dinv + (size_t)r * DD;
+    OT* o = dinv + (size_t)r * DD;
 #pragma unroll
     for (int p = 0; p < DOF; ++p)
 #pragma unroll
-        for (int q = 0; q < DOF; ++q) o[p * DOF + q] = ok ? (float)a[p][DOF + q] : (p == q ? 1.0f : 0.0f);
+        for (int q = 0; q < DOF; ++q) o[p * DOF + q] = ok ? (OT)a[p][DOF + q] : (OT)(p == q ? 1.0 : 0.0);
 }
 
 // B = D^-1 K in FP32 (block Jacobi folded into the inner matrix): block k of
 // block row i becomes D_i^-1 K_k, formed in FP64 from the FP64 scaled matrix.
 // DD lanes per block, one output entry each.
-template <int DOF>
+template <int DOF, typename OT>
 __global__ __launch_bounds__(256) void k_bjscale(size_t nnz, const int* __restrict__ blockRow,
-                                                 const double* __restrict__ K, const float* __restrict__ dinv,
-                                                 float* __restrict__ B)
+                                                 const double* __restrict__ K, const OT* __restrict__ dinv,
+                                                 OT* __restrict__ B)
 {
     constexpr int DD = DOF * DOF;
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -471,39 +517,40 @@ __global__ __launch_bounds__(256) void k_bjscale(size_t nnz, const int* __restri
     if (k >= nnz) return;
     const int e = (int)(t - k * DD);
     const int r = e / DOF, c = e - (e / DOF) * DOF;
-    const float* d = dinv + (size_t)blockRow[k] * DD + r * DOF;
+    const OT* d = dinv + (size_t)blockRow[k] * DD + r * DOF;
     const double* kb = K + k * DD;
     double v = 0.0;
 #pragma unroll
     for (int m = 0; m < DOF; ++m) v += (double)d[m] * kb[m * DOF + c];
-    B[k * DD + e] = (float)v;
+    B[k * DD + e] = (OT)v;
 }
 
 // x = D^-1 x per node (FP32, in place).
-template <int DOF>
-__global__ __launch_bounds__(256) void k_blockdiag_apply(int nNo, const float* __restrict__ dinv,
-                                                         float* __restrict__ x, const int* __restrict__ stop)
+template <int DOF, typename IT>
+__global__ __launch_bounds__(256) void k_blockdiag_apply(int nNo, const IT* __restrict__ dinv,
+                                                         IT* __restrict__ x, const int* __restrict__ stop)
 {
     if (stop && *stop) return;
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= nNo) return;
-    float xi[DOF];
+    IT xi[DOF];
 #pragma unroll
     for (int m = 0; m < DOF; ++m) xi[m] = x[(size_t)r * DOF + m];
-    const float* d = dinv + (size_t)r * DOF * DOF;
+    const IT* d = dinv + (size_t)r * DOF * DOF;
 #pragma unroll
     for (int p = 0; p < DOF; ++p) {
         double v = 0.0;
 #pragma unroll
         for (int m = 0; m < DOF; ++m) v += (double)d[p * DOF + m] * (double)xi[m];
-        x[(size_t)r * DOF + p] = (float)v;
+        x[(size_t)r * DOF + p] = (IT)v;
     }
 }
 
 // Seeded pseudo-random start vector in (-0.5, 0.5), zero where W = 0 (the
 // Dirichlet-masked unknowns, whose rows and columns are zero).
+template <typename IT>
 __global__ __launch_bounds__(256) void k_rand_masked(size_t n, unsigned seed, const double* __restrict__ W,
-                                                     float* __restrict__ v)
+                                                     IT* __restrict__ v)
 {
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
         unsigned x = (unsigned)q * 2654435761u ^ seed;
@@ -513,26 +560,28 @@ __global__ __launch_bounds__(256) void k_rand_masked(size_t n, unsigned seed, co
         x *= 0x846ca68bu;
         x ^= x >> 16;
         const float r = (float)(x >> 8) * (1.0f / 16777216.0f) - 0.5f;
-        v[q] = (W[q] != 0.0) ? r : 0.0f;
+        v[q] = (W[q] != 0.0) ? (IT)r : (IT)0;
     }
 }
 
-int launch_rand_masked(size_t n, unsigned seed, const double* W, float* v, hipStream_t s)
+template <typename IT>
+int launch_rand_masked(size_t n, unsigned seed, const double* W, IT* v, hipStream_t s)
 {
     size_t g = (n + 255) / 256;
     if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(k_rand_masked, dim3((unsigned)g), dim3(256), 0, s, n, seed, W, v);
+    hipLaunchKernelGGL(k_rand_masked<IT>, dim3((unsigned)g), dim3(256), 0, s, n, seed, W, v);
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }
 
-int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, hipStream_t s)
+template <typename OT>
+int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, OT* dinv, hipStream_t s)
 {
     const dim3 g((L.nNo + 255) / 256);
     if (dof == 4)
-        hipLaunchKernelGGL(k_blockdiag_inv<4>, g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
+        hipLaunchKernelGGL((k_blockdiag_inv<4, OT>), g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
     else if (dof == 3)
-        hipLaunchKernelGGL(k_blockdiag_inv<3>, g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
+        hipLaunchKernelGGL((k_blockdiag_inv<3, OT>), g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
     else {
         set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
@@ -541,14 +590,15 @@ int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, float* dinv, 
     return SVFSI_OK;
 }
 
-int launch_bjscale(const Lhs& L, int dof, const double* K, const float* dinv, float* B, hipStream_t s)
+template <typename OT>
+int launch_bjscale(const Lhs& L, int dof, const double* K, const OT* dinv, OT* B, hipStream_t s)
 {
     const size_t nnz = (size_t)L.nnz;
     const dim3 g((unsigned)((nnz * dof * dof + 255) / 256));
     if (dof == 4)
-        hipLaunchKernelGGL(k_bjscale<4>, g, dim3(256), 0, s, nnz, L.blockRow.as<int>(), K, dinv, B);
+        hipLaunchKernelGGL((k_bjscale<4, OT>), g, dim3(256), 0, s, nnz, L.blockRow.as<int>(), K, dinv, B);
     else if (dof == 3)
-        hipLaunchKernelGGL(k_bjscale<3>, g, dim3(256), 0, s, nnz, L.blockRow.as<int>(), K, dinv, B);
+        hipLaunchKernelGGL((k_bjscale<3, OT>), g, dim3(256), 0, s, nnz, L.blockRow.as<int>(), K, dinv, B);
     else {
         set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
@@ -557,13 +607,14 @@ int launch_bjscale(const Lhs& L, int dof, const double* K, const float* dinv, fl
     return SVFSI_OK;
 }
 
-int launch_blockdiag_apply(const Lhs& L, int dof, const float* dinv, float* x, hipStream_t s, const int* stop)
+template <typename IT>
+int launch_blockdiag_apply(const Lhs& L, int dof, const IT* dinv, IT* x, hipStream_t s, const int* stop)
 {
     const dim3 g((L.nNo + 255) / 256);
     if (dof == 4)
-        hipLaunchKernelGGL(k_blockdiag_apply<4>, g, dim3(256), 0, s, L.nNo, dinv, x, stop);
+        hipLaunchKernelGGL((k_blockdiag_apply<4, IT>), g, dim3(256), 0, s, L.nNo, dinv, x, stop);
     else if (dof == 3)
-        hipLaunchKernelGGL(k_blockdiag_apply<3>, g, dim3(256), 0, s, L.nNo, dinv, x, stop);
+        hipLaunchKernelGGL((k_blockdiag_apply<3, IT>), g, dim3(256), 0, s, L.nNo, dinv, x, stop);
     else {
         set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
@@ -571,6 +622,14 @@ int launch_blockdiag_apply(const Lhs& L, int dof, const float* dinv, float* x, h
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }
+template int launch_rand_masked<float>(size_t, unsigned, const double*, float*, hipStream_t);
+template int launch_rand_masked<double>(size_t, unsigned, const double*, double*, hipStream_t);
+template int launch_blockdiag_inv<float>(const Lhs&, int, const double*, float*, hipStream_t);
+template int launch_blockdiag_inv<double>(const Lhs&, int, const double*, double*, hipStream_t);
+template int launch_bjscale<float>(const Lhs&, int, const double*, const float*, float*, hipStream_t);
+template int launch_bjscale<double>(const Lhs&, int, const double*, const double*, double*, hipStream_t);
+template int launch_blockdiag_apply<float>(const Lhs&, int, const float*, float*, hipStream_t, const int*);
+template int launch_blockdiag_apply<double>(const Lhs&, int, const double*, double*, hipStream_t, const int*);
 
 // Transpose product: row i of K^T gathers block (col,i) through the
 // transpose-position map trans[k] (pattern is structurally symmetric, as the
@@ -666,6 +725,29 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
             SV_HIP(hipGetLastError());
             return SVFSI_OK;
         }
+    }
+    // SVFSI_SPMV3_ROW = 1 puts FP64 on the block-per-lane layout as well
+    static const bool row64 = [] {
+        const char* e = getenv("SVFSI_SPMV3_ROW");
+        return e && e[0] == '1';
+    }();
+    if (dof == 3 && (std::is_same<VT, float>::value || row64)) {
+        // FP32: whole blocks per lane (fewer, wider loads; FP32 sums).  FP64
+        // keeps the value-streaming layout below: its 72-byte blocks per lane
+        // spread each load instruction over 4.6 KB and measured slower
+        // (548 vs 373 us on the 40^3 tet10 cube's matrix).
+        const bool nt = (size_t)L.nnz * 9 * sizeof(VT) > ((size_t)128 << 20);
+        const int l3 = spmv3_lanes(L);
+        const unsigned g3 = (unsigned)(((size_t)L.nNo * l3 + 255) / 256);
+#define SV_B3(LL, N)                                                                                           \
+    SV_LAUNCH(ps, (k_bspmv3<VT, XT, VT, LL, N>), dim3(g3), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv)
+        if (l3 == 4) { if (nt) SV_B3(4, true); else SV_B3(4, false); }
+        else if (l3 == 16) { if (nt) SV_B3(16, true); else SV_B3(16, false); }
+        else if (l3 == 32) { if (nt) SV_B3(32, true); else SV_B3(32, false); }
+        else { if (nt) SV_B3(8, true); else SV_B3(8, false); }
+#undef SV_B3
+        SV_HIP(hipGetLastError());
+        return SVFSI_OK;
     }
     if (dof == 2 || dof == 3) {
         // value-streaming layout (tuning knob SVFSI_SPMV_LPR = 8 / 16 / 32;
@@ -1137,6 +1219,76 @@ int precond_rcs(Lhs& L, int dof, double* Val, double* R, double* W1, double* W2,
     case 4: return precond_rcs_t<4>(L, Val, R, W1, W2, Val32, s);
     default: set_error("PRECOND_RCS: dof %d not supported (2..4)", dof); return SVFSI_ERR_NOTSUP;
     }
+}
+
+// r = b - K x (TRANS: b - K^T x) with each row's sum carried in twice the
+// working precision and rounded once: Ogita, Rump and Oishi's Dot2 ("Accurate
+// sum and dot product", SIAM J. Sci. Comput. 26, 2005) -- every product
+// split exactly by an FMA (a x = p + e), the running sum by Knuth's TwoSum,
+// the error terms summed apart.  The computed residual is then as accurate
+// as one rounded from a computation in double-double, which is what
+// iterative refinement needs to reach a solution accurate to the working
+// precision on an ill-conditioned system.  One thread per row component.
+template <int DOF, bool TRANS>
+__global__ __launch_bounds__(256) void k_resid_dd(int nNo, const int* __restrict__ rowPtr,
+                                                  const int* __restrict__ colPtr, const int* __restrict__ trans,
+                                                  const double* __restrict__ V, const double* __restrict__ x,
+                                                  const double* __restrict__ b, double* __restrict__ r)
+{
+#pragma clang fp contract(off)
+    constexpr int DD = DOF * DOF;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int row = t / DOF;
+    if (row >= nNo) return;
+    const int p = t - row * DOF;
+    double hi = b[(size_t)row * DOF + p], lo = 0.0;
+    for (int k = rowPtr[row]; k < rowPtr[row + 1]; ++k) {
+        const int c = colPtr[k];
+        const double* blk = V + (size_t)(TRANS ? trans[k] : k) * DD;
+#pragma unroll
+        for (int q = 0; q < DOF; ++q) {
+            const double a = TRANS ? blk[q * DOF + p] : blk[p * DOF + q];
+            const double xv = x[(size_t)c * DOF + q];
+            const double pr = -(a * xv);
+            const double pe = __builtin_fma(-a, xv, -pr); // -a xv = pr + pe exactly
+            const double sum = hi + pr;                    // TwoSum(hi, pr)
+            const double bb = sum - hi;
+            const double se = (hi - (sum - bb)) + (pr - bb);
+            hi = sum;
+            lo += se + pe;
+        }
+    }
+    r[(size_t)row * DOF + p] = hi + lo;
+}
+
+int launch_resid_dd(const Lhs& L, int dof, const double* K, const double* x, const double* b, double* r,
+                    bool transpose, hipStream_t s)
+{
+    if (L.nNo == 0) return SVFSI_OK;
+    if (transpose && !L.symmetric) {
+        set_error("transpose residual needs a structurally symmetric pattern");
+        return SVFSI_ERR_ARG;
+    }
+    const dim3 g((unsigned)(((size_t)L.nNo * dof + 255) / 256));
+    const int* rp = L.rowPtr.as<int>();
+    const int* cp = L.colPtr.as<int>();
+    const int* tr = transpose ? L.trans.as<int>() : nullptr;
+#define SV_RDD(D)                                                                                          \
+    do {                                                                                                   \
+        if (transpose)                                                                                     \
+            hipLaunchKernelGGL((k_resid_dd<D, true>), g, dim3(256), 0, s, L.nNo, rp, cp, tr, K, x, b, r);  \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_resid_dd<D, false>), g, dim3(256), 0, s, L.nNo, rp, cp, tr, K, x, b, r); \
+    } while (0)
+    switch (dof) {
+    case 2: SV_RDD(2); break;
+    case 3: SV_RDD(3); break;
+    case 4: SV_RDD(4); break;
+    default: set_error("resid_dd: dof %d not supported (2..4)", dof); return SVFSI_ERR_NOTSUP;
+    }
+#undef SV_RDD
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
 }
 
 // Values of K^T on the (structurally symmetric) pattern of K:

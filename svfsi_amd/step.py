@@ -11,7 +11,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import fsils
-from ._lib import BC_TYPE_Dir, BC_TYPE_Neu, FSILS_lsType
+from ._lib import BC_TYPE_Dir, BC_TYPE_Neu, SVFSI_FLUID_P2P1, FSILS_lsType
 from .assembly import DeviceMesh
 from .problems import Problem
 
@@ -55,8 +55,9 @@ class NewtonStep:
     def assemble(self) -> None:
         p = self.prob
         self.dmesh.zero(p.dof, self.Val, self.R)
+        quad = SVFSI_FLUID_P2P1 if p.nFs == 2 else p.quad
         self.dmesh.assemble(p.phys, p.eq, self.Ag, self.Yg, self.Dg, self.Val, self.R,
-                            prop_el=self.prop, quad=p.quad)
+                            prop_el=self.prop, quad=quad)
         for fid, h in self.trac_ids:
             self.dmesh.traction(fid, p.dof, h, self.R)
         for fid in self.bflow_ids:

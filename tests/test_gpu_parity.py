@@ -443,6 +443,34 @@ def test_polynomial_inner_preconditioner(gpu, orc, monkeypatch, bj, which):
     assert np.array_equal(st.R.cpu().numpy(), xg)
 
 
+@pytest.mark.parametrize("poly", ["0", "1"])
+@pytest.mark.parametrize("which", ["fluid", "lelas"])
+def test_nested_solver_with_fp64_inner(gpu, orc, monkeypatch, poly, which):
+    """SVFSI_INNER_FP64=1: the mixed entry point runs the same nested solver
+    (FGMRES outer, polynomial or inner-GMRES preconditioner, block Jacobi as
+    for FP32) with the inner matrix, vectors and products in FP64.  It
+    reaches the FP64 tolerance on the oracle's operator, and takes no more
+    outer steps than the FP32-inner solve (rounding of a FP32 inner
+    operator can only weaken the preconditioner)."""
+    monkeypatch.setenv("SVFSI_INNER_POLY", poly)
+    prob = {"fluid": lambda: problems.fluid_cylinder(h=0.4), "lelas": lambda: problems.lelas_cube(6)}[which]()
+    out = {}
+    for inner64 in ("0", "1"):
+        monkeypatch.setenv("SVFSI_INNER_FP64", inner64)
+        st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc, mixed=True)
+        mask = prob.dir_mask()
+        _, Rs, W = orc.precond_diag(st.rowPtr - 1, st.colPtr - 1, prob.dof, V0, R0, mask)
+        y = np.where(W != 0, xg / np.where(W != 0, W, 1), 0.0)
+        r = Rs - orc.spmv(st.rowPtr - 1, st.colPtr - 1, prob.dof, Vs, y)
+        say("inner fp64", inner64, which, "poly", poly, "outer", ls.RI.itr, "SpMVs", ls.RI.cM,
+            "x relerr", relerr(xg, xo))
+        assert ls.RI.suc == 1
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(Rs) * 1.0001
+        assert relerr(xg, xo) < 1e-4
+        out[inner64] = ls.RI.itr
+    assert out["1"] <= out["0"] + 2, out
+
+
 def test_inner_block_jacobi_reduces_inner_iterations(gpu, orc, monkeypatch):
     """The FP32 inner cycle of the mixed solve is left-preconditioned by the
     inverse 4x4 node blocks.  Both variants must reach the FP64 tolerance;
