@@ -8,9 +8,13 @@ the dense adjoint built on it against central differences of the dense
 solve.
 GPU: the dual-number kernel (svfsi_fluid_dmu_dot_dev) against the oracle;
 the autograd wrapper over the C ABI against the oracle's FP64 CPU adjoint
-and central differences (relative step 1e-4 in mu) on 20 random elements,
-to 1e-5 relative plus an absolute floor of 5e-6 of the largest gradient
-entry (the resolution of solves of this system, condition ~4e8)."""
+and fourth-order central differences (step 4e-2 in z = log(mu / mu0)) on
+20 elements drawn at random among those with |dL/dz_e| >= 1e-3 max|dL/dz|
+(the rest change the outlet-flux loss by less than the rounding of L), each
+to 1e-5 relative with no absolute floor.  Every solve (forward, transposed,
+FD differences) is refined by three residual-correction passes with
+double-double residuals: K's condition number (~4e8) leaves one solve to
+1e-12 only ~4e-4 accurate."""
 import numpy as np
 import pytest
 
@@ -130,29 +134,50 @@ def test_fluid_dmu_kernel_parity(gpu, orc):
     assert np.abs(gg - go).max() <= 1e-12 * np.abs(go).max()
 
 
-def _fd_check(adj, z, du, c, g, elems, h=1e-4, rtol=1e-5, atol_frac=5e-6):
-    """Central differences of L = c . du(z +- h e_k), du(z +- h e_k) from
-    correction-form solves: L+ - L- = c . (d+ - d-).  Returns the largest
-    |fd - g_e| / (rtol |fd| + atol), atol = atol_frac max|g|: with K's
-    condition number ~4e8 a correction solved to 1e-12 carries a relative
-    error up to ~4e-4, which bounds the agreement on the smaller entries."""
+def _fd_check(adj, z, du, c, g, elems, h=4e-2, rtol=1e-5):
+    """Fourth-order central differences of L = c . du in z_e,
+    fd = [8 (D(h) - D(-h)) - (D(2h) - D(-2h))] / (12 h), with
+    D(t) = L(z + t e_k) - L(z) = c . d(t) and d(t) = K(t)^-1 [(R(t) - R0) -
+    (K(t) - K0) du] (FluidViscosityAdjoint.difference: no cancellation in
+    forming D).  Returns the largest |fd - g_e| / (rtol |fd|).
+
+    The step: the computed L(z) carries the rounding of the FP64 assembly
+    (the pressure increments reach 3e8, so eps |K_e| |du_e| is not small
+    against h dK_e/dz du_e), and a difference quotient divides it by h.
+    Second-order differences at h = 1e-4 miss g_e by up to 6e-4 on elements
+    with |g_e| ~ 1e-3 max|g|, and the miss grows as 1/h (8e-4 at 5e-5, 7e-5
+    at 4e-4) -- the same digits with dense LU solves on the CPU, so it is
+    the function's own rounding, not the solves'.  The fourth-order stencil
+    at h = 4e-2 has O(h^4) truncation and divides the rounding by a step
+    400 times larger: 1.6e-6 worst on the CPU over 15 such elements."""
     import torch
 
-    adj.prob.ls = dict(adj.prob.ls, relTol=1e-12, mItr=5000)
-    atol = atol_frac * g.abs().max().item()
+    adj.prob.ls = dict(adj.prob.ls, relTol=1e-12, mItr=max(adj.prob.ls.get("mItr", 0), 200))
     worst = 0.0
     for e in elems:
-        d = []
-        for s in (1.0, -1.0):
+        D = {}
+        for t in (h, -h, 2 * h, -2 * h):
             zz = z.clone()
-            zz[e] += s * h
-            d.append(adj.solve_correction(MU0 * torch.exp(zz), du) - du)
-        fd = torch.sum(c * (d[0] - d[1])).item() / (2 * h)
+            zz[e] += t
+            D[t] = torch.sum(c * adj.difference(MU0 * torch.exp(zz), du)).item()
+        fd = (8.0 * (D[h] - D[-h]) - (D[2 * h] - D[-2 * h])) / (12.0 * h)
         ge = g[e].item()
-        ratio = abs(fd - ge) / (rtol * abs(fd) + atol)
+        ratio = abs(fd - ge) / (rtol * abs(fd))
         worst = max(worst, ratio)
-        print("fd", int(e), fd, ge, abs(fd - ge) / max(abs(fd), 1e-300), ratio, flush=True)
+        print("fd", int(e), fd, ge, "rel", abs(fd - ge) / max(abs(fd), 1e-300), "|g_e|/max",
+              abs(ge) / g.abs().max().item(), flush=True)
     return worst
+
+
+def resolvable_sample(g, n, seed, floor=1e-3):
+    """n elements drawn at random (seeded) among those whose |dL/dz_e| is at
+    least floor * max|dL/dz|.  The loss is a weighted outlet flux: most
+    elements of the long cylinder change it by less than the rounding of L
+    itself (|g_e| down to 1e-10 of the largest), so a uniform draw picks
+    elements whose difference quotient is rounding."""
+    gg = g.abs().cpu().numpy()
+    cand = np.nonzero(gg >= floor * gg.max())[0]
+    return np.random.default_rng(seed).choice(cand, min(n, len(cand)), replace=False)
 
 
 @pytest.mark.gpu
@@ -166,7 +191,7 @@ def test_viscosity_adjoint_vs_cpu_adjoint_and_fd(gpu, orc, mixed):
     prob = fluid_problem(0.5)
     prob.ls = dict(prob.ls, relTol=1e-12, mItr=5000)
     m = prob.mesh
-    adj = FluidViscosityAdjoint(prob, mixed=mixed)
+    adj = FluidViscosityAdjoint(prob, mixed=mixed, passes=3)
     z = torch.tensor(np.log(prob.prop_el / MU0), device=gpu)
     c = torch.tensor(outlet_flux_weights(prob), device=gpu)
     zr = z.clone().requires_grad_(True)
@@ -188,7 +213,7 @@ def test_viscosity_adjoint_vs_cpu_adjoint_and_fd(gpu, orc, mixed):
     err = np.abs(gn - g_o).max() / np.abs(g_o).max()
     print("cpu adjoint relerr", err, flush=True)
     assert err <= 1e-7
-    elems = np.random.default_rng(13).choice(m.nEl, 20, replace=False)
+    elems = resolvable_sample(g, 20, 13)
     assert _fd_check(adj, z, du.detach(), c, g, elems) <= 1.0
 
 
@@ -198,8 +223,8 @@ def test_viscosity_adjoint_large_cylinder(gpu):
     mu_e = 0.04 exp(z_e) with z ~ N(0, 0.1^2) i.i.d., loss = the weighted
     outlet flux of the linearised step, mixed-precision forward and transposed
     solves.  Checked by central differences (relative step 1e-4 in mu) on the
-    five elements of largest |dL/dz| and five random ones, to 1e-5 relative
-    plus the absolute floor of _fd_check."""
+    five elements of largest |dL/dz| and five drawn at random among those
+    with |dL/dz_e| >= 1e-3 max|dL/dz|, each to 1e-5 relative."""
     import time
 
     import torch
@@ -211,7 +236,7 @@ def test_viscosity_adjoint_large_cylinder(gpu):
     prob.ls = dict(prob.ls, relTol=1e-12, mItr=200, sD=100)
     m = prob.mesh
     print("nEl", m.nEl, "nNo", m.nNo, flush=True)
-    adj = FluidViscosityAdjoint(prob, mixed=True)
+    adj = FluidViscosityAdjoint(prob, mixed=True, passes=3)
     z = torch.tensor(np.log(prob.prop_el / MU0), device=gpu)
     c = torch.tensor(outlet_flux_weights(prob), device=gpu)
     zr = z.clone().requires_grad_(True)
@@ -225,5 +250,5 @@ def test_viscosity_adjoint_large_cylinder(gpu):
     g = zr.grad.detach()
     assert torch.isfinite(g).all()
     top = torch.argsort(g.abs(), descending=True)[:5].cpu().numpy()
-    rnd = np.random.default_rng(17).choice(m.nEl, 5, replace=False)
+    rnd = resolvable_sample(g, 5, 17)
     assert _fd_check(adj, z, du.detach(), c, g, np.concatenate([top, rnd])) <= 1.0
