@@ -345,7 +345,7 @@ int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, h
     }
     Work& w = L.ws;
     const size_t need = 19 * nnz + n * (3 + 1 + 3 + 1) + n * (3 + 1) * (size_t)iBmax + n * (3 + 1) * (size_t)nB +
-                        n * (1 + 1 + 1 + 1 + 3) + 520 + (size_t)2 * (2 * nB + 1) * (kGramBlocks + 1);
+                        n * (1 + 1 + 1 + 1 + 1 + 3) + 520 + (size_t)2 * (2 * nB + 1) * (kGramBlocks + 1);
     SV_TRY(w.ns.ensure(need * sizeof(double)));
     double* p = w.ns.as<double>();
     auto take = [&](size_t k) { double* q = p; p += k; return q; };
@@ -362,6 +362,9 @@ int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, h
     double* P = take(n * iBmax);
     double* MU = take(3 * n * nB);
     double* MP = take(n * nB);
+    // the Schur CG's right-hand side / solution at one fixed address, so its
+    // captured iteration graphs are reused across outer iterations
+    double* cR = take(n);
     double* cX = take(n);
     double* cP = take(n);
     double* cP2 = take(n);
@@ -407,7 +410,9 @@ int ns_solve(Lhs& L, FSILS_lsType& ls, int dof, const double* Val, double* Ri, h
         SV_TRY(ns_gmres(c, mK, Rm, VU(i), ls.GM));
         c.vs(mD, VU(i), VP(i));
         hipLaunchKernelGGL(k_ns_sub, grid_of(n), dim3(256), 0, s, n, (const double*)Rc, (const double*)VP(i), VP(i));
-        SV_TRY(cg_schur_solve(L, ls.CG, Gt, mG, mL, VP(i), cX, cP, cP2, cSP, cGP, s));
+        SV_HIP(hipMemcpyAsync(cR, VP(i), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        SV_TRY(cg_schur_solve(L, ls.CG, Gt, mG, mL, cR, cX, cP, cP2, cSP, cGP, s));
+        SV_HIP(hipMemcpyAsync(VP(i), cR, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         c.sv(mG, VP(i), VMU(iB));
         hipLaunchKernelGGL(k_ns_sub, grid_of(3 * n), dim3(256), 0, s, 3 * n, (const double*)Rm,
                            (const double*)VMU(iB), VMU(iBB));
