@@ -49,7 +49,8 @@ def _run(mode: str, payload: bytes) -> bytes:
         with open(fin, "wb") as f:
             f.write(payload)
         env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
-        r = subprocess.run([str(EXE), mode, fin, fout], capture_output=True, text=True, env=env, cwd=td)
+        r = subprocess.run([str(EXE), mode, fin, fout], capture_output=True, text=True, env=env, cwd=td,
+                           timeout=float(os.environ.get("SVFSI_REF_TIMEOUT", "600")))
         if r.returncode != 0 or not os.path.exists(fout):
             raise RuntimeError(f"reference harness failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
         with open(fout, "rb") as f:
