@@ -276,6 +276,18 @@ int svfsi_spmv_f32_dev(void* lhs, int dof, const float* dK, const float* dU,
 /* Transpose product KU = K^T U (adjoint solve; no reference counterpart). */
 int svfsi_spmv_t_dev(void* lhs, int dof, const double* dK, const double* dU,
                      double* dKU, void* stream);
+/* Aggregation multigrid, the inner preconditioner the mixed entry points use
+ * with SVFSI_INNER_AMG=1 (no reference counterpart: svFSILS has diagonal
+ * and row/column scaling only).  svfsi_amg_aggregate: the greedy aggregation
+ * of a 0-based CSR graph (rowPtr nNo+1, colPtr) into aggregates of at most
+ * cap nodes (a node left alone joins the smallest neighbouring one); agg
+ * receives each node's aggregate; returns the number of aggregates (< 0 on
+ * error).  svfsi_amg_levels: the node counts of the hierarchy the last mixed
+ * solve on lhs built (finest first) into nodes[0..maxLevels); returns the
+ * number of levels (0: none). */
+int svfsi_amg_aggregate(int nNo, const int* rowPtr, const int* colPtr, int cap, int* agg);
+int svfsi_amg_levels(void* lhs, int* nodes, int maxLevels);
+
 /* Residual dR = dB - K dU (transpose != 0: dB - K^T dU) with every row's sum
  * carried in double-double (FMA-split products, TwoSum accumulation) and
  * rounded once: the residual of iterative refinement on ill-conditioned

@@ -110,6 +110,8 @@ SIGNATURES = {
     "svfsi_spmv_f32_dev": [_P, _I, _P, _P, _P, _P],
     "svfsi_spmv_t_dev": [_P, _I, _P, _P, _P, _P],
     "svfsi_resid_dd_dev": [_P, _I, _P, _P, _P, _P, _I, _P],
+    "svfsi_amg_aggregate": [_I, _P, _P, _I, _P],
+    "svfsi_amg_levels": [_P, _P, _I],
     "svfsi_dot_dev": [_I, _P, _P, _DP, _P],
     "svfsi_precond_diag_dev": [_P, _I, _P, _P, _P, _IP, _P],
     "svfsi_precond_rcs_dev": [_P, _I, _P, _P, _P, _P, _IP, _P],

@@ -609,6 +609,36 @@ int svfsi_spmv_t_dev(void* lhs, int dof, const double* dK, const double* dU, dou
     return launch_spmv_t(*static_cast<Lhs*>(lhs), dof, dK, dU, dKU, static_cast<hipStream_t>(stream));
 }
 
+int svfsi_amg_aggregate(int nNo, const int* rowPtr, const int* colPtr, int cap, int* agg)
+{
+    SV_ARG(nNo >= 0 && rowPtr && colPtr && agg && cap >= 2, "bad argument");
+    return guarded([&]() -> int {
+        for (int i = 0; i < nNo; ++i)
+            if (rowPtr[i + 1] < rowPtr[i]) {
+                set_error("rowPtr is not monotone at row %d", i);
+                return SVFSI_ERR_ARG;
+            }
+        for (int k = 0; k < rowPtr[nNo]; ++k)
+            if (colPtr[k] < 0 || colPtr[k] >= nNo) {
+                set_error("colPtr[%d] = %d outside 0..%d", k, colPtr[k], nNo - 1);
+                return SVFSI_ERR_ARG;
+            }
+        std::vector<int> a;
+        const int na = amg_aggregate(nNo, rowPtr, colPtr, cap, a);
+        std::copy(a.begin(), a.end(), agg);
+        return na;
+    });
+}
+
+int svfsi_amg_levels(void* lhs, int* nodes, int maxLevels)
+{
+    SV_ARG(lhs && nodes && maxLevels > 0, "bad argument");
+    std::vector<int> lv;
+    const int n = amg_levels(*static_cast<Lhs*>(lhs), lv);
+    for (int i = 0; i < n && i < maxLevels; ++i) nodes[i] = lv[i];
+    return n;
+}
+
 int svfsi_resid_dd_dev(void* lhs, int dof, const double* dK, const double* dU, const double* dB, double* dR,
                        int transpose, void* stream)
 {

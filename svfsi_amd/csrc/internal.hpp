@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <exception>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -123,6 +124,8 @@ struct GraphCache {
     void clear();
 };
 
+struct AmgHier; // aggregation multigrid hierarchy (amg.hip)
+
 // Krylov workspace, grown on demand and reused across solves.
 struct Work {
     GraphCache graphs;   // mixed solve: one graph per outer step
@@ -140,6 +143,7 @@ struct Work {
     DBuf valT; // K^T values for transposed solves
     DBuf dinv32; // inverse diagonal blocks (inner precision) of the inner block-Jacobi
     DBuf valIn;  // FP64 nested solve: the block-Jacobi scaled inner matrix
+    std::shared_ptr<AmgHier> amg; // inner multigrid of the nested solve (pattern part kept)
     DBuf polyCoef, polyVec; // polynomial preconditioner: factors, work vectors
     int maskDof = -1;
     std::vector<int> maskInc;
@@ -266,6 +270,8 @@ int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
 // IT = float (the mixed solve) and IT = double (the same algorithm in FP64).
 template <typename OT>
 int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, OT* dinv, hipStream_t s);
+// the same from FP32 values (the coarse levels of the aggregation hierarchy)
+int launch_blockdiag_inv_f32(const Lhs& L, int dof, const float* val, float* dinv, hipStream_t s);
 template <typename OT>
 int launch_bjscale(const Lhs& L, int dof, const double* K, const OT* dinv, OT* B, hipStream_t s);
 template <typename IT>
@@ -335,6 +341,13 @@ int gmres_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
                 double* R, hipStream_t s, bool transpose, int cpl = 0);
 int mixed_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,
                 float* K32, double* R, hipStream_t s);
+// Aggregation multigrid (amg.hip): greedy aggregation of a CSR graph
+// (0-based) into aggregates of at most cap nodes; set-up of the hierarchy
+// and its values for one solve; one V-cycle z = M^-1 q (FP32).
+int amg_aggregate(int n, const int* rp, const int* cp, int cap, std::vector<int>& agg);
+int amg_setup(Lhs& L, int dof, const double* K64, const float* K32, hipStream_t s);
+int amg_apply(Lhs& L, int dof, const float* q, float* z, hipStream_t s);
+int amg_levels(const Lhs& L, std::vector<int>& nodes);
 // The same nested solver with the inner preconditioner in FP64.
 int nested_solve_fp64(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hipStream_t s);
 

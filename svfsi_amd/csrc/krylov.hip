@@ -1717,8 +1717,16 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     // 3.2M-tet cylinder (72 -> 54).  Off for dof 3, where it added outer
     // steps on the elasticity cube (16 -> 22).  SVFSI_INNER_BJ = 0 / 1
     // overrides.
-    const bool polyWanted = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", 1) != 0;
-    const bool bj = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_BJ", dof == 4 ? 1 : 0) != 0;
+    // SVFSI_INNER_AMG=1: one V-cycle of the aggregation multigrid (amg.hip)
+    // on the FP32 scaled matrix instead (no block-Jacobi rescaling: the
+    // smoother uses the node blocks itself).
+    const bool useAmg = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_AMG", 0) != 0;
+    if (useAmg && !std::is_same<IT, float>::value) {
+        set_error("the multigrid inner preconditioner is FP32 only");
+        return SVFSI_ERR_NOTSUP;
+    }
+    const bool polyWanted = !useAmg && (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", 1) != 0;
+    const bool bj = !useAmg && (dof == 3 || dof == 4) && env_int("SVFSI_INNER_BJ", dof == 4 ? 1 : 0) != 0;
     // With block Jacobi the inner matrix is replaced by B = D^-1 K (formed
     // once per solve from the FP64 matrix into Kin, or into w.valIn for an
     // FP64 inner operator), so the inner products carry no epilogue, and
@@ -1835,11 +1843,22 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
         }
         hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(64), 0, s, w.state.as<GmState>(), 0.0);
     }
+    int amgSpmv = 0;
+    if (useAmg) {
+        if constexpr (std::is_same<IT, float>::value) {
+            SV_TRY(w.polyVec.ensure(3 * n * sizeof(IT)));
+            P0 = w.polyVec.as<IT>();
+            SV_TRY(amg_setup(L, dof, K, Kin, s));
+            std::vector<int> lv;
+            amg_levels(L, lv);
+            amgSpmv = lv.size() > 1 ? 4 : 0; // finest-level products per V-cycle
+        }
+    }
     // graph signature: every pointer and size baked into the step launches.
     // The polynomial steps are launched directly: their ~30 SpMV kernels of
     // ~9 us keep the queue full, and replaying them as graphs measured
     // 0.25 ms slower per Newton step on the 200k-tet cylinder.
-    const bool useGraphs = graphs_enabled() && !usePoly;
+    const bool useGraphs = graphs_enabled() && !usePoly && !useAmg;
     if (useGraphs) {
         double tinv = tin;
         uintptr_t tbits;
@@ -1872,7 +1891,13 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
             // every cycle, so they replay as one graph per i
             auto step = [&, i](hipStream_t cs) -> int {
                 IT* zi = Z + (size_t)(i - 1) * n;
-                if (usePoly) {
+                if (useAmg) {
+                    if constexpr (std::is_same<IT, float>::value) {
+                        hipLaunchKernelGGL((k_cvt<double, IT>), dim3(gv), dim3(256), 0, cs, n,
+                                           (const double*)(Q + (size_t)(i - 1) * n), P0);
+                        SV_TRY(amg_apply(L, dof, P0, zi, cs));
+                    }
+                } else if (usePoly) {
                     // z_i = p(B) D^-1 q_i, factor by factor
                     hipLaunchKernelGGL((k_cvt<double, IT>), dim3(gv), dim3(256), 0, cs, n,
                                        (const double*)(Q + (size_t)(i - 1) * n), P0);
@@ -1929,7 +1954,7 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
             else
                 SV_TRY(step(s));
             outerMv += 1;
-            if (usePoly) ++polySteps;
+            if (usePoly || useAmg) ++polySteps;
             // lagged poll: the state of step i-1 is read while step i runs
             const int q = posted & 1;
             SV_TRY(poll.post(q, st, sizeof(GmState), s));
@@ -1963,7 +1988,9 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
         // SpMVs: inner GMRES steps (or the polynomial's setup Arnoldi steps
         // plus its products), and the outer FP64 products (an outer step
         // launched after convergence exits at once but is counted)
-        ls.cM = usePoly ? (int)(polySetup + polySteps * polySpmv + outerMv) : in.iters + outerMv;
+        ls.cM = useAmg    ? (int)(polySteps * amgSpmv + outerMv)
+                : usePoly ? (int)(polySetup + polySteps * polySpmv + outerMv)
+                          : in.iters + outerMv;
     }
     SV_HIP(hipMemcpyAsync(R, x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     SV_HIP(hipStreamSynchronize(s));

@@ -446,15 +446,15 @@ template int launch_poly_last<double>(size_t, const double*, double*, const doub
 // Inverse of each node's diagonal block (dof 4, Gauss-Jordan with partial
 // pivoting in FP64), stored FP32 row-major.  Rows and columns zeroed by the
 // Dirichlet mask get a unit diagonal first.
-template <int DOF, typename OT>
+template <int DOF, typename OT, typename VT = double>
 __global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __restrict__ diagPtr,
-                                                       const double* __restrict__ val, OT* __restrict__ dinv)
+                                                       const VT* __restrict__ val, OT* __restrict__ dinv)
 {
     constexpr int DD = DOF * DOF;
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= nNo) return;
     double a[DOF][2 * DOF];
-    const double* b = val + (size_t)diagPtr[r] * DD;
+    const VT* b = val + (size_t)diagPtr[r] * DD;
 #pragma unroll
     for (int p = 0; p < DOF; ++p)
 #pragma unroll
@@ -582,6 +582,23 @@ int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, OT* dinv, hip
         hipLaunchKernelGGL((k_blockdiag_inv<4, OT>), g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
     else if (dof == 3)
         hipLaunchKernelGGL((k_blockdiag_inv<3, OT>), g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val, dinv);
+    else {
+        set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
+        return SVFSI_ERR_NOTSUP;
+    }
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+int launch_blockdiag_inv_f32(const Lhs& L, int dof, const float* val, float* dinv, hipStream_t s)
+{
+    const dim3 g((L.nNo + 255) / 256);
+    if (dof == 4)
+        hipLaunchKernelGGL((k_blockdiag_inv<4, float, float>), g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val,
+                           dinv);
+    else if (dof == 3)
+        hipLaunchKernelGGL((k_blockdiag_inv<3, float, float>), g, dim3(256), 0, s, L.nNo, L.diagPtr.as<int>(), val,
+                           dinv);
     else {
         set_error("block-Jacobi: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
