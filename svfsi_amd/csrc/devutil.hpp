@@ -7,6 +7,19 @@ namespace svfsi {
 
 constexpr int kWave = 64;
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8).  xcd_block maps the hardware block index to a logical one so that
+// each XCD processes one contiguous eighth of the grid (a bijection for any
+// grid size): the rows of an XCD are then contiguous, and the vector entries
+// they gather mostly stay in that XCD's own L2.
+__device__ __forceinline__ int xcd_block(int on)
+{
+    const unsigned b = blockIdx.x;
+    if (!on) return (int)b;
+    const unsigned nb = gridDim.x, q = nb >> 3, r = nb & 7u, x = b & 7u, j = b >> 3;
+    return (int)(x * q + (x < r ? x : r) + j);
+}
+
 // Sum over the 64 lanes of a wavefront (butterfly; every lane gets the sum).
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -108,6 +121,46 @@ __device__ __forceinline__ VT ld_nt(const VT* p)
     else return *p;
 }
 
+// The nine values of a 3x3 block with the fewest load instructions: the
+// blocks of a row are only 4-byte (FP32) or 8-byte (FP64) aligned, and
+// vector types declared with that alignment let the compiler issue
+// dwordx4 loads on them (two plus one dword in FP32, four plus one dwordx2
+// in FP64) instead of nine scalar loads.
+typedef float f4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef double d2_a8 __attribute__((ext_vector_type(2), aligned(8)));
+
+template <bool NT>
+__device__ __forceinline__ void load9(const float* __restrict__ u, float v[9])
+{
+    f4_a4 a, b;
+    if constexpr (NT) {
+        a = __builtin_nontemporal_load(reinterpret_cast<const f4_a4*>(u));
+        b = __builtin_nontemporal_load(reinterpret_cast<const f4_a4*>(u + 4));
+        v[8] = __builtin_nontemporal_load(u + 8);
+    } else {
+        a = *reinterpret_cast<const f4_a4*>(u);
+        b = *reinterpret_cast<const f4_a4*>(u + 4);
+        v[8] = u[8];
+    }
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+template <bool NT>
+__device__ __forceinline__ void load9(const double* __restrict__ u, double v[9])
+{
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        d2_a8 a;
+        if constexpr (NT) a = __builtin_nontemporal_load(reinterpret_cast<const d2_a8*>(u + 2 * h));
+        else a = *reinterpret_cast<const d2_a8*>(u + 2 * h);
+        v[2 * h] = a.x;
+        v[2 * h + 1] = a.y;
+    }
+    if constexpr (NT) v[8] = __builtin_nontemporal_load(u + 8);
+    else v[8] = u[8];
+}
+
 template <typename VT, typename XT, typename AT, int L, bool NT>
 __device__ __forceinline__ void row3(const int* __restrict__ rowPtr, const int* __restrict__ colPtr,
                                      const VT* __restrict__ val, const XT* __restrict__ x, int row, int l,
@@ -118,26 +171,28 @@ __device__ __forceinline__ void row3(const int* __restrict__ rowPtr, const int* 
         const int e = rowPtr[row + 1];
         int k = rowPtr[row] + l;
         for (; k + L < e; k += 2 * L) {
-            const VT* u = val + (size_t)k * 9;
-            const VT* w = u + 9 * L;
+            VT a[9], b[9];
+            load9<NT>(val + (size_t)k * 9, a);
+            load9<NT>(val + (size_t)(k + L) * 9, b);
             const XT* xu = x + (size_t)colPtr[k] * 3;
             const XT* xw = x + (size_t)colPtr[k + L] * 3;
             const AT u0 = (AT)xu[0], u1 = (AT)xu[1], u2 = (AT)xu[2];
             const AT w0 = (AT)xw[0], w1 = (AT)xw[1], w2 = (AT)xw[2];
-            s0 += (AT)ld_nt<NT>(u + 0) * u0 + (AT)ld_nt<NT>(u + 1) * u1 + (AT)ld_nt<NT>(u + 2) * u2;
-            s1 += (AT)ld_nt<NT>(u + 3) * u0 + (AT)ld_nt<NT>(u + 4) * u1 + (AT)ld_nt<NT>(u + 5) * u2;
-            s2 += (AT)ld_nt<NT>(u + 6) * u0 + (AT)ld_nt<NT>(u + 7) * u1 + (AT)ld_nt<NT>(u + 8) * u2;
-            t0 += (AT)ld_nt<NT>(w + 0) * w0 + (AT)ld_nt<NT>(w + 1) * w1 + (AT)ld_nt<NT>(w + 2) * w2;
-            t1 += (AT)ld_nt<NT>(w + 3) * w0 + (AT)ld_nt<NT>(w + 4) * w1 + (AT)ld_nt<NT>(w + 5) * w2;
-            t2 += (AT)ld_nt<NT>(w + 6) * w0 + (AT)ld_nt<NT>(w + 7) * w1 + (AT)ld_nt<NT>(w + 8) * w2;
+            s0 += (AT)a[0] * u0 + (AT)a[1] * u1 + (AT)a[2] * u2;
+            s1 += (AT)a[3] * u0 + (AT)a[4] * u1 + (AT)a[5] * u2;
+            s2 += (AT)a[6] * u0 + (AT)a[7] * u1 + (AT)a[8] * u2;
+            t0 += (AT)b[0] * w0 + (AT)b[1] * w1 + (AT)b[2] * w2;
+            t1 += (AT)b[3] * w0 + (AT)b[4] * w1 + (AT)b[5] * w2;
+            t2 += (AT)b[6] * w0 + (AT)b[7] * w1 + (AT)b[8] * w2;
         }
         if (k < e) {
-            const VT* u = val + (size_t)k * 9;
+            VT a[9];
+            load9<NT>(val + (size_t)k * 9, a);
             const XT* xu = x + (size_t)colPtr[k] * 3;
             const AT u0 = (AT)xu[0], u1 = (AT)xu[1], u2 = (AT)xu[2];
-            s0 += (AT)ld_nt<NT>(u + 0) * u0 + (AT)ld_nt<NT>(u + 1) * u1 + (AT)ld_nt<NT>(u + 2) * u2;
-            s1 += (AT)ld_nt<NT>(u + 3) * u0 + (AT)ld_nt<NT>(u + 4) * u1 + (AT)ld_nt<NT>(u + 5) * u2;
-            s2 += (AT)ld_nt<NT>(u + 6) * u0 + (AT)ld_nt<NT>(u + 7) * u1 + (AT)ld_nt<NT>(u + 8) * u2;
+            s0 += (AT)a[0] * u0 + (AT)a[1] * u1 + (AT)a[2] * u2;
+            s1 += (AT)a[3] * u0 + (AT)a[4] * u1 + (AT)a[5] * u2;
+            s2 += (AT)a[6] * u0 + (AT)a[7] * u1 + (AT)a[8] * u2;
         }
     }
     a[0] = s0 + t0;

@@ -83,11 +83,11 @@ __global__ __launch_bounds__(256) void k_bspmv_vs(int nNo, const int* __restrict
                                                   const XT* __restrict__ x,
                                                   XT* __restrict__ y,
                                                   const int* __restrict__ stop,
-                                                  const VT* __restrict__ dinv = nullptr)
+                                                  const VT* __restrict__ dinv = nullptr, int xcd = 0)
 {
     if (stop && *stop) return;
     constexpr int DD = DOF * DOF;
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = xcd_block(xcd) * 256 + threadIdx.x;
     const int row = t / LPR;
     const int l = t - row * LPR;
     double acc[DOF];
@@ -152,10 +152,11 @@ template <typename VT, typename XT, typename AT, int L, bool NT>
 __global__ __launch_bounds__(256) void k_bspmv3(int nNo, const int* __restrict__ rowPtr,
                                                 const int* __restrict__ colPtr, const VT* __restrict__ val,
                                                 const XT* __restrict__ x, XT* __restrict__ y,
-                                                const int* __restrict__ stop, const VT* __restrict__ dinv)
+                                                const int* __restrict__ stop, const VT* __restrict__ dinv,
+                                                int xcd)
 {
     if (stop && *stop) return;
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = xcd_block(xcd) * 256 + threadIdx.x;
     const int row = t / L;
     const int l = t - row * L;
     const bool active = row < nNo;
@@ -172,6 +173,20 @@ __global__ __launch_bounds__(256) void k_bspmv3(int nNo, const int* __restrict__
 
 // Lanes per block row of the block-per-lane dof-3 kernels; SVFSI_SPMV3_L =
 // 4, 8, 16 or 32 overrides.
+// The XCD-contiguous block order of the SpMV-shaped kernels (xcd_block),
+// SVFSI_XCD_MAP = 1; off by default: measured no faster (dof-3 cube
+// matrices 82.6 / 171.3 us against 83.3 / 170.8 us in hardware order, the
+// 3.2M-tet cylinder's mixed step 252 against 248 ms): the gathered vector
+// entries already come from L2 or the Infinity Cache.
+int xcd_map()
+{
+    static const int v = [] {
+        const char* e = getenv("SVFSI_XCD_MAP");
+        return e ? (atoi(e) != 0 ? 1 : 0) : 0;
+    }();
+    return v;
+}
+
 int spmv3_lanes(const Lhs& L)
 {
     static const int forced = [] {
@@ -180,11 +195,10 @@ int spmv3_lanes(const Lhs& L)
         return (v == 4 || v == 8 || v == 16 || v == 32) ? v : 0;
     }();
     if (forced) return forced;
-    (void)L;
-    // 16 lanes per block row measured fastest in FP32 on both the tet4
-    // (15 blocks per row: 83 us against 108 at 8 and 135 at 4 lanes) and the
-    // tet10 cube matrices (28 blocks per row: 190 us against 210 and 252)
-    return 16;
+    // FP32 measurements: tet4 cube matrix (15 blocks per row) 83 us at 16
+    // lanes against 108 at 8, 135 at 4 and 110 at 32; tet10 cube matrix (28
+    // blocks per row) 169 us at 32 against 190 at 16 and 210 at 8
+    return (size_t)L.nnz > (size_t)20 * L.nNo ? 32 : 16;
 }
 
 // dof = 4 specialisation: 4 lanes per block row, lane i owns output
@@ -201,12 +215,12 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
                                                 const VT* __restrict__ x,
                                                 VT* __restrict__ y,
                                                 const int* __restrict__ stop,
-                                                const VT* __restrict__ dinv)
+                                                const VT* __restrict__ dinv, int xcd)
 {
     // 4*H lanes per block row: lane (h, i) takes blocks k = s + h, s + h + H,
     // ... and output component i; the H partial sums meet by lane shuffles.
     if (stop && *stop) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block(xcd) * blockDim.x + threadIdx.x;
     const int row = t / (4 * H);
     const int l = t - row * 4 * H;
     const int h = l >> 2, i = l & 3;
@@ -260,11 +274,11 @@ __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ 
                                                const IT* __restrict__ prodOld, IT* __restrict__ w,
                                                IT* __restrict__ y, IT* __restrict__ out,
                                                const double* __restrict__ coef, int step,
-                                               const int* __restrict__ stop)
+                                               const int* __restrict__ stop, int xcd)
 {
     if (stop && *stop) return;
     const double c1 = coef[2 * step], c2 = coef[2 * step + 1];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block(xcd) * blockDim.x + threadIdx.x;
     const int row = t / (4 * H);
     const int l = t - row * 4 * H;
     const int h = l >> 2, i = l & 3;
@@ -321,11 +335,11 @@ __global__ __launch_bounds__(256) void k_poly3(int nNo, const int* __restrict__ 
                                                const IT* __restrict__ prodOld, IT* __restrict__ w,
                                                IT* __restrict__ y, IT* __restrict__ out,
                                                const double* __restrict__ coef, int step,
-                                               const int* __restrict__ stop)
+                                               const int* __restrict__ stop, int xcd)
 {
     if (stop && *stop) return;
     const double c1 = coef[2 * step], c2 = coef[2 * step + 1];
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = xcd_block(xcd) * 256 + threadIdx.x;
     const int row = t / L;
     const int l = t - row * L;
     const bool skipSpmv = (MODE == 2) && c2 == 0.0;
@@ -368,6 +382,7 @@ int launch_poly(const Lhs& L, int dof, int mode, const IT* K, const IT* dinv, co
                 const IT* prodOld, IT* w, IT* y, IT* out, const double* coef, int step, hipStream_t s,
                 const int* stop)
 {
+    const int xm = xcd_map();
     if (dof != 3 && dof != 4) {
         set_error("polynomial preconditioner: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
@@ -385,7 +400,7 @@ int launch_poly(const Lhs& L, int dof, int mode, const IT* K, const IT* dinv, co
     const unsigned g3 = (unsigned)(((size_t)L.nNo * l3 + 255) / 256);
 #define SV_POLY3L(M, N, LL)                                                                                    \
     SV_LAUNCH(ps, (k_poly3<IT, M, N, LL>), dim3(g3), dim3(256), 0, s, L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, \
-              coef, step, stop)
+              coef, step, stop, xm)
 #define SV_POLY3(M, N)                                                                                         \
     do {                                                                                                       \
         if (l3 == 4) SV_POLY3L(M, N, 4);                                                                       \
@@ -402,7 +417,7 @@ int launch_poly(const Lhs& L, int dof, int mode, const IT* K, const IT* dinv, co
     }();
 #define SV_POLYH(M, N, HH)                                                                                       \
     SV_LAUNCH(ps, (k_poly4<IT, M, N, HH>), dim3((unsigned)(((size_t)L.nNo * 4 * HH + 255) / 256)), dim3(256), 0, s, \
-              L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, step, stop)
+              L.nNo, rp, cp, K, dinv, x, prodOld, w, y, out, coef, step, stop, xm)
 #define SV_POLY4(M, N)                                                                                           \
     do {                                                                                                         \
         if (ph == 2) SV_POLYH(M, N, 2);                                                                          \
@@ -710,6 +725,7 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
                                  (dinv ? (double)L.nNo * dof * dof * sizeof(VT) : 0.0));
     const int* rp = L.rowPtr.as<int>();
     const int* cp = L.colPtr.as<int>();
+    const int xm = xcd_map();
     if constexpr (std::is_same<VT, XT>::value) {
         if (dof == 4) {
             // lanes per output component (tuning knob SVFSI_SPMV_H = 1, 2 or 4;
@@ -728,17 +744,17 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
             const bool nt = ntMode >= 0 ? ntMode != 0 : (size_t)L.nnz * 16 * sizeof(VT) > ((size_t)128 << 20);
             const int g4 = (int)(((size_t)L.nNo * 4 * H + 255) / 256);
             if (H == 1 && nt)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 1, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 1, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv, xm);
             else if (H == 1)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 1, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 1, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv, xm);
             else if (H == 2 && nt)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 2, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 2, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv, xm);
             else if (H == 2)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 2, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 2, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv, xm);
             else if (nt)
-                SV_LAUNCH(ps, (k_bspmv4<VT, 4, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 4, true>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv, xm);
             else
-                SV_LAUNCH(ps, (k_bspmv4<VT, 4, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv);
+                SV_LAUNCH(ps, (k_bspmv4<VT, 4, false>), dim3(g4), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv, xm);
             SV_HIP(hipGetLastError());
             return SVFSI_OK;
         }
@@ -757,7 +773,7 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
         const int l3 = spmv3_lanes(L);
         const unsigned g3 = (unsigned)(((size_t)L.nNo * l3 + 255) / 256);
 #define SV_B3(LL, N)                                                                                           \
-    SV_LAUNCH(ps, (k_bspmv3<VT, XT, VT, LL, N>), dim3(g3), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv)
+    SV_LAUNCH(ps, (k_bspmv3<VT, XT, VT, LL, N>), dim3(g3), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop, dinv, xm)
         if (l3 == 4) { if (nt) SV_B3(4, true); else SV_B3(4, false); }
         else if (l3 == 16) { if (nt) SV_B3(16, true); else SV_B3(16, false); }
         else if (l3 == 32) { if (nt) SV_B3(32, true); else SV_B3(32, false); }
@@ -781,10 +797,10 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
     do {                                                                                                      \
         if (nt)                                                                                               \
             SV_LAUNCH(ps, (k_bspmv_vs<D, VT, XT, P, true>), dim3(gvs), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, \
-                      stop, dinv);                                                                            \
+                      stop, dinv, xm);                                                                        \
         else                                                                                                  \
             SV_LAUNCH(ps, (k_bspmv_vs<D, VT, XT, P, false>), dim3(gvs), dim3(256), 0, s, L.nNo, rp, cp, K, U,    \
-                      KU, stop, dinv);                                                                        \
+                      KU, stop, dinv, xm);                                                                    \
     } while (0)
             if (dof == 2) {
                 if (lpr == 8) SV_VS(2, 8); else if (lpr == 32) SV_VS(2, 32); else SV_VS(2, 16);
