@@ -91,8 +91,10 @@ __device__ __forceinline__ double dot4_nt(const double* __restrict__ v, const do
     return a.x * xa.x + a.y * xa.y + b.x * xb.x + b.y * xb.y;
 }
 
+// FP32 operands, FP32 sums (the inner operator of the mixed solve computes in
+// its storage precision)
 template <bool NT>
-__device__ __forceinline__ double dot4_nt(const float* __restrict__ v, const float* __restrict__ x)
+__device__ __forceinline__ float dot4_nt(const float* __restrict__ v, const float* __restrict__ x)
 {
     typedef float f4 __attribute__((ext_vector_type(4)));
     f4 a;
@@ -101,8 +103,7 @@ __device__ __forceinline__ double dot4_nt(const float* __restrict__ v, const flo
     else
         a = *reinterpret_cast<const f4*>(v);
     const f4 xa = *reinterpret_cast<const f4*>(x);
-    return (double)a.x * (double)xa.x + (double)a.y * (double)xa.y + (double)a.z * (double)xa.z
-         + (double)a.w * (double)xa.w;
+    return a.x * xa.x + a.y * xa.y + a.z * xa.z + a.w * xa.w;
 }
 
 

@@ -224,7 +224,8 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
     const int row = t / (4 * H);
     const int l = t - row * 4 * H;
     const int h = l >> 2, i = l & 3;
-    double a0 = 0.0, a1 = 0.0;
+    using AT = VT; // sums in the storage precision
+    AT a0 = 0, a1 = 0;
     if (row < nNo) {
         const int e = rowPtr[row + 1];
         int k = rowPtr[row] + h;
@@ -235,20 +236,20 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
         }
         if (k < e) a0 += dot4_nt<NT>(val + (size_t)k * 16 + i * 4, x + (size_t)colPtr[k] * 4);
     }
-    double acc = a0 + a1;
+    AT acc = a0 + a1;
 #pragma unroll
     for (int o = 4; o < 4 * H; o <<= 1) acc += __shfl_xor(acc, o, kWave);
     if (dinv) {
         // every lane holds (K x)(row, i); gather the row's four components
         const int base = (threadIdx.x & (kWave - 1)) & ~3;
-        double kx[4];
+        AT kx[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) kx[m] = __shfl(acc, base + m, kWave);
-        double v = 0.0;
+        AT v = 0;
         if (row < nNo) {
             const VT* d = dinv + (size_t)row * 16 + i * 4;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) v += (double)d[m] * kx[m];
+            for (int m = 0; m < 4; ++m) v += d[m] * kx[m];
         }
         acc = v;
     }
@@ -256,7 +257,8 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Polynomial preconditioner steps (dof 4, FP32 vectors, FP64 arithmetic).
+// Polynomial preconditioner steps (dof 4; the block products summed in the
+// inner precision, the factor's vector update in FP64 registers).
 // p(B) v with 1 - a p(a) = prod_s f_s(a) is accumulated factor by factor:
 // prod <- f_s(B) prod, y <- y + (the matching part of p).  For a quadratic
 // factor 1 - c1 a + c2 a^2:
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ 
     const int l = t - row * 4 * H;
     const int h = l >> 2, i = l & 3;
     const bool skipSpmv = (MODE == 2) && c2 == 0.0;
-    double a0 = 0.0, a1 = 0.0;
+    IT a0 = 0, a1 = 0; // sums in the inner precision
     if (row < nNo && !skipSpmv) {
         const int e = rowPtr[row + 1];
         int k = rowPtr[row] + h;
@@ -294,19 +296,19 @@ __global__ __launch_bounds__(256) void k_poly4(int nNo, const int* __restrict__ 
         }
         if (k < e) a0 += dot4_nt<NT>(val + (size_t)k * 16 + i * 4, x + (size_t)colPtr[k] * 4);
     }
-    double acc = a0 + a1;
+    IT acc = a0 + a1;
 #pragma unroll
     for (int o = 4; o < 4 * H; o <<= 1) acc += __shfl_xor(acc, o, kWave);
     if (dinv && !skipSpmv) {
         const int base = (threadIdx.x & (kWave - 1)) & ~3;
-        double kx[4];
+        IT kx[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) kx[m] = __shfl(acc, base + m, kWave);
-        double v = 0.0;
+        IT v = 0;
         if (row < nNo) {
             const IT* d = dinv + (size_t)row * 16 + i * 4;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) v += (double)d[m] * kx[m];
+            for (int m = 0; m < 4; ++m) v += d[m] * kx[m];
         }
         acc = v;
     }
