@@ -68,10 +68,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["fluid", "fluid-large", "struct", "lelas"], default="fluid",
+    ap.add_argument("--workload", choices=["fluid", "fluid-large", "struct", "lelas", "fluid-th"], default="fluid",
                     help="fluid: VMS cylinder ~200k tet4 (the metric's configuration); fluid-large: the same physics "
                          "on ~3.2M tet4 with GMRES(100); struct: neo-Hookean tet10 cube 40^3 x 6, 4-point rule; "
-                         "lelas: linear elasticity cube 20^3 x 6 tet4 with CG")
+                         "lelas: linear elasticity cube 20^3 x 6 tet4 with CG; fluid-th: Taylor-Hood (P2P1) "
+                         "channel on the tet10 cube 20^3 x 6 with GMRES(50)")
     ap.add_argument("--n", type=int, default=0, help="cube cells per side for struct/lelas (default 40/20)")
     ap.add_argument("--h", type=float, default=0.11, help="cylinder lattice spacing (0.11 -> ~200k tets)")
     ap.add_argument("--sD", type=int, default=0, help="GMRES restart length (default: 50 fluid, 100 struct)")
@@ -125,6 +126,10 @@ def main() -> int:
         if args.workload == "fluid-large":
             prob.ls = dict(prob.ls, mItr=2000)
         wl = "FLUID3D VMS tet4 cylinder r=1 L=10 (h=%g), one Newton step, GMRES(%d) to 1e-8" % (h, args.sD)
+    elif args.workload == "fluid-th":
+        n = args.n or 20
+        prob = problems.fluid_th_cube(n, sD=args.sD)
+        wl = f"FLUID Taylor-Hood (P2P1) channel, tet10 cube {n}^3 x 6, one Newton step, GMRES({args.sD}) to 1e-8"
     elif args.workload == "struct":
         n = args.n or 40
         prob = problems.struct_cube(n, quad=1, sD=args.sD)

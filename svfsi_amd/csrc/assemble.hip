@@ -1300,7 +1300,18 @@ __global__ __launch_bounds__(256) void k_gather(size_t nItems, const int* __rest
     const int q = (int)(t - j * DD);
     const int k1 = ptr[j + 1];
     double sum = 0.0;
-    for (int k = ptr[j]; k < k1; ++k) sum += src[(size_t)k * DD + q];
+    int k = ptr[j];
+    // four loads in flight, added in element order (the sums associate as
+    // DOASSEM's element loop does)
+    for (; k + 4 <= k1; k += 4) {
+        const double a0 = src[(size_t)k * DD + q], a1 = src[(size_t)(k + 1) * DD + q];
+        const double a2 = src[(size_t)(k + 2) * DD + q], a3 = src[(size_t)(k + 3) * DD + q];
+        sum += a0;
+        sum += a1;
+        sum += a2;
+        sum += a3;
+    }
+    for (; k < k1; ++k) sum += src[(size_t)k * DD + q];
     dst[j * DD + q] += sum;
 }
 

@@ -270,6 +270,10 @@ int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
 // IT = float (the mixed solve) and IT = double (the same algorithm in FP64).
 template <typename OT>
 int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, OT* dinv, hipStream_t s);
+// number of unmasked dofs of the scaled matrix with a zero diagonal entry
+// (dCount: one device int of scratch)
+int count_zero_diag(const Lhs& L, int dof, const double* val, const double* W, int* dCount, int& hCount,
+                    hipStream_t s);
 // 1 when the SpMV-shaped kernels map blocks XCD-contiguously (xcd_block)
 int xcd_map();
 // the same from FP32 values (the coarse levels of the aggregation hierarchy)

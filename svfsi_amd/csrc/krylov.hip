@@ -1725,8 +1725,21 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
         set_error("the multigrid inner preconditioner is FP32 only");
         return SVFSI_ERR_NOTSUP;
     }
-    const bool polyWanted = !useAmg && (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", 1) != 0;
-    const bool bj = !useAmg && (dof == 3 || dof == 4) && env_int("SVFSI_INNER_BJ", dof == 4 ? 1 : 0) != 0;
+    // A saddle-point system without pressure stabilisation (unmasked rows
+    // with a zero diagonal: the Taylor-Hood fluid) defeats both the node
+    // block inverses and the fixed polynomial: on the tet10 channel they
+    // made the outer loop stall or overflow, while one inner GMRES cycle of
+    // the diagonally scaled operator converged (371 outer steps at 10^3).
+    // Such systems get the inner GMRES cycle without block Jacobi unless
+    // SVFSI_INNER_POLY / SVFSI_INNER_BJ say otherwise.
+    int zeroDiag = 0;
+    SV_TRY(w.ostate.ensure(256));
+    SV_TRY(count_zero_diag(L, dof, K, w.W.as<double>(), w.ostate.as<int>(), zeroDiag, s));
+    const bool saddle = zeroDiag > 0;
+    const bool polyWanted =
+        !useAmg && (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", saddle ? 0 : 1) != 0;
+    const bool bj =
+        !useAmg && (dof == 3 || dof == 4) && env_int("SVFSI_INNER_BJ", (dof == 4 && !saddle) ? 1 : 0) != 0;
     // With block Jacobi the inner matrix is replaced by B = D^-1 K (formed
     // once per solve from the FP64 matrix into Kin, or into w.valIn for an
     // FP64 inner operator), so the inner products carry no epilogue, and

@@ -463,6 +463,8 @@ template int launch_poly_last<double>(size_t, const double*, double*, const doub
 // Inverse of each node's diagonal block (dof 4, Gauss-Jordan with partial
 // pivoting in FP64), stored FP32 row-major.  Rows and columns zeroed by the
 // Dirichlet mask get a unit diagonal first.
+constexpr double kBlockCond = 1e6;
+
 template <int DOF, typename OT, typename VT = double>
 __global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __restrict__ diagPtr,
                                                        const VT* __restrict__ val, OT* __restrict__ dinv)
@@ -486,6 +488,11 @@ __global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __res
         for (int q = 0; q < DOF; ++q) zero = zero && a[p][q] == 0.0 && a[q][p] == 0.0;
         if (zero) a[p][p] = 1.0;
     }
+    double amax = 0.0;
+#pragma unroll
+    for (int p = 0; p < DOF; ++p)
+#pragma unroll
+        for (int q = 0; q < DOF; ++q) amax = fmax(amax, fabs(a[p][q]));
     bool ok = true;
 #pragma unroll
     for (int c = 0; c < DOF; ++c) {
@@ -513,11 +520,28 @@ __global__ __launch_bounds__(256) void k_blockdiag_inv(int nNo, const int* __res
                 for (int q = 0; q < 2 * DOF; ++q) a[p][q] -= f * a[c][q];
             }
     }
+    // A block whose inverse is ill-conditioned (max|B| max|B^-1| above
+    // kBlockCond: a saddle-point node block with a zero pressure diagonal
+    // and weak coupling) would turn the inner operator's rounding into
+    // noise; such a node keeps the inverse of its diagonal instead.
+    double imax = 0.0;
+#pragma unroll
+    for (int p = 0; p < DOF; ++p)
+#pragma unroll
+        for (int q = 0; q < DOF; ++q) imax = fmax(imax, fabs(a[p][DOF + q]));
+    ok = ok && amax * imax <= kBlockCond;
     OT* o = dinv + (size_t)r * DD;
 #pragma unroll
     for (int p = 0; p < DOF; ++p)
 #pragma unroll
-        for (int q = 0; q < DOF; ++q) o[p * DOF + q] = ok ? (OT)a[p][DOF + q] : (OT)(p == q ? 1.0 : 0.0);
+        for (int q = 0; q < DOF; ++q) {
+            if (ok) {
+                o[p * DOF + q] = (OT)a[p][DOF + q];
+            } else {
+                const double dpp = (double)b[p * DOF + p];
+                o[p * DOF + q] = (OT)(p == q ? (dpp != 0.0 ? 1.0 / dpp : 1.0) : 0.0);
+            }
+        }
 }
 
 // B = D^-1 K in FP32 (block Jacobi folded into the inner matrix): block k of
@@ -604,6 +628,32 @@ int launch_blockdiag_inv(const Lhs& L, int dof, const double* val, OT* dinv, hip
         return SVFSI_ERR_NOTSUP;
     }
     SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+// number of unmasked dofs (W != 0) whose diagonal entry is zero: rows of a
+// saddle-point system without pressure stabilisation
+__global__ __launch_bounds__(256) void k_count_zero_diag(int nNo, int dof, const int* __restrict__ diagPtr,
+                                                         const double* __restrict__ val,
+                                                         const double* __restrict__ W, int* __restrict__ count)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nNo * dof) return;
+    const int i = t / dof, c = t - (t / dof) * dof;
+    if (W[t] != 0.0 && val[(size_t)diagPtr[i] * dof * dof + c * dof + c] == 0.0) atomicAdd(count, 1);
+}
+
+int count_zero_diag(const Lhs& L, int dof, const double* val, const double* W, int* dCount, int& hCount,
+                    hipStream_t s)
+{
+    hCount = 0;
+    if (L.nNo == 0) return SVFSI_OK;
+    SV_HIP(hipMemsetAsync(dCount, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_count_zero_diag, dim3((unsigned)(((size_t)L.nNo * dof + 255) / 256)), dim3(256), 0, s,
+                       L.nNo, dof, L.diagPtr.as<int>(), val, W, dCount);
+    SV_HIP(hipGetLastError());
+    SV_HIP(hipMemcpyAsync(&hCount, dCount, sizeof(int), hipMemcpyDeviceToHost, s));
+    SV_HIP(hipStreamSynchronize(s));
     return SVFSI_OK;
 }
 

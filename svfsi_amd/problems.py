@@ -134,6 +134,33 @@ def fluid_cylinder(h: float = 0.11, seed: int = 0, rho: float = 1.06, mu: float 
         ls=dict(LS_type=LS_TYPE_GMRES, relTol=1e-8, absTol=1e-30, mItr=1000, sD=sD))
 
 
+def fluid_th_cube(n: int = 20, seed: int = 0, rho: float = 1.06, mu: float = 0.04, u_peak: float = 1.0,
+                  dt: float = 1e-3, rho_inf: float = 0.2, sD: int = 50, bstab: float = 0.2) -> Problem:
+    """Taylor-Hood (P2P1) Navier-Stokes in the unit-cube channel on the tet10
+    Kuhn mesh (nFs = 2: tet10 velocity, tet4 pressure on the corner nodes):
+    inlet x = 0 with u_x = 16 u_peak y(1-y) z(1-z), no-slip walls y, z = 0, 1,
+    zero-traction outlet x = 1.  State: Ag = 0, Yg velocity drawn from
+    U(-1, 1) (seeded) with the Dirichlet values imposed, p = 0.  GMRES(sD)
+    with diagonal scaling to relative residual 1e-8."""
+    m = meshmod.cube_tet10(n)
+    eq = dict(gen_alpha(rho_inf, False), dt=dt, rho=rho, mu=mu, bstab=bstab)
+    rng = np.random.default_rng(seed + 1)
+    x = m.x
+    tol = 1e-12
+    inlet = x[:, 0] < tol
+    wall = (x[:, 1] < tol) | (x[:, 1] > 1 - tol) | (x[:, 2] < tol) | (x[:, 2] > 1 - tol)
+    Y = np.zeros((m.nNo, 4))
+    Y[:, :3] = rng.uniform(-1.0, 1.0, (m.nNo, 3))
+    Y[inlet, :3] = 0.0
+    Y[inlet, 0] = 16.0 * u_peak * x[inlet, 1] * (1 - x[inlet, 1]) * x[inlet, 2] * (1 - x[inlet, 2])
+    Y[wall, :3] = 0.0
+    return Problem(
+        name=f"fluid_th_cube{n}", phys=SVFSI_PHYS_FLUID, dof=4, mesh=m, eq=eq, Ag=np.zeros((m.nNo, 4)), Yg=Y,
+        Dg=np.zeros((m.nNo, 4)),
+        dir_faces=[(np.where(inlet)[0].astype(np.int32), 3), (np.where(wall & ~inlet)[0].astype(np.int32), 3)],
+        nFs=2, ls=dict(LS_type=LS_TYPE_GMRES, relTol=1e-8, absTol=1e-30, mItr=1000, sD=sD))
+
+
 def struct_cube(n: int = 4, C10: float = 1e5, Kpen: float = 1e7, rho: float = 1.0, dt: float = 1e-3,
                 amp: float = 1e-3, seed: int = 0, rho_inf: float = 0.2, quad: int = 1,
                 volType: int = 2, sD: int = 100) -> Problem:

@@ -4,7 +4,9 @@ output pins (tests/test_ref_golden.py, ref_fluid_th10_*): jittered tet10
 cubes with random velocity, acceleration and pressure, the three viscosity
 models and a per-element viscosity, FP64 to 1e-12 relative and bitwise
 reproducible; the mid-edge pressure rows are the identity; and an FSILS
-GMRES solve of the assembled system matches the oracle's."""
+GMRES solve of the assembled system matches the oracle's; the mixed solve
+(which picks the inner GMRES cycle without block Jacobi for a system with
+zero pressure diagonals) reaches the FP64 criterion on the tet10 channel."""
 import sys
 
 import numpy as np
@@ -112,3 +114,28 @@ def test_th_needs_the_flag_on_tet10(gpu):
     st = NewtonStep(p)
     with pytest.raises(SvfsiError, match="P2P1"):
         st.dmesh.assemble(p.phys, p.eq, st.Ag, st.Yg, st.Dg, st.Val, st.R, quad=0)
+
+
+def test_th_mixed_solve_reaches_fp64_tolerance(gpu, orc):
+    import torch
+
+    p = problems.fluid_th_cube(4)
+    st = NewtonStep(p)
+    st.assemble()
+    torch.cuda.synchronize()
+    V0, R0 = st.Val.cpu().numpy().copy(), st.R.cpu().numpy().copy()
+    ls = st.solve(mixed=True)
+    torch.cuda.synchronize()
+    xg = st.R.cpu().numpy()
+    rp, cp = st.rowPtr - 1, st.colPtr - 1
+    mask = p.dir_mask()
+    _, Rs, W = orc.precond_diag(rp, cp, 4, V0, R0, mask)
+    xo, Vs, so = orc.fsils_solve(rp, cp, 4, mask, p.ls, R0, V0)
+    y = np.where(W != 0, xg / np.where(W != 0, W, 1), 0.0)
+    r = Rs - orc.spmv(rp, cp, 4, Vs, y)
+    say("Taylor-Hood mixed: outer", ls.RI.itr, "SpMVs", ls.RI.cM, "scaled residual",
+        np.linalg.norm(r) / np.linalg.norm(Rs), "x rel err", np.abs(xg - xo).max() / np.abs(xo).max())
+    assert ls.RI.suc == 1 and so.suc == 1
+    assert np.all(np.isfinite(xg))
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(Rs) * 1.0001
+    assert np.abs(xg - xo).max() <= 1e-4 * np.abs(xo).max()
