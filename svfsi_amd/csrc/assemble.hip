@@ -1332,7 +1332,7 @@ static bool gather_enabled(int eNoN)
     return eNoN == 4;
 }
 
-static int launch_gather(Mesh& M, int dof, double* val, double* R, hipStream_t s)
+int launch_gather(Mesh& M, int dof, double* val, double* R, hipStream_t s)
 {
     const size_t nb = (size_t)M.lhs->nnz, nr = (size_t)M.nNo;
     const double* ke = M.ke.as<double>();

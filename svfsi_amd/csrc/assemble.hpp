@@ -67,6 +67,9 @@ constexpr int kThRec = 45;
 int make_th_rule(std::vector<double>& tab);
 // CONSTRUCT_FLUID with nFs = 2 on a tet10 mesh, colour by colour, then
 // THOOD_ValRC on the mid-edge nodes.
+// Gather assembly's second pass: every Val block and R row summed from the
+// stored element blocks (M.ke) and rows (M.re) in element-id order.
+int launch_gather(Mesh& M, int dof, double* val, double* R, hipStream_t s);
 int launch_asm_fluid_th(Mesh& M, const AsmParams& P, int tDof, const double* Ag, const double* Yg,
                         const double* muEl, double* val, double* R, hipStream_t s);
 // kern: one of SVFSI_STRUCT_*.  The MFMA kernels keep material-point

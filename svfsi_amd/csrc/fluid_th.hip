@@ -187,7 +187,9 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
                                                       const double* __restrict__ Ag, const double* __restrict__ Yg,
                                                       const double* __restrict__ muEl, AsmParams P,
                                                       const double* __restrict__ rule, double* __restrict__ val,
-                                                      double* __restrict__ R)
+                                                      double* __restrict__ R, double* __restrict__ ke,
+                                                      double* __restrict__ re, const int* __restrict__ gSlot,
+                                                      const int* __restrict__ rSlot)
 {
     const int p = begin + blockIdx.x;
     const int t = threadIdx.x;
@@ -260,13 +262,17 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
     for (int q = 0; q < 16; ++q) k[q] = 0.0;
     for (int q = 0; q < 4; ++q) lr[q] = 0.0;
 
-    // ---- integration 1: FLUID3D_M at the 15 points of the tet10 rule
-    for (int g = 0; g < kNG1; ++g) {
+    // ---- integration 1: FLUID3D_M at the 15 points of the tet10 rule.
+    // Point kinematics first, one thread per point (the same expressions in
+    // the same order as a per-lane evaluation would use), into LDS
+    __shared__ double sU[kNG1][3], sUd[kNG1][3], sUx[kNG1][3][3], sEs[kNG1][3][3], sD2u2[kNG1][3];
+    __shared__ double sPp[kNG1], sMu[kNG1], sMuG[kNG1], sMuX[kNG1][3], sTauM[kNG1], sUp[kNG1][3];
+    __shared__ double sRM[kNG1][3][3], sRV[kNG1][3];
+    if (t < kNG1) {
+        const int g = t;
         const double* r = rule + (size_t)g * kThRec;
         const double* Nw = r + 1;
         const double* Nq = r + 41;
-        const double w = sW[g];
-        const double wl = w * T1c, wr = w * rho;
         // velocity, its gradient and second derivatives, inertia (FLUID.f:228-291)
         double ud[3], u[3], ux[3][3], uxx[3][3][3];
         for (int c = 0; c < 3; ++c) {
@@ -350,6 +356,41 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
                 rM[i][c] = mu * es[i][c] - rho * up[c] * u[i] + rv * up[i] - (i == c ? pp : 0.0);
         }
         for (int c = 0; c < 3; ++c) rV[c] = ud[c] + u[0] * ux[0][c] + u[1] * ux[1][c] + u[2] * ux[2][c];
+        for (int c = 0; c < 3; ++c) {
+            sU[g][c] = u[c];
+            sUd[g][c] = ud[c];
+            sD2u2[g][c] = d2u2[c];
+            sMuX[g][c] = mu_x[c];
+            sUp[g][c] = up[c];
+            sRV[g][c] = rV[c];
+            for (int i = 0; i < 3; ++i) {
+                sUx[g][i][c] = ux[i][c];
+                sEs[g][i][c] = es[i][c];
+                sRM[g][i][c] = rM[i][c];
+            }
+        }
+        sPp[g] = pp;
+        sMu[g] = mu;
+        sMuG[g] = mu_g;
+        sTauM[g] = tauM;
+        (void)Nw;
+    }
+    __syncthreads();
+    for (int g = 0; g < kNG1; ++g) {
+        const double* r = rule + (size_t)g * kThRec;
+        const double* Nw = r + 1;
+        const double* Nq = r + 41;
+        const double w = sW[g];
+        const double wl = w * T1c, wr = w * rho;
+        const double* u = sU[g];
+        const double* d2u2 = sD2u2[g];
+        const double* mu_x = sMuX[g];
+        const double* up = sUp[g];
+        const double* rV = sRV[g];
+        const double (*es)[3] = sEs[g];
+        const double (*rM)[3] = sRM[g];
+        const double mu = sMu[g], mu_g = sMuG[g], tauM = sTauM[g];
+        const double tauC = 0.0, tauB = 0.0;
         if (!own) continue;
         const double* Xa = sNwx[g][a];
         const double* Xb = sNwx[g][b];
@@ -416,6 +457,17 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
     }
 
     if (!own) return;
+    if (ke) {
+        // gather assembly: the element block at its gather position (the
+        // sums into Val follow in element-id order)
+        double* kb = ke + (size_t)gSlot[((size_t)p * kNW + a) * kNW + b] * 16;
+        for (int q = 0; q < 16; ++q) kb[q] = k[q];
+        if (b == 0) {
+            double* rb = re + (size_t)rSlot[(size_t)p * kNW + a] * 4;
+            for (int c = 0; c < 4; ++c) rb[c] = lr[c];
+        }
+        return;
+    }
     // DOASSEM through the scatter map (colours keep the updates disjoint)
     double* vb = val + (size_t)emap[((size_t)p * kNW + a) * kNW + b] * 16;
     for (int q = 0; q < 16; ++q) vb[q] += k[q];
@@ -453,15 +505,30 @@ int launch_asm_fluid_th(Mesh& M, const AsmParams& P, int tDof, const double* Ag,
         SV_TRY(M.ruleTH.ensure(tab.size() * sizeof(double)));
         SV_HIP(hipMemcpy(M.ruleTH.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     }
-    const int nC = (int)M.colorOff.size() - 1;
-    for (int c = 0; c < nC; ++c) {
-        const int begin = M.colorOff[c], count = M.colorOff[c + 1] - begin;
-        if (count == 0) continue;
-        hipLaunchKernelGGL(k_asm_fluid_th, dim3(count), dim3(128), 0, s, begin, M.ien.as<int>(),
-                           M.elemId.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Yg, muEl, P,
-                           M.ruleTH.as<double>(), val, R);
+    // Gather (default): one launch over all elements stores the element
+    // blocks, launch_gather sums them in element-id order; SVFSI_ASM_GATHER=0
+    // keeps the coloured read-modify-write, one launch per colour.
+    const char* ge = getenv("SVFSI_ASM_GATHER");
+    const bool gather = !(ge && ge[0] == '0');
+    if (gather) {
+        SV_TRY(M.ke.ensure((size_t)M.nEl * 100 * 16 * sizeof(double)));
+        SV_TRY(M.re.ensure((size_t)M.nEl * 10 * 4 * sizeof(double)));
+        hipLaunchKernelGGL(k_asm_fluid_th, dim3(M.nEl), dim3(128), 0, s, 0, M.ien.as<int>(), M.elemId.as<int>(),
+                           M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Yg, muEl, P, M.ruleTH.as<double>(), val, R,
+                           M.ke.as<double>(), M.re.as<double>(), M.gSlot.as<int>(), M.rSlot.as<int>());
+        SV_HIP(hipGetLastError());
+        SV_TRY(launch_gather(M, 4, val, R, s));
+    } else {
+        const int nC = (int)M.colorOff.size() - 1;
+        for (int c = 0; c < nC; ++c) {
+            const int begin = M.colorOff[c], count = M.colorOff[c + 1] - begin;
+            if (count == 0) continue;
+            hipLaunchKernelGGL(k_asm_fluid_th, dim3(count), dim3(128), 0, s, begin, M.ien.as<int>(),
+                               M.elemId.as<int>(), M.emap.as<int>(), M.x.as<double>(), tDof, Ag, Yg, muEl, P,
+                               M.ruleTH.as<double>(), val, R, nullptr, nullptr, nullptr, nullptr);
+        }
+        SV_HIP(hipGetLastError());
     }
-    SV_HIP(hipGetLastError());
     if (M.nMid > 0) {
         hipLaunchKernelGGL(k_thood_valrc, dim3((M.nMid + 255) / 256), dim3(256), 0, s, M.nMid, M.midNodes.as<int>(),
                            M.lhs->rowPtr.as<int>(), M.lhs->colPtr.as<int>(), val, R);

@@ -61,10 +61,13 @@ def rel_val(V, Vo):
     return np.abs(V - Vo).max() / np.abs(Vo[:, :15]).max()
 
 
-@pytest.mark.parametrize("visc", ["const", "cy", "cass"])
-def test_th_assembly_matches_oracle(gpu, orc, visc):
+@pytest.mark.parametrize("visc,gather", [("const", "1"), ("cy", "1"), ("cass", "1"), ("const", "0")])
+def test_th_assembly_matches_oracle(gpu, orc, monkeypatch, visc, gather):
+    """gather "1": one launch storing element blocks, summed per Val block in
+    element order (the default); "0": coloured read-modify-write."""
     import torch
 
+    monkeypatch.setenv("SVFSI_ASM_GATHER", gather)
     p = th_problem(4, 11, visc, per_element=(visc == "const"))
     say("Taylor-Hood tet10 assembly", visc, "nEl", p.mesh.nEl, "nNo", p.nNo)
     st = NewtonStep(p)
