@@ -159,6 +159,55 @@ __global__ __launch_bounds__(256) void k_cg_spmv_dot_vs(int nNo, const int* __re
     if (threadIdx.x == 0) partial[blockIdx.x] = sb;
 }
 
+// dof 3 on the SpMV's dof-4 lane mapping (k_bspmv3r, H = 4): 16 lanes per
+// block row, lane (h, i) takes blocks s + h, s + h + 4, ... and block row
+// i < 3; grid-stride over rows, per-block partials of <P, K P>.
+__global__ __launch_bounds__(256) void k_cg_spmv_dot3r(int nNo, const int* __restrict__ rowPtr,
+                                                       const int* __restrict__ colPtr,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ p,
+                                                       double* __restrict__ kp,
+                                                       double* __restrict__ partial,
+                                                       const CgState* __restrict__ st)
+{
+    if (st->done) return;
+    constexpr int H = 4, RPB = 256 / (4 * H);
+    __shared__ double lds[4];
+    const int l = threadIdx.x % (4 * H);
+    const int h = l >> 2, i = l & 3;
+    double contrib = 0.0;
+    for (int rb = blockIdx.x; rb * RPB < nNo; rb += gridDim.x) {
+        const int row = rb * RPB + threadIdx.x / (4 * H);
+        double a0 = 0.0, a1 = 0.0;
+        if (row < nNo && i < 3) {
+            const int e = rowPtr[row + 1];
+            int k = rowPtr[row] + h;
+            for (; k + H < e; k += 2 * H) {
+                const double* ua = val + (size_t)k * 9 + i * 3;
+                const double* ub = val + (size_t)(k + H) * 9 + i * 3;
+                const double* xa = p + (size_t)colPtr[k] * 3;
+                const double* xb = p + (size_t)colPtr[k + H] * 3;
+                a0 += ua[0] * xa[0] + ua[1] * xa[1] + ua[2] * xa[2];
+                a1 += ub[0] * xb[0] + ub[1] * xb[1] + ub[2] * xb[2];
+            }
+            if (k < e) {
+                const double* ua = val + (size_t)k * 9 + i * 3;
+                const double* xa = p + (size_t)colPtr[k] * 3;
+                a0 += ua[0] * xa[0] + ua[1] * xa[1] + ua[2] * xa[2];
+            }
+        }
+        double acc = a0 + a1;
+#pragma unroll
+        for (int o = 4; o < 4 * H; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+        if (row < nNo && h == 0 && i < 3) {
+            kp[(size_t)row * 3 + i] = acc;
+            contrib += p[(size_t)row * 3 + i] * acc;
+        }
+    }
+    const double sb = block_sum<256>(contrib, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sb;
+}
+
 // alpha = |R|^2 / <P, K P> ; X += alpha P ; R -= alpha KP ; partials of |R|^2
 // (CGRAD.f:216-226).  par: parity of the iteration.
 __global__ __launch_bounds__(256) void k_cg_update(size_t n, double* __restrict__ x,
@@ -352,7 +401,7 @@ int cg_core(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hi
     }
     const size_t n = (size_t)L.nNo * dof;
     Work& w = L.ws;
-    const bool vs = dof <= 3;  // value-streaming SpMV for dof 2 / 3
+    const bool vs = dof <= 3;  // dof 2 value streaming, dof 3 row lanes: 16 lanes per row either way
     const int rowsPerBlock = vs ? 256 / 16 : 4 * (kWave / (dof * dof));
     int gs = (L.nNo + rowsPerBlock - 1) / rowsPerBlock;
     if (gs > 1024) gs = 1024;
@@ -400,7 +449,7 @@ int cg_core(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hi
             const int par = it & 1;
             switch (dof) {
             case 2: hipLaunchKernelGGL((k_cg_spmv_dot_vs<2, 16>), dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
-            case 3: hipLaunchKernelGGL((k_cg_spmv_dot_vs<3, 16>), dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
+            case 3: hipLaunchKernelGGL(k_cg_spmv_dot3r, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
             default: hipLaunchKernelGGL(k_cg_spmv_dot<4>, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
             }
             hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, cs, n, X, R, (const double*)P, (const double*)KP,

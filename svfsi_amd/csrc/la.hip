@@ -256,6 +256,46 @@ __global__ __launch_bounds__(256) void k_bspmv4(int nNo, const int* __restrict__
     if (row < nNo && h == 0) y[(size_t)row * 4 + i] = (VT)acc;
 }
 
+// dof 3 on the dof-4 lane mapping: 4*H lanes per block row, lane (h, i)
+// takes blocks s + h, s + h + H, ... and block row i < 3 (three contiguous
+// values), lane i = 3 idles; x(:, col) is read as three contiguous values.  The H partial sums meet by lane
+// shuffles.
+template <typename VT, int H, bool NT>
+__global__ __launch_bounds__(256) void k_bspmv3r(int nNo, const int* __restrict__ rowPtr,
+                                                 const int* __restrict__ colPtr, const VT* __restrict__ val,
+                                                 const VT* __restrict__ x, VT* __restrict__ y,
+                                                 const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int row = t / (4 * H);
+    const int l = t - row * 4 * H;
+    const int h = l >> 2, i = l & 3;
+    VT a0 = 0, a1 = 0;
+    if (row < nNo && i < 3) {
+        const int e = rowPtr[row + 1];
+        int k = rowPtr[row] + h;
+        for (; k + H < e; k += 2 * H) {
+            const int c0 = colPtr[k], c1 = colPtr[k + H];
+            const VT* ua = val + (size_t)k * 9 + i * 3;
+            const VT* ub = val + (size_t)(k + H) * 9 + i * 3;
+            const VT* xa = x + (size_t)c0 * 3;
+            const VT* xb = x + (size_t)c1 * 3;
+            a0 += ld_nt<NT>(ua) * xa[0] + ld_nt<NT>(ua + 1) * xa[1] + ld_nt<NT>(ua + 2) * xa[2];
+            a1 += ld_nt<NT>(ub) * xb[0] + ld_nt<NT>(ub + 1) * xb[1] + ld_nt<NT>(ub + 2) * xb[2];
+        }
+        if (k < e) {
+            const VT* ua = val + (size_t)k * 9 + i * 3;
+            const VT* xa = x + (size_t)colPtr[k] * 3;
+            a0 += ld_nt<NT>(ua) * xa[0] + ld_nt<NT>(ua + 1) * xa[1] + ld_nt<NT>(ua + 2) * xa[2];
+        }
+    }
+    VT acc = a0 + a1;
+#pragma unroll
+    for (int o = 4; o < 4 * H; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+    if (row < nNo && h == 0 && i < 3) y[(size_t)row * 3 + i] = acc;
+}
+
 // ---------------------------------------------------------------------------
 // Polynomial preconditioner steps (dof 4; the block products summed in the
 // inner precision, the factor's vector update in FP64 registers).
@@ -811,6 +851,30 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
             return SVFSI_OK;
         }
     }
+    // dof 3 in FP64: the dof-4 lane mapping (k_bspmv3r, H = 4) measured
+    // fastest: 331 us on the 40^3 tet10 cube matrix against 373 us value
+    // streaming and 458 us block per lane, 182 against 187 us on the 80^3
+    // tet4 one.  FP32 keeps block per lane (169 / 83 us against 169 / 99).
+    // SVFSI_SPMV3_RH = 0 / 4 / 8 overrides (0: value streaming).
+    static const int rhEnv = [] {
+        const char* e = getenv("SVFSI_SPMV3_RH");
+        if (!e) return -1;
+        const int v = atoi(e);
+        return (v == 4 || v == 8) ? v : 0;
+    }();
+    const int rh = rhEnv >= 0 ? rhEnv : (std::is_same<VT, double>::value ? 4 : 0);
+    if (dof == 3 && rh && !dinv) {
+        const bool nt = (size_t)L.nnz * 9 * sizeof(VT) > ((size_t)128 << 20);
+        if constexpr (std::is_same<VT, XT>::value) {
+            const unsigned g = (unsigned)(((size_t)L.nNo * 4 * rh + 255) / 256);
+#define SV_R3(HH, N) SV_LAUNCH(ps, (k_bspmv3r<VT, HH, N>), dim3(g), dim3(256), 0, s, L.nNo, rp, cp, K, U, KU, stop)
+            if (rh == 4) { if (nt) SV_R3(4, true); else SV_R3(4, false); }
+            else { if (nt) SV_R3(8, true); else SV_R3(8, false); }
+#undef SV_R3
+            SV_HIP(hipGetLastError());
+            return SVFSI_OK;
+        }
+    }
     // SVFSI_SPMV3_ROW = 1 puts FP64 on the block-per-lane layout as well
     static const bool row64 = [] {
         const char* e = getenv("SVFSI_SPMV3_ROW");
@@ -818,9 +882,8 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
     }();
     if (dof == 3 && (std::is_same<VT, float>::value || row64)) {
         // FP32: whole blocks per lane (fewer, wider loads; FP32 sums).  FP64
-        // keeps the value-streaming layout below: its 72-byte blocks per lane
-        // spread each load instruction over 4.6 KB and measured slower
-        // (548 vs 373 us on the 40^3 tet10 cube's matrix).
+        // uses k_bspmv3r above: its 72-byte blocks per lane spread each load
+        // instruction over 4.6 KB and measured slower (458 us at 32 lanes).
         const bool nt = (size_t)L.nnz * 9 * sizeof(VT) > ((size_t)128 << 20);
         const int l3 = spmv3_lanes(L);
         const unsigned g3 = (unsigned)(((size_t)L.nNo * l3 + 255) / 256);
