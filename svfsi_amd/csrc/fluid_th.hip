@@ -72,31 +72,40 @@ __device__ void th_gnn(const double (*xl)[3], const double* Nxi, double (*Nx)[3]
 
 // GETGNNxx, TET10 (svFSI/NN.f:1285-1295): constant d2N/dxi_i dxi_j, order
 // (11, 22, 33, 12, 23, 13).
+// (a table in constant memory: indexed by a run-time node, a local array
+// would be materialised in registers)
+__constant__ double kTet10Xixi[10][6] = {{4, 0, 0, 0, 0, 0}, {0, 4, 0, 0, 0, 0}, {0, 0, 4, 0, 0, 0},
+                                         {4, 4, 4, 4, 4, 4}, {0, 0, 0, 4, 0, 0}, {0, 0, 0, 0, 4, 0},
+                                         {0, 0, 0, 0, 0, 4}, {-8, 0, 0, -4, 0, -4}, {0, -8, 0, -4, -4, 0},
+                                         {0, 0, -8, 0, -4, -4}};
+
 __device__ __forceinline__ double tet10_xixi(int a, int i)
 {
-    constexpr double t[10][6] = {{4, 0, 0, 0, 0, 0}, {0, 4, 0, 0, 0, 0}, {0, 0, 4, 0, 0, 0},
-                                 {4, 4, 4, 4, 4, 4}, {0, 0, 0, 4, 0, 0}, {0, 0, 0, 0, 4, 0},
-                                 {0, 0, 0, 0, 0, 4}, {-8, 0, 0, -4, 0, -4}, {0, -8, 0, -4, -4, 0},
-                                 {0, 0, -8, 0, -4, -4}};
-    return t[a][i];
+    return kTet10Xixi[a][i];
 }
 
 // GNNxx (svFSI/NN.f:1568-1678), 3-D: the 6 x 6 chain-rule system K Nxx(a) =
 // Nxi2(a) - Nx(a) . xXi2 for the physical second derivatives (order xx, yy,
 // zz, xy, yz, xz), solved by LU with partial pivoting (the algorithm of
 // the DGESV call at NN.f:1672) for the ten right-hand sides.
-__device__ void th_gnnxx(const double (*xl)[3], const double* Nxi, const double (*Nx)[3], double (*Nxx)[6])
+__device__ void th_gnnxx_lu(const double (*xl)[3], const double* Nxi, double* LU, int* pivOut, double* X2)
 {
     double xXi[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double xXi2[3][6];
+#pragma unroll
     for (int k = 0; k < 3; ++k)
+#pragma unroll
         for (int i = 0; i < 6; ++i) xXi2[k][i] = 0.0;
     for (int a = 0; a < kNW; ++a)
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
+#pragma unroll
             for (int i = 0; i < 3; ++i) xXi[k][i] += xl[a][k] * Nxi[a * 3 + i];
+#pragma unroll
             for (int i = 0; i < 6; ++i) xXi2[k][i] += xl[a][k] * tet10_xixi(a, i);
         }
     double K[6][6];
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         K[i][0] = xXi[0][i] * xXi[0][i];
         K[i][1] = xXi[1][i] * xXi[1][i];
@@ -106,6 +115,7 @@ __device__ void th_gnnxx(const double (*xl)[3], const double* Nxi, const double 
         K[i][5] = 2.0 * xXi[0][i] * xXi[2][i];
     }
     constexpr int pr[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+#pragma unroll
     for (int r = 0; r < 3; ++r) {
         const int i = pr[r][0], j = pr[r][1];
         K[3 + r][0] = xXi[0][i] * xXi[0][j];
@@ -115,41 +125,76 @@ __device__ void th_gnnxx(const double (*xl)[3], const double* Nxi, const double 
         K[3 + r][4] = xXi[1][i] * xXi[2][j] + xXi[1][j] * xXi[2][i];
         K[3 + r][5] = xXi[0][i] * xXi[2][j] + xXi[0][j] * xXi[2][i];
     }
-    for (int a = 0; a < kNW; ++a)
-        for (int i = 0; i < 6; ++i)
-            Nxx[a][i] = tet10_xixi(a, i) - Nx[a][0] * xXi2[0][i] - Nx[a][1] * xXi2[1][i] - Nx[a][2] * xXi2[2][i];
-    int piv[6];
+    // LU with partial pivoting, row swaps over compile-time indices (a
+    // lane-dependent index would move the matrix to scratch memory)
+#pragma unroll
     for (int k = 0; k < 6; ++k) {
         int p = k;
+        double best = fabs(K[k][k]);
+#pragma unroll
         for (int i = k + 1; i < 6; ++i)
-            if (fabs(K[i][k]) > fabs(K[p][k])) p = i;
-        piv[k] = p;
-        if (p != k)
-            for (int j = 0; j < 6; ++j) {
-                const double t = K[k][j];
-                K[k][j] = K[p][j];
-                K[p][j] = t;
+            if (fabs(K[i][k]) > best) {
+                best = fabs(K[i][k]);
+                p = i;
             }
+        pivOut[k] = p;
+#pragma unroll
+        for (int i = k + 1; i < 6; ++i)
+            if (i == p)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
+                    const double tt = K[k][j];
+                    K[k][j] = K[i][j];
+                    K[i][j] = tt;
+                }
+#pragma unroll
         for (int i = k + 1; i < 6; ++i) {
             K[i][k] /= K[k][k];
+#pragma unroll
             for (int j = k + 1; j < 6; ++j) K[i][j] -= K[i][k] * K[k][j];
         }
     }
-    for (int a = 0; a < kNW; ++a) {
-        double* b = Nxx[a];
-        for (int k = 0; k < 6; ++k)
-            if (piv[k] != k) {
-                const double t = b[k];
-                b[k] = b[piv[k]];
-                b[piv[k]] = t;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) LU[i * 6 + j] = K[i][j];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) X2[k * 6 + i] = xXi2[k][i];
+}
+
+// one right-hand side of GNNxx: shape function a, with the point's LU
+// factors, pivots and x_xixi from th_gnnxx_lu
+__device__ void th_gnnxx_rhs(int a, const double* Nxa, const double* LU, const int* piv, const double* X2,
+                             double* out)
+{
+    double b[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) b[i] = tet10_xixi(a, i) - Nxa[0] * X2[i] - Nxa[1] * X2[6 + i] - Nxa[2] * X2[12 + i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int pk = piv[k];
+#pragma unroll
+        for (int i = k + 1; i < 6; ++i)
+            if (i == pk) {
+                const double tt = b[k];
+                b[k] = b[i];
+                b[i] = tt;
             }
-        for (int i = 1; i < 6; ++i)
-            for (int j = 0; j < i; ++j) b[i] -= K[i][j] * b[j];
-        for (int i = 5; i >= 0; --i) {
-            for (int j = i + 1; j < 6; ++j) b[i] -= K[i][j] * b[j];
-            b[i] /= K[i][i];
-        }
     }
+#pragma unroll
+    for (int i = 1; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < i; ++j) b[i] -= LU[i * 6 + j] * b[j];
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+#pragma unroll
+        for (int j = i + 1; j < 6; ++j) b[i] -= LU[i * 6 + j] * b[j];
+        b[i] /= LU[i * 6 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) out[i] = b[i];
 }
 
 // GETVISCOSITY (svFSI/FLUID.f:1690-1739) at the shear rate gam, then
@@ -200,6 +245,8 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
     __shared__ double sKs[kNG1 + kNG2][3][3];
     __shared__ double sNqx[kNQ][3];
     __shared__ int sNode[kNW];
+    __shared__ double sLU[kNG1][36], sX2[kNG1][18];
+    __shared__ int sPiv[kNG1][6];
     if (t < kNW) {
         const int n = ien[(size_t)p * kNW + t];
         sNode[t] = n;
@@ -213,13 +260,9 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
     if (t < kNG1) {
         // momentum point t: the tet10 map gives Nwx, Jac and ksix, GNNxx Nwxx
         const double* r = rule + (size_t)t * kThRec;
-        double Nx[kNW][3], Nxx[kNW][6], ks[3][3], J;
-        th_gnn<kNW>(sx, r + 11, Nx, J, ks);
-        th_gnnxx(sx, r + 11, Nx, Nxx);
-        for (int a = 0; a < kNW; ++a)
-            for (int i = 0; i < 3; ++i) sNwx[t][a][i] = Nx[a][i];
-        for (int a = 0; a < kNW; ++a)
-            for (int i = 0; i < 6; ++i) sNwxx[t][a][i] = Nxx[a][i];
+        double ks[3][3], J;
+        th_gnn<kNW>(sx, r + 11, sNwx[t], J, ks);
+        th_gnnxx_lu(sx, r + 11, sLU[t], sPiv[t], sX2[t]);
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) sKs[t][i][j] = ks[i][j];
         sW[t] = r[0] * J;
@@ -232,14 +275,11 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
         if (t < kNG1 + kNG2) {
             const int g = t - kNG1;
             const double* r = rule + (size_t)t * kThRec;
-            double Nx[kNW][3];
-            th_gnn<kNW>(sx, r + 11, Nx, J, ks);
+            th_gnn<kNW>(sx, r + 11, sNwx[t], J, ks);
             if (g == 0) {
                 double Nq[kNQ][3];
                 th_gnn<kNQ>(sx, Nqxi, Nq, J, ks);
             }
-            for (int a = 0; a < kNW; ++a)
-                for (int i = 0; i < 3; ++i) sNwx[t][a][i] = Nx[a][i];
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j) sKs[t][i][j] = ks[i][j];
             sW[t] = r[0] * J;
@@ -249,6 +289,12 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
             for (int a = 0; a < kNQ; ++a)
                 for (int i = 0; i < 3; ++i) sNqx[a][i] = Nq[a][i];
         }
+    }
+    __syncthreads();
+    // GNNxx right-hand sides: (point, shape function) pairs over the threads
+    for (int q = t; q < kNG1 * kNW; q += 128) {
+        const int g = q / kNW, a = q - (q / kNW) * kNW;
+        th_gnnxx_rhs(a, sNwx[g][a], sLU[g], sPiv[g], sX2[g], sNwxx[g][a]);
     }
     __syncthreads();
 
@@ -283,6 +329,9 @@ __global__ __launch_bounds__(128) void k_asm_fluid_th(int begin, const int* __re
                 for (int j = 0; j < 3; ++j) uxx[i][c][j] = 0.0;
             }
         }
+        // node loop kept rolled: unrolled, its loads filled the register
+        // file (one wave per SIMD); rolled, the kernel runs two
+#pragma unroll 1
         for (int n = 0; n < kNW; ++n) {
             for (int c = 0; c < 3; ++c) {
                 ud[c] += Nw[n] * (sa[n][c] - 0.0);
