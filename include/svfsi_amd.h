@@ -181,7 +181,9 @@ int fsils_gpu_solve_dev(void* lhs, FSILS_lsType* ls, int dof, double* dRi,
  * restart length ls->RI.sD (30 when 0), whose right preconditioner works on
  * an FP32 copy of the scaled matrix: a fixed GMRES polynomial of the
  * (block-Jacobi scaled) operator, or one cycle of an inner FP32 GMRES(30)
- * stopped at a relative reduction of 0.1.  LS_type GMRES, CG and BICGS all
+ * stopped at a relative reduction of 0.1 (chosen, without block Jacobi and
+ * stopped at 0.01, when unmasked rows of the scaled matrix have a zero
+ * diagonal: a saddle-point system such as the Taylor-Hood fluid).  LS_type GMRES, CG and BICGS all
  * select this one solver (it replaces each of them; their defaults of
  * fsils_gpu_ls_create still apply); LS_TYPE_NS is refused
  * (SVFSI_ERR_NOTSUP).  The reported residual is the FP64 one in the

@@ -1733,7 +1733,6 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     // outer Krylov dimension: the caller's restart length (dimKry), 30 when
     // none is set (CG requests); SVFSI_FGMRES_M overrides
     const int mo = env_int("SVFSI_FGMRES_M", ls.sD > 0 ? std::min(ls.sD, kMaxSD) : 30);
-    const double tin = env_dbl("SVFSI_FGMRES_INNER_TOL", 0.1);
     const int cgsPasses = env_int("SVFSI_OUTER_CGS", 2) == 1 ? 1 : 2;
     const int mi = env_int("SVFSI_FGMRES_INNER_M", 30);     // inner Krylov dimension
     if (mo > kMaxSD || mi > kMaxSD) {
@@ -1785,6 +1784,9 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     SV_TRY(w.ostate.ensure(256));
     SV_TRY(count_zero_diag(L, dof, K, w.W.as<double>(), w.ostate.as<int>(), zeroDiag, s));
     const bool saddle = zeroDiag > 0;
+    // inner GMRES tolerance: 0.1; 0.01 on saddle-point systems (the P2P1
+    // channel at 10^3: 227 outer steps and 138 ms against 371 and 197 ms)
+    const double tin = env_dbl("SVFSI_FGMRES_INNER_TOL", saddle ? 0.01 : 0.1);
     const bool polyWanted =
         !useAmg && (dof == 3 || dof == 4) && env_int("SVFSI_INNER_POLY", saddle ? 0 : 1) != 0;
     const bool bj =
