@@ -224,6 +224,7 @@ int build_hierarchy(const Lhs& L, int dof, int cap, AmgHier& H)
         SV_TRY(upload(l->A.diagPtr, c.diag));
         l->A.hRowPtr = std::move(c.rp);
         l->A.hColPtr = std::move(c.cp);
+        SV_TRY(build_row_tiles(l->A));
         l->pat = &l->A;
         SV_TRY(l->val.ensure((size_t)l->A.nnz * dof * dof * sizeof(float)));
         SV_TRY(alloc_vectors(*l, dof));

@@ -457,7 +457,8 @@ int fsils_gpu_lhs_create(void** lhs, int gnNo, int nNo, int nnz, const int* gNod
     }
     int rc = SVFSI_OK;
     if ((rc = upload(L->rowPtr, L->hRowPtr)) || (rc = upload(L->colPtr, L->hColPtr)) ||
-        (rc = upload(L->diagPtr, diag)) || (rc = upload(L->blockRow, brow)) || (rc = upload(L->trans, trans))) {
+        (rc = upload(L->diagPtr, diag)) || (rc = upload(L->blockRow, brow)) || (rc = upload(L->trans, trans)) ||
+        (rc = build_row_tiles(*L))) {
         delete L;
         return rc;
     }

@@ -173,6 +173,11 @@ struct Lhs {
     std::vector<Face> faces;
     int facesVersion = 0;
     bool symmetric = false; // every block (i,j) has a (j,i) partner: trans[] valid
+    // Row tiles of the CSR-stream block products: tile b is rows
+    // tiles[t][b] .. tiles[t][b+1]-1, at most 256 (t = 0) or 512 (t = 1)
+    // blocks together; nTiles[t] = 0 when some row holds more
+    DBuf tiles[2];
+    int nTiles[2] = {0, 0};
     CplSet cpl;
     Work ws;
 };
@@ -315,6 +320,11 @@ int build_mask_into(const Lhs& L, int dof, const int* incL, double* mask, void* 
 int launch_fill(size_t n, double v, double* x, hipStream_t s);
 // lanes per block row of the dof-3 SpMV kernels for this pattern
 int spmv3_lanes(const Lhs& L);
+// the CSR-stream row tiles of L (from hRowPtr; rows must be non-empty)
+int build_row_tiles(Lhs& L);
+// the row tiles the dof-3/4 products use for this pattern and precision, or
+// nullptr (then the row-per-lane-group kernels run); T: blocks per tile
+const int* stream_tiles(const Lhs& L, int dof, bool fp64, int& T, int& nT);
 int launch_scale_vec(size_t n, const double* W, double* R, hipStream_t s);
 
 int cg_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K,

@@ -11,6 +11,7 @@
 // reproducible run to run.
 #include "internal.hpp"
 #include "devutil.hpp"
+#include "stream.hpp"
 
 #include <chrono>
 #include <cmath>
@@ -204,6 +205,31 @@ __global__ __launch_bounds__(256) void k_cg_spmv_dot3r(int nNo, const int* __res
             contrib += p[(size_t)row * 3 + i] * acc;
         }
     }
+    const double sb = block_sum<256>(contrib, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sb;
+}
+
+// The same on the row tiles (stream_tile): workgroups stride over the tiles,
+// one partial of <P, K P> per workgroup.
+template <int T>
+__global__ __launch_bounds__(256) void k_cg_sspmv_dot3(int nTiles, const int* __restrict__ tile,
+                                                       const int* __restrict__ rowPtr,
+                                                       const int* __restrict__ colPtr,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ p, double* __restrict__ kp,
+                                                       double* __restrict__ partial,
+                                                       const CgState* __restrict__ st)
+{
+    if (st->done) return;
+    __shared__ double lds[4];
+    double contrib = 0.0;
+    for (int b = blockIdx.x; b < nTiles; b += gridDim.x)
+        stream_tile<double, 3, T, false>(b, tile, rowPtr, colPtr, val, p, true,
+                                         [&](int row, int i, double v, double, double, double, double) {
+                                             const size_t q = (size_t)row * 3 + i;
+                                             kp[q] = v;
+                                             contrib += p[q] * v;
+                                         });
     const double sb = block_sum<256>(contrib, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = sb;
 }
@@ -404,6 +430,10 @@ int cg_core(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hi
     const bool vs = dof <= 3;  // dof 2 value streaming, dof 3 row lanes: 16 lanes per row either way
     const int rowsPerBlock = vs ? 256 / 16 : 4 * (kWave / (dof * dof));
     int gs = (L.nNo + rowsPerBlock - 1) / rowsPerBlock;
+    // dof 3: the CSR-stream product when the pattern has row tiles
+    int sT = 0, nT = 0;
+    const int* tl = dof == 3 ? stream_tiles(L, 3, true, sT, nT) : nullptr;
+    if (tl) gs = nT;
     if (gs > 1024) gs = 1024;
     if (gs < 1) gs = 1;
     unsigned gv = vec_grid(n);
@@ -449,7 +479,14 @@ int cg_core(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hi
             const int par = it & 1;
             switch (dof) {
             case 2: hipLaunchKernelGGL((k_cg_spmv_dot_vs<2, 16>), dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
-            case 3: hipLaunchKernelGGL(k_cg_spmv_dot3r, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
+            case 3:
+                if (tl && sT == 512)
+                    hipLaunchKernelGGL(k_cg_sspmv_dot3<512>, dim3(gs), dim3(256), 0, cs, nT, tl, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st);
+                else if (tl)
+                    hipLaunchKernelGGL(k_cg_sspmv_dot3<256>, dim3(gs), dim3(256), 0, cs, nT, tl, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st);
+                else
+                    hipLaunchKernelGGL(k_cg_spmv_dot3r, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st);
+                break;
             default: hipLaunchKernelGGL(k_cg_spmv_dot<4>, dim3(gs), dim3(256), 0, cs, L.nNo, L.rowPtr.as<int>(), L.colPtr.as<int>(), K, (const double*)P, KP, part1, (const CgState*)st); break;
             }
             hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(256), 0, cs, n, X, R, (const double*)P, (const double*)KP,
@@ -1866,7 +1903,12 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     const double* coefD = nullptr;
     bool usePoly = polyWanted;
     if (usePoly) {
-        const int d = env_int("SVFSI_POLY_DEG", 30);
+        // degree 25 when the inner matrix fits half the Infinity Cache, else
+        // 30 (measured, ms per Newton step: 200k-tet cylinder 6.7 at 25
+        // against 7.2 at 30; 3.2M-tet cylinder 227 at 30 against 243 at 25;
+        // LELAS 20^3 3.32 against 3.34)
+        const bool cached = (size_t)L.nnz * dof * dof * sizeof(IT) <= ((size_t)128 << 20);
+        const int d = env_int("SVFSI_POLY_DEG", cached ? 25 : 30);
         if (d < 2 || d > kMaxSD) {
             set_error("SVFSI_POLY_DEG must be in 2..%d", kMaxSD);
             return SVFSI_ERR_ARG;

@@ -11,6 +11,7 @@
 // vector U keeps the L2.
 #include "internal.hpp"
 #include "devutil.hpp"
+#include "stream.hpp"
 
 #include <hip/hip_ext.h>
 
@@ -199,6 +200,95 @@ int spmv3_lanes(const Lhs& L)
     // lanes against 108 at 8, 135 at 4 and 110 at 32; tet10 cube matrix (28
     // blocks per row) 169 us at 32 against 190 at 16 and 210 at 8
     return (size_t)L.nnz > (size_t)20 * L.nNo ? 32 : 16;
+}
+
+// Greedy row tiles: a tile takes rows while its blocks stay within T (and its
+// rows within T, so that the row offsets fit the workgroup's LDS table).
+static int row_tiles(const std::vector<int>& rp, int nNo, int T, std::vector<int>& out)
+{
+    out.clear();
+    out.push_back(0);
+    int r = 0;
+    while (r < nNo) {
+        if (rp[r + 1] - rp[r] > T) {
+            out.clear();
+            return 0;
+        }
+        int r1 = r + 1;
+        while (r1 < nNo && r1 - r < T && rp[r1 + 1] - rp[r] <= T) ++r1;
+        out.push_back(r1);
+        r = r1;
+    }
+    return (int)out.size() - 1;
+}
+
+int build_row_tiles(Lhs& L)
+{
+    static const int Ts[2] = {256, 512};
+    for (int t = 0; t < 2; ++t) {
+        std::vector<int> tl;
+        L.nTiles[t] = 0;
+        if ((int)L.hRowPtr.size() != L.nNo + 1 || L.nNo == 0) continue;
+        const int n = row_tiles(L.hRowPtr, L.nNo, Ts[t], tl);
+        if (n > 0) {
+            SV_TRY(L.tiles[t].ensure(tl.size() * sizeof(int)));
+            SV_HIP(hipMemcpy(L.tiles[t].p, tl.data(), tl.size() * sizeof(int), hipMemcpyHostToDevice));
+            L.nTiles[t] = n;
+        }
+    }
+    return SVFSI_OK;
+}
+
+// CSR-stream products (stream_tile) measured against the row-per-lane-group
+// kernels, us per SpMV (tet10 cube 40^3 / tet4 cube 80^3 for dof 3, the
+// 200k / 3.2M-tet cylinders for dof 4):
+//   dof 3 FP32  111 / 58 (T = 512) against 178 / 86
+//   dof 3 FP64  203 / 111 (T = 512) against 331 / 182
+//   dof 4 FP32  6.2 (T = 256) / 107 (T = 512) against 7.2 / 120
+//   dof 4 FP64  13.1 / 224 (T = 512) against 13.2 / 239
+// FP64 wants two blocks in flight per lane (T = 512: 16.2 us at T = 256 on
+// the small cylinder); FP32 prefers more, smaller tiles below 2M blocks.
+// FP64 dof 4 keeps k_bspmv4 unless forced: it gains little there, and the
+// FP64 GMRESV baseline of the fluid keeps its summation order (a GMRES of
+// thousands of iterations moves its count by a few per cent with the
+// rounding of the products).
+// SVFSI_SPMV_STREAM = 0 selects the older kernels, 256 or 512 forces T.
+const int* stream_tiles(const Lhs& L, int dof, bool fp64, int& T, int& nT)
+{
+    static const int env = [] {
+        const char* e = getenv("SVFSI_SPMV_STREAM");
+        return e ? atoi(e) : -1;
+    }();
+    if (env == 0 || (dof != 3 && dof != 4)) return nullptr;
+    if (fp64 && dof == 4 && env < 0) return nullptr;
+    int t = (fp64 || (size_t)L.nnz >= ((size_t)2 << 20)) ? 1 : 0;
+    if (env == 256) t = 0;
+    else if (env == 512) t = 1;
+    if (!L.nTiles[t]) t ^= 1;
+    if (!L.nTiles[t]) return nullptr;
+    T = t ? 512 : 256;
+    nT = L.nTiles[t];
+    return L.tiles[t].as<int>();
+}
+
+// K x on the row tiles (optionally D^-1 K x with per-node inverse diagonal
+// blocks, row-major)
+template <typename VT, int DOF, int T>
+__global__ __launch_bounds__(256) void k_sspmv(const int* __restrict__ tile, const int* __restrict__ rowPtr,
+                                               const int* __restrict__ colPtr, const VT* __restrict__ val,
+                                               const VT* __restrict__ x, VT* __restrict__ y,
+                                               const int* __restrict__ stop, const VT* __restrict__ dinv)
+{
+    if (stop && *stop) return;
+    stream_tile<VT, DOF, T, false>(blockIdx.x, tile, rowPtr, colPtr, val, x, true,
+                                   [&](int row, int i, VT v, VT s0, VT s1, VT s2, VT s3) {
+                                       if (dinv) {
+                                           const VT* d = dinv + (size_t)row * DOF * DOF + i * DOF;
+                                           v = d[0] * s0 + d[1] * s1 + d[2] * s2;
+                                           if constexpr (DOF == 4) v += d[3] * s3;
+                                       }
+                                       y[(size_t)row * DOF + i] = v;
+                                   });
 }
 
 // dof = 4 specialisation: 4 lanes per block row, lane i owns output
@@ -407,6 +497,42 @@ __global__ __launch_bounds__(256) void k_poly3(int nNo, const int* __restrict__ 
     }
 }
 
+// The factor steps of k_poly4 / k_poly3 on the row tiles (stream_tile),
+// sums in the inner precision IT.
+template <typename IT, int DOF, int MODE, int T>
+__global__ __launch_bounds__(256) void k_spoly(const int* __restrict__ tile, const int* __restrict__ rowPtr,
+                                               const int* __restrict__ colPtr, const IT* __restrict__ val,
+                                               const IT* __restrict__ dinv, const IT* __restrict__ x,
+                                               const IT* __restrict__ prodOld, IT* __restrict__ w,
+                                               IT* __restrict__ y, IT* __restrict__ out,
+                                               const double* __restrict__ coef, int step,
+                                               const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const double c1 = coef[2 * step], c2 = coef[2 * step + 1];
+    const bool skipSpmv = (MODE == 2) && c2 == 0.0;
+    stream_tile<IT, DOF, T, false>(
+        blockIdx.x, tile, rowPtr, colPtr, val, x, !skipSpmv, [&](int row, int i, IT a, IT s0, IT s1, IT s2, IT s3) {
+            if (dinv && !skipSpmv) {
+                const IT* d = dinv + (size_t)row * DOF * DOF + i * DOF;
+                a = d[0] * s0 + d[1] * s1 + d[2] * s2;
+                if constexpr (DOF == 4) a += d[3] * s3;
+            }
+            const size_t q = (size_t)row * DOF + i;
+            if constexpr (MODE == 0) {
+                const double pv = (double)x[q];
+                y[q] = (IT)((double)y[q] + c1 * pv);
+                out[q] = (IT)(pv - c1 * a);
+            } else if constexpr (MODE == 1) {
+                const double pv = (double)x[q];
+                w[q] = a;
+                y[q] = (IT)((double)y[q] + c1 * pv - c2 * a);
+            } else {
+                out[q] = (IT)((double)prodOld[q] - c1 * (double)x[q] + c2 * a);
+            }
+        });
+}
+
 // y += c1 prod (the last, linear factor: no product needed)
 template <typename IT>
 __global__ __launch_bounds__(256) void k_poly_last(size_t n, const IT* __restrict__ prod, IT* __restrict__ y,
@@ -438,6 +564,30 @@ int launch_poly(const Lhs& L, int dof, int mode, const IT* K, const IT* dinv, co
     const double base = (double)L.nnz * dd * sz + 4.0 * L.nnz + 4.0 * (L.nNo + 1) + (dinv ? L.nNo * dd * sz : 0.0);
     const double vec = (double)L.nNo * dof * sz;
     ProfScope ps(std::is_same<IT, float>::value ? 1 : 0, base + vec * (mode == 0 ? 4 : mode == 1 ? 4 : 3));
+    int sT = 0, nT = 0;
+    if (const int* tl = stream_tiles(L, dof, std::is_same<IT, double>::value, sT, nT)) {
+#define SV_SPOLY(D, M, TT)                                                                                     \
+    SV_LAUNCH(ps, (k_spoly<IT, D, M, TT>), dim3((unsigned)nT), dim3(256), 0, s, tl, rp, cp, K, dinv, x, prodOld, w, y, \
+              out, coef, step, stop)
+#define SV_SPOLYT(D, M)                                                                                        \
+    do {                                                                                                       \
+        if (sT == 512) SV_SPOLY(D, M, 512);                                                                    \
+        else SV_SPOLY(D, M, 256);                                                                              \
+    } while (0)
+        if (dof == 4) {
+            if (mode == 0) SV_SPOLYT(4, 0);
+            else if (mode == 1) SV_SPOLYT(4, 1);
+            else SV_SPOLYT(4, 2);
+        } else {
+            if (mode == 0) SV_SPOLYT(3, 0);
+            else if (mode == 1) SV_SPOLYT(3, 1);
+            else SV_SPOLYT(3, 2);
+        }
+#undef SV_SPOLYT
+#undef SV_SPOLY
+        SV_HIP(hipGetLastError());
+        return SVFSI_OK;
+    }
     const int l3 = spmv3_lanes(L);
     const unsigned g3 = (unsigned)(((size_t)L.nNo * l3 + 255) / 256);
 #define SV_POLY3L(M, N, LL)                                                                                    \
@@ -819,6 +969,17 @@ static int spmv_dispatch(const Lhs& L, int dof, const VT* K, const XT* U, XT* KU
     const int* cp = L.colPtr.as<int>();
     const int xm = xcd_map();
     if constexpr (std::is_same<VT, XT>::value) {
+        int sT = 0, nT = 0;
+        const int* tl = stream_tiles(L, dof, std::is_same<VT, double>::value, sT, nT);
+        if (tl) {
+#define SV_SSPMV(D, TT)                                                                                        \
+    SV_LAUNCH(ps, (k_sspmv<VT, D, TT>), dim3((unsigned)nT), dim3(256), 0, s, tl, rp, cp, K, U, KU, stop, dinv)
+            if (dof == 4) { if (sT == 512) SV_SSPMV(4, 512); else SV_SSPMV(4, 256); }
+            else { if (sT == 512) SV_SSPMV(3, 512); else SV_SSPMV(3, 256); }
+#undef SV_SSPMV
+            SV_HIP(hipGetLastError());
+            return SVFSI_OK;
+        }
         if (dof == 4) {
             // lanes per output component (tuning knob SVFSI_SPMV_H = 1, 2 or 4;
             // 4 measured fastest on the 200k-tet cylinder)
