@@ -553,7 +553,10 @@ def bench_ns(args, prob, t0) -> int:
     full = max(1, min(cnt.value, lsp.CG.itr))  # launches queued after convergence exit at once
     bpl = byt.value / max(cnt.value, 1)
     gbs = bpl * full / (tot.value * 1e-3) / 1e9 if cnt.value else None
-    kname = "k_cgs_sp"
+    # the row-tile kernel from 2M blocks on (512-block tiles), the 8-lanes-per-row one below
+    ct = os.environ.get("SVFSI_CGS_TILE")
+    ct = int(ct) if ct in ("0", "512", "1024") else (512 if nnzb >= (2 << 20) else 0)
+    kname = f"k_cgs_sp_t<{ct}>" if ct else "k_cgs_sp<8>"
     roof = None if gbs is None else {
         "kernel": f"{kname} (Schur-complement product SP = L P - Gt GP with its partial dot)", "bound": "hbm",
         "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),

@@ -174,10 +174,10 @@ struct Lhs {
     int facesVersion = 0;
     bool symmetric = false; // every block (i,j) has a (j,i) partner: trans[] valid
     // Row tiles of the CSR-stream block products: tile b is rows
-    // tiles[t][b] .. tiles[t][b+1]-1, at most 256 (t = 0) or 512 (t = 1)
-    // blocks together; nTiles[t] = 0 when some row holds more
-    DBuf tiles[2];
-    int nTiles[2] = {0, 0};
+    // tiles[t][b] .. tiles[t][b+1]-1, at most 256 (t = 0), 512 (t = 1) or
+    // 1024 (t = 2) blocks together; nTiles[t] = 0 when some row holds more
+    DBuf tiles[3];
+    int nTiles[3] = {0, 0, 0};
     CplSet cpl;
     Work ws;
 };

@@ -546,6 +546,13 @@ int cg_core(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, double* R, hi
     return SVFSI_OK;
 }
 
+// an integer tuning knob from the environment, read at each call
+static int env_int(const char* k, int d)
+{
+    const char* e = getenv(k);
+    return e ? atoi(e) : d;
+}
+
 // ---------------------------------------------------------------------------
 // CGRAD_SCHUR (svFSILS/CGRAD.f:51-123): CG on S P = L P - Gt (G P), the
 // pressure Schur complement of the NS solver (Gt = -G^T blockwise, so
@@ -659,6 +666,192 @@ __global__ __launch_bounds__(256) void k_cgs_pgp(int nNo, const int* __restrict_
     }
 }
 
+// The same two row passes on the row tiles of the pattern (tiles of whole
+// rows with at most T blocks, one 256-lane workgroup per tile, workgroups
+// striding over the tiles): every lane loads T/256 of the tile's blocks with
+// all loads issued before the first product, the per-block products go to
+// LDS, and 8 lanes per row sum them with the strides and shuffles of the
+// kernels above.  The rows' L P and Gt GP sums stay separate, as there.
+constexpr int kCgsS = 8;
+
+template <int T>
+__global__ __launch_bounds__(256) void k_cgs_sp_t(int nTiles, const int* __restrict__ tile,
+                                                  const int* __restrict__ rowPtr, const int* __restrict__ colPtr,
+                                                  const double* __restrict__ Lm, const double* __restrict__ Gt,
+                                                  const double* __restrict__ P, const double* __restrict__ GP,
+                                                  double* __restrict__ SP, double* __restrict__ part,
+                                                  const CgState* __restrict__ st)
+{
+    if (st->done) return;
+    constexpr int U = T / 256;
+    __shared__ double sA[T], sD[T];
+    __shared__ int sR[T + 1];
+    __shared__ double lds[4];
+    const int tid = threadIdx.x;
+    const int g = tid / kCgsS, l = tid % kCgsS;
+    double contrib = 0.0;
+    for (int b = blockIdx.x; b < nTiles; b += gridDim.x) {
+        const int r0 = tile[b], r1 = tile[b + 1];
+        const int k0 = rowPtr[r0];
+        const int nb = rowPtr[r1] - k0, nr = r1 - r0;
+        for (int i = tid; i <= nr; i += 256) sR[i] = rowPtr[r0 + i] - k0;
+        int c[U];
+        double lv[U], g0[U], g1[U], g2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = u * 256 + tid;
+            c[u] = j < nb ? colPtr[k0 + j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = u * 256 + tid;
+            if (j < nb) {
+                const size_t k = (size_t)(k0 + j);
+                lv[u] = Lm[k];
+                g0[u] = Gt[k * 3];
+                g1[u] = Gt[k * 3 + 1];
+                g2[u] = Gt[k * 3 + 2];
+            } else {
+                lv[u] = g0[u] = g1[u] = g2[u] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = u * 256 + tid;
+            const double pv = P[c[u]];
+            const double* q = GP + (size_t)c[u] * 3;
+            const double q0 = q[0], q1 = q[1], q2 = q[2];
+            if (j < nb) {
+                sA[j] = lv[u] * pv;
+                sD[j] = g0[u] * q0 + g1[u] * q1 + g2[u] * q2;
+            }
+        }
+        __syncthreads();
+        for (int rr = g; rr < nr; rr += 256 / kCgsS) {
+            double a = 0.0, d = 0.0;
+            const int ke = sR[rr + 1];
+            for (int j = sR[rr] + l; j < ke; j += kCgsS) {
+                a += sA[j];
+                d += sD[j];
+            }
+#pragma unroll
+            for (int o = kCgsS / 2; o > 0; o >>= 1) {
+                a += __shfl_xor(a, o, kWave);
+                d += __shfl_xor(d, o, kWave);
+            }
+            if (l == 0) {
+                const int i = r0 + rr;
+                const double sp = a + (-1.0) * d;
+                SP[i] = sp;
+                contrib += P[i] * sp;
+            }
+        }
+        __syncthreads();
+    }
+    const double sb = block_sum<256>(contrib, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = sb;
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_cgs_pgp_t(int nTiles, const int* __restrict__ tile,
+                                                   const int* __restrict__ rowPtr, const int* __restrict__ colPtr,
+                                                   const double* __restrict__ G, const double* __restrict__ Pin,
+                                                   const double* __restrict__ R, const double* __restrict__ part2,
+                                                   int m2, double* __restrict__ Pout, double* __restrict__ GP,
+                                                   CgState* __restrict__ st, int par, int first)
+{
+    constexpr int U = T / 256;
+    __shared__ double s0[T], s1[T], s2[T];
+    __shared__ int sR[T + 1];
+    __shared__ double lds[4];
+    __shared__ int done_s;
+    if (threadIdx.x == 0) done_s = *(volatile int*)&st->done;
+    __syncthreads();
+    if (done_s) return;
+    double r1 = 0.0, r2 = 1.0;
+    if (!first) {
+        const double t = sum_partials(part2, m2, lds);
+        const double nrm = sqrt(t);
+        const double err = nrm * nrm;
+        const double errO = st->e[par ^ 1];
+        r1 = errO / err;
+        r2 = err / errO;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->errO = errO;
+            st->err = err;
+            st->e[par] = err;
+            st->itr += 1;
+            if (st->itr >= st->mItr) {
+                st->done = 1;
+            } else if (err < st->eps) {
+                st->done = 1;
+                st->suc = 1;
+            }
+        }
+    }
+    const int tid = threadIdx.x;
+    const int g = tid / kCgsS, l = tid % kCgsS;
+    for (int b = blockIdx.x; b < nTiles; b += gridDim.x) {
+        const int r0 = tile[b], r1t = tile[b + 1];
+        const int k0 = rowPtr[r0];
+        const int nb = rowPtr[r1t] - k0, nr = r1t - r0;
+        for (int i = tid; i <= nr; i += 256) sR[i] = rowPtr[r0 + i] - k0;
+        int c[U];
+        double g0[U], g1[U], g2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = u * 256 + tid;
+            c[u] = j < nb ? colPtr[k0 + j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = u * 256 + tid;
+            if (j < nb) {
+                const size_t k = (size_t)(k0 + j);
+                g0[u] = G[k * 3];
+                g1[u] = G[k * 3 + 1];
+                g2[u] = G[k * 3 + 2];
+            } else {
+                g0[u] = g1[u] = g2[u] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = u * 256 + tid;
+            const double uv = (Pin[c[u]] + r1 * R[c[u]]) * r2;
+            if (j < nb) {
+                s0[j] = g0[u] * uv;
+                s1[j] = g1[u] * uv;
+                s2[j] = g2[u] * uv;
+            }
+        }
+        __syncthreads();
+        for (int rr = g; rr < nr; rr += 256 / kCgsS) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            const int ke = sR[rr + 1];
+            for (int j = sR[rr] + l; j < ke; j += kCgsS) {
+                a0 += s0[j];
+                a1 += s1[j];
+                a2 += s2[j];
+            }
+#pragma unroll
+            for (int o = kCgsS / 2; o > 0; o >>= 1) {
+                a0 += __shfl_xor(a0, o, kWave);
+                a1 += __shfl_xor(a1, o, kWave);
+                a2 += __shfl_xor(a2, o, kWave);
+            }
+            if (l == 0) {
+                const int i = r0 + rr;
+                Pout[i] = (Pin[i] + r1 * R[i]) * r2;
+                GP[(size_t)i * 3] = a0;
+                GP[(size_t)i * 3 + 1] = a1;
+                GP[(size_t)i * 3 + 2] = a2;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* G, const double* Lm, double* R,
                    double* X, double* P, double* P2, double* SP, double* GP, hipStream_t s)
 {
@@ -672,8 +865,27 @@ int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* 
         const int v = e ? atoi(e) : 8;
         return (v == 16 || v == 32) ? v : 8;
     }();
+    // Row tiles of 512 blocks from 2M blocks on, the row-per-lane-group
+    // kernels below: ms per NS step (3.2M-tet cylinder / 200k-tet one)
+    // 131.9 / 8.35 with 512-block tiles, 144.6 / 8.55 with 1024, 154.9 /
+    // 8.27 without; on the small matrix the tile kernels' longer dependency
+    // chain (tile bounds, LDS pass, barriers) outweighs their loads in
+    // flight.  Streaming (non-temporal) loads of L, Gt, G measured slower
+    // (104 against 92 us per k_cgs_sp_t on the large cylinder), and so did
+    // an XCD-contiguous tile order (94 us).  SVFSI_CGS_TILE = 0, 512 or 1024
+    // forces the choice (read at each call).
+    const int cgsEnv = [] {
+        const char* e = getenv("SVFSI_CGS_TILE");
+        if (!e) return -1;
+        const int v = atoi(e);
+        return (v == 512 || v == 1024) ? v : 0;
+    }();
+    const int cgsTile = cgsEnv >= 0 ? cgsEnv : ((size_t)L.nnz >= ((size_t)2 << 20) ? 512 : 0);
+    const int ti = cgsTile == 1024 ? 2 : 1;
+    const int* tl = (cgsTile && L.nTiles[ti]) ? L.tiles[ti].as<int>() : nullptr;
+    const int nT = tl ? L.nTiles[ti] : 0;
     constexpr int kMaxGs = 2048;
-    int gs = (nNo + 256 / lanes - 1) / (256 / lanes);
+    int gs = tl ? nT : (nNo + 256 / lanes - 1) / (256 / lanes);
     if (gs > kMaxGs) gs = kMaxGs;
     if (gs < 1) gs = 1;
     unsigned gv = vec_grid(n);
@@ -724,7 +936,13 @@ int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* 
             const double* Pin = buf[(it + 1) & 1];
             double* Pc = buf[it & 1];
             const int first = (firstChunk && it == 0) ? 1 : 0;
-            if (lanes == 8)
+            if (tl && cgsTile == 1024)
+                hipLaunchKernelGGL(k_cgs_pgp_t<1024>, dim3(gs), dim3(256), 0, cs, nT, tl, rp, cp, G, Pin,
+                                   (const double*)R, (const double*)part2, (int)gv, Pc, GP, st, par, first);
+            else if (tl)
+                hipLaunchKernelGGL(k_cgs_pgp_t<512>, dim3(gs), dim3(256), 0, cs, nT, tl, rp, cp, G, Pin,
+                                   (const double*)R, (const double*)part2, (int)gv, Pc, GP, st, par, first);
+            else if (lanes == 8)
                 hipLaunchKernelGGL(k_cgs_pgp<8>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, G, Pin, (const double*)R,
                                    (const double*)part2, (int)gv, Pc, GP, st, par, first);
             else if (lanes == 32)
@@ -738,7 +956,13 @@ int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* 
             {
                 // algorithmic bytes: L, Gt, colPtr, rowPtr; P and GP once; SP
                 ProfScope ps(4, 36.0 * L.nnz + 4.0 * (nNo + 1) + 40.0 * nNo);
-                if (lanes == 8)
+                if (tl && cgsTile == 1024)
+                    SV_LAUNCH(ps, k_cgs_sp_t<1024>, dim3(gs), dim3(256), 0, cs, nT, tl, rp, cp, Lm, Gt,
+                              (const double*)Pc, (const double*)GP, SP, part1, (const CgState*)st);
+                else if (tl)
+                    SV_LAUNCH(ps, k_cgs_sp_t<512>, dim3(gs), dim3(256), 0, cs, nT, tl, rp, cp, Lm, Gt,
+                              (const double*)Pc, (const double*)GP, SP, part1, (const CgState*)st);
+                else if (lanes == 8)
                     SV_LAUNCH(ps, k_cgs_sp<8>, dim3(gs), dim3(256), 0, cs, nNo, rp, cp, Lm, Gt, (const double*)Pc,
                               (const double*)GP, SP, part1, (const CgState*)st);
                 else if (lanes == 32)
@@ -759,7 +983,7 @@ int cg_schur_solve(Lhs& L, FSILS_subLsType& ls, const double* Gt, const double* 
         std::vector<uintptr_t> sig = {(uintptr_t)n, (uintptr_t)Gt, (uintptr_t)G, (uintptr_t)Lm, (uintptr_t)R,
                                       (uintptr_t)X, (uintptr_t)P, (uintptr_t)P2, (uintptr_t)SP, (uintptr_t)GP,
                                       (uintptr_t)part1, (uintptr_t)st, (uintptr_t)rp, (uintptr_t)cp, (uintptr_t)cpl,
-                                      (uintptr_t)lanes};
+                                      (uintptr_t)lanes, (uintptr_t)tl};
         if (cpl) cpl_sig(L, sig);
         if (sig != w.csGraphs.sig) {
             w.csGraphs.clear();
@@ -1731,12 +1955,6 @@ __global__ __launch_bounds__(256) void k_fg_xupdate(size_t n, double* __restrict
         for (int jj = 0; jj < i; ++jj) v = v + A.y[jj] * (double)Z[(size_t)jj * n + q];
         X[q] = v;
     }
-}
-
-static int env_int(const char* k, int d)
-{
-    const char* e = getenv(k);
-    return e ? atoi(e) : d;
 }
 
 static double env_dbl(const char* k, double d)

@@ -224,8 +224,8 @@ static int row_tiles(const std::vector<int>& rp, int nNo, int T, std::vector<int
 
 int build_row_tiles(Lhs& L)
 {
-    static const int Ts[2] = {256, 512};
-    for (int t = 0; t < 2; ++t) {
+    static const int Ts[3] = {256, 512, 1024};
+    for (int t = 0; t < 3; ++t) {
         std::vector<int> tl;
         L.nTiles[t] = 0;
         if ((int)L.hRowPtr.size() != L.nNo + 1 || L.nNo == 0) continue;

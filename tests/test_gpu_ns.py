@@ -113,6 +113,25 @@ def test_ns_solver_larger_mesh(gpu, orc):
     assert relerr(xg, xo) < 1e-6
 
 
+@pytest.mark.parametrize("tile", ["512", "1024"])
+@pytest.mark.parametrize("which", ["default", "tight"])
+def test_ns_solver_row_tiles(gpu, orc, monkeypatch, which, tile):
+    """The Schur CG's row-tile kernels (the default from 2M blocks on),
+    forced on a small mesh: the same outer iterations and solution as the
+    oracle, CG counts within 5 %."""
+    monkeypatch.setenv("SVFSI_CGS_TILE", tile)
+    prob = fluid(0.6)
+    say("ns solve row tiles", tile, which, "nNo", prob.nNo)
+    st, V0, R0, xg, ls, xo, lo = _solve_both(orc, prob, SETTINGS[which], fsils.PRECOND_FSILS)
+    say("  itr", ls.RI.itr, lo.RI.itr, "GM", ls.GM.itr, lo.GM.itr, "CG", ls.CG.itr, lo.CG.itr, "x relerr",
+        relerr(xg, xo))
+    assert ls.RI.suc == lo.RI.suc == 1
+    assert ls.RI.itr == lo.RI.itr
+    assert abs(ls.GM.itr - lo.GM.itr) <= max(2, lo.GM.itr // 20)
+    assert abs(ls.CG.itr - lo.CG.itr) <= max(2, lo.CG.itr // 20)
+    assert relerr(xg, xo) < (1e-6 if which == "default" else 1e-5)
+
+
 def test_ns_solver_rejects_unsupported(gpu):
     """NS needs dof 4 and runs FP64 only: dof 3 and the mixed path are
     refused with an error (the system is left untouched by the refusal)."""
