@@ -472,7 +472,9 @@ int svfsi_prof_enable(int on);
  * fused with its multi-dot, and the polynomial preconditioner's steps),
  * 2 = STRUCT element kernels (one per colour), 3 = STRUCT material-point
  * kernel, 4 = the NS solver's Schur-complement product SP = L P - Gt GP
- * (with its partial dot).  Returns launches and total ms. */
+ * (with its partial dot), 5 = FP32 SpMV of the multigrid inner
+ * preconditioner's coarse levels (its finest level counts as 1).  Returns
+ * launches and total ms. */
 int svfsi_prof_get(int kernel, long long* count, double* total_ms);
 /* Algorithmic bytes of the profiled launches of `kernel` (values, indices,
  * row pointers, x and y once; for the fused inner step also the basis

@@ -52,6 +52,7 @@ struct AmgLevel {
 struct AmgHier {
     std::vector<std::unique_ptr<AmgLevel>> lv;
     int dof = 0, nNo0 = 0, nnz0 = 0, cap = 0;
+    int deg = 2;                 // Chebyshev degree of the smoothing
     const void* key = nullptr;   // the fine pattern's device rowPtr
     int nDense = 0;              // coarsest operator dofs when inverted densely (0: smoothed)
     DBuf dense, perm;            // coarsest inverse (nDense^2 FP64), pivot rows
@@ -519,6 +520,10 @@ int apply_dinv(const AmgLevel& l, float* v, hipStream_t s)
     return launch_blockdiag_apply<float>(*l.pat, DOF, l.dinv.as<float>(), v, s, nullptr);
 }
 
+// profiling slot of a level's products: 1 (the inner FP32 SpMV) on the
+// finest level, 5 on the coarse ones
+inline int spmv_slot(const AmgLevel& l) { return l.pat == &l.A ? 5 : 1; }
+
 // Chebyshev smoothing of A x = b on D^-1 A, degree deg: x given (setx:
 // starts from zero) and r = b - A x on entry when !setx; on exit r holds
 // b - A x minus the last step's update, x the smoothed iterate.
@@ -532,7 +537,7 @@ int cheb(AmgLevel& l, int deg, bool setx, hipStream_t s)
     hipLaunchKernelGGL(k_amg_cheb_init<DOF>, dim3(grid_of(l.nNo)), dim3(256), 0, s, l.nNo, dinv, (const float*)l.r,
                        l.d, l.x, (float)(1.0 / th), setx ? 1 : 0);
     for (int k = 1; k < deg; ++k) {
-        SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.d, l.t, s));
+        SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.d, l.t, s, nullptr, nullptr, spmv_slot(l)));
         const double rho1 = 1.0 / (2.0 * s1 - rho0);
         hipLaunchKernelGGL(k_amg_cheb_step<DOF>, dim3(grid_of(l.nNo)), dim3(256), 0, s, l.nNo, dinv,
                            (const float*)l.t, l.r, l.d, l.x, (float)(rho1 * rho0), (float)(2.0 * rho1 / de));
@@ -562,9 +567,9 @@ int vcycle(AmgHier& H, size_t li, hipStream_t s)
     AmgLevel& c = *H.lv[li + 1];
     // pre-smoothing from x = 0 (r = b), which leaves r = b - A x current
     SV_HIP(hipMemcpyAsync(l.r, l.b, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-    SV_TRY(cheb<DOF>(l, 2, true, s));
+    SV_TRY(cheb<DOF>(l, H.deg, true, s));
     // the last step's update d is not yet in r
-    SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.x, l.t, s));
+    SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.x, l.t, s, nullptr, nullptr, spmv_slot(l)));
     hipLaunchKernelGGL(k_amg_sub, dim3(grid_of(n)), dim3(256), 0, s, n, (const float*)l.b, (const float*)l.t, l.r);
     hipLaunchKernelGGL(k_amg_restrict<DOF>, dim3(grid_of((size_t)c.nNo * DOF)), dim3(256), 0, s, c.nNo,
                        l.mPtr.as<int>(), l.mIdx.as<int>(), (const float*)l.r, c.b);
@@ -572,9 +577,9 @@ int vcycle(AmgHier& H, size_t li, hipStream_t s)
     hipLaunchKernelGGL(k_amg_prolong<DOF>, dim3(grid_of(n)), dim3(256), 0, s, l.nNo, l.dAgg.as<int>(),
                        (const float*)c.x, l.x);
     // post-smoothing from the corrected x
-    SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.x, l.t, s));
+    SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.x, l.t, s, nullptr, nullptr, spmv_slot(l)));
     hipLaunchKernelGGL(k_amg_sub, dim3(grid_of(n)), dim3(256), 0, s, n, (const float*)l.b, (const float*)l.t, l.r);
-    SV_TRY(cheb<DOF>(l, 2, false, s));
+    SV_TRY(cheb<DOF>(l, H.deg, false, s));
     SV_HIP(hipGetLastError());
     return SVFSI_OK;
 }
@@ -590,7 +595,7 @@ int estimate_lambda(AmgHier& H, AmgLevel& l, hipStream_t s, double& lam)
     double* nrm = part + 256;
     hipLaunchKernelGGL(k_amg_rand, dim3(grid_of(n)), dim3(256), 0, s, n, l.d);
     for (int it = 0; it < kPowerSteps; ++it) {
-        SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.d, l.t, s));
+        SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.d, l.t, s, nullptr, nullptr, spmv_slot(l)));
         SV_TRY(apply_dinv<DOF>(l, l.t, s));
         hipLaunchKernelGGL(k_amg_sumsq, dim3(nb), dim3(256), 0, s, n, (const float*)l.t, part);
         hipLaunchKernelGGL(k_amg_normalize, dim3(nb), dim3(256), 0, s, n, (const float*)l.t, l.d,
@@ -653,8 +658,11 @@ int amg_setup(Lhs& L, int dof, const double* K64, const float* K32, hipStream_t 
         set_error("aggregation multigrid: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
     }
+    // aggregate size cap (SVFSI_AMG_CAP): measured on the 3.2M-tet cylinder,
+    // ms per mixed step at caps 4 / 5 / 6 / 7 / 8 / 12: 194 / 177 / 191 /
+    // 204 / 215 / 330
     const char* e = getenv("SVFSI_AMG_CAP");
-    int cap = e ? atoi(e) : 8;
+    int cap = e ? atoi(e) : 5;
     if (cap < 2) cap = 2;
     Work& w = L.ws;
     if (!w.amg || w.amg->key != L.rowPtr.p || w.amg->dof != dof || w.amg->nNo0 != L.nNo || w.amg->nnz0 != L.nnz ||
@@ -668,6 +676,9 @@ int amg_setup(Lhs& L, int dof, const double* K64, const float* K32, hipStream_t 
     }
     AmgHier& H = *w.amg;
     H.lv[0]->pat = &L;
+    // Chebyshev degree of the pre- and post-smoothing (SVFSI_AMG_DEG)
+    const char* ed = getenv("SVFSI_AMG_DEG");
+    H.deg = ed ? std::max(1, std::min(8, atoi(ed))) : 2;
     return dof == 3 ? setup_values<3>(H, K64, K32, s) : setup_values<4>(H, K64, K32, s);
 }
 

@@ -1007,7 +1007,7 @@ int svfsi_prof_enable(int on)
 
 int svfsi_prof_get(int kernel, long long* count, double* total_ms)
 {
-    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..4");
+    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..%d", Prof::kSlots - 1);
     prof_resolve();
     if (count) *count = prof().count[kernel];
     if (total_ms) *total_ms = prof().ms[kernel];
@@ -1016,7 +1016,7 @@ int svfsi_prof_get(int kernel, long long* count, double* total_ms)
 
 int svfsi_prof_get_bytes(int kernel, double* bytes)
 {
-    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..4");
+    SV_ARG(kernel >= 0 && kernel < Prof::kSlots, "kernel must be 0..%d", Prof::kSlots - 1);
     prof_resolve();
     if (bytes) *bytes = prof().bytes[kernel];
     return SVFSI_OK;

@@ -227,8 +227,9 @@ struct ProfPending {
 struct Prof {
     bool on = false;
     // slots: 0 FP64 SpMV, 1 FP32 SpMV (inner solver), 2 STRUCT element
-    // kernels, 3 STRUCT material-point kernel, 4 NS Schur-complement product
-    static constexpr int kSlots = 5;
+    // kernels, 3 STRUCT material-point kernel, 4 NS Schur-complement product,
+    // 5 FP32 SpMV of the multigrid's coarse levels
+    static constexpr int kSlots = 6;
     long long count[kSlots] = {};
     double ms[kSlots] = {};
     double bytes[kSlots] = {}; // algorithmic bytes (slot 2: MFMA flops) of the timed launches
@@ -270,7 +271,7 @@ int launch_spmv(const Lhs& L, int dof, const double* K, const double* U,
                 double* KU, hipStream_t s, const int* stop = nullptr);
 int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U,
                     float* KU, hipStream_t s, const int* stop = nullptr,
-                    const float* dinv = nullptr);
+                    const float* dinv = nullptr, int profSlot = 1);
 // The inner (preconditioner) kernels of the nested solve, instantiated for
 // IT = float (the mixed solve) and IT = double (the same algorithm in FP64).
 template <typename OT>

@@ -1198,6 +1198,32 @@ __global__ __launch_bounds__(256) void k_gm_mdot(size_t n, int i, const BT* __re
     }
 }
 
+// Many row chunks (large systems): one workgroup per coefficient jj sums
+// the nb partials of k_gm_mdot in a fixed order and leaves the sum in the
+// coefficient's first slot, so that k_gm_givens reads one value per
+// coefficient instead of summing (i+1) nb partials in a single workgroup
+// (about 100 us per Arnoldi step at 3.5k chunks).
+__global__ __launch_bounds__(256) void k_gm_fold(int nb, double* __restrict__ partial,
+                                                 const GmState* __restrict__ st)
+{
+    if (st->stop) return;
+    __shared__ double lds[4];
+    double* pp = partial + (size_t)blockIdx.x * nb;
+    double t0 = 0.0, t1 = 0.0;
+    int b = threadIdx.x;
+    for (; b + 256 < nb; b += 512) {
+        t0 += pp[b];
+        t1 += pp[b + 256];
+    }
+    if (b < nb) t0 += pp[b];
+    const double sum = block_sum<256>(t0 + t1, lds);
+    if (threadIdx.x == 0) pp[0] = sum;
+}
+
+// Row chunks above which the multi-dot partials are folded first
+// (SVFSI_FOLD_CHUNKS overrides, read at each solve).
+constexpr int kFoldChunks = 1024;
+
 // One block: sums the per-block partials in block order (deterministic),
 // then the Pythagorean h(i+1,i) (GMRES.f:342-347), the previous rotations,
 // the new Givens rotation, the residual estimate and the convergence test
@@ -1207,7 +1233,9 @@ __global__ __launch_bounds__(256) void k_gm_mdot(size_t n, int i, const BT* __re
 // solver): pass 1 only stores the first coefficients (in y, free during a
 // cycle) for k_gm_update; pass 2 takes h(i+1,i) from the twice-orthogonalised
 // vector and adds the two coefficient sets before the Givens rotations.
-__global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, const double* __restrict__ partial,
+// Coefficient jj's nb partials are at partial[jj * ldp ...]: ldp = nb for
+// the partials of k_gm_mdot itself, nb = 1 once k_gm_fold has summed them.
+__global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, int ldp, const double* __restrict__ partial,
                                                    GmState* __restrict__ st, GmArrays A, int pass)
 {
     if (st->stop) return;
@@ -1225,7 +1253,7 @@ __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, const 
         const int b0 = threadIdx.x % T;
         double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
         if (jj <= i) {
-            const double* pp = partial + (size_t)jj * nb;
+            const double* pp = partial + (size_t)jj * ldp;
             int b = b0;
             for (; b + 3 * T < nb; b += 4 * T) {
                 t0 += pp[b];
@@ -1679,6 +1707,7 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
     Work& w = L.ws;
     // row chunks of the multi-dot: 256 lanes x E elements each
     const size_t nbr = (n + (size_t)256 * VecE<BT>::E - 1) / ((size_t)256 * VecE<BT>::E);
+    const long long foldAt = env_int("SVFSI_FOLD_CHUNKS", kFoldChunks);
     const unsigned gv = vec_grid(n);
     SV_TRY(basisBuf.ensure((size_t)(sD + 1) * n * sizeof(BT)));
     SV_TRY(xBuf.ensure(n * sizeof(BT)));
@@ -1725,7 +1754,7 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
         std::vector<uintptr_t> sig = {
             (uintptr_t)n, (uintptr_t)dof, (uintptr_t)sD, (uintptr_t)op.K, (uintptr_t)sizeof(BT),
             (uintptr_t)op_transposed(op), (uintptr_t)basis, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.hess.p,
-            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)L.trans.p, (uintptr_t)op_cpl(op)};
+            (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)L.trans.p, (uintptr_t)op_cpl(op), (uintptr_t)((long long)nbr > foldAt)};
         if (op_cpl(op)) cpl_sig(L, sig);
         if (sig != w.gmGraphs.sig) {
             w.gmGraphs.clear();
@@ -1751,8 +1780,15 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
                     SV_TRY(op.apply(basis + (size_t)(i - 1) * n, basis + (size_t)i * n, cs, stopFlag));
                     hipLaunchKernelGGL(k_gm_mdot<BT>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0, cs,
                                        n, i, (const BT*)basis, part, (const GmState*)st);
-                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, sD, (int)nbr, (const double*)part,
-                                       st, A, 0);
+                    if ((long long)nbr > foldAt) {
+                        hipLaunchKernelGGL(k_gm_fold, dim3((unsigned)(i + 1)), dim3(256), 0, cs, (int)nbr, part,
+                                           (const GmState*)st);
+                        hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, sD, 1, (int)nbr,
+                                           (const double*)part, st, A, 0);
+                    } else {
+                        hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, sD, (int)nbr, (int)nbr,
+                                           (const double*)part, st, A, 0);
+                    }
                     hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, cs, n, i, sD, basis,
                                        (const GmState*)st, A);
                 }
@@ -1996,6 +2032,7 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     }
     const unsigned gv = vec_grid(n);
     const size_t nbr = (n + (size_t)256 * VecE<double>::E - 1) / ((size_t)256 * VecE<double>::E);
+    const long long foldAt = env_int("SVFSI_FOLD_CHUNKS", kFoldChunks);
     SV_TRY(w.X.ensure(n * sizeof(double)));
     SV_TRY(w.tmp.ensure(n * sizeof(double)));
     SV_TRY(w.basis.ensure((size_t)(mo + 1) * n * sizeof(double)));
@@ -2020,14 +2057,6 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     // 3.2M-tet cylinder (72 -> 54).  Off for dof 3, where it added outer
     // steps on the elasticity cube (16 -> 22).  SVFSI_INNER_BJ = 0 / 1
     // overrides.
-    // SVFSI_INNER_AMG=1: one V-cycle of the aggregation multigrid (amg.hip)
-    // on the FP32 scaled matrix instead (no block-Jacobi rescaling: the
-    // smoother uses the node blocks itself).
-    const bool useAmg = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_AMG", 0) != 0;
-    if (useAmg && !std::is_same<IT, float>::value) {
-        set_error("the multigrid inner preconditioner is FP32 only");
-        return SVFSI_ERR_NOTSUP;
-    }
     // A saddle-point system without pressure stabilisation (unmasked rows
     // with a zero diagonal: the Taylor-Hood fluid) defeats both the node
     // block inverses and the fixed polynomial: on the tet10 channel they
@@ -2039,6 +2068,21 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     SV_TRY(w.ostate.ensure(256));
     SV_TRY(count_zero_diag(L, dof, K, w.W.as<double>(), w.ostate.as<int>(), zeroDiag, s));
     const bool saddle = zeroDiag > 0;
+    // One V-cycle of the aggregation multigrid (amg.hip) on the FP32 scaled
+    // matrix (no block-Jacobi rescaling: the smoother uses the node blocks
+    // itself): the default for dof-4 systems whose FP32 matrix exceeds half
+    // the Infinity Cache, where the polynomial's outer steps grow with the
+    // mesh (3.2M-tet cylinder: 177 ms, 89 outer steps and 446 fine SpMVs
+    // against 227-233 ms at degree 30 and 216 ms at 50; 200k-tet cylinder
+    // 15-16 ms against 6.7-7.0 for the polynomial).  SVFSI_INNER_AMG = 0 / 1
+    // overrides.
+    const bool bigFluid = dof == 4 && std::is_same<IT, float>::value && !saddle &&
+                          (size_t)L.nnz * 16 * sizeof(float) > ((size_t)128 << 20);
+    const bool useAmg = (dof == 3 || dof == 4) && env_int("SVFSI_INNER_AMG", bigFluid ? 1 : 0) != 0;
+    if (useAmg && !std::is_same<IT, float>::value) {
+        set_error("the multigrid inner preconditioner is FP32 only");
+        return SVFSI_ERR_NOTSUP;
+    }
     // inner GMRES tolerance: 0.1; 0.01 on saddle-point systems (the P2P1
     // channel at 10^3: 227 outer steps and 138 ms against 371 and 197 ms)
     const double tin = env_dbl("SVFSI_FGMRES_INNER_TOL", saddle ? 0.01 : 0.1);
@@ -2192,7 +2236,7 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
             (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
             (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
             (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv,
-            (uintptr_t)cgsPasses, (uintptr_t)sizeof(IT)};
+            (uintptr_t)cgsPasses, (uintptr_t)sizeof(IT), (uintptr_t)((long long)nbr > foldAt)};
         cpl_sig(L, sig);
         if (sig != w.graphs.sig) {
             w.graphs.clear();
@@ -2265,8 +2309,15 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
                     const int pass = cgsPasses == 1 ? 0 : pp;
                     hipLaunchKernelGGL(k_gm_mdot<double>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0,
                                        cs, n, i, (const double*)Q, part, (const GmState*)st);
-                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, mo, (int)nbr, (const double*)part,
-                                       st, A, pass);
+                    if ((long long)nbr > foldAt) {
+                        hipLaunchKernelGGL(k_gm_fold, dim3((unsigned)(i + 1)), dim3(256), 0, cs, (int)nbr, part,
+                                           (const GmState*)st);
+                        hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, mo, 1, (int)nbr,
+                                           (const double*)part, st, A, pass);
+                    } else {
+                        hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, mo, (int)nbr, (int)nbr,
+                                           (const double*)part, st, A, pass);
+                    }
                     hipLaunchKernelGGL(k_gm_update<double>, dim3(gv), dim3(256), 0, cs, n, i, mo, Q,
                                        (const GmState*)st, A);
                 }

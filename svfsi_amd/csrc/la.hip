@@ -1110,13 +1110,13 @@ int launch_spmv(const Lhs& L, int dof, const double* K, const double* U, double*
 }
 
 int launch_spmv_f32(const Lhs& L, int dof, const float* K, const float* U, float* KU,
-                    hipStream_t s, const int* stop, const float* dinv)
+                    hipStream_t s, const int* stop, const float* dinv, int profSlot)
 {
     if (dinv && dof != 4 && dof != 3) {
         set_error("block-Jacobi SpMV: dof %d not supported (3, 4)", dof);
         return SVFSI_ERR_NOTSUP;
     }
-    return spmv_dispatch<float, float>(L, dof, K, U, KU, s, 1, stop, dinv);
+    return spmv_dispatch<float, float>(L, dof, K, U, KU, s, profSlot, stop, dinv);
 }
 
 int launch_spmv_t(const Lhs& L, int dof, const double* K, const double* U, double* KU,

@@ -367,6 +367,27 @@ def test_gmres_solve_parity(gpu, orc):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+def test_gmres_folded_partials(gpu, orc, monkeypatch, mixed):
+    """The two-stage sum of the Arnoldi multi-dot partials (k_gm_fold, the
+    default above 1024 row chunks, i.e. for large systems) forced on a small
+    one: GMRESV and the mixed solve still match the oracle."""
+    monkeypatch.setenv("SVFSI_FOLD_CHUNKS", "4")
+    prob = problems.fluid_cylinder(h=0.4)
+    say("gmres folded partials", prob.nNo, "mixed", mixed)
+    st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc, mixed=mixed)
+    say("itr gpu", ls.RI.itr, "cpu", so.itr, "x relerr", relerr(xg, xo))
+    assert ls.RI.suc == 1 and so.suc == 1
+    if not mixed:
+        assert abs(ls.RI.itr - so.itr) <= max(3, so.itr // 50), (ls.RI.itr, so.itr)
+    mask = prob.dir_mask()
+    _, Rs, W = orc.precond_diag(st.rowPtr - 1, st.colPtr - 1, prob.dof, V0, R0, mask)
+    y = np.where(W != 0, xg / np.where(W != 0, W, 1), 0.0)
+    r = Rs - orc.spmv(st.rowPtr - 1, st.colPtr - 1, prob.dof, Vs, y)
+    assert np.linalg.norm(r) <= 2e-8 * np.linalg.norm(Rs)
+    assert relerr(xg, xo) < 1e-4
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_struct_solve(gpu, orc, mixed):
     prob = problems.struct_cube(4, quad=1)
     say("struct solve", prob.nNo, "mixed", mixed)
