@@ -1233,6 +1233,8 @@ constexpr int kFoldChunks = 1024;
 // solver): pass 1 only stores the first coefficients (in y, free during a
 // cycle) for k_gm_update; pass 2 takes h(i+1,i) from the twice-orthogonalised
 // vector and adds the two coefficient sets before the Givens rotations.
+// pass 3: a single classical pass with the explicit norm (after pass 1 and
+// k_gm_update with its |w|^2 partials): h(i+1,i) = |w|.
 // Coefficient jj's nb partials are at partial[jj * ldp ...]: ldp = nb for
 // the partials of k_gm_mdot itself, nb = 1 once k_gm_fold has summed them.
 __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, int ldp, const double* __restrict__ partial,
@@ -1241,6 +1243,17 @@ __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, int ld
     if (st->stop) return;
     __shared__ double hsh[kMaxSD + 2];
     __shared__ double cs[2][kMaxSD];
+    __shared__ double lds[4];
+    if (pass == 3) {
+        // single pass with the explicit norm: partial[0..nb) hold the
+        // partial sums of |w|^2 after k_gm_update, the coefficients are the
+        // pass-1 ones kept in y
+        double t = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 256) t += partial[b];
+        t = block_sum<256>(t, lds);
+        for (int jj = threadIdx.x; jj < i; jj += 256) hsh[jj] = A.y[jj];
+        if (threadIdx.x == 0) hsh[i] = t;
+    }
     // T lanes per coefficient (power of two <= 64), fixed-order sums
     int npow = 1;
     while (npow < i + 1) npow <<= 1;
@@ -1248,7 +1261,7 @@ __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, int ld
     if (T > 64) T = 64;
     if (T < 1) T = 1;
     const int per = 256 / T;
-    for (int jj0 = 0; jj0 <= i; jj0 += per) {
+    for (int jj0 = 0; jj0 <= i && pass != 3; jj0 += per) {
         const int jj = jj0 + threadIdx.x / T;
         const int b0 = threadIdx.x % T;
         double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
@@ -1282,7 +1295,8 @@ __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, int ld
     if (threadIdx.x != 0) return;
     st->iters += 1;
     double hii = hsh[i];
-    for (int jj = 0; jj < i; ++jj) hii = hii - hsh[jj] * hsh[jj];
+    if (pass != 3)
+        for (int jj = 0; jj < i; ++jj) hii = hii - hsh[jj] * hsh[jj];
     hii = sqrt(fabs(hii));
     if (pass == 2)
         for (int jj = 0; jj < i; ++jj) hsh[jj] = A.y[jj] + hsh[jj];
@@ -1320,13 +1334,18 @@ __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, int ld
 // The pre-rotation coefficients were saved in hraw by k_gm_givens.  Each
 // lane updates E consecutive elements with 16-byte loads; the sweep over j
 // keeps the reference's order per element.
+// With normPart given, each block also leaves the partial sum of the
+// updated vector's squares in normPart[blockIdx.x].
 template <typename BT>
 __global__ __launch_bounds__(256) void k_gm_update(size_t n, int i, int sD, BT* __restrict__ basis,
-                                                   const GmState* __restrict__ st, GmArrays A)
+                                                   const GmState* __restrict__ st, GmArrays A,
+                                                   double* __restrict__ normPart)
 {
     if (st->stop) return;
     constexpr int E = VecE<BT>::E;
     __shared__ double hc[kMaxSD + 1];
+    __shared__ double lds[4];
+    double nrm = 0.0;
     for (int jj = threadIdx.x; jj < i; jj += 256) hc[jj] = -A.hraw[jj];
     __syncthreads();
     const double inv = A.hraw[sD + 2];
@@ -1357,8 +1376,28 @@ __global__ __launch_bounds__(256) void k_gm_update(size_t n, int i, int sD, BT* 
         }
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            if (q + e < n) w[q + e] = (BT)(v[e] * inv);
+            if (q + e < n) {
+                const double o = v[e] * inv;
+                w[q + e] = (BT)o;
+                nrm += o * o;
+            }
     }
+    if (normPart) {
+        const double sb = block_sum<256>(nrm, lds);
+        if (threadIdx.x == 0) normPart[blockIdx.x] = sb;
+    }
+}
+
+// w *= 1/h(i+1,i) (the explicit-norm single pass)
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_scale(size_t n, int i, int sD, BT* __restrict__ basis,
+                                                  const GmState* __restrict__ st, GmArrays A)
+{
+    if (st->stop) return;
+    const double inv = A.hraw[sD + 2];
+    BT* w = basis + (size_t)i * n;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        w[q] = (BT)((double)w[q] * inv);
 }
 
 // Step end of the device-driven inner cycle, fused: every block sums the
@@ -1790,7 +1829,7 @@ static int gmres_core(Lhs& L, const GmOp<BT>& op, int dof, BT* R, DBuf& basisBuf
                                            (const double*)part, st, A, 0);
                     }
                     hipLaunchKernelGGL(k_gm_update<BT>, dim3(gv), dim3(256), 0, cs, n, i, sD, basis,
-                                       (const GmState*)st, A);
+                                       (const GmState*)st, A, nullptr);
                 }
                 SV_HIP(hipGetLastError());
                 return SVFSI_OK;
@@ -2024,7 +2063,6 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     // outer Krylov dimension: the caller's restart length (dimKry), 30 when
     // none is set (CG requests); SVFSI_FGMRES_M overrides
     const int mo = env_int("SVFSI_FGMRES_M", ls.sD > 0 ? std::min(ls.sD, kMaxSD) : 30);
-    const int cgsPasses = env_int("SVFSI_OUTER_CGS", 2) == 1 ? 1 : 2;
     const int mi = env_int("SVFSI_FGMRES_INNER_M", 30);     // inner Krylov dimension
     if (mo > kMaxSD || mi > kMaxSD) {
         set_error("FGMRES: Krylov dimension exceeds %d", kMaxSD);
@@ -2068,6 +2106,22 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
     SV_TRY(w.ostate.ensure(256));
     SV_TRY(count_zero_diag(L, dof, K, w.W.as<double>(), w.ostate.as<int>(), zeroDiag, s));
     const bool saddle = zeroDiag > 0;
+    // outer Gram-Schmidt (SVFSI_OUTER_CGS): 3 a single classical pass with
+    // the explicit norm (the default), 2 two passes (CGS2, the default on
+    // saddle-point systems), 1 the reference's single pass with the
+    // Pythagorean norm.  Measured (outer steps, ms per mixed step):
+    // 200k-tet cylinder 22 / 6.64 (3) against 22 / 6.77 (2) and 54 / 17
+    // (1, before the later kernels); 3.2M-tet cylinder 89 / 139.5 against
+    // 89 / 161.2; LELAS 20^3 18 steps either way; the Taylor-Hood channel
+    // 390 / 504 against 354 / 460, where the second pass pays.  The
+    // Pythagorean h(i+1,i) loses the basis' orthogonality on these
+    // operators; the explicit norm keeps it as well as the second pass does
+    // on the others, at half the basis reads.
+    const int cgsMode = [saddle] {
+        const int v = env_int("SVFSI_OUTER_CGS", saddle ? 2 : 3);
+        return (v == 1 || v == 2) ? v : 3;
+    }();
+    const int cgsPasses = cgsMode == 2 ? 2 : 1;
     // One V-cycle of the aggregation multigrid (amg.hip) on the FP32 scaled
     // matrix (no block-Jacobi rescaling: the smoother uses the node blocks
     // itself): the default for dof-4 systems whose FP32 matrix exceeds half
@@ -2236,7 +2290,7 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
             (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
             (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
             (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv,
-            (uintptr_t)cgsPasses, (uintptr_t)sizeof(IT), (uintptr_t)((long long)nbr > foldAt)};
+            (uintptr_t)cgsMode, (uintptr_t)sizeof(IT), (uintptr_t)((long long)nbr > foldAt)};
         cpl_sig(L, sig);
         if (sig != w.graphs.sig) {
             w.graphs.clear();
@@ -2306,7 +2360,7 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
                 // outer Arnoldi: classical Gram-Schmidt, two passes (CGS2) by
                 // default; SVFSI_OUTER_CGS=1 runs the reference's single pass
                 for (int pp = 1; pp <= cgsPasses; ++pp) {
-                    const int pass = cgsPasses == 1 ? 0 : pp;
+                    const int pass = cgsMode == 1 ? 0 : (cgsMode == 3 ? 1 : pp);
                     hipLaunchKernelGGL(k_gm_mdot<double>, dim3((unsigned)nbr, (unsigned)((i + 8) / 8)), dim3(256), 0,
                                        cs, n, i, (const double*)Q, part, (const GmState*)st);
                     if ((long long)nbr > foldAt) {
@@ -2319,7 +2373,13 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
                                            (const double*)part, st, A, pass);
                     }
                     hipLaunchKernelGGL(k_gm_update<double>, dim3(gv), dim3(256), 0, cs, n, i, mo, Q,
-                                       (const GmState*)st, A);
+                                       (const GmState*)st, A, cgsMode == 3 ? part : nullptr);
+                    if (cgsMode == 3) {
+                        hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, mo, (int)gv, (int)gv,
+                                           (const double*)part, st, A, 3);
+                        hipLaunchKernelGGL(k_gm_scale<double>, dim3(gv), dim3(256), 0, cs, n, i, mo, Q,
+                                           (const GmState*)st, A);
+                    }
                 }
                 SV_HIP(hipGetLastError());
                 return SVFSI_OK;

@@ -248,10 +248,11 @@ int build_row_tiles(Lhs& L)
 //   dof 4 FP64  13.1 / 224 (T = 512) against 13.2 / 239
 // FP64 wants two blocks in flight per lane (T = 512: 16.2 us at T = 256 on
 // the small cylinder); FP32 prefers more, smaller tiles below 2M blocks.
-// FP64 dof 4 keeps k_bspmv4 unless forced: it gains little there, and the
-// FP64 GMRESV baseline of the fluid keeps its summation order (a GMRES of
-// thousands of iterations moves its count by a few per cent with the
-// rounding of the products).
+// FP64 dof 4 keeps k_bspmv4 below 2M blocks unless forced: it gains little
+// there (13.1 against 13.2 us), and those systems keep the summation order
+// their GMRESV iteration counts were pinned with (a GMRES of thousands of
+// iterations moves its count by a few per cent with the rounding of the
+// products); from 2M blocks on the stream kernel saves 6 % (224 / 239 us).
 // SVFSI_SPMV_STREAM = 0 selects the older kernels, 256 or 512 forces T.
 const int* stream_tiles(const Lhs& L, int dof, bool fp64, int& T, int& nT)
 {
@@ -260,7 +261,7 @@ const int* stream_tiles(const Lhs& L, int dof, bool fp64, int& T, int& nT)
         return e ? atoi(e) : -1;
     }();
     if (env == 0 || (dof != 3 && dof != 4)) return nullptr;
-    if (fp64 && dof == 4 && env < 0) return nullptr;
+    if (fp64 && dof == 4 && env < 0 && (size_t)L.nnz < ((size_t)2 << 20)) return nullptr;
     int t = (fp64 || (size_t)L.nnz >= ((size_t)2 << 20)) ? 1 : 0;
     if (env == 256) t = 0;
     else if (env == 512) t = 1;

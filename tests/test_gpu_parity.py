@@ -366,6 +366,26 @@ def test_gmres_solve_parity(gpu, orc):
     assert abs(ls.RI.iNorm - so.iNorm) <= 1e-12 * so.iNorm
 
 
+@pytest.mark.parametrize("cgs", ["2", "3"])
+@pytest.mark.parametrize("which", ["fluid", "lelas"])
+def test_mixed_outer_gram_schmidt(gpu, orc, monkeypatch, cgs, which):
+    """The outer loop's Gram-Schmidt variants, two passes (2) and one pass
+    with the explicit norm (3, the default): both stop on the residual
+    estimate, and the FP64 residual recomputed on the CPU meets it."""
+    monkeypatch.setenv("SVFSI_OUTER_CGS", cgs)
+    prob = problems.fluid_cylinder(h=0.4) if which == "fluid" else problems.lelas_cube(6)
+    say("outer gram-schmidt", cgs, which, prob.nNo)
+    st, ls, xg, xo, so, V0, R0, Vs = _solve_pair(prob, orc, mixed=True)
+    say("  outer", ls.RI.itr, "spmv", ls.RI.cM, "x relerr", relerr(xg, xo))
+    assert ls.RI.suc == 1
+    mask = prob.dir_mask()
+    _, Rs, W = orc.precond_diag(st.rowPtr - 1, st.colPtr - 1, prob.dof, V0, R0, mask)
+    y = np.where(W != 0, xg / np.where(W != 0, W, 1), 0.0)
+    r = Rs - orc.spmv(st.rowPtr - 1, st.colPtr - 1, prob.dof, Vs, y)
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(Rs) * 1.0001
+    assert relerr(xg, xo) < 1e-4
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_gmres_folded_partials(gpu, orc, monkeypatch, mixed):
     """The two-stage sum of the Arnoldi multi-dot partials (k_gm_fold, the
