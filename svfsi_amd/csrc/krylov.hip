@@ -2018,6 +2018,61 @@ __global__ void k_cvt(size_t n, const TI* __restrict__ a, TO* __restrict__ b)
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) b[q] = (TO)a[q];
 }
 
+// Start of an outer step's preconditioner application, one pass per node:
+// p = D^-1 (IT) q (p = (IT) q without dinv; the product summed in FP64 as
+// k_blockdiag_apply does) and, when z is given, z = 0.
+template <typename IT, int DOF>
+__global__ __launch_bounds__(256) void k_fg_start(int nNo, const double* __restrict__ q, const IT* __restrict__ dinv,
+                                                  IT* __restrict__ p, IT* __restrict__ z)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nNo) return;
+    IT xi[DOF];
+#pragma unroll
+    for (int m = 0; m < DOF; ++m) xi[m] = (IT)q[(size_t)r * DOF + m];
+    if (dinv) {
+        const IT* d = dinv + (size_t)r * DOF * DOF;
+#pragma unroll
+        for (int a = 0; a < DOF; ++a) {
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < DOF; ++m) v += (double)d[a * DOF + m] * (double)xi[m];
+            p[(size_t)r * DOF + a] = (IT)v;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < DOF; ++m) p[(size_t)r * DOF + m] = xi[m];
+    }
+    if (z)
+#pragma unroll
+        for (int m = 0; m < DOF; ++m) z[(size_t)r * DOF + m] = (IT)0;
+}
+
+template <typename IT>
+static void launch_fg_start(int nNo, int dof, const double* q, const IT* dinv, IT* p, IT* z, hipStream_t s)
+{
+    const unsigned g = (unsigned)((nNo + 255) / 256);
+    if (dof == 4) hipLaunchKernelGGL((k_fg_start<IT, 4>), dim3(g), dim3(256), 0, s, nNo, q, dinv, p, z);
+    else if (dof == 3) hipLaunchKernelGGL((k_fg_start<IT, 3>), dim3(g), dim3(256), 0, s, nNo, q, dinv, p, z);
+    else if (dof == 2) hipLaunchKernelGGL((k_fg_start<IT, 2>), dim3(g), dim3(256), 0, s, nNo, q, dinv, p, z);
+    else hipLaunchKernelGGL((k_fg_start<IT, 1>), dim3(g), dim3(256), 0, s, nNo, q, dinv, p, z);
+}
+
+// The last, linear factor of the polynomial (y += c1 prod, as k_poly_last)
+// with the FP64 copy of the result the outer product reads.
+template <typename IT>
+__global__ __launch_bounds__(256) void k_poly_last_cvt(size_t n, const IT* __restrict__ prod, IT* __restrict__ y,
+                                                       double* __restrict__ y64, const double* __restrict__ coef,
+                                                       int step)
+{
+    const double c1 = coef[2 * step];
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const IT v = (IT)((double)y[q] + c1 * (double)prod[q]);
+        y[q] = v;
+        y64[q] = (double)v;
+    }
+}
+
 // x += sum_j y_j z_j with the inner-precision directions z_j (flexible
 // GMRES update)
 template <typename IT>
@@ -2313,6 +2368,7 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
             // every cycle, so they replay as one graph per i
             auto step = [&, i](hipStream_t cs) -> int {
                 IT* zi = Z + (size_t)(i - 1) * n;
+                bool t64Done = false;
                 if (useAmg) {
                     if constexpr (std::is_same<IT, float>::value) {
                         hipLaunchKernelGGL((k_cvt<double, IT>), dim3(gv), dim3(256), 0, cs, n,
@@ -2321,10 +2377,7 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
                     }
                 } else if (usePoly) {
                     // z_i = p(B) D^-1 q_i, factor by factor
-                    hipLaunchKernelGGL((k_cvt<double, IT>), dim3(gv), dim3(256), 0, cs, n,
-                                       (const double*)(Q + (size_t)(i - 1) * n), P0);
-                    if (dinv) SV_TRY(launch_blockdiag_apply<IT>(L, dof, dinv, P0, cs, &st->stop));
-                    hipLaunchKernelGGL(k_set_zero<IT>, dim3(gv), dim3(256), 0, cs, n, zi);
+                    launch_fg_start<IT>(L.nNo, dof, (const double*)(Q + (size_t)(i - 1) * n), dinv, P0, zi, cs);
                     IT* cur = P0;
                     IT* nxt = P1;
                     const int nf = (int)kinds.size();
@@ -2332,7 +2385,10 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
                         const bool last = f == nf - 1;
                         if (kinds[f] == 1) {
                             if (last) {
-                                SV_TRY(launch_poly_last<IT>(n, cur, zi, coefD, f, cs, &st->stop));
+                                // with the FP64 copy for the outer product
+                                hipLaunchKernelGGL(k_poly_last_cvt<IT>, dim3(gv), dim3(256), 0, cs, n,
+                                                   (const IT*)cur, zi, t64, coefD, f);
+                                t64Done = true;
                             } else {
                                 SV_TRY(launch_poly<IT>(L, dof, 0, (const IT*)Kin, nullptr, cur, nullptr, nullptr, zi, nxt, coefD, f, cs,
                                                     &st->stop));
@@ -2349,12 +2405,12 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
                         }
                     }
                 } else {
-                    hipLaunchKernelGGL((k_cvt<double, IT>), dim3(gv), dim3(256), 0, cs, n,
-                                       (const double*)(Q + (size_t)(i - 1) * n), zi);
-                    if (dinv) SV_TRY(launch_blockdiag_apply<IT>(L, dof, dinv, zi, cs, &st->stop));
+                    launch_fg_start<IT>(L.nNo, dof, (const double*)(Q + (size_t)(i - 1) * n), dinv, zi,
+                                        (IT*)nullptr, cs);
                     SV_TRY(gmres_inner_dev<IT>(L, opIn, dof, zi, w.basis32, mi, tin, &st->stop, cs));
                 }
-                hipLaunchKernelGGL((k_cvt<IT, double>), dim3(gv), dim3(256), 0, cs, n, (const IT*)zi, t64);
+                if (!t64Done)
+                    hipLaunchKernelGGL((k_cvt<IT, double>), dim3(gv), dim3(256), 0, cs, n, (const IT*)zi, t64);
                 SV_TRY(launch_spmv(L, dof, K, t64, Q + (size_t)i * n, cs, &st->stop));
                 SV_TRY(cpl_apply(L, 0, dof, t64, Q + (size_t)i * n, cs, &st->stop));
                 // outer Arnoldi: classical Gram-Schmidt, two passes (CGS2) by
