@@ -449,9 +449,12 @@ def main() -> int:
         poly = dof in (3, 4) and os.environ.get("SVFSI_INNER_POLY", "1") != "0"
         if amg_default:
             tt = 512 if nnzb >= (2 << 20) else 256
-            kname = f"k_sspmv<float, {dof}, {tt}"
-            what = ("FP32 CSR-stream block SpMV over row tiles: the products of the multigrid V-cycle's finest "
-                    "level (Chebyshev smoothing and residuals) plus the coarse levels' products")
+            fused = os.environ.get("SVFSI_AMG_FUSED", "") != "0"
+            kname = (f"k_amg_resid_t<{dof}, {tt}> / k_amg_cheb_t<{dof}, {tt}" if fused else
+                     f"k_sspmv<float, {dof}, {tt}")
+            what = ("the finest level of the multigrid V-cycle: FP32 CSR-stream block SpMVs over row tiles "
+                    + ("with the residual (r = b - A x) or the Chebyshev step fused in" if fused else
+                       "of the Chebyshev smoothing and the residuals"))
         elif poly and os.environ.get("SVFSI_SPMV_STREAM", "") != "0":
             # the CSR-stream kernels over row tiles of 256 blocks (512 from 2M blocks on)
             tt = 512 if nnzb >= (2 << 20) else 256

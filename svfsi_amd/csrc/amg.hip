@@ -24,6 +24,7 @@
 
 #include "devutil.hpp"
 #include "internal.hpp"
+#include "stream.hpp"
 
 namespace svfsi {
 
@@ -320,6 +321,80 @@ __global__ __launch_bounds__(256) void k_amg_cheb_step(int nNo, const float* __r
     }
 }
 
+// The same steps fused with their products on the pattern's row tiles
+// (stream_tile; the row sums are those of the CSR-stream SpMV, so the
+// results are the separate kernels' bit for bit).
+// r = b, d = c0 D^-1 b, x = d: the first step from x = 0 with the copy of b
+template <int DOF>
+__global__ __launch_bounds__(256) void k_amg_cheb_init_b(int nNo, const float* __restrict__ dinv,
+                                                         const float* __restrict__ b, float* __restrict__ r,
+                                                         float* __restrict__ d, float* __restrict__ x, float c0)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nNo) return;
+    float rv[DOF];
+    double z[DOF];
+#pragma unroll
+    for (int p = 0; p < DOF; ++p) {
+        rv[p] = b[(size_t)i * DOF + p];
+        r[(size_t)i * DOF + p] = rv[p];
+    }
+    dinv_apply<DOF>(dinv + (size_t)i * DOF * DOF, rv, z);
+#pragma unroll
+    for (int p = 0; p < DOF; ++p) {
+        const size_t q = (size_t)i * DOF + p;
+        const float dv = (float)((double)c0 * z[p]);
+        d[q] = dv;
+        x[q] = dv;
+    }
+}
+
+// r = b - A x
+template <int DOF, int T>
+__global__ __launch_bounds__(256) void k_amg_resid_t(const int* __restrict__ tile, const int* __restrict__ rowPtr,
+                                                     const int* __restrict__ colPtr, const float* __restrict__ val,
+                                                     const float* __restrict__ x, const float* __restrict__ b,
+                                                     float* __restrict__ r)
+{
+    stream_tile<float, DOF, T, false>(blockIdx.x, tile, rowPtr, colPtr, val, x, true,
+                                      [&](int row, int i, float v, float, float, float, float) {
+                                          const size_t q = (size_t)row * DOF + i;
+                                          r[q] = b[q] - v;
+                                      });
+}
+
+// t = A d; r -= t; dn = c1 d + c2 D^-1 r; x += dn.  dn is another buffer
+// than d: other workgroups still gather d.  Every lane of a row reads the
+// row's DOF residual entries before its own store (one wave, in order).
+template <int DOF, int T>
+__global__ __launch_bounds__(256) void k_amg_cheb_t(const int* __restrict__ tile, const int* __restrict__ rowPtr,
+                                                    const int* __restrict__ colPtr, const float* __restrict__ val,
+                                                    const float* __restrict__ dinv, const float* __restrict__ d,
+                                                    float* __restrict__ dn, float* r, float* __restrict__ x,
+                                                    float c1, float c2)
+{
+    stream_tile<float, DOF, T, false>(
+        blockIdx.x, tile, rowPtr, colPtr, val, d, true, [&](int row, int i, float v, float s0, float s1, float s2,
+                                                            float s3) {
+            const size_t base = (size_t)row * DOF;
+            float rv[DOF];
+            rv[0] = r[base] - s0;
+            rv[1] = r[base + 1] - s1;
+            rv[2] = r[base + 2] - s2;
+            if constexpr (DOF == 4) rv[DOF - 1] = r[base + DOF - 1] - s3;
+            const float* di = dinv + (size_t)row * DOF * DOF + (size_t)i * DOF;
+            double z = 0.0;
+#pragma unroll
+            for (int m = 0; m < DOF; ++m) z += (double)di[m] * (double)rv[m];
+            const size_t q = base + i;
+            const float ri = r[q] - v;
+            const float dv = (float)((double)c1 * (double)d[q] + (double)c2 * z);
+            dn[q] = dv;
+            x[q] += dv;
+            r[q] = ri;
+        });
+}
+
 __global__ __launch_bounds__(256) void k_amg_sub(size_t n, const float* __restrict__ b, const float* __restrict__ t,
                                                  float* __restrict__ r)
 {
@@ -527,20 +602,83 @@ inline int spmv_slot(const AmgLevel& l) { return l.pat == &l.A ? 5 : 1; }
 // Chebyshev smoothing of A x = b on D^-1 A, degree deg: x given (setx:
 // starts from zero) and r = b - A x on entry when !setx; on exit r holds
 // b - A x minus the last step's update, x the smoothed iterate.
+// The level's row tiles for the fused kernels (those of its CSR-stream
+// SpMV), or nullptr when the pattern has none (SVFSI_AMG_FUSED=0 also
+// selects the separate kernels).
+inline const int* level_tiles(const AmgLevel& l, int dof, int& T, int& nT)
+{
+    const char* e = getenv("SVFSI_AMG_FUSED");
+    const bool fused = !(e && e[0] == '0');
+    return fused ? stream_tiles(*l.pat, dof, false, T, nT) : nullptr;
+}
+
+// bytes of the level's FP32 SpMV (values, indices, row pointers, x and y once)
+inline double level_spmv_bytes(const AmgLevel& l, int dof)
+{
+    const Lhs& P = *l.pat;
+    return (double)P.nnz * dof * dof * 4.0 + 4.0 * P.nnz + 4.0 * (P.nNo + 1) + 2.0 * P.nNo * dof * 4.0;
+}
+
+// r = b - A x (the residual and its product in one launch when tiled)
 template <int DOF>
-int cheb(AmgLevel& l, int deg, bool setx, hipStream_t s)
+int level_resid(AmgLevel& l, hipStream_t s)
+{
+    const size_t n = (size_t)l.nNo * DOF;
+    int T = 0, nT = 0;
+    if (const int* tl = level_tiles(l, DOF, T, nT)) {
+        ProfScope ps(spmv_slot(l), level_spmv_bytes(l, DOF) + n * 4.0);
+        const int* rp = l.pat->rowPtr.as<int>();
+        const int* cp = l.pat->colPtr.as<int>();
+        if (T == 512)
+            SV_LAUNCH(ps, (k_amg_resid_t<DOF, 512>), dim3((unsigned)nT), dim3(256), 0, s, tl, rp, cp, l.v,
+                      (const float*)l.x, (const float*)l.b, l.r);
+        else
+            SV_LAUNCH(ps, (k_amg_resid_t<DOF, 256>), dim3((unsigned)nT), dim3(256), 0, s, tl, rp, cp, l.v,
+                      (const float*)l.x, (const float*)l.b, l.r);
+        SV_HIP(hipGetLastError());
+        return SVFSI_OK;
+    }
+    SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.x, l.t, s, nullptr, nullptr, spmv_slot(l)));
+    hipLaunchKernelGGL(k_amg_sub, dim3(grid_of(n)), dim3(256), 0, s, n, (const float*)l.b, (const float*)l.t, l.r);
+    SV_HIP(hipGetLastError());
+    return SVFSI_OK;
+}
+
+template <int DOF>
+int cheb(AmgLevel& l, int deg, bool setx, hipStream_t s, bool fromB = false)
 {
     const double th = 0.5 * (l.hi + l.lo), de = 0.5 * (l.hi - l.lo);
     const double s1 = th / de;
     double rho0 = 1.0 / s1;
     const float* dinv = l.dinv.as<float>();
-    hipLaunchKernelGGL(k_amg_cheb_init<DOF>, dim3(grid_of(l.nNo)), dim3(256), 0, s, l.nNo, dinv, (const float*)l.r,
-                       l.d, l.x, (float)(1.0 / th), setx ? 1 : 0);
+    const size_t n = (size_t)l.nNo * DOF;
+    if (fromB)
+        hipLaunchKernelGGL(k_amg_cheb_init_b<DOF>, dim3(grid_of(l.nNo)), dim3(256), 0, s, l.nNo, dinv,
+                           (const float*)l.b, l.r, l.d, l.x, (float)(1.0 / th));
+    else
+        hipLaunchKernelGGL(k_amg_cheb_init<DOF>, dim3(grid_of(l.nNo)), dim3(256), 0, s, l.nNo, dinv,
+                           (const float*)l.r, l.d, l.x, (float)(1.0 / th), setx ? 1 : 0);
+    int T = 0, nT = 0;
+    const int* tl = level_tiles(l, DOF, T, nT);
     for (int k = 1; k < deg; ++k) {
-        SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.d, l.t, s, nullptr, nullptr, spmv_slot(l)));
         const double rho1 = 1.0 / (2.0 * s1 - rho0);
-        hipLaunchKernelGGL(k_amg_cheb_step<DOF>, dim3(grid_of(l.nNo)), dim3(256), 0, s, l.nNo, dinv,
-                           (const float*)l.t, l.r, l.d, l.x, (float)(rho1 * rho0), (float)(2.0 * rho1 / de));
+        const float c1 = (float)(rho1 * rho0), c2 = (float)(2.0 * rho1 / de);
+        if (tl) {
+            ProfScope ps(spmv_slot(l), level_spmv_bytes(l, DOF) + n * 4.0 * 5.0 + (double)l.nNo * DOF * DOF * 4.0);
+            const int* rp = l.pat->rowPtr.as<int>();
+            const int* cp = l.pat->colPtr.as<int>();
+            if (T == 512)
+                SV_LAUNCH(ps, (k_amg_cheb_t<DOF, 512>), dim3((unsigned)nT), dim3(256), 0, s, tl, rp, cp, l.v, dinv,
+                          (const float*)l.d, l.t, l.r, l.x, c1, c2);
+            else
+                SV_LAUNCH(ps, (k_amg_cheb_t<DOF, 256>), dim3((unsigned)nT), dim3(256), 0, s, tl, rp, cp, l.v, dinv,
+                          (const float*)l.d, l.t, l.r, l.x, c1, c2);
+            std::swap(l.d, l.t); // the new direction is in t
+        } else {
+            SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.d, l.t, s, nullptr, nullptr, spmv_slot(l)));
+            hipLaunchKernelGGL(k_amg_cheb_step<DOF>, dim3(grid_of(l.nNo)), dim3(256), 0, s, l.nNo, dinv,
+                               (const float*)l.t, l.r, l.d, l.x, c1, c2);
+        }
         rho0 = rho1;
     }
     SV_HIP(hipGetLastError());
@@ -565,20 +703,17 @@ int vcycle(AmgHier& H, size_t li, hipStream_t s)
         return SVFSI_OK;
     }
     AmgLevel& c = *H.lv[li + 1];
-    // pre-smoothing from x = 0 (r = b), which leaves r = b - A x current
-    SV_HIP(hipMemcpyAsync(l.r, l.b, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-    SV_TRY(cheb<DOF>(l, H.deg, true, s));
-    // the last step's update d is not yet in r
-    SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.x, l.t, s, nullptr, nullptr, spmv_slot(l)));
-    hipLaunchKernelGGL(k_amg_sub, dim3(grid_of(n)), dim3(256), 0, s, n, (const float*)l.b, (const float*)l.t, l.r);
+    // pre-smoothing from x = 0 (r = b, copied by the first step), which
+    // leaves r = b - A x current but for the last step's update
+    SV_TRY(cheb<DOF>(l, H.deg, true, s, true));
+    SV_TRY(level_resid<DOF>(l, s));
     hipLaunchKernelGGL(k_amg_restrict<DOF>, dim3(grid_of((size_t)c.nNo * DOF)), dim3(256), 0, s, c.nNo,
                        l.mPtr.as<int>(), l.mIdx.as<int>(), (const float*)l.r, c.b);
     SV_TRY(vcycle<DOF>(H, li + 1, s));
     hipLaunchKernelGGL(k_amg_prolong<DOF>, dim3(grid_of(n)), dim3(256), 0, s, l.nNo, l.dAgg.as<int>(),
                        (const float*)c.x, l.x);
     // post-smoothing from the corrected x
-    SV_TRY(launch_spmv_f32(*l.pat, DOF, l.v, l.x, l.t, s, nullptr, nullptr, spmv_slot(l)));
-    hipLaunchKernelGGL(k_amg_sub, dim3(grid_of(n)), dim3(256), 0, s, n, (const float*)l.b, (const float*)l.t, l.r);
+    SV_TRY(level_resid<DOF>(l, s));
     SV_TRY(cheb<DOF>(l, H.deg, false, s));
     SV_HIP(hipGetLastError());
     return SVFSI_OK;

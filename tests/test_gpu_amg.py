@@ -61,3 +61,26 @@ def test_amg_inner_reaches_fp64_tolerance(gpu, orc, monkeypatch, which):
     torch.cuda.synchronize()
     assert (ls2.RI.itr, ls2.RI.cM) == (ls.RI.itr, ls.RI.cM)
     assert np.array_equal(st.R.cpu().numpy(), xg)
+
+
+@pytest.mark.parametrize("which", ["lelas", "fluid"])
+def test_amg_fused_kernels_match_separate_ones(gpu, monkeypatch, which):
+    """The V-cycle's residual and Chebyshev steps fused with their products
+    on the row tiles (the default) give the separate kernels' results bit
+    for bit (SVFSI_AMG_FUSED=0): the same row sums, the same arithmetic."""
+    import torch
+
+    monkeypatch.setenv("SVFSI_INNER_AMG", "1")
+    prob = {"fluid": lambda: problems.fluid_cylinder(h=0.4), "lelas": lambda: problems.lelas_cube(8)}[which]()
+    st = NewtonStep(prob)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("SVFSI_AMG_FUSED", fused)
+        st.assemble()
+        ls = st.solve(mixed=True)
+        torch.cuda.synchronize()
+        out[fused] = (ls.RI.itr, ls.RI.cM, ls.RI.suc, st.R.cpu().numpy().copy())
+        say("amg fused", fused, which, "outer", ls.RI.itr, "SpMVs", ls.RI.cM)
+    assert out["1"][:3] == out["0"][:3]
+    assert out["1"][2] == 1
+    assert np.array_equal(out["1"][3], out["0"][3])
