@@ -1224,6 +1224,51 @@ __global__ __launch_bounds__(256) void k_gm_fold(int nb, double* __restrict__ pa
 // (SVFSI_FOLD_CHUNKS overrides, read at each solve).
 constexpr int kFoldChunks = 1024;
 
+// The serial part of an Arnoldi step (one thread): h(i+1,i) (Pythagorean
+// for passes 0-2, hsh[i] = |w|^2 for pass 3), the previous rotations, the
+// new Givens rotation, the Hessenberg column, the residual estimate and the
+// convergence test (GMRES.f:342-366).  hsh: the step's coefficients;
+// c, sn: the previous rotations.
+__device__ void givens_tail(int i, int sD, int pass, double* hsh, const double* c, const double* sn,
+                            GmState* __restrict__ st, GmArrays A)
+{
+    st->iters += 1;
+    double hii = hsh[i];
+    if (pass != 3)
+        for (int jj = 0; jj < i; ++jj) hii = hii - hsh[jj] * hsh[jj];
+    hii = sqrt(fabs(hii));
+    if (pass == 2)
+        for (int jj = 0; jj < i; ++jj) hsh[jj] = A.y[jj] + hsh[jj];
+    hsh[i] = hii;
+    A.hraw[sD + 2] = 1.0 / hii;  // OMPMULV(1/h(i+1,i)) factor for k_gm_update
+    for (int jj = 1; jj <= i - 1; ++jj) {
+        const double tmp = c[jj - 1] * hsh[jj - 1] + sn[jj - 1] * hsh[jj];
+        hsh[jj] = -sn[jj - 1] * hsh[jj - 1] + c[jj - 1] * hsh[jj];
+        hsh[jj - 1] = tmp;
+    }
+    const double tmp = sqrt(hsh[i - 1] * hsh[i - 1] + hsh[i] * hsh[i]);
+    const double ci = hsh[i - 1] / tmp;
+    const double si = hsh[i] / tmp;
+    hsh[i - 1] = tmp;
+    hsh[i] = 0.0;
+    A.c[i - 1] = ci;
+    A.sn[i - 1] = si;
+    double* Hc = A.H + (size_t)(i - 1) * (sD + 1); // column i: Hc[j-1] = h(j,i)
+    for (int jj = 0; jj <= i; ++jj) Hc[jj] = hsh[jj];
+    const double ei = A.err[i - 1];
+    const double eNew = -si * ei;
+    A.err[i] = eNew;
+    A.err[i - 1] = ci * ei;
+    if (fabs(eNew) < st->eps) {
+        st->suc = 1;
+        st->stop = 1;
+        st->iconv = i;
+    } else if (i >= sD) {
+        st->stop = 1;
+        st->iconv = sD;
+    }
+}
+
 // One block: sums the per-block partials in block order (deterministic),
 // then the Pythagorean h(i+1,i) (GMRES.f:342-347), the previous rotations,
 // the new Givens rotation, the residual estimate and the convergence test
@@ -1293,41 +1338,7 @@ __global__ __launch_bounds__(256) void k_gm_givens(int i, int sD, int nb, int ld
         return;
     }
     if (threadIdx.x != 0) return;
-    st->iters += 1;
-    double hii = hsh[i];
-    if (pass != 3)
-        for (int jj = 0; jj < i; ++jj) hii = hii - hsh[jj] * hsh[jj];
-    hii = sqrt(fabs(hii));
-    if (pass == 2)
-        for (int jj = 0; jj < i; ++jj) hsh[jj] = A.y[jj] + hsh[jj];
-    hsh[i] = hii;
-    A.hraw[sD + 2] = 1.0 / hii;  // OMPMULV(1/h(i+1,i)) factor for k_gm_update
-    for (int jj = 1; jj <= i - 1; ++jj) {
-        const double tmp = cs[0][jj - 1] * hsh[jj - 1] + cs[1][jj - 1] * hsh[jj];
-        hsh[jj] = -cs[1][jj - 1] * hsh[jj - 1] + cs[0][jj - 1] * hsh[jj];
-        hsh[jj - 1] = tmp;
-    }
-    const double tmp = sqrt(hsh[i - 1] * hsh[i - 1] + hsh[i] * hsh[i]);
-    const double ci = hsh[i - 1] / tmp;
-    const double si = hsh[i] / tmp;
-    hsh[i - 1] = tmp;
-    hsh[i] = 0.0;
-    A.c[i - 1] = ci;
-    A.sn[i - 1] = si;
-    double* Hc = A.H + (size_t)(i - 1) * (sD + 1); // column i: Hc[j-1] = h(j,i)
-    for (int jj = 0; jj <= i; ++jj) Hc[jj] = hsh[jj];
-    const double ei = A.err[i - 1];
-    const double eNew = -si * ei;
-    A.err[i] = eNew;
-    A.err[i - 1] = ci * ei;
-    if (fabs(eNew) < st->eps) {
-        st->suc = 1;
-        st->stop = 1;
-        st->iconv = i;
-    } else if (i >= sD) {
-        st->stop = 1;
-        st->iconv = sD;
-    }
+    givens_tail(i, sD, pass, hsh, cs[0], cs[1], st, A);
 }
 
 // u_{i+1} = (u_{i+1} - sum_j h(j,i) u_j) * (1/h(i+1,i))   (GMRES.f:342-349)
@@ -1386,6 +1397,39 @@ __global__ __launch_bounds__(256) void k_gm_update(size_t n, int i, int sD, BT* 
         const double sb = block_sum<256>(nrm, lds);
         if (threadIdx.x == 0) normPart[blockIdx.x] = sb;
     }
+}
+
+// The explicit-norm pass's end in one launch: every workgroup sums the nb
+// partials of |w|^2 (the order of k_gm_givens pass 3) and scales its rows
+// of w by 1/|w|; workgroup 0 also makes the rotation and the convergence
+// test.  Once that has set stop the step's result is not used, so
+// workgroups that start later and skip their rows change nothing.
+template <typename BT>
+__global__ __launch_bounds__(256) void k_gm_normscale(size_t n, int i, int sD, int nb,
+                                                      const double* __restrict__ partial, BT* __restrict__ basis,
+                                                      GmState* __restrict__ st, GmArrays A)
+{
+    if (st->stop) return;
+    __shared__ double lds[4];
+    __shared__ double hsh[kMaxSD + 2];
+    __shared__ double cs[2][kMaxSD];
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) t += partial[b];
+    t = block_sum<256>(t, lds);
+    const double inv = 1.0 / sqrt(fabs(t));
+    BT* w = basis + (size_t)i * n;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256)
+        w[q] = (BT)((double)w[q] * inv);
+    if (blockIdx.x != 0) return;
+    for (int jj = threadIdx.x; jj < i; jj += 256) hsh[jj] = A.y[jj];
+    for (int jj = threadIdx.x; jj < i - 1; jj += 256) {
+        cs[0][jj] = A.c[jj];
+        cs[1][jj] = A.sn[jj];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    hsh[i] = t;
+    givens_tail(i, sD, 3, hsh, cs[0], cs[1], st, A);
 }
 
 // w *= 1/h(i+1,i) (the explicit-norm single pass)
@@ -2177,6 +2221,9 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
         return (v == 1 || v == 2) ? v : 3;
     }();
     const int cgsPasses = cgsMode == 2 ? 2 : 1;
+    // the explicit-norm pass's end as one kernel (k_gm_normscale; the same
+    // arithmetic as k_gm_givens pass 3 then k_gm_scale, SVFSI_GM_NORMSCALE=0)
+    const bool fuseNorm = env_int("SVFSI_GM_NORMSCALE", 1) != 0;
     // One V-cycle of the aggregation multigrid (amg.hip) on the FP32 scaled
     // matrix (no block-Jacobi rescaling: the smoother uses the node blocks
     // itself): the default for dof-4 systems whose FP32 matrix exceeds half
@@ -2345,7 +2392,7 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
             (uintptr_t)Q, (uintptr_t)Z, (uintptr_t)t64, (uintptr_t)w.basis32.p, (uintptr_t)w.partial.p,
             (uintptr_t)w.hess.p, (uintptr_t)w.state.p, (uintptr_t)part, (uintptr_t)st, (uintptr_t)w.ohess.p,
             (uintptr_t)L.rowPtr.p, (uintptr_t)L.colPtr.p, (uintptr_t)fused_inner_enabled(), (uintptr_t)dinv,
-            (uintptr_t)cgsMode, (uintptr_t)sizeof(IT), (uintptr_t)((long long)nbr > foldAt)};
+            (uintptr_t)cgsMode, (uintptr_t)sizeof(IT), (uintptr_t)((long long)nbr > foldAt), (uintptr_t)fuseNorm};
         cpl_sig(L, sig);
         if (sig != w.graphs.sig) {
             w.graphs.clear();
@@ -2430,7 +2477,10 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
                     }
                     hipLaunchKernelGGL(k_gm_update<double>, dim3(gv), dim3(256), 0, cs, n, i, mo, Q,
                                        (const GmState*)st, A, cgsMode == 3 ? part : nullptr);
-                    if (cgsMode == 3) {
+                    if (cgsMode == 3 && fuseNorm) {
+                        hipLaunchKernelGGL(k_gm_normscale<double>, dim3(gv), dim3(256), 0, cs, n, i, mo, (int)gv,
+                                           (const double*)part, Q, st, A);
+                    } else if (cgsMode == 3) {
                         hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(256), 0, cs, i, mo, (int)gv, (int)gv,
                                            (const double*)part, st, A, 3);
                         hipLaunchKernelGGL(k_gm_scale<double>, dim3(gv), dim3(256), 0, cs, n, i, mo, Q,

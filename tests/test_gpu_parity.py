@@ -386,6 +386,26 @@ def test_mixed_outer_gram_schmidt(gpu, orc, monkeypatch, cgs, which):
     assert relerr(xg, xo) < 1e-4
 
 
+@pytest.mark.parametrize("which", ["fluid", "lelas"])
+def test_outer_norm_scale_fused(gpu, monkeypatch, which):
+    """The explicit-norm pass's end in one kernel (k_gm_normscale, the
+    default) against k_gm_givens pass 3 then k_gm_scale: bit for bit."""
+    import torch
+
+    prob = problems.fluid_cylinder(h=0.4) if which == "fluid" else problems.lelas_cube(6)
+    st = NewtonStep(prob)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("SVFSI_GM_NORMSCALE", fused)
+        st.assemble()
+        ls = st.solve(mixed=True)
+        torch.cuda.synchronize()
+        out[fused] = (ls.RI.itr, ls.RI.cM, ls.RI.suc, st.R.cpu().numpy().copy())
+        say("normscale", fused, which, "outer", ls.RI.itr)
+    assert out["1"][:3] == out["0"][:3] and out["1"][2] == 1
+    assert np.array_equal(out["1"][3], out["0"][3])
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_gmres_folded_partials(gpu, orc, monkeypatch, mixed):
     """The two-stage sum of the Arnoldi multi-dot partials (k_gm_fold, the
