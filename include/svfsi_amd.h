@@ -176,11 +176,13 @@ int fsils_gpu_solve_dev(void* lhs, FSILS_lsType* ls, int dof, double* dRi,
 /* Mixed-precision form of fsils_gpu_solve_dev (no reference counterpart;
  * the reference is FP64-only, svFSILS/FSILS_TYPEDEF.h:55-57).  Same
  * arguments, scaling, convergence test and outputs.  The solver is always
- * flexible GMRES in FP64 (outer residual f - Ku, two-pass classical
- * Gram-Schmidt Arnoldi, Hessenberg/Givens least squares, solution), with
+ * flexible GMRES in FP64 (outer residual f - Ku, classical Gram-Schmidt
+ * Arnoldi in one pass with the explicit norm (two passes on saddle-point
+ * systems), Hessenberg/Givens least squares, solution), with
  * restart length ls->RI.sD (30 when 0), whose right preconditioner works on
  * an FP32 copy of the scaled matrix: a fixed GMRES polynomial of the
- * (block-Jacobi scaled) operator, or one cycle of an inner FP32 GMRES(30)
+ * (block-Jacobi scaled) operator, one V-cycle of an aggregation multigrid
+ * (dof 4 with an FP32 matrix above 128 MiB), or one cycle of an inner FP32 GMRES(30)
  * stopped at a relative reduction of 0.1 (chosen, without block Jacobi and
  * stopped at 0.01, when unmasked rows of the scaled matrix have a zero
  * diagonal: a saddle-point system such as the Taylor-Hood fluid).  LS_type GMRES, CG and BICGS all
