@@ -1,6 +1,10 @@
 // la.hip -- block-CSR SpMV, reductions and diagonal scaling for gfx950.
 //
-// Block SpMV (FSILS_SPARMULVV, svFSILS/SPARMUL.f:51-133).  Mapping: a group
+// Block SpMV (FSILS_SPARMULVV, svFSILS/SPARMUL.f:51-133).  dof 3 and 4 run
+// on the pattern's row tiles (k_sspmv, k_spoly: stream_tile in stream.hpp,
+// one workgroup per tile of whole rows, flat block loads, row sums in LDS)
+// where stream_tiles selects them; the row-per-lane-group kernels below
+// serve the other cases.  Generic mapping (k_bspmv): a group
 // of DOF*DOF lanes owns one block row; lane r = i*DOF + j always reads entry
 // (i,j) of each block, so a group streams the row's contiguous blocks with
 // one coalesced DOF*DOF-element load per block (72 B FP64 dof 3, 128 B FP64
