@@ -2379,9 +2379,11 @@ static int nested_solve(Lhs& L, FSILS_subLsType& ls, int dof, const double* K, I
         }
     }
     // graph signature: every pointer and size baked into the step launches.
-    // The polynomial steps are launched directly: their ~30 SpMV kernels of
-    // ~9 us keep the queue full, and replaying them as graphs measured
-    // 0.25 ms slower per Newton step on the 200k-tet cylinder.
+    // The polynomial steps are launched directly: their ~25 SpMV kernels of
+    // ~8 us keep the queue full, and replaying them as graphs (keyed by the
+    // factor sequence, bit for bit the same results) measured slower: 6.65
+    // against 6.41 ms per Newton step on the 200k-tet cylinder, 3.07 against
+    // 3.05 ms on the LELAS cube.
     const bool useGraphs = graphs_enabled() && !usePoly && !useAmg;
     if (useGraphs) {
         double tinv = tin;
